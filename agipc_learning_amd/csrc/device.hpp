@@ -1,0 +1,104 @@
+// Device-side data layout shared by the kernels and the host orchestrator.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace masx
+{
+
+enum Status : int
+{
+    kRunning    = 0,
+    kDone       = 1,
+    kCandidate  = 2,  // recursive residual met the tolerance; awaiting a true-residual check
+    kRestarting = 3,  // r := b - A x is pending
+    kInit       = 4,  // first restriction pending
+};
+
+enum Failure : int
+{
+    kOk = 0, kBadRz = 1, kBadCurvature = 2, kNonFinite = 3, kIterationCap = 4
+};
+
+enum Rule : int
+{
+    kRuleRefRtz  = 0,  // pcg_solver.cu:277 -- |rz_prev| <= tol * rz0, one-iteration lag
+    kRuleTrueRel = 1,  // ||r|| <= tol ||b||, FP64 true residual confirms, restart on drift
+};
+
+enum Precision : int
+{
+    kPrecFp32 = 0,
+    kPrecFp16 = 1,
+    kPrecBf16 = 2,
+};
+
+// One per system; 128-byte aligned so systems never share a cache line.
+struct alignas(128) SysState
+{
+    double   rz, rz0, pAp, alpha, beta, rr, bb, true_rr;
+    double   tol;
+    int      k, status, failure, rule;
+    int      max_iter, k_check, check_every, beta_zero;
+    int      need_check, restarts, n_true_checks, pad0;
+    unsigned counter[6];
+};
+
+enum Counter : int { kCntPre = 0, kCntSpmv = 1, kCntUpd = 2, kCntTrue = 3, kCntLoad = 4, kCntAux = 5 };
+
+enum WgType : int { kWgPlain = 0, kWgFemMas = 1, kWgAbd = 2 };
+
+constexpr int kBlock      = 256;  // threads per workgroup of the row kernels
+constexpr int kBodiesPerWg = 21;  // 21 bodies x 12 DOF rows = 252 threads
+
+struct DevData
+{
+    // sizes
+    int n_rows, n_sys, bank, n_banks, n_fine_banks;
+    long long nnz, plane;  // plane = padded entry stride of the SoA value planes
+    // structure
+    const int* row_ptr;
+    const int* col;
+    const int* entry_src;   // full entry -> upper block | transposed<<31
+    const int* row_sys;
+    // values
+    const float*  vals32;   // 9 planes of `plane` floats, entry-major, block row-major (r*3+c)
+    const double* vals64;   // upper blocks, col-major (r + 3c), global upper order
+    // vectors (float3 per internal row)
+    float*  x;
+    float*  r;
+    float*  z;
+    float*  p0;
+    float*  p1;
+    float*  Ap;
+    float*  rt;             // true residual (restart source)
+    double* b64;
+    float*  Rc;             // coarse restriction, float3 per coarse node
+    float*  Zc;
+    // MAS
+    const void*     minv;   // n_banks * nblk * 9 values of the storage precision
+    const int*      bank_vec;
+    const int*      bank_sys;
+    const int*      comp_ptr;
+    const uint32_t* comp_mask;
+    const int*      comp_parent;
+    const int*      coarse_parent;
+    const int*      sys_level_ptr;    // n_sys+1 into level ranges
+    const int*      sys_level_banks;  // pairs [b0, b1) of global bank ids
+    // ABD and diagonal block-Jacobi
+    const float* abd_inv;   // 144 per body, row-major
+    const int*   body_row;
+    const float* diag_inv;  // 9 per internal row, row-major (identity where unused)
+    const int*   row_mode;  // per internal row: 0 identity, 1 diag inverse
+    // work distribution: int4 {sys, aux, begin, end}
+    const int4* wg_pre;
+    const int4* wg_spmv;
+    const int4* wg_upd;
+    const int*  wg_pre_range;   // per system [begin,end) of its workgroups in wg_pre (2 ints)
+    const int*  wg_spmv_range;
+    const int*  wg_upd_range;
+    double*     partial;        // per workgroup partial sums (sized for the largest grid)
+    SysState*   state;
+};
+
+}  // namespace masx

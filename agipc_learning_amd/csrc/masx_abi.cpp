@@ -1,0 +1,558 @@
+// C ABI of the masx library (include/masx.h).
+#include "masx.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "solver.hpp"
+
+namespace
+{
+thread_local std::string g_error;
+
+template <typename F>
+int guard(F&& f)
+{
+    try
+    {
+        f();
+        return 0;
+    }
+    catch(const std::exception& e)
+    {
+        g_error = e.what();
+        return 1;
+    }
+    catch(...)
+    {
+        g_error = "unknown error";
+        return 1;
+    }
+}
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+void require(bool ok, const char* msg)
+{
+    if(!ok) throw std::invalid_argument(msg);
+}
+}  // namespace
+
+struct masx_solver
+{
+    masx::Solver impl;
+};
+
+struct masx_mas
+{
+    int bank = 32;
+    int vert_num = 0, node_offset = 0, neighbor_total = 0, part_map_size = 0, levels = 0;
+    std::vector<uint32_t> nbr_list, nbr_start, nbr_num;
+    std::vector<int>      part_to_real, real_to_part;
+    bool uploaded = false, matrix_ready = false;
+    // structure of the last set_preconditioner_bcoo call
+    std::vector<int>      rows, cols, pairs;
+    std::unique_ptr<masx_solver> solver;
+    uint32_t* d_sel = nullptr;  // selected block indices (device)
+    size_t    d_sel_n = 0;
+    ~masx_mas()
+    {
+        if(d_sel) (void)hipFree(d_sel);
+    }
+};
+
+extern "C" {
+
+const char* masx_last_error(void) { return g_error.c_str(); }
+
+int masx_device_info(char* name, int name_len, int* cu_count, long long* hbm_bytes)
+{
+    return guard([&] {
+        int dev = 0;
+        if(hipGetDevice(&dev) != hipSuccess) throw std::runtime_error("no HIP device");
+        hipDeviceProp_t p;
+        if(hipGetDeviceProperties(&p, dev) != hipSuccess) throw std::runtime_error("hipGetDeviceProperties failed");
+        if(name && name_len > 0)
+        {
+            std::strncpy(name, p.gcnArchName, name_len - 1);
+            name[name_len - 1] = 0;
+        }
+        if(cu_count) *cu_count = p.multiProcessorCount;
+        if(hbm_bytes) *hbm_bytes = (long long)p.totalGlobalMem;
+    });
+}
+
+void masx_default_solve_params(masx_solve_params* p)
+{
+    masx::SolveParams d;
+    p->rule        = d.rule;
+    p->tol         = d.tol;
+    p->max_iter    = d.max_iter;
+    p->check_every = d.check_every;
+    p->chunk       = d.chunk;
+    p->use_graph   = d.use_graph;
+}
+
+int masx_create(masx_solver** out)
+{
+    return guard([&] {
+        require(out != nullptr, "null output pointer");
+        *out = new masx_solver();
+    });
+}
+
+int masx_destroy(masx_solver* s)
+{
+    return guard([&] { delete s; });
+}
+
+int masx_add_system(masx_solver* s, const masx_system_desc* desc, const int* rows, const int* cols, int* sys)
+{
+    return guard([&] {
+        require(s && desc, "null argument");
+        require(desc->nb >= 0 && desc->block_rows > 0, "empty system");
+        require(desc->nb == 0 || (rows && cols), "null rows/cols");
+        masx::SystemDesc d;
+        d.block_rows  = desc->block_rows;
+        d.nb          = desc->nb;
+        d.n_abd       = desc->n_abd;
+        d.fem_offset  = 4 * desc->n_abd;
+        d.n_fem       = desc->n_fem;
+        d.use_mas     = desc->use_mas != 0;
+        d.global_diag = desc->global_diag != 0;
+        d.bank        = desc->bank > 0 ? desc->bank : 32;
+        d.levels      = desc->levels;
+        require(d.bank == 32 || d.bank == 16, "bank size must be 32 or 16");
+        d.rows.assign(rows, rows + desc->nb);
+        d.cols.assign(cols, cols + desc->nb);
+        const int idx = s->impl.add_system(d);
+        if(sys) *sys = idx;
+    });
+}
+
+int masx_set_topology(masx_solver* s, int sys, const uint32_t* nbr_start, const uint32_t* nbr_num,
+                      const uint32_t* nbr_list, int list_len, const int* part_to_real, int part_map_size,
+                      const int* real_to_part)
+{
+    return guard([&] {
+        require(s && sys >= 0 && sys < s->impl.n_systems(), "bad system index");
+        auto& d = s->impl.system(sys);
+        require(part_to_real && real_to_part && part_map_size >= d.n_fem, "bad partition");
+        require(part_map_size % d.bank == 0, "partition size must be a multiple of the bank size");
+        if(d.n_fem > 0) require(nbr_start && nbr_num, "null neighbor arrays");
+        d.nbr_start.assign(nbr_start, nbr_start + d.n_fem);
+        d.nbr_num.assign(nbr_num, nbr_num + d.n_fem);
+        d.nbr_list.assign(nbr_list, nbr_list + list_len);
+        for(int v = 0; v < d.n_fem; ++v)
+            require((long long)nbr_start[v] + nbr_num[v] <= list_len, "neighbor range outside the list");
+        d.part_to_real.assign(part_to_real, part_to_real + part_map_size);
+        d.real_to_part.assign(real_to_part, real_to_part + d.n_fem);
+        for(int v = 0; v < d.n_fem; ++v)
+        {
+            const int slot = real_to_part[v];
+            require(slot >= 0 && slot < part_map_size && part_to_real[slot] == v, "inconsistent partition");
+        }
+    });
+}
+
+int masx_set_topology_morton(masx_solver* s, int sys, const uint32_t* nbr_start, const uint32_t* nbr_num,
+                             const uint32_t* nbr_list, int list_len, const double* positions)
+{
+    return guard([&] {
+        require(s && sys >= 0 && sys < s->impl.n_systems(), "bad system index");
+        auto& d = s->impl.system(sys);
+        std::vector<int> p2r, r2p;
+        masx::morton_partition(d.n_fem, positions, d.bank, p2r, r2p);
+        if(masx_set_topology(s, sys, nbr_start, nbr_num, nbr_list, list_len, p2r.data(), (int)p2r.size(),
+                             r2p.data()))
+            throw std::runtime_error(g_error);
+    });
+}
+
+int masx_set_contacts(masx_solver* s, int sys, const int* pairs, int n_pairs, int node_offset)
+{
+    return guard([&] {
+        require(s && sys >= 0 && sys < s->impl.n_systems(), "bad system index");
+        require(n_pairs == 0 || pairs, "null pairs");
+        auto& d = s->impl.system(sys);
+        d.pairs.assign(pairs, pairs + 4 * (size_t)n_pairs);
+        d.node_offset = node_offset;
+    });
+}
+
+int masx_setup(masx_solver* s)
+{
+    return guard([&] {
+        require(s != nullptr, "null solver");
+        s->impl.setup();
+    });
+}
+
+int masx_set_values(masx_solver* s, int sys, const double* d_values, void* stream)
+{
+    return guard([&] {
+        require(s && sys >= 0 && sys < s->impl.n_systems(), "bad system index");
+        s->impl.set_values(sys, d_values, as_stream(stream));
+    });
+}
+
+int masx_set_values_host(masx_solver* s, int sys, const double* h_values, void* stream)
+{
+    return guard([&] {
+        require(s && sys >= 0 && sys < s->impl.n_systems(), "bad system index");
+        s->impl.set_values_host(sys, h_values, as_stream(stream));
+    });
+}
+
+int masx_build(masx_solver* s, int precision, void* stream)
+{
+    return guard([&] {
+        require(s != nullptr, "null solver");
+        s->impl.build(precision, as_stream(stream));
+    });
+}
+
+int masx_solve(masx_solver* s, const double* d_b, double* d_x, const masx_solve_params* params,
+               masx_solve_stats* stats, void* stream)
+{
+    return guard([&] {
+        require(s && d_b && d_x, "null argument");
+        masx::SolveParams p;
+        if(params)
+        {
+            p.rule        = params->rule;
+            p.tol         = params->tol;
+            p.max_iter    = params->max_iter;
+            p.check_every = params->check_every;
+            p.chunk       = params->chunk;
+            p.use_graph   = params->use_graph;
+        }
+        require(p.rule == MASX_RULE_REF_RTZ || p.rule == MASX_RULE_TRUE_REL, "unknown stopping rule");
+        require(p.tol >= 0.0, "negative tolerance");
+        std::vector<masx::SolveStats> st;
+        s->impl.solve(d_b, d_x, p, stats ? &st : nullptr, as_stream(stream));
+        if(stats)
+            for(size_t i = 0; i < st.size(); ++i)
+            {
+                stats[i].iterations        = st[i].iterations;
+                stats[i].status            = st[i].status == masx::kDone ? MASX_DONE : MASX_RUNNING;
+                stats[i].failure           = st[i].failure;
+                stats[i].restarts          = st[i].restarts;
+                stats[i].true_checks       = st[i].true_checks;
+                stats[i].rz0               = st[i].rz0;
+                stats[i].rz                = st[i].rz;
+                stats[i].rel_res_recursive = st[i].rel_res_recursive;
+                stats[i].rel_res_true      = st[i].rel_res_true;
+            }
+    });
+}
+
+int masx_spmv(masx_solver* s, double a, const double* d_x, double b, double* d_y, void* stream)
+{
+    return guard([&] {
+        require(s && d_x && d_y, "null argument");
+        s->impl.spmv(a, d_x, b, d_y, as_stream(stream));
+    });
+}
+
+int masx_apply_preconditioner(masx_solver* s, const double* d_r, double* d_z, void* stream)
+{
+    return guard([&] {
+        require(s && d_r && d_z, "null argument");
+        s->impl.apply_preconditioner(d_r, d_z, as_stream(stream));
+    });
+}
+
+int masx_set_spmv_tiling(masx_solver* s, int lanes_per_row, int rows_per_wg)
+{
+    return guard([&] {
+        require(s != nullptr, "null solver");
+        s->impl.set_spmv_tiling(lanes_per_row, rows_per_wg);
+    });
+}
+
+int masx_time_kernel(masx_solver* s, int which, int reps, double* ms)
+{
+    return guard([&] {
+        require(s && ms && reps > 0, "bad argument");
+        *ms = s->impl.time_kernel(which, reps);
+    });
+}
+
+int masx_plan_info_get(masx_solver* s, masx_plan_info* info)
+{
+    return guard([&] {
+        require(s && info && s->impl.is_setup(), "solver not set up");
+        const auto& P       = s->impl.plan();
+        info->n_rows        = P.n_rows;
+        info->n_ext         = P.n_ext;
+        info->n_sys         = P.n_sys;
+        info->bank          = P.bank;
+        info->nnz           = P.nnz;
+        info->nb_total      = P.nb_total;
+        info->n_banks       = P.n_banks;
+        info->n_fine_banks  = P.n_fine_banks;
+        info->n_coarse      = P.n_coarse;
+        info->n_bodies      = P.n_bodies;
+        info->n_gal         = P.n_gal;
+        info->inverse_bytes = (long long)s->impl.inverse_bytes();
+    });
+}
+
+int masx_get_hierarchy(masx_solver* s, int sys, int* levels, int* total, int* level_count, int* level_begin,
+                       uint32_t* fine_mask, int* prefix, int* going_next, int* coarse_table)
+{
+    return guard([&] {
+        require(s && s->impl.is_setup() && sys >= 0 && sys < s->impl.n_systems(), "bad system index");
+        const auto& h = s->impl.system(sys).hier;
+        require(s->impl.system(sys).use_mas, "system has no MAS hierarchy");
+        if(levels) *levels = h.levels;
+        if(total) *total = h.total;
+        if(level_count)
+        {
+            level_count[0] = 0;
+            for(int l = 1; l <= h.levels; ++l) level_count[l] = h.level_count[l];
+        }
+        if(level_begin) std::memcpy(level_begin, h.level_begin.data(), sizeof(int) * (h.levels + 1));
+        if(fine_mask) std::memcpy(fine_mask, h.fine_mask.data(), sizeof(uint32_t) * h.n_fem);
+        if(prefix) std::memcpy(prefix, h.prefix.data(), sizeof(int) * h.prefix.size());
+        if(going_next) std::memcpy(going_next, h.going_next.data(), sizeof(int) * h.total);
+        if(coarse_table) std::memcpy(coarse_table, h.coarse_table.data(), sizeof(int) * 6 * (size_t)h.n_fem);
+    });
+}
+
+int masx_get_inverse(masx_solver* s, float* out, long long capacity)
+{
+    return guard([&] {
+        require(s && out, "null argument");
+        std::vector<float> v;
+        s->impl.download_inverse(v);
+        require((long long)v.size() <= capacity, "output buffer too small");
+        std::memcpy(out, v.data(), v.size() * sizeof(float));
+    });
+}
+
+int masx_bank_of_slot(masx_solver* s, int sys, int slot, int* bank)
+{
+    return guard([&] {
+        require(s && bank && s->impl.is_setup(), "bad argument");
+        const auto& P = s->impl.plan();
+        const auto& d = s->impl.system(sys);
+        require(d.use_mas, "system has no MAS hierarchy");
+        const auto& h = d.hier;
+        require(slot >= 0 && slot < h.total, "slot out of range");
+        if(slot < h.n_slots)
+        {
+            int fine_base = 0;
+            for(int t = 0; t < sys; ++t)
+                if(s->impl.system(t).use_mas) fine_base += s->impl.system(t).hier.n_slots / P.bank;
+            *bank = fine_base + slot / P.bank;
+        }
+        else
+            *bank = P.n_fine_banks + (P.coarse_base[sys] + slot - h.n_slots) / P.bank;
+    });
+}
+
+int masx_factor_status(masx_solver* s, long long* count)
+{
+    return guard([&] {
+        require(s && count, "null argument");
+        *count = s->impl.factor_status();
+    });
+}
+
+int masx_hierarchy_host(int n_fem, int n_slots, int bank, int levels, const uint32_t* nbr_start,
+                        const uint32_t* nbr_num, const uint32_t* nbr_list, const int* part_to_real,
+                        const int* real_to_part, const int* pairs, int n_pairs, int node_offset, int* out_levels,
+                        int* out_total, int* level_count, int* level_begin, uint32_t* fine_mask, int* prefix,
+                        int* going_next, long long going_next_cap, int* coarse_table)
+{
+    return guard([&] {
+        require(n_fem >= 1 && n_slots >= n_fem && part_to_real && real_to_part, "bad argument");
+        masx::Hierarchy h;
+        masx::build_hierarchy(h, n_fem, n_slots, bank, levels, nbr_start, nbr_num, nbr_list, part_to_real,
+                              real_to_part, pairs, n_pairs, node_offset);
+        if(out_levels) *out_levels = h.levels;
+        if(out_total) *out_total = h.total;
+        if(!going_next) return;
+        require(going_next_cap >= (long long)std::max(h.total, n_fem), "going_next buffer too small");
+        if(level_count)
+        {
+            level_count[0] = 0;
+            for(int l = 1; l <= h.levels; ++l) level_count[l] = h.level_count[l];
+        }
+        if(level_begin) std::memcpy(level_begin, h.level_begin.data(), sizeof(int) * (h.levels + 1));
+        if(fine_mask) std::memcpy(fine_mask, h.fine_mask.data(), sizeof(uint32_t) * n_fem);
+        if(prefix) std::memcpy(prefix, h.prefix.data(), sizeof(int) * h.prefix.size());
+        std::memcpy(going_next, h.going_next.data(), sizeof(int) * std::max(h.total, n_fem));
+        if(coarse_table) std::memcpy(coarse_table, h.coarse_table.data(), sizeof(int) * 6 * (size_t)n_fem);
+    });
+}
+
+// ------------------------------------------------------------------ MAS ----
+
+int masx_mas_create(masx_mas** out, int bank)
+{
+    return guard([&] {
+        require(out != nullptr, "null output pointer");
+        require(bank == 32 || bank == 16, "bank size must be 32 or 16");
+        auto* m = new masx_mas();
+        m->bank = bank;
+        *out    = m;
+    });
+}
+
+int masx_mas_destroy(masx_mas* m)
+{
+    return guard([&] { delete m; });
+}
+
+int masx_mas_init_neighbor(masx_mas* m, int vert_num, int collision_node_offset, int total_neighbor_num,
+                           int part_map_size)
+{
+    return guard([&] {
+        require(m && vert_num >= 1 && total_neighbor_num >= 0, "bad argument");
+        require(part_map_size >= vert_num && part_map_size % m->bank == 0, "bad partition size");
+        m->vert_num       = vert_num;
+        m->node_offset    = collision_node_offset;
+        m->neighbor_total = total_neighbor_num;
+        m->part_map_size  = part_map_size;
+    });
+}
+
+int masx_mas_upload(masx_mas* m, const uint32_t* neighbor_list, const uint32_t* neighbor_start,
+                    const uint32_t* neighbor_num, const int* part_to_real, const int* real_to_part)
+{
+    return guard([&] {
+        require(m && m->vert_num > 0, "init_neighbor must come first");
+        require(neighbor_start && neighbor_num && part_to_real && real_to_part, "null argument");
+        require(m->neighbor_total == 0 || neighbor_list, "null neighbor list");
+        m->nbr_list.assign(neighbor_list, neighbor_list + m->neighbor_total);
+        m->nbr_start.assign(neighbor_start, neighbor_start + m->vert_num);
+        m->nbr_num.assign(neighbor_num, neighbor_num + m->vert_num);
+        m->part_to_real.assign(part_to_real, part_to_real + m->part_map_size);
+        m->real_to_part.assign(real_to_part, real_to_part + m->vert_num);
+        m->uploaded = true;
+    });
+}
+
+int masx_mas_init_matrix(masx_mas* m, int levels)
+{
+    return guard([&] {
+        require(m && m->uploaded, "upload must come first");
+        m->levels       = levels;
+        m->matrix_ready = true;
+    });
+}
+
+int masx_mas_set_preconditioner_bcoo(masx_mas* m, const double* d_values, const int* d_rows, const int* d_cols,
+                                     const uint32_t* d_indices, int offset, int triplet_num, const int* d_pairs,
+                                     int cp_num, int precision, void* stream)
+{
+    return guard([&] {
+        require(m && m->matrix_ready, "init_matrix must come first");
+        require(d_values && d_rows && d_cols && triplet_num >= 0, "bad matrix arguments");
+        hipStream_t st = as_stream(stream);
+        // structure back to the host (the reference also synchronizes here:
+        // i_preconditioner.cu:83-86 and TraditionalMAS32Preconditioner.cu:2211)
+        std::vector<uint32_t> idx(triplet_num);
+        if(d_indices)
+        {
+            if(triplet_num)
+                if(hipMemcpyAsync(idx.data(), d_indices, sizeof(uint32_t) * triplet_num, hipMemcpyDeviceToHost, st)
+                   != hipSuccess)
+                    throw std::runtime_error("index copy failed");
+        }
+        else
+            for(int i = 0; i < triplet_num; ++i) idx[i] = (uint32_t)i;
+        if(hipStreamSynchronize(st) != hipSuccess) throw std::runtime_error("stream sync failed");
+        uint32_t max_idx = 0;
+        for(uint32_t v : idx) max_idx = std::max(max_idx, v);
+        const size_t n_all = triplet_num ? (size_t)max_idx + 1 : 0;
+        std::vector<int> all_r(n_all), all_c(n_all);
+        if(n_all)
+        {
+            if(hipMemcpy(all_r.data(), d_rows, sizeof(int) * n_all, hipMemcpyDeviceToHost) != hipSuccess ||
+               hipMemcpy(all_c.data(), d_cols, sizeof(int) * n_all, hipMemcpyDeviceToHost) != hipSuccess)
+                throw std::runtime_error("structure copy failed");
+        }
+        std::vector<int> rows(triplet_num), cols(triplet_num), pairs(4 * (size_t)cp_num);
+        for(int i = 0; i < triplet_num; ++i)
+        {
+            rows[i] = all_r[idx[i]] - offset;
+            cols[i] = all_c[idx[i]] - offset;
+            require(rows[i] >= 0 && cols[i] >= 0 && rows[i] < m->vert_num && cols[i] < m->vert_num,
+                    "selected block outside the FEM range");
+        }
+        if(cp_num)
+        {
+            require(d_pairs != nullptr, "null contact pairs");
+            if(hipMemcpy(pairs.data(), d_pairs, sizeof(int) * 4 * cp_num, hipMemcpyDeviceToHost) != hipSuccess)
+                throw std::runtime_error("pair copy failed");
+        }
+        const bool same = m->solver && rows == m->rows && cols == m->cols && pairs == m->pairs;
+        if(!same)
+        {
+            m->solver = std::make_unique<masx_solver>();
+            masx::SystemDesc d;
+            d.block_rows   = m->vert_num;
+            d.nb           = triplet_num;
+            d.n_abd        = 0;
+            d.fem_offset   = 0;
+            d.n_fem        = m->vert_num;
+            d.use_mas      = true;
+            d.bank         = m->bank;
+            d.levels       = m->levels;
+            d.rows         = rows;
+            d.cols         = cols;
+            d.nbr_start    = m->nbr_start;
+            d.nbr_num      = m->nbr_num;
+            d.nbr_list     = m->nbr_list;
+            d.part_to_real = m->part_to_real;
+            d.real_to_part = m->real_to_part;
+            d.pairs        = pairs;
+            d.node_offset  = m->node_offset;
+            m->solver->impl.add_system(d);
+            m->solver->impl.setup();
+            m->rows  = std::move(rows);
+            m->cols  = std::move(cols);
+            m->pairs = std::move(pairs);
+            if(m->d_sel) (void)hipFree(m->d_sel);
+            m->d_sel   = nullptr;
+            m->d_sel_n = 0;
+        }
+        if(triplet_num)
+        {
+            if(m->d_sel_n < (size_t)triplet_num)
+            {
+                if(m->d_sel) (void)hipFree(m->d_sel);
+                if(hipMalloc((void**)&m->d_sel, sizeof(uint32_t) * triplet_num) != hipSuccess)
+                    throw std::runtime_error("allocation failed");
+                m->d_sel_n = triplet_num;
+            }
+            if(hipMemcpyAsync(m->d_sel, idx.data(), sizeof(uint32_t) * triplet_num, hipMemcpyHostToDevice, st)
+               != hipSuccess)
+                throw std::runtime_error("index upload failed");
+            m->solver->impl.gather_values(0, d_values, m->d_sel, st);
+        }
+        m->solver->impl.build(precision, st);
+        if(hipStreamSynchronize(st) != hipSuccess) throw std::runtime_error("build failed");
+    });
+}
+
+int masx_mas_preconditioning(masx_mas* m, const double* d_R, double* d_Z, void* stream)
+{
+    return guard([&] {
+        require(m && m->solver, "set_preconditioner_bcoo must come first");
+        m->solver->impl.apply_preconditioner(d_R, d_Z, as_stream(stream));
+    });
+}
+
+masx_solver* masx_mas_solver(masx_mas* m) { return m ? m->solver.get() : nullptr; }
+
+}  // extern "C"

@@ -1,0 +1,647 @@
+// Host-side structure analysis (see plan.hpp).
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+#include <unordered_map>
+
+namespace masx
+{
+namespace
+{
+inline int pad_to(int v, int b) { return (v + b - 1) / b * b; }
+inline int packed_index(int bank, int r, int c) { return bank * r - r * (r + 1) / 2 + c; }
+inline uint32_t lane_bit(int l) { return 1u << l; }
+}  // namespace
+
+int reference_level_count(int vert_num, int part_map_size, int bank)
+{
+    // TraditionalMAS32Preconditioner.cu:2142-2161 with the max of :2898
+    const int max_nodes = std::max(vert_num, part_map_size);
+    int       levels    = 1;
+    int       sz        = pad_to(max_nodes, bank);
+    while(sz > bank)
+    {
+        sz = pad_to(sz / bank, bank);
+        ++levels;
+    }
+    return std::min(levels + 1, 6);
+}
+
+int decode_pair(const int* p, int v[4])
+{
+    // sign encoding of the collision pair list (TraditionalMAS32Preconditioner.cu:1550-1960)
+    const int x = p[0], y = p[1], z = p[2], w = p[3];
+    if(x >= 0)
+    {
+        v[0] = x; v[1] = y; v[2] = z; v[3] = w < 0 ? -w - 1 : w;
+        return 4;
+    }
+    v[0] = -x - 1;
+    if(z < 0)
+    {
+        if(y < 0)
+        {
+            v[1] = -y - 1; v[2] = -z - 1; v[3] = -w - 1;
+            return 4;
+        }
+        v[1] = y;
+        return 2;
+    }
+    if(w < 0)
+    {
+        if(y < 0)
+        {
+            v[1] = -y - 1; v[2] = z; v[3] = -w - 1;
+            return 4;
+        }
+        v[1] = y; v[2] = z;
+        return 3;
+    }
+    v[1] = y; v[2] = z; v[3] = w;
+    return 4;
+}
+
+void morton_partition(int n, const double* pos, int bank, std::vector<int>& p2r, std::vector<int>& r2p)
+{
+    // gl_main.cu:1176-1243: 10 bits per axis over the bounding box, x | y<<1 | z<<2,
+    // ties broken by index, banks padded with -1 at the end
+    auto spread = [](uint32_t v) {
+        uint64_t x = v & 0x3ffu, out = 0;
+        for(int b = 0; b < 10; ++b) out |= ((x >> b) & 1u) << (3 * b);
+        return (uint32_t)out;
+    };
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for(int d = 0; d < 3 && n > 0; ++d) lo[d] = hi[d] = pos[d];
+    for(int v = 1; v < n; ++v)
+        for(int d = 0; d < 3; ++d)
+        {
+            lo[d] = std::min(lo[d], pos[3 * v + d]);
+            hi[d] = std::max(hi[d], pos[3 * v + d]);
+        }
+    std::vector<std::pair<uint32_t, int>> key(n);
+    for(int v = 0; v < n; ++v)
+    {
+        uint32_t code = 0;
+        for(int d = 0; d < 3; ++d)
+        {
+            const double e = hi[d] - lo[d];
+            const double t = e > 0.0 ? std::clamp((pos[3 * v + d] - lo[d]) / e, 0.0, 1.0) : 0.0;
+            code |= spread((uint32_t)(t * 1023.0 + 0.5)) << d;
+        }
+        key[v] = {code, v};
+    }
+    std::sort(key.begin(), key.end());
+    p2r.assign(pad_to(n, bank), -1);
+    r2p.assign(n, 0);
+    for(int s = 0; s < n; ++s)
+    {
+        p2r[s]               = key[s].second;
+        r2p[key[s].second] = s;
+    }
+}
+
+// Components inside one bank by label propagation over lane bit sets.
+// adj[l] holds lane l's own bit plus its neighbours' bits.
+static void bank_components(const uint32_t* adj, uint32_t valid, uint32_t* comp)
+{
+    uint32_t done = 0;
+    for(int l = 0; l < 32; ++l)
+    {
+        if(!(valid & lane_bit(l)) || (done & lane_bit(l))) continue;
+        uint32_t set = lane_bit(l), frontier = lane_bit(l);
+        while(frontier)
+        {
+            uint32_t next = 0;
+            for(uint32_t f = frontier; f; f &= f - 1) next |= adj[__builtin_ctz(f)];
+            frontier = next & ~set;
+            set |= next;
+        }
+        for(uint32_t f = set; f; f &= f - 1) comp[__builtin_ctz(f)] = set;
+        done |= set;
+    }
+}
+
+void build_hierarchy(Hierarchy& h, int n_fem, int n_slots, int bank, int levels, const uint32_t* nstart,
+                     const uint32_t* nnum, const uint32_t* nlist, const int* p2r, const int* r2p,
+                     const int* pairs, int n_pairs, int node_offset)
+{
+    if(bank < 2 || bank > 32) throw std::invalid_argument("bank size must be in [2, 32]");
+    if(n_slots % bank || n_slots < n_fem) throw std::invalid_argument("partition size must be a bank multiple >= vertex count");
+    h.bank    = bank;
+    h.n_fem   = n_fem;
+    h.n_slots = n_slots;
+    h.levels  = levels > 0 ? std::min(levels, 6) : reference_level_count(n_fem, n_slots, bank);
+    const int L = h.levels;
+    h.level_count.assign(L + 1, 0);
+    h.level_begin.assign(L + 1, 0);
+    h.level_count[0] = n_slots;
+    h.fine_mask.assign(n_fem, 0);
+    h.prefix.assign(n_slots / bank, 0);
+    h.coarse_table.assign((size_t)n_fem * 6, 0);
+
+    // undirected links between fine vertices: mesh edges + every pair of
+    // distinct in-range vertices of each contact
+    std::vector<std::pair<int, int>> links;
+    links.reserve(nstart ? (size_t)n_fem * 8 : 0);
+    for(int v = 0; v < n_fem && nstart; ++v)
+        for(uint32_t i = 0; i < nnum[v]; ++i)
+        {
+            const int u = (int)nlist[nstart[v] + i];
+            if(u < 0 || u >= n_fem) throw std::invalid_argument("neighbor id out of range");
+            if(u > v) links.push_back({v, u});
+        }
+    for(int k = 0; k < n_pairs; ++k)
+    {
+        int       vv[4];
+        const int cnt = decode_pair(pairs + 4 * k, vv);
+        for(int i = 0; i < cnt; ++i) vv[i] -= node_offset;
+        for(int i = 0; i < cnt; ++i)
+            for(int j = i + 1; j < cnt; ++j)
+                if(vv[i] >= 0 && vv[j] >= 0 && vv[i] < n_fem && vv[j] < n_fem && vv[i] != vv[j])
+                    links.push_back({vv[i], vv[j]});
+    }
+
+    std::vector<int> id(r2p, r2p + n_fem);  // fine vertex -> node at the current level
+    int count = n_slots, begin = 0;
+    std::vector<uint32_t> adj, comp;
+    std::vector<int>      newid;
+    std::vector<std::vector<std::pair<uint32_t, int>>> bank_comps;  // per slot bank, all levels
+    const int max_slots = n_slots * 2 + bank * (L + 2);
+    bank_comps.reserve(max_slots / bank + 1);
+    h.going_next.assign(max_slots, -1);
+    for(int level = 0; level < L; ++level)
+    {
+        const int nbanks = (count + bank - 1) / bank;
+        adj.assign((size_t)nbanks * bank, 0u);
+        comp.assign((size_t)nbanks * bank, 0u);
+        std::vector<uint32_t> valid(nbanks, 0u);
+        for(int c = 0; c < count; ++c)
+        {
+            const bool ok = level > 0 || p2r[c] >= 0;
+            if(!ok) continue;
+            adj[c] |= lane_bit(c % bank);
+            valid[c / bank] |= lane_bit(c % bank);
+        }
+        for(const auto& e : links)
+        {
+            const int a = id[e.first], b = id[e.second];
+            if(a != b && a / bank == b / bank)
+            {
+                adj[a] |= lane_bit(b % bank);
+                adj[b] |= lane_bit(a % bank);
+            }
+        }
+        newid.assign((size_t)nbanks * bank, -1);
+        int next = 0;
+        const int next_begin = level == 0 ? pad_to(n_slots, bank) : begin + pad_to(count, bank);
+        for(int b = 0; b < nbanks; ++b)
+        {
+            bank_components(&adj[(size_t)b * bank], valid[b], &comp[(size_t)b * bank]);
+            int ncomp = 0;
+            std::vector<std::pair<uint32_t, int>> comps;
+            for(int l = 0; l < bank; ++l)
+            {
+                if(!(valid[b] & lane_bit(l))) continue;
+                const uint32_t m = comp[(size_t)b * bank + l];
+                if(__builtin_ctz(m) == l)  // lowest lane leads the component
+                {
+                    comps.push_back({m, next_begin + next});
+                    for(uint32_t f = m; f; f &= f - 1) newid[(size_t)b * bank + __builtin_ctz(f)] = next;
+                    ++next;
+                    ++ncomp;
+                }
+            }
+            if(level == 0) h.prefix[b] = ncomp;
+            if(level + 1 < L)
+                bank_comps.push_back(std::move(comps));
+            else
+                bank_comps.push_back({});
+        }
+        if(level == 0)
+            for(int v = 0; v < n_fem; ++v) h.fine_mask[v] = comp[r2p[v]];
+        h.level_count[level + 1 <= L ? level + 1 : L] = level + 1 <= L ? next : 0;
+        if(level + 1 <= L) h.level_begin[level + 1] = next_begin;
+        if(level == 0)
+        {
+            for(int v = 0; v < n_fem; ++v) h.going_next[v] = newid[r2p[v]] + next_begin;
+        }
+        else
+        {
+            if(begin + count > (int)h.going_next.size()) h.going_next.resize(begin + count + bank, -1);
+            for(int c = 0; c < count; ++c) h.going_next[begin + c] = newid[c] + next_begin;
+        }
+        for(int v = 0; v < n_fem; ++v)
+        {
+            id[v] = newid[id[v]];
+            if(level < L - 1) h.coarse_table[6 * (size_t)v + level] = 0;  // filled below
+        }
+        // drop links that became internal to one node
+        links.erase(std::remove_if(links.begin(), links.end(),
+                                   [&](const std::pair<int, int>& e) { return id[e.first] == id[e.second]; }),
+                    links.end());
+        begin = next_begin;
+        count = next;
+    }
+    h.total = h.level_begin[L];
+    h.going_next.resize(std::max(h.total, n_fem), -1);
+    for(int v = 0; v < n_fem; ++v)
+    {
+        int cur = v;
+        for(int l = 0; l < L - 1; ++l)
+        {
+            cur                             = h.going_next[cur];
+            h.coarse_table[6 * (size_t)v + l] = cur;
+        }
+    }
+    // component lists for the banks of levels 0..L-2 (slots [0, level_begin[L-1]))
+    const int total_banks = h.total / bank;
+    h.comp_ptr.assign(total_banks + 1, 0);
+    h.comp_mask.clear();
+    h.comp_parent.clear();
+    // bank_comps is ordered by level then bank; map to slot banks
+    int idx = 0;
+    std::vector<int> bank_of_entry;
+    for(int level = 0; level < L; ++level)
+    {
+        const int b0 = (level == 0 ? 0 : h.level_begin[level]) / bank;
+        const int nb = (level == 0 ? n_slots : pad_to(h.level_count[level], bank)) / bank;
+        for(int b = 0; b < nb; ++b, ++idx)
+        {
+            h.comp_ptr[b0 + b + 1] = (int)bank_comps[idx].size();
+        }
+    }
+    for(int b = 0; b < total_banks; ++b) h.comp_ptr[b + 1] += h.comp_ptr[b];
+    h.comp_mask.resize(h.comp_ptr[total_banks]);
+    h.comp_parent.resize(h.comp_ptr[total_banks]);
+    idx = 0;
+    for(int level = 0; level < L; ++level)
+    {
+        const int b0 = (level == 0 ? 0 : h.level_begin[level]) / bank;
+        const int nb = (level == 0 ? n_slots : pad_to(h.level_count[level], bank)) / bank;
+        for(int b = 0; b < nb; ++b, ++idx)
+            for(size_t k = 0; k < bank_comps[idx].size(); ++k)
+            {
+                h.comp_mask[h.comp_ptr[b0 + b] + k]   = bank_comps[idx][k].first;
+                h.comp_parent[h.comp_ptr[b0 + b] + k] = bank_comps[idx][k].second;
+            }
+    }
+}
+
+// ----------------------------------------------------------------- plan ----
+
+void build_plan(Plan& P, std::vector<SystemDesc>& sys)
+{
+    const int S = (int)sys.size();
+    if(S == 0) throw std::invalid_argument("no systems");
+    const int bank = sys[0].bank;
+    for(auto& s : sys)
+        if(s.use_mas && s.bank != bank) throw std::invalid_argument("all systems of a batch share one bank size");
+    P     = Plan{};
+    P.bank = bank;
+    P.n_sys = S;
+    P.row_off.assign(S + 1, 0);
+    P.ext_off.assign(S + 1, 0);
+    P.fem_row0.assign(S, 0);
+    P.abd_row0.assign(S, 0);
+    P.src_off.assign(S + 1, 0);
+    P.coarse_base.assign(S + 1, 0);
+    // ---- rows
+    for(int s = 0; s < S; ++s)
+    {
+        auto& d = sys[s];
+        if(d.fem_offset != 4 * d.n_abd || d.block_rows != d.fem_offset + d.n_fem)
+            throw std::invalid_argument("block rows must be [4 per affine body][FEM vertices]");
+        if(d.use_mas)
+        {
+            if((int)d.part_to_real.size() % bank || (int)d.real_to_part.size() != d.n_fem)
+                throw std::invalid_argument("partition arrays have the wrong size");
+        }
+        const int abd_pad = pad_to(4 * d.n_abd, bank);
+        const int fem_rows = d.use_mas ? (int)d.part_to_real.size() : pad_to(d.n_fem, bank);
+        P.abd_row0[s]     = P.row_off[s];
+        P.fem_row0[s]     = P.row_off[s] + abd_pad;
+        P.row_off[s + 1]  = P.row_off[s] + abd_pad + fem_rows;
+        P.ext_off[s + 1]  = P.ext_off[s] + d.block_rows;
+        P.src_off[s + 1]  = P.src_off[s] + d.nb;
+    }
+    P.n_rows   = P.row_off[S];
+    P.n_ext    = P.ext_off[S];
+    P.nb_total = P.src_off[S];
+    if(P.nb_total >= (1ll << 31) - 1) throw std::invalid_argument("too many blocks for 32-bit indices");
+    P.ext_to_int.assign(P.n_ext, -1);
+    P.row_sys.assign(P.n_rows, 0);
+    for(int s = 0; s < S; ++s)
+    {
+        auto& d = sys[s];
+        for(int r = 0; r < 4 * d.n_abd; ++r) P.ext_to_int[P.ext_off[s] + r] = P.abd_row0[s] + r;
+        for(int v = 0; v < d.n_fem; ++v)
+            P.ext_to_int[P.ext_off[s] + d.fem_offset + v] = P.fem_row0[s] + (d.use_mas ? d.real_to_part[v] : v);
+        for(int r = P.row_off[s]; r < P.row_off[s + 1]; ++r) P.row_sys[r] = s;
+    }
+    // ---- full block CSR (both triangles)
+    std::vector<int> cnt(P.n_rows + 1, 0);
+    P.dest1.assign(P.nb_total, -1);
+    P.dest2.assign(P.nb_total, -1);
+    for(int s = 0; s < S; ++s)
+    {
+        auto& d = sys[s];
+        if((long long)d.rows.size() != d.nb || (long long)d.cols.size() != d.nb)
+            throw std::invalid_argument("row/col arrays do not match the block count");
+        for(long long k = 0; k < d.nb; ++k)
+        {
+            if(d.rows[k] < 0 || d.rows[k] >= d.block_rows || d.cols[k] < 0 || d.cols[k] >= d.block_rows)
+                throw std::invalid_argument("block index out of range");
+            const int R = P.ext_to_int[P.ext_off[s] + d.rows[k]];
+            const int C = P.ext_to_int[P.ext_off[s] + d.cols[k]];
+            cnt[R + 1]++;
+            if(d.rows[k] != d.cols[k]) cnt[C + 1]++;
+        }
+    }
+    P.row_ptr.assign(P.n_rows + 1, 0);
+    for(int r = 0; r < P.n_rows; ++r) P.row_ptr[r + 1] = P.row_ptr[r] + cnt[r + 1];
+    P.nnz = P.row_ptr[P.n_rows];
+    P.col.assign(P.nnz, 0);
+    P.entry_src.assign(P.nnz, 0);
+    {
+        std::vector<int> fill(P.row_ptr.begin(), P.row_ptr.end() - 1);
+        for(int s = 0; s < S; ++s)
+        {
+            auto& d = sys[s];
+            for(long long k = 0; k < d.nb; ++k)
+            {
+                const int g = (int)(P.src_off[s] + k);
+                const int R = P.ext_to_int[P.ext_off[s] + d.rows[k]];
+                const int C = P.ext_to_int[P.ext_off[s] + d.cols[k]];
+                int e = fill[R]++;
+                P.col[e] = C;
+                P.entry_src[e] = g;
+                if(d.rows[k] != d.cols[k])
+                {
+                    e = fill[C]++;
+                    P.col[e] = R;
+                    P.entry_src[e] = g | (int)0x80000000u;
+                }
+            }
+        }
+        // sort each row by column (stable) for gather locality
+        std::vector<std::pair<int, int>> tmp;
+        for(int r = 0; r < P.n_rows; ++r)
+        {
+            const int b = P.row_ptr[r], e = P.row_ptr[r + 1];
+            tmp.resize(e - b);
+            for(int i = b; i < e; ++i) tmp[i - b] = {P.col[i], P.entry_src[i]};
+            std::stable_sort(tmp.begin(), tmp.end(), [](auto& x, auto& y) { return x.first < y.first; });
+            for(int i = b; i < e; ++i)
+            {
+                P.col[i]       = tmp[i - b].first;
+                P.entry_src[i] = tmp[i - b].second;
+            }
+        }
+        for(long long e = 0; e < P.nnz; ++e)
+        {
+            const int src = P.entry_src[e] & 0x7fffffff;
+            if(P.entry_src[e] < 0)
+                P.dest2[src] = (int)e;
+            else if(P.dest1[src] < 0)
+                P.dest1[src] = (int)e;
+            else
+                P.dest2[src] = (int)e;  // duplicate diagonal input blocks cannot occur: rows==cols gives one entry
+        }
+    }
+    // ---- global 3x3 block-Jacobi and ABD bodies
+    P.diag_src.assign(P.n_rows, -1);
+    P.row_needs_diag.assign(P.n_rows, 0);
+    for(int s = 0; s < S; ++s)
+    {
+        auto& d = sys[s];
+        for(long long k = 0; k < d.nb; ++k)
+            if(d.rows[k] == d.cols[k]) P.diag_src[P.ext_to_int[P.ext_off[s] + d.rows[k]]] = (int)(P.src_off[s] + k);
+        for(int r = P.row_off[s]; r < P.row_off[s + 1]; ++r)
+        {
+            const bool abd = r >= P.abd_row0[s] && r < P.abd_row0[s] + 4 * d.n_abd;
+            const bool fem = r >= P.fem_row0[s];
+            P.row_needs_diag[r] = d.global_diag && !abd && !(fem && d.use_mas) ? 1 : 0;
+        }
+        if(d.n_abd > 0)
+        {
+            std::unordered_map<long long, int> where;
+            for(long long k = 0; k < d.nb; ++k)
+                if(d.rows[k] < 4 * d.n_abd && d.rows[k] / 4 == d.cols[k] / 4)
+                    where[(long long)d.rows[k] * d.block_rows + d.cols[k]] = (int)(P.src_off[s] + k);
+            for(int b = 0; b < d.n_abd; ++b)
+            {
+                P.body_row.push_back(P.abd_row0[s] + 4 * b);
+                P.body_sys.push_back(s);
+                for(int q = 0; q < 4; ++q)
+                    for(int t = 0; t < 4; ++t)
+                    {
+                        int src = -1;
+                        auto it = where.find((long long)(4 * b + q) * d.block_rows + 4 * b + t);
+                        if(it != where.end())
+                            src = it->second;
+                        else
+                        {
+                            it = where.find((long long)(4 * b + t) * d.block_rows + 4 * b + q);
+                            if(it != where.end()) src = it->second | (int)0x80000000u;
+                        }
+                        P.body_src.push_back(src);
+                    }
+            }
+        }
+    }
+    P.n_bodies = (int)P.body_row.size();
+
+    // ---- MAS: hierarchy per system, banks, components, Galerkin maps
+    int n_fine_banks = 0, n_coarse = 0;
+    std::vector<int> fine_bank_base(S, 0);
+    for(int s = 0; s < S; ++s)
+    {
+        auto& d = sys[s];
+        fine_bank_base[s] = n_fine_banks;
+        P.coarse_base[s]  = n_coarse;
+        if(!d.use_mas) continue;
+        build_hierarchy(d.hier, d.n_fem, (int)d.part_to_real.size(), bank, d.levels, d.nbr_start.data(),
+                        d.nbr_num.data(), d.nbr_list.data(), d.part_to_real.data(), d.real_to_part.data(),
+                        d.pairs.data(), (int)d.pairs.size() / 4, d.node_offset);
+        n_fine_banks += d.hier.n_slots / bank;
+        n_coarse += d.hier.total - d.hier.n_slots;
+    }
+    P.coarse_base[S] = n_coarse;
+    P.n_fine_banks   = n_fine_banks;
+    P.n_coarse       = n_coarse;
+    P.n_banks        = n_fine_banks + n_coarse / bank;
+    P.bank_vec.assign(P.n_banks, 0);
+    P.bank_sys.assign(P.n_banks, 0);
+    P.coarse_parent.assign(n_coarse, -1);
+    P.comp_ptr.assign(P.n_banks + 1, 0);
+    auto gbank_of_slot = [&](int s, int slot) {
+        const auto& h = sys[s].hier;
+        return slot < h.n_slots ? fine_bank_base[s] + slot / bank
+                                : n_fine_banks + (P.coarse_base[s] + slot - h.n_slots) / bank;
+    };
+    auto gcoarse = [&](int s, int slot) { return P.coarse_base[s] + slot - sys[s].hier.n_slots; };
+    for(int s = 0; s < S; ++s)
+    {
+        auto& d = sys[s];
+        if(!d.use_mas) continue;
+        const auto& h = d.hier;
+        for(int b = 0; b < h.total / bank; ++b)
+        {
+            const int gb = gbank_of_slot(s, b * bank);
+            P.bank_sys[gb] = s;
+            P.bank_vec[gb] = b * bank < h.n_slots ? P.fem_row0[s] + b * bank : gcoarse(s, b * bank);
+            P.comp_ptr[gb + 1] = h.comp_ptr[b + 1] - h.comp_ptr[b];
+        }
+        for(int slot = h.n_slots; slot < h.total; ++slot)
+        {
+            // parent exists below the top level
+            const int lvl_top_begin = h.level_begin[h.levels - 1];
+            if(slot < lvl_top_begin && h.going_next[slot] >= 0) P.coarse_parent[gcoarse(s, slot)] = gcoarse(s, h.going_next[slot]);
+        }
+    }
+    for(int b = 0; b < P.n_banks; ++b) P.comp_ptr[b + 1] += P.comp_ptr[b];
+    P.comp_mask.resize(P.comp_ptr[P.n_banks]);
+    P.comp_parent.resize(P.comp_ptr[P.n_banks]);
+    P.sys_level_ptr.assign(S + 1, 0);
+    for(int s = 0; s < S; ++s)
+    {
+        auto& d = sys[s];
+        if(d.use_mas)
+        {
+            const auto& h = d.hier;
+            for(int b = 0; b < h.total / bank; ++b)
+            {
+                const int gb = gbank_of_slot(s, b * bank);
+                for(int k = h.comp_ptr[b]; k < h.comp_ptr[b + 1]; ++k)
+                {
+                    P.comp_mask[P.comp_ptr[gb] + k - h.comp_ptr[b]]   = h.comp_mask[k];
+                    P.comp_parent[P.comp_ptr[gb] + k - h.comp_ptr[b]] = gcoarse(s, h.comp_parent[k]);
+                }
+            }
+            // coarse levels 1..L-1 as global coarse bank ranges
+            for(int l = 1; l < h.levels; ++l)
+            {
+                const int b0 = gbank_of_slot(s, h.level_begin[l]);
+                const int nb = pad_to(h.level_count[l], bank) / bank;
+                P.sys_level_banks.push_back(b0);
+                P.sys_level_banks.push_back(b0 + nb);
+            }
+        }
+        P.sys_level_ptr[s + 1] = (int)P.sys_level_banks.size() / 2;
+    }
+
+    // Galerkin coarse operators A_l = P_l^T A P_l (all entries, not only in-bank),
+    // built level by level from the previous level; group g holds level g+1 of
+    // every system.
+    int max_levels = 0;
+    for(auto& d : sys)
+        if(d.use_mas) max_levels = std::max(max_levels, d.hier.levels);
+    struct Ent { long long key; int src; int mode; };
+    // level-(l) entries per system: (I,J) slots, global E index
+    std::vector<std::vector<std::pair<int, int>>> prev_ij(S);   // (I,J) of the previous level's entries
+    std::vector<std::vector<int>>                 prev_idx(S);  // their global E index
+    P.gal_level_ptr.assign(1, 0);
+    P.gal_ptr.assign(1, 0);
+    P.gal_src.clear();
+    std::vector<std::pair<int, int>> E_ij;     // all E entries (slots, system-local)
+    std::vector<int>                 E_sys;
+    for(int lvl = 1; lvl < std::max(max_levels, 1); ++lvl)
+    {
+        for(int s = 0; s < S; ++s)
+        {
+            auto& d = sys[s];
+            if(!d.use_mas || lvl >= d.hier.levels) continue;
+            const auto& h = d.hier;
+            std::vector<Ent> ents;
+            auto push = [&](int I, int J, int src, bool src_diag) {
+                // canonical (min, max): as is, transposed, or B + B^T when both
+                // ends merge into one node
+                if(I < J) ents.push_back({(long long)I * h.total + J, src, 0});
+                else if(I > J) ents.push_back({(long long)J * h.total + I, src, 1});
+                else ents.push_back({(long long)I * h.total + I, src, src_diag ? 0 : 2});
+            };
+            if(lvl == 1)
+            {
+                for(long long k = 0; k < d.nb; ++k)
+                {
+                    const int r = d.rows[k] - d.fem_offset, c = d.cols[k] - d.fem_offset;
+                    if(r < 0 || c < 0) continue;
+                    push(h.going_next[r], h.going_next[c], (int)(P.src_off[s] + k), r == c);
+                }
+            }
+            else
+            {
+                for(size_t e = 0; e < prev_ij[s].size(); ++e)
+                {
+                    const int I = prev_ij[s][e].first, J = prev_ij[s][e].second;
+                    push(h.going_next[I], h.going_next[J], prev_idx[s][e], I == J);
+                }
+            }
+            std::stable_sort(ents.begin(), ents.end(), [](const Ent& a, const Ent& b) { return a.key < b.key; });
+            prev_ij[s].clear();
+            prev_idx[s].clear();
+            for(size_t i = 0; i < ents.size();)
+            {
+                size_t j = i;
+                while(j < ents.size() && ents[j].key == ents[i].key) ++j;
+                const int I = (int)(ents[i].key / h.total), J = (int)(ents[i].key % h.total);
+                const int eidx = (int)E_ij.size();
+                E_ij.push_back({I, J});
+                E_sys.push_back(s);
+                prev_ij[s].push_back({I, J});
+                prev_idx[s].push_back(eidx);
+                for(size_t t = i; t < j; ++t) P.gal_src.push_back((ents[t].src << 2) | ents[t].mode);
+                P.gal_ptr.push_back((int)P.gal_src.size());
+                i = j;
+            }
+        }
+        P.gal_level_ptr.push_back((int)E_ij.size());
+        P.gal_src_is_fine.push_back(lvl == 1 ? 1 : 0);
+    }
+    P.n_gal = (long long)E_ij.size();
+    if((long long)P.gal_src.size() >= (1ll << 29)) throw std::invalid_argument("Galerkin map too large");
+
+    // per-bank entries for the dense local assembly
+    std::vector<std::vector<std::pair<int, int>>> be(P.n_banks);  // (pos|T<<16, src)
+    for(int s = 0; s < S; ++s)
+    {
+        auto& d = sys[s];
+        if(!d.use_mas) continue;
+        const auto& h = d.hier;
+        for(long long k = 0; k < d.nb; ++k)
+        {
+            const int r = d.rows[k] - d.fem_offset, c = d.cols[k] - d.fem_offset;
+            if(r < 0 || c < 0) continue;
+            const int sr = d.real_to_part[r], sc = d.real_to_part[c];
+            if(sr / bank != sc / bank) continue;
+            const int gb = gbank_of_slot(s, sr);
+            const int lr = sr % bank, lc = sc % bank;
+            const int pos = lr <= lc ? packed_index(bank, lr, lc) : packed_index(bank, lc, lr);
+            be[gb].push_back({pos | ((lr > lc ? 1 : 0) << 16), (int)(P.src_off[s] + k)});
+        }
+        (void)h;
+    }
+    for(size_t e = 0; e < E_ij.size(); ++e)
+    {
+        const int s = E_sys[e];
+        const int I = E_ij[e].first, J = E_ij[e].second;
+        if(I / bank != J / bank) continue;
+        const int gb = gbank_of_slot(s, I);
+        be[gb].push_back({packed_index(bank, I % bank, J % bank), (int)e});
+    }
+    P.bank_ent_ptr.assign(P.n_banks + 1, 0);
+    for(int b = 0; b < P.n_banks; ++b) P.bank_ent_ptr[b + 1] = P.bank_ent_ptr[b] + (int)be[b].size();
+    P.bank_ent_pos.resize(P.bank_ent_ptr[P.n_banks]);
+    P.bank_ent_src.resize(P.bank_ent_ptr[P.n_banks]);
+    for(int b = 0; b < P.n_banks; ++b)
+        for(size_t k = 0; k < be[b].size(); ++k)
+        {
+            P.bank_ent_pos[P.bank_ent_ptr[b] + k] = be[b][k].first;
+            P.bank_ent_src[P.bank_ent_ptr[b] + k] = be[b][k].second;
+        }
+}
+
+}  // namespace masx

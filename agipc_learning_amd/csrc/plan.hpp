@@ -1,0 +1,131 @@
+// Host-side structure analysis for the MAS-PCG solve.
+//
+// Everything here depends only on the sparsity structure, the mesh topology,
+// the contact pairs and the vertex partition -- not on matrix values.  It runs
+// once per structure ("new graph"); value refreshes with an unchanged
+// structure reuse it (the reference's refresh_fixed_graph_bcoo path,
+// TraditionalMAS32Preconditioner.cu:2877-2887).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace masx
+{
+
+// --------------------------------------------------------------- hierarchy --
+// Multilevel additive-Schwarz hierarchy over fixed-size banks of slots
+// (TraditionalMAS32Preconditioner.cu:2142-2230 defines the algorithm: level 0
+// banks are consecutive slots of the vertex partition; the connected
+// components inside each bank -- through mesh edges and contact pairs --
+// become the nodes of the next level, numbered by (bank, lowest lane)).
+struct Hierarchy
+{
+    int bank   = 32;
+    int levels = 0;
+    int n_fem  = 0;
+    int n_slots = 0;
+    std::vector<int>      level_count;   // levels+1 (index 0 unused = n_slots)
+    std::vector<int>      level_begin;   // levels+1, slot offset of each level
+    int                   total = 0;     // level_begin[levels]
+    std::vector<uint32_t> fine_mask;     // per real vertex: component lanes in its level-0 bank
+    std::vector<int>      prefix;        // components per level-0 bank
+    std::vector<int>      going_next;    // [0,n_fem): real vertex -> level-1 slot;
+                                         // [n_slots,total): coarse slot -> next-level slot
+    std::vector<int>      coarse_table;  // n_fem*6: ancestor slots at levels 1..levels-1
+    // per bank (all levels but the top): components as (lane mask, parent slot)
+    std::vector<int>      comp_ptr;      // total/bank + 1
+    std::vector<uint32_t> comp_mask;
+    std::vector<int>      comp_parent;   // slot at the next level
+};
+
+int  reference_level_count(int vert_num, int part_map_size, int bank);
+// pairs: int4 sign-encoded collision pairs (global vertex ids), node_offset is
+// subtracted (TraditionalMAS32Preconditioner.cu:1550-1960).
+void build_hierarchy(Hierarchy& h, int n_fem, int n_slots, int bank, int levels,
+                     const uint32_t* nbr_start, const uint32_t* nbr_num, const uint32_t* nbr_list,
+                     const int* part_to_real, const int* real_to_part, const int* pairs, int n_pairs,
+                     int node_offset);
+int  decode_pair(const int* p, int v[4]);
+void morton_partition(int n, const double* pos, int bank, std::vector<int>& part_to_real,
+                      std::vector<int>& real_to_part);
+
+// ---------------------------------------------------------------- systems --
+struct SystemDesc
+{
+    int  block_rows = 0;   // external block rows (= 4*n_abd + n_fem for FEM at the end)
+    long long nb    = 0;   // upper 3x3 blocks
+    int  n_abd      = 0;   // affine bodies occupying block rows [0, 4*n_abd)
+    int  fem_offset = 0;   // first FEM block row
+    int  n_fem      = 0;   // FEM vertices [fem_offset, fem_offset + n_fem)
+    bool use_mas    = false;
+    bool global_diag = false;
+    int  bank       = 32;
+    int  levels     = 0;   // <= 0: reference rule
+    // host copies of structure
+    std::vector<int>      rows, cols;
+    std::vector<uint32_t> nbr_start, nbr_num, nbr_list;
+    std::vector<int>      part_to_real, real_to_part;
+    std::vector<int>      pairs;        // int4 per pair
+    int                   node_offset = 0;
+    Hierarchy             hier;
+};
+
+// Global packed layout over a batch of systems (everything the kernels index).
+struct Plan
+{
+    int bank = 32;
+    int n_sys = 0;
+    // rows: internal block rows, per system [ABD rows padded to bank][FEM slots]
+    std::vector<int> row_off;        // n_sys+1 internal row offsets
+    std::vector<int> ext_off;        // n_sys+1 external row offsets (concatenated b/x)
+    std::vector<int> ext_to_int;     // per external row (global) -> internal row (global)
+    int n_rows = 0;                  // internal rows total
+    int n_ext  = 0;
+    std::vector<int> fem_row0;       // per system: first internal FEM row (global)
+    std::vector<int> abd_row0;       // per system: first internal ABD row (global)
+    std::vector<int> row_sys;        // per internal row: system
+    // full (both-triangle) block CSR over internal rows
+    std::vector<int> row_ptr;        // n_rows+1
+    std::vector<int> col;            // nnz
+    std::vector<long long> src_off;  // n_sys+1: offset of each system's upper blocks in the
+                                     // concatenated value array
+    std::vector<int> dest1, dest2;   // per upper block (global index): full entries (dest2 -1 if diag)
+    std::vector<int> entry_src;      // per full entry: global upper block | transposed flag (bit 31)
+    long long nnz = 0, nb_total = 0;
+    // MAS banks: fine banks first (all systems), then coarse banks
+    int n_fine_banks = 0, n_banks = 0, n_coarse = 0;   // n_coarse = coarse slots total
+    std::vector<int> bank_vec;       // per bank: float3 offset into fine vector or coarse array
+    std::vector<int> bank_sys;
+    std::vector<int> coarse_base;    // per system: global coarse index of its level-1 begin
+    std::vector<int> comp_ptr;       // per bank (global) -> components
+    std::vector<uint32_t> comp_mask;
+    std::vector<int> comp_parent;    // global coarse index
+    std::vector<int> coarse_parent;  // per global coarse node: parent global coarse index or -1
+    // restriction schedule for levels >= 2 (per system, coarse banks grouped by level)
+    std::vector<int> sys_level_banks;   // flattened: per system, per level l>=1, [bank begin, end)
+    std::vector<int> sys_level_ptr;     // n_sys+1 offsets into sys_level_banks (pairs)
+    // Galerkin: coarse operator entries E (all systems, all levels >= 1), 3x3 row-major FP64
+    std::vector<int> gal_level_ptr;     // per "global level step" boundaries in E (launch groups)
+    std::vector<int> gal_ptr;           // per E entry: range in gal_src
+    std::vector<int> gal_src;           // source index << 2 | mode (0 as is, 1 transpose, 2 sym)
+    std::vector<int> gal_src_is_fine;   // per launch group: 1 = sources are fine upper blocks
+    long long n_gal = 0;
+    // per bank entry lists for the dense local assembly
+    std::vector<int> bank_ent_ptr;      // n_banks+1
+    std::vector<int> bank_ent_pos;      // packed block position | transposed << 16
+    std::vector<int> bank_ent_src;      // fine upper block index (fine banks) or E index (coarse)
+    // ABD
+    int n_bodies = 0;
+    std::vector<int> body_row;          // internal row of each body's first block row
+    std::vector<int> body_src;          // 16 per body: upper block index of (q,t) q<=t in the 4x4
+                                        // grid, with transposed flag (bit 31), -1 if absent
+    std::vector<int> body_sys;
+    // diagonal 3x3 inverse (global block-Jacobi): per internal row, diagonal source or -1
+    std::vector<int> diag_src;
+    std::vector<int> row_needs_diag;    // 1 if no local preconditioner covers the row
+};
+
+void build_plan(Plan& plan, std::vector<SystemDesc>& systems);
+
+}  // namespace masx

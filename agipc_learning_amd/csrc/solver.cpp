@@ -1,0 +1,550 @@
+// Host orchestrator of the MAS-PCG solve (see solver.hpp).
+#include "solver.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+#include "kernels.hpp"
+
+namespace masx
+{
+
+#define MASX_HIP(expr)                                                                                  \
+    do                                                                                                  \
+    {                                                                                                   \
+        const hipError_t e_ = (expr);                                                                   \
+        if(e_ != hipSuccess)                                                                            \
+            throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(e_));               \
+    } while(0)
+
+
+
+template <typename T>
+void DeviceArray<T>::alloc(size_t count)
+{
+    release();
+    if(count == 0) count = 1;  // keep a valid pointer for empty arrays
+    MASX_HIP(hipMalloc((void**)&ptr, count * sizeof(T)));
+    n = count;
+}
+
+template <typename T>
+void DeviceArray<T>::upload(const std::vector<T>& h, hipStream_t st)
+{
+    alloc(h.size());
+    if(!h.empty()) MASX_HIP(hipMemcpyAsync(ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
+template struct DeviceArray<int>;
+template struct DeviceArray<float>;
+template struct DeviceArray<double>;
+template struct DeviceArray<char>;
+template struct DeviceArray<uint32_t>;
+template struct DeviceArray<int4>;
+template struct DeviceArray<SysState>;
+
+Solver::Solver()
+{
+    MASX_HIP(hipStreamCreateWithFlags(&work_, hipStreamNonBlocking));
+    MASX_HIP(hipEventCreate(&ev_a_));
+    MASX_HIP(hipEventCreate(&ev_b_));
+}
+
+Solver::~Solver()
+{
+    if(graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if(h_state_) (void)hipHostFree(h_state_);
+    if(ev_a_) (void)hipEventDestroy(ev_a_);
+    if(ev_b_) (void)hipEventDestroy(ev_b_);
+    if(work_) (void)hipStreamDestroy(work_);
+}
+
+int Solver::add_system(const SystemDesc& d)
+{
+    systems_.push_back(d);
+    setup_done_ = false;
+    built_      = false;
+    return (int)systems_.size() - 1;
+}
+
+void Solver::set_spmv_tiling(int lanes_per_row, int rows_per_wg)
+{
+    if(lanes_per_row != 4 && lanes_per_row != 8 && lanes_per_row != 16 && lanes_per_row != 32)
+        throw std::invalid_argument("lanes per row must be 4, 8, 16 or 32");
+    if(rows_per_wg < 256 / lanes_per_row || rows_per_wg % (256 / lanes_per_row))
+        throw std::invalid_argument("rows per workgroup must be a multiple of 256 / lanes per row");
+    lanes_       = lanes_per_row;
+    rows_per_wg_ = rows_per_wg;
+    if(setup_done_)
+    {
+        build_work_tables();
+        make_devdata();
+        if(graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+        graph_exec_ = nullptr;
+    }
+}
+
+void Solver::setup()
+{
+    build_plan(plan_, systems_);
+    const Plan& P = plan_;
+    hipStream_t st = work_;
+    d_row_ptr_.upload(P.row_ptr, st);
+    d_col_.upload(P.col, st);
+    d_entry_src_.upload(P.entry_src, st);
+    d_row_sys_.upload(P.row_sys, st);
+    d_dest1_.upload(P.dest1, st);
+    d_dest2_.upload(P.dest2, st);
+    d_e2i_.upload(P.ext_to_int, st);
+    const long long plane = (P.nnz + 63) / 64 * 64;
+    d_vals32_.alloc(9 * (size_t)plane);
+    d_vals64_.alloc(9 * (size_t)P.nb_total);
+    d_E_.alloc(9 * (size_t)std::max<long long>(P.n_gal, 1));
+    const size_t nv = 3 * (size_t)P.n_rows;
+    d_b64_.alloc(nv);
+    d_tmp64a_.alloc(nv);
+    d_tmp64b_.alloc(nv);
+    for(auto* a : {&d_x_, &d_r_, &d_z_, &d_p0_, &d_p1_, &d_Ap_, &d_rt_}) a->alloc(nv);
+    d_Rc_.alloc(3 * (size_t)std::max(P.n_coarse, 1));
+    d_Zc_.alloc(3 * (size_t)std::max(P.n_coarse, 1));
+    MASX_HIP(hipMemsetAsync(d_Rc_.ptr, 0, d_Rc_.n * sizeof(float), st));
+    MASX_HIP(hipMemsetAsync(d_Zc_.ptr, 0, d_Zc_.n * sizeof(float), st));
+    d_bank_vec_.upload(P.bank_vec, st);
+    d_bank_sys_.upload(P.bank_sys, st);
+    d_comp_ptr_.upload(P.comp_ptr, st);
+    d_comp_mask_.upload(P.comp_mask, st);
+    d_comp_parent_.upload(P.comp_parent, st);
+    d_coarse_parent_.upload(P.coarse_parent, st);
+    d_sys_level_ptr_.upload(P.sys_level_ptr, st);
+    d_sys_level_banks_.upload(P.sys_level_banks, st);
+    d_gal_ptr_.upload(P.gal_ptr, st);
+    d_gal_src_.upload(P.gal_src, st);
+    d_ent_ptr_.upload(P.bank_ent_ptr, st);
+    d_ent_pos_.upload(P.bank_ent_pos, st);
+    d_ent_src_.upload(P.bank_ent_src, st);
+    d_body_row_.upload(P.body_row, st);
+    d_body_src_.upload(P.body_src, st);
+    d_diag_src_.upload(P.diag_src, st);
+    d_needs_diag_.upload(P.row_needs_diag, st);
+    d_row_mode_.upload(P.row_needs_diag, st);
+    d_abd_inv_.alloc(144 * (size_t)std::max(P.n_bodies, 1));
+    d_diag_inv_.alloc(9 * (size_t)P.n_rows);
+    d_state_.alloc(P.n_sys);
+    d_max_iter_.alloc(P.n_sys);
+    d_factor_status_.alloc(1);
+    MASX_HIP(hipMemsetAsync(d_state_.ptr, 0, d_state_.n * sizeof(SysState), st));
+    if(h_state_) (void)hipHostFree(h_state_);
+    MASX_HIP(hipHostMalloc((void**)&h_state_, sizeof(SysState) * P.n_sys, hipHostMallocDefault));
+    build_work_tables();
+    make_devdata();
+    dd_.plane = plane;
+    MASX_HIP(hipStreamSynchronize(st));
+    if(graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    graph_exec_ = nullptr;
+    setup_done_ = true;
+    built_      = false;
+}
+
+void Solver::build_work_tables()
+{
+    const Plan& P = plan_;
+    const int   S = P.n_sys;
+    std::vector<int4> pre, spmv, upd;
+    std::vector<int>  pre_r(2 * S), spmv_r(2 * S), upd_r(2 * S);
+    int body_base = 0;
+    std::vector<int> fine_bank_base(S, 0);
+    {
+        int acc = 0;
+        for(int s = 0; s < S; ++s)
+        {
+            fine_bank_base[s] = acc;
+            if(systems_[s].use_mas) acc += (P.row_off[s + 1] - P.fem_row0[s]) / P.bank;
+        }
+    }
+    auto rows_chunks = [&](std::vector<int4>& out, int s, int b, int e, int type, int step, int bank_base) {
+        for(int r = b; r < e; r += step)
+        {
+            const int aux = type == kWgFemMas ? bank_base + (r - b) / P.bank : 0;
+            out.push_back(make_int4(s, (type << 28) | aux, r, std::min(e, r + step)));
+        }
+    };
+    for(int s = 0; s < S; ++s)
+    {
+        const auto& d = systems_[s];
+        const int abd_rows_end = P.abd_row0[s] + 4 * d.n_abd;
+        const int fem_type = d.use_mas ? kWgFemMas : kWgPlain;
+        // update kernel
+        upd_r[2 * s] = (int)upd.size();
+        rows_chunks(upd, s, P.abd_row0[s], P.fem_row0[s], kWgPlain, kBlock, 0);
+        rows_chunks(upd, s, P.fem_row0[s], P.row_off[s + 1], fem_type, kBlock, fine_bank_base[s]);
+        upd_r[2 * s + 1] = (int)upd.size();
+        // preconditioner finish
+        pre_r[2 * s] = (int)pre.size();
+        for(int b = 0; b < d.n_abd; b += kBodiesPerWg)
+            pre.push_back(make_int4(s, kWgAbd << 28, body_base + b, body_base + std::min(d.n_abd, b + kBodiesPerWg)));
+        rows_chunks(pre, s, abd_rows_end, P.fem_row0[s], kWgPlain, kBlock, 0);
+        rows_chunks(pre, s, P.fem_row0[s], P.row_off[s + 1], fem_type, kBlock, fine_bank_base[s]);
+        pre_r[2 * s + 1] = (int)pre.size();
+        body_base += d.n_abd;
+        // SpMV
+        spmv_r[2 * s] = (int)spmv.size();
+        rows_chunks(spmv, s, P.row_off[s], P.row_off[s + 1], kWgPlain, rows_per_wg_, 0);
+        spmv_r[2 * s + 1] = (int)spmv.size();
+        if(upd_r[2 * s] == upd_r[2 * s + 1] || pre_r[2 * s] == pre_r[2 * s + 1] || spmv_r[2 * s] == spmv_r[2 * s + 1])
+            throw std::runtime_error("empty system");
+    }
+    hipStream_t st = work_;
+    d_wg_pre_.upload(pre, st);
+    d_wg_spmv_.upload(spmv, st);
+    d_wg_upd_.upload(upd, st);
+    d_wg_pre_range_.upload(pre_r, st);
+    d_wg_spmv_range_.upload(spmv_r, st);
+    d_wg_upd_range_.upload(upd_r, st);
+    n_wg_pre_  = (int)pre.size();
+    n_wg_spmv_ = (int)spmv.size();
+    n_wg_upd_  = (int)upd.size();
+    d_partial_.alloc(std::max({n_wg_pre_, n_wg_spmv_, n_wg_upd_}) + 1);
+    MASX_HIP(hipStreamSynchronize(st));
+}
+
+void Solver::make_devdata()
+{
+    const Plan& P = plan_;
+    DevData& d = dd_;
+    d.n_rows = P.n_rows;
+    d.n_sys = P.n_sys;
+    d.bank = P.bank;
+    d.n_banks = P.n_banks;
+    d.n_fine_banks = P.n_fine_banks;
+    d.nnz = P.nnz;
+    d.plane = (P.nnz + 63) / 64 * 64;
+    d.row_ptr = d_row_ptr_.ptr;
+    d.col = d_col_.ptr;
+    d.entry_src = d_entry_src_.ptr;
+    d.row_sys = d_row_sys_.ptr;
+    d.vals32 = d_vals32_.ptr;
+    d.vals64 = d_vals64_.ptr;
+    d.x = d_x_.ptr;
+    d.r = d_r_.ptr;
+    d.z = d_z_.ptr;
+    d.p0 = d_p0_.ptr;
+    d.p1 = d_p1_.ptr;
+    d.Ap = d_Ap_.ptr;
+    d.rt = d_rt_.ptr;
+    d.b64 = d_b64_.ptr;
+    d.Rc = d_Rc_.ptr;
+    d.Zc = d_Zc_.ptr;
+    d.minv = d_minv_.ptr;
+    d.bank_vec = d_bank_vec_.ptr;
+    d.bank_sys = d_bank_sys_.ptr;
+    d.comp_ptr = d_comp_ptr_.ptr;
+    d.comp_mask = d_comp_mask_.ptr;
+    d.comp_parent = d_comp_parent_.ptr;
+    d.coarse_parent = d_coarse_parent_.ptr;
+    d.sys_level_ptr = d_sys_level_ptr_.ptr;
+    d.sys_level_banks = d_sys_level_banks_.ptr;
+    d.abd_inv = d_abd_inv_.ptr;
+    d.body_row = d_body_row_.ptr;
+    d.diag_inv = d_diag_inv_.ptr;
+    d.row_mode = d_row_mode_.ptr;
+    d.wg_pre = d_wg_pre_.ptr;
+    d.wg_spmv = d_wg_spmv_.ptr;
+    d.wg_upd = d_wg_upd_.ptr;
+    d.wg_pre_range = d_wg_pre_range_.ptr;
+    d.wg_spmv_range = d_wg_spmv_range_.ptr;
+    d.wg_upd_range = d_wg_upd_range_.ptr;
+    d.partial = d_partial_.ptr;
+    d.state = d_state_.ptr;
+}
+
+void Solver::set_values(int s, const double* d_values, hipStream_t st)
+{
+    if(!setup_done_) throw std::runtime_error("setup() must precede set_values()");
+    const long long nb = systems_.at(s).nb;
+    if(nb) MASX_HIP(hipMemcpyAsync(d_vals64_.ptr + 9 * plan_.src_off[s], d_values, 9 * nb * sizeof(double),
+                                   hipMemcpyDeviceToDevice, st ? st : work_));
+    built_ = false;
+}
+
+void Solver::set_values_host(int s, const double* h_values, hipStream_t st)
+{
+    if(!setup_done_) throw std::runtime_error("setup() must precede set_values()");
+    const long long nb = systems_.at(s).nb;
+    if(nb) MASX_HIP(hipMemcpyAsync(d_vals64_.ptr + 9 * plan_.src_off[s], h_values, 9 * nb * sizeof(double),
+                                   hipMemcpyHostToDevice, st ? st : work_));
+    built_ = false;
+}
+
+void Solver::gather_values(int s, const double* d_values, const uint32_t* d_indices, hipStream_t st)
+{
+    if(!setup_done_) throw std::runtime_error("setup() must precede set_values()");
+    MASX_HIP(launch_gather_blocks(d_values, d_indices, d_vals64_.ptr + 9 * plan_.src_off[s], systems_.at(s).nb,
+                                  st ? st : work_));
+    built_ = false;
+}
+
+size_t Solver::inverse_bytes() const
+{
+    const size_t vals = (size_t)plan_.n_banks * plan_.bank * (plan_.bank + 1) / 2 * 9;
+    return vals * (precision_ == kPrecFp32 ? 4 : 2);
+}
+
+void Solver::build(int precision, hipStream_t user)
+{
+    if(!setup_done_) throw std::runtime_error("setup() must precede build()");
+    if(precision < kPrecFp32 || precision > kPrecBf16) throw std::invalid_argument("unknown precision");
+    hipStream_t st = work_;
+    if(user)
+    {
+        MASX_HIP(hipEventRecord(ev_a_, user));
+        MASX_HIP(hipStreamWaitEvent(st, ev_a_, 0));
+    }
+    if(precision != precision_ || !d_minv_.ptr)
+    {
+        precision_ = precision;
+        d_minv_.alloc(std::max<size_t>(inverse_bytes(), 16));
+        make_devdata();
+        if(graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+        graph_exec_ = nullptr;
+    }
+    const Plan& P = plan_;
+    MASX_HIP(launch_pack_values(dd_, d_dest1_.ptr, d_dest2_.ptr, P.nb_total, st));
+    for(size_t g = 0; g + 1 < P.gal_level_ptr.size(); ++g)
+        MASX_HIP(launch_gal_gather(d_vals64_.ptr, d_E_.ptr, d_gal_ptr_.ptr, d_gal_src_.ptr, P.gal_level_ptr[g],
+                                   P.gal_level_ptr[g + 1], P.gal_src_is_fine[g], st));
+    MASX_HIP(hipMemsetAsync(d_factor_status_.ptr, 0, sizeof(int), st));
+    MASX_HIP(launch_mas_factor(dd_, d_E_.ptr, d_ent_ptr_.ptr, d_ent_pos_.ptr, d_ent_src_.ptr, d_minv_.ptr, precision_,
+                               d_factor_status_.ptr, st));
+    MASX_HIP(launch_abd_factor(dd_, d_body_src_.ptr, d_abd_inv_.ptr, P.n_bodies, st));
+    MASX_HIP(launch_diag_factor(dd_, d_diag_src_.ptr, d_needs_diag_.ptr, d_diag_inv_.ptr, st));
+    if(user)
+    {
+        MASX_HIP(hipEventRecord(ev_b_, st));
+        MASX_HIP(hipStreamWaitEvent(user, ev_b_, 0));
+    }
+    built_ = true;
+}
+
+long long Solver::factor_status() const
+{
+    int v = 0;
+    MASX_HIP(hipMemcpy(&v, d_factor_status_.ptr, sizeof(int), hipMemcpyDeviceToHost));
+    return v;
+}
+
+void Solver::launch_iteration(hipStream_t st, int i)
+{
+    const float* p_old = (i & 1) ? d_p1_.ptr : d_p0_.ptr;
+    float*       p_new = (i & 1) ? d_p0_.ptr : d_p1_.ptr;
+    MASX_HIP(launch_mas_local(dd_, precision_, 0, st));
+    MASX_HIP(launch_precond_finish(dd_, n_wg_pre_, 0, st));
+    MASX_HIP(launch_spmv_fused(dd_, n_wg_spmv_, lanes_, p_old, p_new, st));
+    MASX_HIP(launch_update(dd_, n_wg_upd_, p_new, 0, st));
+}
+
+void Solver::run_chunk(hipStream_t st, int chunk, bool with_check)
+{
+    for(int i = 0; i < chunk; ++i) launch_iteration(st, i);
+    if(with_check)
+    {
+        MASX_HIP(launch_true_residual(dd_, n_wg_spmv_, 0, st));
+        MASX_HIP(launch_update(dd_, n_wg_upd_, d_p0_.ptr, 1, st));
+    }
+}
+
+void Solver::solve(const double* d_b, double* d_x, const SolveParams& prm, std::vector<SolveStats>* stats,
+                   hipStream_t user)
+{
+    if(!built_) throw std::runtime_error("build() must precede solve()");
+    const Plan& P = plan_;
+    hipStream_t st = work_;
+    if(user)
+    {
+        MASX_HIP(hipEventRecord(ev_a_, user));
+        MASX_HIP(hipStreamWaitEvent(st, ev_a_, 0));
+    }
+    const int chunk = std::max(2, prm.chunk + (prm.chunk & 1));
+    std::vector<int> max_iter(P.n_sys);
+    for(int s = 0; s < P.n_sys; ++s)
+        max_iter[s] = prm.max_iter > 0 ? prm.max_iter : std::max(1, (int)(0.3 * 3.0 * systems_[s].block_rows));
+    MASX_HIP(hipMemcpyAsync(d_max_iter_.ptr, max_iter.data(), sizeof(int) * P.n_sys, hipMemcpyHostToDevice, st));
+    const size_t nvb = 3 * (size_t)P.n_rows * sizeof(float);
+    for(auto* a : {&d_x_, &d_r_, &d_z_, &d_p0_, &d_p1_, &d_Ap_, &d_rt_}) MASX_HIP(hipMemsetAsync(a->ptr, 0, nvb, st));
+    MASX_HIP(hipMemsetAsync(d_b64_.ptr, 0, 3 * (size_t)P.n_rows * sizeof(double), st));
+    MASX_HIP(launch_init_state(dd_, prm.rule, prm.tol, d_max_iter_.ptr, prm.rule == kRuleTrueRel ? prm.check_every : 0,
+                               st));
+    MASX_HIP(launch_load_rhs(dd_, d_e2i_.ptr, d_b, P.n_ext, st));
+    MASX_HIP(launch_update(dd_, n_wg_upd_, d_p0_.ptr, 1, st));
+    const bool with_check = prm.rule == kRuleTrueRel;
+    if(prm.use_graph && (!graph_exec_ || graph_chunk_ != chunk || graph_check_ != with_check))
+    {
+        if(graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+        graph_exec_ = nullptr;
+        hipGraph_t g = nullptr;
+        MASX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        run_chunk(st, chunk, with_check);
+        MASX_HIP(hipStreamEndCapture(st, &g));
+        MASX_HIP(hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+        graph_chunk_ = chunk;
+        graph_check_ = with_check;
+    }
+    int max_total = 0;
+    for(int m : max_iter) max_total = std::max(max_total, m);
+    const int max_chunks = max_total / chunk + 8 + (with_check ? max_total / 10 : 0);
+    for(int c = 0; c < max_chunks; ++c)
+    {
+        if(prm.use_graph)
+            MASX_HIP(hipGraphLaunch(graph_exec_, st));
+        else
+            run_chunk(st, chunk, with_check);
+        MASX_HIP(hipMemcpyAsync(h_state_, d_state_.ptr, sizeof(SysState) * P.n_sys, hipMemcpyDeviceToHost, st));
+        MASX_HIP(hipStreamSynchronize(st));
+        bool all_done = true;
+        for(int s = 0; s < P.n_sys; ++s) all_done &= h_state_[s].status == kDone;
+        if(all_done) break;
+    }
+    MASX_HIP(launch_store_x(dd_, d_e2i_.ptr, d_x, P.n_ext, st));
+    MASX_HIP(launch_true_residual(dd_, n_wg_spmv_, 1, st));
+    MASX_HIP(hipMemcpyAsync(h_state_, d_state_.ptr, sizeof(SysState) * P.n_sys, hipMemcpyDeviceToHost, st));
+    MASX_HIP(hipStreamSynchronize(st));
+    if(stats)
+    {
+        stats->resize(P.n_sys);
+        for(int s = 0; s < P.n_sys; ++s)
+        {
+            const SysState& h = h_state_[s];
+            SolveStats& o = (*stats)[s];
+            o.iterations = h.k;
+            if(h.rule == kRuleRefRtz && h.failure == kIterationCap) o.iterations = h.max_iter;
+            o.status = h.status;
+            o.failure = h.failure;
+            o.restarts = h.restarts;
+            o.true_checks = h.n_true_checks;
+            o.rz0 = h.rz0;
+            o.rz = h.rz;
+            o.rel_res_recursive = h.bb > 0 ? std::sqrt(h.rr / h.bb) : 0.0;
+            o.rel_res_true = h.bb > 0 ? std::sqrt(h.true_rr / h.bb) : std::sqrt(h.true_rr);
+        }
+    }
+    if(user)
+    {
+        MASX_HIP(hipEventRecord(ev_b_, st));
+        MASX_HIP(hipStreamWaitEvent(user, ev_b_, 0));
+    }
+}
+
+void Solver::spmv(double a, const double* d_x, double b, double* d_y, hipStream_t user)
+{
+    if(!setup_done_) throw std::runtime_error("setup() must precede spmv()");
+    hipStream_t st = user ? user : work_;
+    MASX_HIP(hipMemsetAsync(d_tmp64a_.ptr, 0, 3 * (size_t)plan_.n_rows * sizeof(double), st));
+    MASX_HIP(launch_gather_ext64(d_e2i_.ptr, d_x, d_tmp64a_.ptr, plan_.n_ext, st));
+    MASX_HIP(launch_spmv64(dd_, d_tmp64a_.ptr, d_tmp64b_.ptr, st));
+    MASX_HIP(launch_axpby_int_to_ext(d_e2i_.ptr, d_tmp64b_.ptr, d_y, a, b, plan_.n_ext, st));
+    if(!user) MASX_HIP(hipStreamSynchronize(st));
+}
+
+void Solver::apply_preconditioner(const double* d_r, double* d_z, hipStream_t user)
+{
+    if(!built_) throw std::runtime_error("build() must precede apply_preconditioner()");
+    hipStream_t st = user ? user : work_;
+    const size_t nvb = 3 * (size_t)plan_.n_rows * sizeof(float);
+    MASX_HIP(hipMemsetAsync(d_rt_.ptr, 0, nvb, st));
+    MASX_HIP(hipMemsetAsync(d_b64_.ptr, 0, 3 * (size_t)plan_.n_rows * sizeof(double), st));
+    std::vector<int> ones(plan_.n_sys, 1);
+    MASX_HIP(hipMemcpyAsync(d_max_iter_.ptr, ones.data(), sizeof(int) * plan_.n_sys, hipMemcpyHostToDevice, st));
+    MASX_HIP(launch_init_state(dd_, kRuleTrueRel, 0.0, d_max_iter_.ptr, 0, st));
+    MASX_HIP(launch_ext_to_r32(d_e2i_.ptr, d_r, d_rt_.ptr, plan_.n_ext, st));
+    MASX_HIP(launch_update(dd_, n_wg_upd_, d_p0_.ptr, 1, st));
+    MASX_HIP(launch_mas_local(dd_, precision_, 1, st));
+    MASX_HIP(launch_precond_finish(dd_, n_wg_pre_, 1, st));
+    MASX_HIP(launch_z32_to_ext(d_e2i_.ptr, d_z_.ptr, d_z, plan_.n_ext, st));
+    MASX_HIP(hipStreamSynchronize(st));
+}
+
+void Solver::download_inverse(std::vector<float>& out)
+{
+    const size_t vals = (size_t)plan_.n_banks * plan_.bank * (plan_.bank + 1) / 2 * 9;
+    out.resize(vals);
+    if(precision_ == kPrecFp32)
+    {
+        MASX_HIP(hipMemcpy(out.data(), d_minv_.ptr, vals * sizeof(float), hipMemcpyDeviceToHost));
+        return;
+    }
+    std::vector<uint16_t> h(vals);
+    MASX_HIP(hipMemcpy(h.data(), d_minv_.ptr, vals * sizeof(uint16_t), hipMemcpyDeviceToHost));
+    for(size_t i = 0; i < vals; ++i)
+    {
+        const uint16_t v = h[i];
+        float f;
+        if(precision_ == kPrecBf16)
+        {
+            const uint32_t u = (uint32_t)v << 16;
+            std::memcpy(&f, &u, 4);
+        }
+        else
+        {
+            const uint32_t sgn = (v >> 15) & 1u, ex = (v >> 10) & 0x1fu, man = v & 0x3ffu;
+            float m;
+            if(ex == 0) m = std::ldexp((float)man, -24);
+            else if(ex == 31) m = man ? NAN : INFINITY;
+            else m = std::ldexp((float)(man | 0x400u), (int)ex - 25);
+            f = sgn ? -m : m;
+        }
+        out[i] = f;
+    }
+}
+
+void Solver::download_coarse_entries(std::vector<double>& out)
+{
+    out.resize(9 * (size_t)plan_.n_gal);
+    if(plan_.n_gal) MASX_HIP(hipMemcpy(out.data(), d_E_.ptr, out.size() * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+double Solver::time_kernel(int which, int reps)
+{
+    if(!built_) throw std::runtime_error("build() must precede time_kernel()");
+    hipStream_t st = work_;
+    // put every system in the running state with finite scalars
+    std::vector<SysState> h(plan_.n_sys);
+    MASX_HIP(hipMemcpy(h.data(), d_state_.ptr, sizeof(SysState) * plan_.n_sys, hipMemcpyDeviceToHost));
+    for(auto& s : h)
+    {
+        s.status = kRunning;
+        s.rz = s.rz0 = 1.0;
+        s.beta = 0.5;
+        s.alpha = 0.0;
+        s.max_iter = 1 << 30;
+        s.k = 1;
+        s.need_check = 0;
+        s.check_every = 0;
+        s.tol = 0.0;
+        for(auto& c : s.counter) c = 0u;
+    }
+    MASX_HIP(hipMemcpy(d_state_.ptr, h.data(), sizeof(SysState) * plan_.n_sys, hipMemcpyHostToDevice));
+    auto launch = [&]() {
+        switch(which)
+        {
+            case 0: MASX_HIP(launch_mas_local(dd_, precision_, 0, st)); break;
+            case 1: MASX_HIP(launch_precond_finish(dd_, n_wg_pre_, 0, st)); break;
+            case 2: MASX_HIP(launch_spmv_fused(dd_, n_wg_spmv_, lanes_, d_p0_.ptr, d_p1_.ptr, st)); break;
+            case 3: MASX_HIP(launch_update(dd_, n_wg_upd_, d_p1_.ptr, 0, st)); break;
+            default: throw std::invalid_argument("unknown kernel id");
+        }
+    };
+    launch();
+    MASX_HIP(hipStreamSynchronize(st));
+    // the update kernel may end a system; keep them running between launches
+    MASX_HIP(hipEventRecord(ev_a_, st));
+    for(int i = 0; i < reps; ++i) launch();
+    MASX_HIP(hipEventRecord(ev_b_, st));
+    MASX_HIP(hipEventSynchronize(ev_b_));
+    float ms = 0.f;
+    MASX_HIP(hipEventElapsedTime(&ms, ev_a_, ev_b_));
+    return ms / reps;
+}
+
+}  // namespace masx

@@ -1,0 +1,132 @@
+// Host orchestrator: a batch of independent linear systems solved by
+// MAS-preconditioned CG on one GPU.  Owns all device memory of the batch.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+#include "plan.hpp"
+
+namespace masx
+{
+
+template <typename T>
+struct DeviceArray
+{
+    T*     ptr = nullptr;
+    size_t n   = 0;
+    DeviceArray() = default;
+    DeviceArray(const DeviceArray&) = delete;
+    DeviceArray& operator=(const DeviceArray&) = delete;
+    ~DeviceArray() { release(); }
+    void release()
+    {
+        if(ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        n   = 0;
+    }
+    void alloc(size_t count);
+    void upload(const std::vector<T>& h, hipStream_t st);
+};
+
+struct SolveParams
+{
+    int    rule        = kRuleTrueRel;
+    double tol         = 1e-4;
+    int    max_iter    = 0;     // <= 0: floor(0.3 * dof) per system (pcg_solver.h:15)
+    int    check_every = 50;    // true-residual recompute period (TRUE rule)
+    int    chunk       = 8;     // iterations per captured graph (even)
+    int    use_graph   = 1;
+};
+
+struct SolveStats
+{
+    int    iterations;
+    int    status;
+    int    failure;
+    int    restarts;
+    int    true_checks;
+    double rz0, rz;
+    double rel_res_recursive;
+    double rel_res_true;
+};
+
+class Solver
+{
+  public:
+    Solver();
+    ~Solver();
+
+    int  add_system(const SystemDesc& d);
+    SystemDesc& system(int s) { return systems_.at(s); }
+    int  n_systems() const { return (int)systems_.size(); }
+    void setup();                      // structure analysis + allocation
+    bool is_setup() const { return setup_done_; }
+    void set_values(int s, const double* d_values, hipStream_t st);      // device FP64 col-major
+    void set_values_host(int s, const double* h_values, hipStream_t st);
+    void gather_values(int s, const double* d_values, const uint32_t* d_indices, hipStream_t st);
+    void build(int precision, hipStream_t st);
+    void solve(const double* d_b, double* d_x, const SolveParams& prm, std::vector<SolveStats>* stats,
+               hipStream_t st);
+    void spmv(double a, const double* d_x, double b, double* d_y, hipStream_t st);
+    void apply_preconditioner(const double* d_r, double* d_z, hipStream_t st);
+
+    // introspection for tests and the benchmark
+    const Plan& plan() const { return plan_; }
+    int  precision() const { return precision_; }
+    void download_inverse(std::vector<float>& out);  // FP32 view of the packed bank inverses
+    void download_coarse_entries(std::vector<double>& out);
+    size_t inverse_bytes() const;
+    int  spmv_lanes() const { return lanes_; }
+    void set_spmv_tiling(int lanes_per_row, int rows_per_wg);
+    // average device time of one kernel of the iteration over `reps` launches
+    // (0 MAS local, 1 preconditioner finish, 2 SpMV, 3 update), milliseconds
+    double time_kernel(int which, int reps);
+    long long factor_status() const;
+
+  private:
+    void build_work_tables();
+    void make_devdata();
+    void run_chunk(hipStream_t st, int chunk, bool with_check);
+    void launch_iteration(hipStream_t st, int i);
+
+    std::vector<SystemDesc> systems_;
+    Plan plan_;
+    bool setup_done_ = false;
+    int  precision_  = kPrecFp32;
+    int  lanes_      = 16;
+    int  rows_per_wg_ = 64;
+    bool built_      = false;
+
+    // device
+    DeviceArray<int>       d_row_ptr_, d_col_, d_entry_src_, d_row_sys_, d_dest1_, d_dest2_, d_e2i_;
+    DeviceArray<float>     d_vals32_;
+    DeviceArray<double>    d_vals64_, d_E_, d_b64_, d_tmp64a_, d_tmp64b_;
+    DeviceArray<float>     d_x_, d_r_, d_z_, d_p0_, d_p1_, d_Ap_, d_rt_, d_Rc_, d_Zc_;
+    DeviceArray<char>      d_minv_;
+    DeviceArray<int>       d_bank_vec_, d_bank_sys_, d_comp_ptr_, d_comp_parent_, d_coarse_parent_;
+    DeviceArray<uint32_t>  d_comp_mask_;
+    DeviceArray<int>       d_sys_level_ptr_, d_sys_level_banks_;
+    DeviceArray<int>       d_gal_ptr_, d_gal_src_, d_ent_ptr_, d_ent_pos_, d_ent_src_;
+    DeviceArray<float>     d_abd_inv_, d_diag_inv_;
+    DeviceArray<int>       d_body_row_, d_body_src_, d_diag_src_, d_needs_diag_, d_row_mode_;
+    DeviceArray<int4>      d_wg_pre_, d_wg_spmv_, d_wg_upd_;
+    DeviceArray<int>       d_wg_pre_range_, d_wg_spmv_range_, d_wg_upd_range_;
+    DeviceArray<double>    d_partial_;
+    DeviceArray<SysState>  d_state_;
+    DeviceArray<int>       d_max_iter_, d_factor_status_;
+    int n_wg_pre_ = 0, n_wg_spmv_ = 0, n_wg_upd_ = 0;
+    DevData dd_{};
+
+    hipStream_t work_ = nullptr;
+    hipEvent_t  ev_a_ = nullptr, ev_b_ = nullptr;
+    SysState*   h_state_ = nullptr;  // pinned
+    hipGraphExec_t graph_exec_ = nullptr;
+    int graph_chunk_ = 0;
+    bool graph_check_ = false;
+};
+
+}  // namespace masx

@@ -1,0 +1,342 @@
+"""ctypes front end of libmasx.so (include/masx.h).
+
+The product path has no CPU fallback: if the HIP library cannot be loaded, or
+a call fails, an exception is raised.  Device pointers are taken from torch
+tensors (`tensor.data_ptr()`); torch only provides device memory and streams.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from . import build as _build
+
+LIB_PATH = os.path.join(_build.LIB, "libmasx.so")
+HEADER = os.path.join(_build.ROOT, "include", "masx.h")
+
+FP32, FP16, BF16 = 0, 1, 2
+RULE_REF_RTZ, RULE_TRUE_REL = 0, 1
+PRECISIONS = {"fp32": FP32, "fp16": FP16, "bf16": BF16}
+
+
+class SystemDesc(C.Structure):
+    _fields_ = [("block_rows", C.c_int), ("nb", C.c_longlong), ("n_abd", C.c_int), ("n_fem", C.c_int),
+                ("use_mas", C.c_int), ("global_diag", C.c_int), ("bank", C.c_int), ("levels", C.c_int)]
+
+
+class SolveParams(C.Structure):
+    _fields_ = [("rule", C.c_int), ("tol", C.c_double), ("max_iter", C.c_int), ("check_every", C.c_int),
+                ("chunk", C.c_int), ("use_graph", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("status", C.c_int), ("failure", C.c_int), ("restarts", C.c_int),
+                ("true_checks", C.c_int), ("rz0", C.c_double), ("rz", C.c_double),
+                ("rel_res_recursive", C.c_double), ("rel_res_true", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [("n_rows", C.c_int), ("n_ext", C.c_int), ("n_sys", C.c_int), ("bank", C.c_int),
+                ("nnz", C.c_longlong), ("nb_total", C.c_longlong), ("n_banks", C.c_int),
+                ("n_fine_banks", C.c_int), ("n_coarse", C.c_int), ("n_bodies", C.c_int),
+                ("n_gal", C.c_longlong), ("inverse_bytes", C.c_longlong)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def header_functions():
+    """Names of the functions include/masx.h declares."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(masx_[a-z0-9_]+)\s*\(", text)))
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        _build.ensure()
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"masx HIP library missing: {LIB_PATH}")
+    L = C.CDLL(LIB_PATH)
+    vp, ip, up, dp, fp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_uint32), C.POINTER(C.c_double), \
+        C.POINTER(C.c_float)
+    sig = {
+        "masx_last_error": ([], C.c_char_p),
+        "masx_device_info": ([C.c_char_p, C.c_int, ip, C.POINTER(C.c_longlong)], C.c_int),
+        "masx_default_solve_params": ([C.POINTER(SolveParams)], None),
+        "masx_create": ([C.POINTER(vp)], C.c_int),
+        "masx_destroy": ([vp], C.c_int),
+        "masx_add_system": ([vp, C.POINTER(SystemDesc), ip, ip, ip], C.c_int),
+        "masx_set_topology": ([vp, C.c_int, up, up, up, C.c_int, ip, C.c_int, ip], C.c_int),
+        "masx_set_topology_morton": ([vp, C.c_int, up, up, up, C.c_int, dp], C.c_int),
+        "masx_set_contacts": ([vp, C.c_int, ip, C.c_int, C.c_int], C.c_int),
+        "masx_setup": ([vp], C.c_int),
+        "masx_set_values": ([vp, C.c_int, vp, vp], C.c_int),
+        "masx_set_values_host": ([vp, C.c_int, dp, vp], C.c_int),
+        "masx_build": ([vp, C.c_int, vp], C.c_int),
+        "masx_solve": ([vp, vp, vp, C.POINTER(SolveParams), C.POINTER(SolveStats), vp], C.c_int),
+        "masx_spmv": ([vp, C.c_double, vp, C.c_double, vp, vp], C.c_int),
+        "masx_apply_preconditioner": ([vp, vp, vp, vp], C.c_int),
+        "masx_set_spmv_tiling": ([vp, C.c_int, C.c_int], C.c_int),
+        "masx_time_kernel": ([vp, C.c_int, C.c_int, dp], C.c_int),
+        "masx_plan_info_get": ([vp, C.POINTER(PlanInfo)], C.c_int),
+        "masx_get_hierarchy": ([vp, C.c_int, ip, ip, ip, ip, up, ip, ip, ip], C.c_int),
+        "masx_get_inverse": ([vp, fp, C.c_longlong], C.c_int),
+        "masx_bank_of_slot": ([vp, C.c_int, C.c_int, ip], C.c_int),
+        "masx_factor_status": ([vp, C.POINTER(C.c_longlong)], C.c_int),
+        "masx_hierarchy_host": ([C.c_int, C.c_int, C.c_int, C.c_int, up, up, up, ip, ip, ip, C.c_int, C.c_int,
+                                 ip, ip, ip, ip, up, ip, ip, C.c_longlong, ip], C.c_int),
+        "masx_mas_create": ([C.POINTER(vp), C.c_int], C.c_int),
+        "masx_mas_destroy": ([vp], C.c_int),
+        "masx_mas_init_neighbor": ([vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
+        "masx_mas_upload": ([vp, up, up, up, ip, ip], C.c_int),
+        "masx_mas_init_matrix": ([vp, C.c_int], C.c_int),
+        "masx_mas_set_preconditioner_bcoo": ([vp, vp, vp, vp, vp, C.c_int, C.c_int, vp, C.c_int, C.c_int, vp],
+                                             C.c_int),
+        "masx_mas_preconditioning": ([vp, vp, vp, vp], C.c_int),
+        "masx_mas_solver": ([vp], vp),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise RuntimeError("masx: " + load().masx_last_error().decode())
+
+
+def _ptr(a, ctype):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _dev(t):
+    """Device pointer of a torch tensor (must live on the GPU)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+def hierarchy_host(n_fem, n_slots, nbr_start, nbr_num, nbr_list, part_to_real, real_to_part, *, pairs=None,
+                   node_offset=0, bank=32, levels=0):
+    """Library hierarchy computed on the host (no device needed)."""
+    L = load()
+    s, n, l = _u32(nbr_start), _u32(nbr_num), _u32(nbr_list if len(nbr_list) else np.zeros(1))
+    p2r, r2p = _i32(part_to_real), _i32(real_to_part)
+    pr = _i32(np.zeros((0, 4)) if pairs is None else pairs).reshape(-1)
+    npairs = len(pr) // 4
+    pr_p = _ptr(pr, C.c_int) if npairs else None
+    lv, tot = C.c_int(), C.c_int()
+    check(L.masx_hierarchy_host(n_fem, n_slots, bank, levels, _ptr(s, C.c_uint32), _ptr(n, C.c_uint32),
+                                _ptr(l, C.c_uint32), _ptr(p2r, C.c_int), _ptr(r2p, C.c_int), pr_p, npairs,
+                                node_offset, C.byref(lv), C.byref(tot), None, None, None, None, None, 0, None))
+    Lv, T = lv.value, tot.value
+    lc = np.zeros(Lv + 1, np.int32)
+    lb = np.zeros(Lv + 1, np.int32)
+    fm = np.zeros(n_fem, np.uint32)
+    px = np.zeros(n_slots // bank, np.int32)
+    gn = np.zeros(max(T, n_fem), np.int32)
+    ct = np.zeros((n_fem, 6), np.int32)
+    check(L.masx_hierarchy_host(n_fem, n_slots, bank, levels, _ptr(s, C.c_uint32), _ptr(n, C.c_uint32),
+                                _ptr(l, C.c_uint32), _ptr(p2r, C.c_int), _ptr(r2p, C.c_int), pr_p, npairs,
+                                node_offset, C.byref(lv), C.byref(tot), _ptr(lc, C.c_int), _ptr(lb, C.c_int),
+                                _ptr(fm, C.c_uint32), _ptr(px, C.c_int), _ptr(gn, C.c_int), len(gn),
+                                _ptr(ct, C.c_int)))
+    return dict(levels=Lv, total=T, level_count=lc, level_begin=lb, fine_mask=fm, prefix=px, going_next=gn,
+                coarse_table=ct)
+
+
+class Solver:
+    """A batch of independent Newton systems solved by MAS-PCG on the GPU."""
+
+    def __init__(self):
+        self.L = load()
+        h = C.c_void_p()
+        check(self.L.masx_create(C.byref(h)))
+        self.h = h
+        self.systems = []
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.L.masx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_system(self, *, block_rows, rows, cols, n_abd=0, n_fem=None, use_mas=True, global_diag=False,
+                   bank=32, levels=0):
+        rows, cols = _i32(rows), _i32(cols)
+        if n_fem is None:
+            n_fem = block_rows - 4 * n_abd
+        d = SystemDesc(block_rows, len(rows), n_abd, n_fem, int(use_mas), int(global_diag), bank, levels)
+        idx = C.c_int()
+        check(self.L.masx_add_system(self.h, C.byref(d), _ptr(rows, C.c_int), _ptr(cols, C.c_int), C.byref(idx)))
+        self.systems.append(dict(block_rows=block_rows, nb=len(rows), n_abd=n_abd, n_fem=n_fem))
+        return idx.value
+
+    def set_topology(self, sys, nbr_start, nbr_num, nbr_list, part_to_real, real_to_part):
+        s, n, l = _u32(nbr_start), _u32(nbr_num), _u32(nbr_list if len(nbr_list) else np.zeros(1))
+        p2r, r2p = _i32(part_to_real), _i32(real_to_part)
+        check(self.L.masx_set_topology(self.h, sys, _ptr(s, C.c_uint32), _ptr(n, C.c_uint32), _ptr(l, C.c_uint32),
+                                       len(nbr_list), _ptr(p2r, C.c_int), len(p2r), _ptr(r2p, C.c_int)))
+
+    def set_topology_morton(self, sys, nbr_start, nbr_num, nbr_list, positions):
+        s, n, l = _u32(nbr_start), _u32(nbr_num), _u32(nbr_list if len(nbr_list) else np.zeros(1))
+        pos = np.ascontiguousarray(positions, np.float64)
+        check(self.L.masx_set_topology_morton(self.h, sys, _ptr(s, C.c_uint32), _ptr(n, C.c_uint32),
+                                              _ptr(l, C.c_uint32), len(nbr_list), _ptr(pos, C.c_double)))
+
+    def set_contacts(self, sys, pairs, node_offset=0):
+        p = _i32(pairs).reshape(-1)
+        check(self.L.masx_set_contacts(self.h, sys, _ptr(p, C.c_int) if len(p) else None, len(p) // 4,
+                                       node_offset))
+
+    def setup(self):
+        check(self.L.masx_setup(self.h))
+
+    def set_values(self, sys, values, stream=None):
+        """values: device float64 tensor (nb, 9) or host numpy array."""
+        if isinstance(values, np.ndarray):
+            v = np.ascontiguousarray(values, np.float64)
+            check(self.L.masx_set_values_host(self.h, sys, _ptr(v, C.c_double), stream))
+            self._keep = v
+            import torch  # the copy is asynchronous on the library stream
+            torch.cuda.synchronize()
+        else:
+            check(self.L.masx_set_values(self.h, sys, _dev(values), stream))
+
+    def build(self, precision="fp32", stream=None):
+        check(self.L.masx_build(self.h, PRECISIONS.get(precision, precision), stream))
+
+    def solve(self, b, x, *, rule=RULE_TRUE_REL, tol=1e-4, max_iter=0, check_every=50, chunk=8, use_graph=True,
+              stream=None):
+        p = SolveParams(rule, tol, max_iter, check_every, chunk, int(use_graph))
+        st = (SolveStats * len(self.systems))()
+        check(self.L.masx_solve(self.h, _dev(b), _dev(x), C.byref(p), st, stream))
+        return [s.as_dict() for s in st]
+
+    def spmv(self, x, y, a=1.0, b=0.0, stream=None):
+        check(self.L.masx_spmv(self.h, a, _dev(x), b, _dev(y), stream))
+
+    def apply_preconditioner(self, r, z, stream=None):
+        check(self.L.masx_apply_preconditioner(self.h, _dev(r), _dev(z), stream))
+
+    def set_spmv_tiling(self, lanes_per_row, rows_per_wg):
+        check(self.L.masx_set_spmv_tiling(self.h, lanes_per_row, rows_per_wg))
+
+    def time_kernel(self, which, reps=20):
+        ms = C.c_double()
+        check(self.L.masx_time_kernel(self.h, which, reps, C.byref(ms)))
+        return ms.value
+
+    def info(self):
+        i = PlanInfo()
+        check(self.L.masx_plan_info_get(self.h, C.byref(i)))
+        return i.as_dict()
+
+    def hierarchy(self, sys=0):
+        lv, tot = C.c_int(), C.c_int()
+        lb = np.zeros(8, np.int32)
+        check(self.L.masx_get_hierarchy(self.h, sys, C.byref(lv), C.byref(tot), None, _ptr(lb, C.c_int), None, None,
+                                        None, None))
+        d = self.systems[sys]
+        Lv, T, n_slots = lv.value, tot.value, int(lb[1])
+        lc = np.zeros(Lv + 1, np.int32)
+        fm = np.zeros(d["n_fem"], np.uint32)
+        px = np.zeros(max(n_slots // 32, 1), np.int32)
+        gn = np.zeros(T, np.int32)
+        ct = np.zeros((d["n_fem"], 6), np.int32)
+        check(self.L.masx_get_hierarchy(self.h, sys, C.byref(lv), C.byref(tot), _ptr(lc, C.c_int), _ptr(lb, C.c_int),
+                                        _ptr(fm, C.c_uint32), _ptr(px, C.c_int), _ptr(gn, C.c_int), _ptr(ct, C.c_int)))
+        return dict(levels=Lv, total=T, level_count=lc, level_begin=lb[:Lv + 1].copy(), fine_mask=fm, prefix=px,
+                    going_next=gn, coarse_table=ct)
+
+    def inverse(self):
+        i = self.info()
+        nblk = i["bank"] * (i["bank"] + 1) // 2
+        out = np.zeros((i["n_banks"], nblk, 9), np.float32)
+        check(self.L.masx_get_inverse(self.h, _ptr(out, C.c_float), out.size))
+        return out
+
+    def bank_of_slot(self, sys, slot):
+        b = C.c_int()
+        check(self.L.masx_bank_of_slot(self.h, sys, slot, C.byref(b)))
+        return b.value
+
+    def factor_status(self):
+        c = C.c_longlong()
+        check(self.L.masx_factor_status(self.h, C.byref(c)))
+        return c.value
+
+
+class MasPreconditioner:
+    """TraditionalMAS32Preconditioner-shaped object over the C ABI
+    (TraditionalMAS32Preconditioner.cuh:78-116)."""
+
+    def __init__(self, bank=32):
+        self.L = load()
+        h = C.c_void_p()
+        check(self.L.masx_mas_create(C.byref(h), bank))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h is not None and self.h.value:
+                self.L.masx_mas_destroy(self.h)
+        except Exception:
+            pass
+
+    def init_neighbor(self, vert_num, collision_node_offset, total_neighbor_num, part_map_size):
+        check(self.L.masx_mas_init_neighbor(self.h, vert_num, collision_node_offset, total_neighbor_num,
+                                            part_map_size))
+
+    def upload(self, neighbor_list, neighbor_start, neighbor_num, part_to_real, real_to_part):
+        self._keep = [_u32(neighbor_list if len(neighbor_list) else np.zeros(1)), _u32(neighbor_start),
+                      _u32(neighbor_num), _i32(part_to_real), _i32(real_to_part)]
+        a, b, c, d, e = self._keep
+        check(self.L.masx_mas_upload(self.h, _ptr(a, C.c_uint32), _ptr(b, C.c_uint32), _ptr(c, C.c_uint32),
+                                     _ptr(d, C.c_int), _ptr(e, C.c_int)))
+
+    def init_matrix(self, levels=0):
+        check(self.L.masx_mas_init_matrix(self.h, levels))
+
+    def set_preconditioner_bcoo(self, values, rows, cols, indices, offset, triplet_num, pairs=None, cp_num=0,
+                                precision="fp32", stream=None):
+        check(self.L.masx_mas_set_preconditioner_bcoo(self.h, _dev(values), _dev(rows), _dev(cols), _dev(indices),
+                                                      offset, triplet_num, _dev(pairs), cp_num,
+                                                      PRECISIONS.get(precision, precision), stream))
+
+    def preconditioning(self, R, Z, stream=None):
+        check(self.L.masx_mas_preconditioning(self.h, _dev(R), _dev(Z), stream))

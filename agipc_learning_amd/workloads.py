@@ -158,3 +158,26 @@ def batch64(seed=0, count=64, lo=5000, hi=80000, contact_fraction=0.05):
         pairs = int(round(contact_fraction * nv / 4))
         systems.append(fem_grid(n, young_sigma=0.5, n_contacts=pairs, seed=seed * 1000 + i + 1))
     return systems
+
+
+def neighbors(n, tets, tris=None):
+    """Vertex neighbor CSR (unique undirected tet/triangle edges), the input
+    the MAS hierarchy expects (load_mesh.cpp:934-1040 builds the same sets)."""
+    tets = np.asarray(tets, np.int64).reshape(-1, 4)
+    pairs = [tets[:, [a, b]] for a, b in ((0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3))]
+    if tris is not None and len(tris):
+        tris = np.asarray(tris, np.int64).reshape(-1, 3)
+        pairs += [tris[:, [a, b]] for a, b in ((0, 1), (1, 2), (2, 0))]
+    e = np.concatenate(pairs) if pairs else np.zeros((0, 2), np.int64)
+    e = np.sort(e, axis=1)
+    e = e[e[:, 0] != e[:, 1]]
+    e = np.unique(e[:, 0] * n + e[:, 1])
+    u, v = e // n, e % n
+    src = np.concatenate([u, v])
+    dst = np.concatenate([v, u])
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    num = np.bincount(src, minlength=n).astype(np.uint32)
+    start = np.zeros(n, np.uint32)
+    start[1:] = np.cumsum(num)[:-1]
+    return start, num, dst.astype(np.uint32)

@@ -1,0 +1,203 @@
+/*
+ * masx -- MI355X-native MAS-preconditioned conjugate gradient (C ABI).
+ *
+ * Drop-in boundary for StiffGIPC's Newton-step linear solve.  Plain C types
+ * only; device pointers are HIP device pointers; every call returns 0 on
+ * success and a nonzero code on failure, with the message available from
+ * masx_last_error() (the reference aborts instead: cuda_tools/cuda_tools.h:38-143).
+ * Streams are explicit (hipStream_t passed as void*; NULL = the library's
+ * own stream, synchronous for the caller).
+ *
+ * Matrix contract (linear_system/linear_system/global_matrix.h:18-203): 3x3
+ * block triplets, 9 doubles per block in Eigen column-major order, symmetric
+ * semantics y += B x_j at (i,j) and, for i != j, B^T x_i at (j,i)
+ * (linear_system/utils/spmv.cu:28-53); block rows ordered [4 per affine body]
+ * [FEM vertices] (GIPC.cu:10271-10272).
+ */
+#ifndef MASX_H
+#define MASX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct masx_solver masx_solver;
+typedef struct masx_mas    masx_mas;
+
+enum masx_precision { MASX_FP32 = 0, MASX_FP16 = 1, MASX_BF16 = 2 };
+enum masx_rule      { MASX_RULE_REF_RTZ = 0, MASX_RULE_TRUE_REL = 1 };
+enum masx_status    { MASX_RUNNING = 0, MASX_DONE = 1 };
+enum masx_failure   { MASX_OK = 0, MASX_BAD_RZ = 1, MASX_BAD_CURVATURE = 2, MASX_NONFINITE = 3,
+                      MASX_ITERATION_CAP = 4 };
+
+typedef struct masx_system_desc
+{
+    int       block_rows;   /* 4 * n_abd + n_fem */
+    long long nb;           /* number of 3x3 block triplets */
+    int       n_abd;        /* affine bodies: block rows [0, 4*n_abd) get the 12x12 block-Jacobi */
+    int       n_fem;        /* FEM vertices: block rows [4*n_abd, block_rows) */
+    int       use_mas;      /* 1: MAS preconditioner on the FEM range */
+    int       global_diag;  /* 1: 3x3 block-Jacobi on rows no local preconditioner covers */
+    int       bank;         /* MAS subdomain size: 32 (MAS32) or 16 (CEMAS16) */
+    int       levels;       /* MAS levels, <= 0 for the reference rule */
+} masx_system_desc;
+
+typedef struct masx_solve_params
+{
+    int    rule;         /* MASX_RULE_REF_RTZ: |r.z| <= tol * r0.z0 with the reference's one-iteration
+                            lag (linear_system/solver/pcg_solver.cu:277);
+                            MASX_RULE_TRUE_REL: ||b - A x|| <= tol ||b||, FP64 true residual */
+    double tol;          /* 1e-4 in the reference (gipc/gipc.cu:77-79) */
+    int    max_iter;     /* <= 0: floor(0.3 * dof) (linear_system/solver/pcg_solver.h:15) */
+    int    check_every;  /* TRUE rule: true-residual recompute period (restart on 10x tol drift) */
+    int    chunk;        /* iterations per captured hipGraph launch (even, >= 2) */
+    int    use_graph;    /* 1: replay a captured graph per chunk */
+} masx_solve_params;
+
+typedef struct masx_solve_stats
+{
+    int    iterations;   /* SpMV count, the value PCGSolver::solve returns */
+    int    status;       /* masx_status */
+    int    failure;      /* masx_failure */
+    int    restarts;
+    int    true_checks;
+    double rz0;
+    double rz;
+    double rel_res_recursive;
+    double rel_res_true; /* ||b - A x|| / ||b|| in FP64 */
+} masx_solve_stats;
+
+const char* masx_last_error(void);
+int         masx_device_info(char* name, int name_len, int* cu_count, long long* hbm_bytes);
+void        masx_default_solve_params(masx_solve_params* p);
+
+/* ---------------------------------------------------------------------------
+ * Global solver (GlobalLinearSystem + PCGSolver).  A solver holds a batch of
+ * independent systems; a single Newton system is a batch of one.
+ * -------------------------------------------------------------------------*/
+
+/* GlobalLinearSystem ctor (linear_system/linear_system/global_linear_system.h:14-119) */
+int masx_create(masx_solver** out);
+int masx_destroy(masx_solver* s);
+
+/* Structure of one system: host rows/cols of the upper block triplets after
+ * Converter::convert (linear_system/utils/converter.cu:137-156).  Returns the
+ * system index in *sys. */
+int masx_add_system(masx_solver* s, const masx_system_desc* desc, const int* rows, const int* cols, int* sys);
+
+/* MAS topology: neighbor CSR (load_mesh.cpp:934-1040) and the 32-slot vertex
+ * partition (gl_main.cu:1201-1243), host arrays, as uploaded by
+ * gl_main.cu:1502-1536 before initPreconditioner_Matrix. */
+int masx_set_topology(masx_solver* s, int sys, const uint32_t* nbr_start, const uint32_t* nbr_num,
+                      const uint32_t* nbr_list, int list_len, const int* part_to_real, int part_map_size,
+                      const int* real_to_part);
+/* Same, with the partition computed from vertex positions by Morton order. */
+int masx_set_topology_morton(masx_solver* s, int sys, const uint32_t* nbr_start, const uint32_t* nbr_num,
+                             const uint32_t* nbr_list, int list_len, const double* positions);
+/* Contact pairs (int4 per pair, sign-encoded types, global vertex ids;
+ * node_offset is subtracted), the _collisonPairs of GIPC.cuh:49-51 consumed by
+ * TraditionalMAS32Preconditioner.cu:1550-1960.  Host array. */
+int masx_set_contacts(masx_solver* s, int sys, const int* pairs, int n_pairs, int node_offset);
+
+/* Structure analysis: hierarchy (ReorderRealtime, TraditionalMAS32Preconditioner.cu
+ * :2189-2230), internal slot order, block layout, Galerkin maps; allocation. */
+int masx_setup(masx_solver* s);
+
+/* Block values of one system (device, FP64 col-major, same order as rows/cols). */
+int masx_set_values(masx_solver* s, int sys, const double* d_values, void* stream);
+/* Host-side variant of masx_set_values. */
+int masx_set_values_host(masx_solver* s, int sys, const double* h_values, void* stream);
+
+/* Preconditioner assembly: GlobalLinearSystem::build_linear_system's
+ * preconditioner step (global_linear_system.cu:37-146), i.e.
+ * ABDPreconditioner::assemble (abd_preconditioner.cu:33),
+ * TraditionalMAS32_Preconditioner::assemble (traditional_mas32_preconditioner.cu:20-31),
+ * DiagPreconditioner::assemble (diag_preconditioner.cu:79-86). */
+int masx_build(masx_solver* s, int precision, void* stream);
+
+/* IterativeSolver::solve(x, b) (linear_system/linear_system/i_linear_system_solver.h:43-44)
+ * -> PCGSolver::solve (pcg_solver.cu:160-173).  b, x: device FP64, systems
+ * concatenated in order.  stats: one per system (may be NULL). */
+int masx_solve(masx_solver* s, const double* d_b, double* d_x, const masx_solve_params* params,
+               masx_solve_stats* stats, void* stream);
+
+/* IterativeSolver::spmv(a, x, b, y) (i_linear_system_solver.h:46), FP64. */
+int masx_spmv(masx_solver* s, double a, const double* d_x, double b, double* d_y, void* stream);
+
+/* IterativeSolver::apply_preconditioner(z, r) (i_linear_system_solver.h:52-53)
+ * -> GlobalLinearSystem::apply_preconditioner (global_linear_system.cu:293-306). */
+int masx_apply_preconditioner(masx_solver* s, const double* d_r, double* d_z, void* stream);
+
+/* SpMV tiling: lanes per block row (4, 8, 16, 32) and rows per workgroup. */
+int masx_set_spmv_tiling(masx_solver* s, int lanes_per_row, int rows_per_wg);
+/* Average device time of one kernel (0 MAS local, 1 preconditioner finish,
+ * 2 fused SpMV, 3 update) over `reps` back-to-back launches, milliseconds. */
+int masx_time_kernel(masx_solver* s, int which, int reps, double* ms);
+
+/* Introspection (tests, benchmark). */
+typedef struct masx_plan_info
+{
+    int       n_rows, n_ext, n_sys, bank;
+    long long nnz, nb_total;
+    int       n_banks, n_fine_banks, n_coarse, n_bodies;
+    long long n_gal, inverse_bytes;
+} masx_plan_info;
+int masx_plan_info_get(masx_solver* s, masx_plan_info* info);
+/* Hierarchy of one system, same arrays as the oracle: level_count and
+ * level_begin (levels+1), fine_mask (n_fem), prefix (n_slots/bank), going_next
+ * (total; [0,n_fem) by real vertex, [n_slots,total) by slot), coarse_table
+ * (n_fem*6).  Any pointer may be NULL; *levels and *total are always set. */
+int masx_get_hierarchy(masx_solver* s, int sys, int* levels, int* total, int* level_count, int* level_begin,
+                       uint32_t* fine_mask, int* prefix, int* going_next, int* coarse_table);
+/* Packed bank inverses (FP32 view), bank order = fine banks of all systems,
+ * then coarse banks; within a bank, packed upper 3x3 blocks, row-major. */
+int masx_get_inverse(masx_solver* s, float* out, long long capacity);
+/* Global bank index of a system's slot (fine slots and coarse slots). */
+int masx_bank_of_slot(masx_solver* s, int sys, int slot, int* bank);
+/* Number of Cholesky pivots floored or non-finite in the last build. */
+int masx_factor_status(masx_solver* s, long long* count);
+
+/* Host-only hierarchy build (no device needed): the same computation
+ * masx_setup runs for each MAS system, exposed for CPU-side verification.
+ * Call once with going_next == NULL to obtain *levels and *total, then again
+ * with buffers of level_count/level_begin (levels+1), fine_mask (n_fem),
+ * prefix (n_slots/bank), going_next (going_next_cap >= max(total, n_fem)),
+ * coarse_table (n_fem*6). */
+int masx_hierarchy_host(int n_fem, int n_slots, int bank, int levels, const uint32_t* nbr_start,
+                        const uint32_t* nbr_num, const uint32_t* nbr_list, const int* part_to_real,
+                        const int* real_to_part, const int* pairs, int n_pairs, int node_offset, int* out_levels,
+                        int* out_total, int* level_count, int* level_begin, uint32_t* fine_mask, int* prefix,
+                        int* going_next, long long going_next_cap, int* coarse_table);
+
+/* ---------------------------------------------------------------------------
+ * MAS preconditioner object (TraditionalMAS32Preconditioner /
+ * MASPreconditioner API, TraditionalMAS32Preconditioner.cuh:78-116), for a
+ * caller that keeps its own PCG and BCOO matrix.
+ * -------------------------------------------------------------------------*/
+
+int masx_mas_create(masx_mas** out, int bank);
+int masx_mas_destroy(masx_mas* m);  /* FreeMAS */
+/* initPreconditioner_Neighbor (TraditionalMAS32Preconditioner.cu:2889-2926) */
+int masx_mas_init_neighbor(masx_mas* m, int vert_num, int collision_node_offset, int total_neighbor_num,
+                           int part_map_size);
+/* the host uploads of gl_main.cu:1502-1536 */
+int masx_mas_upload(masx_mas* m, const uint32_t* neighbor_list, const uint32_t* neighbor_start,
+                    const uint32_t* neighbor_num, const int* part_to_real, const int* real_to_part);
+/* initPreconditioner_Matrix (:2928-2977); levels <= 0 keeps the reference rule */
+int masx_mas_init_matrix(masx_mas* m, int levels);
+/* setPreconditioner_bcoo (:2643-2681): device values/rows/cols/indices of the
+ * global BCOO, FEM block offset, selected count, device contact pairs. */
+int masx_mas_set_preconditioner_bcoo(masx_mas* m, const double* d_values, const int* d_rows, const int* d_cols,
+                                     const uint32_t* d_indices, int offset, int triplet_num, const int* d_pairs,
+                                     int cp_num, int precision, void* stream);
+/* preconditioning(R, Z) (:2684-2725): device double3 arrays of the FEM range. */
+int masx_mas_preconditioning(masx_mas* m, const double* d_R, double* d_Z, void* stream);
+/* the solver the MAS object drives (introspection) */
+masx_solver* masx_mas_solver(masx_mas* m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,73 @@
+"""CPU-side checks of the HIP library: it loads, exports every symbol the C
+header declares, and its host-side MAS hierarchy equals the oracle's bit for
+bit.  No device calls."""
+import numpy as np
+import pytest
+
+from agipc_learning_amd import native
+from agipc_learning_amd import workloads as W
+from test_oracle import _random_graph_case, kat35
+
+
+def test_library_exports_every_declared_symbol():
+    L = native.load()
+    names = native.header_functions()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(L, n)]
+    assert missing == []
+
+
+def _compare(lib_h, orc, levels):
+    assert lib_h["levels"] == levels
+    o = orc.hierarchy()
+    assert lib_h["total"] == orc.total_clusters
+    assert np.array_equal(lib_h["level_count"][1:], o["level_count"][1:])
+    assert np.array_equal(lib_h["level_begin"], o["level_begin"])
+    assert np.array_equal(lib_h["fine_mask"], o["fine_mask"])
+    assert np.array_equal(lib_h["prefix"], o["prefix"])
+    assert np.array_equal(lib_h["coarse_table"], o["coarse_table"])
+    n_fem = len(lib_h["fine_mask"])
+    n_slots = int(o["level_begin"][1])
+    T = orc.total_clusters
+    gl, go = lib_h["going_next"], o["going_next"]
+    assert np.array_equal(gl[:n_fem], go[:n_fem])
+    assert np.array_equal(gl[n_slots:T], go[n_slots:T])
+
+
+def test_hierarchy_reference_kat_library(oracle):
+    valid, padded, start, num, lst, p2r, r2p = kat35()
+    h = native.hierarchy_host(valid, padded, start, num, lst, p2r, r2p)
+    assert list(h["prefix"]) == [1, 1]
+    assert h["level_count"][1] == 2 and h["level_begin"][1] == padded
+    assert all(h["fine_mask"][v] == (0xFFFFFFFF if v < 32 else 0x7) for v in range(valid))
+    assert np.array_equal(h["going_next"][:valid], padded + np.array([0 if v < 32 else 1 for v in range(valid)]))
+
+
+@pytest.mark.parametrize("seed,n,extra,npairs,scramble", [
+    (0, 70, 10, 0, False), (1, 200, 60, 15, False), (2, 300, 400, 40, True), (3, 1000, 100, 30, False),
+    (4, 129, 5, 8, True), (5, 2100, 300, 50, False), (6, 5000, 3000, 200, True)])
+def test_library_hierarchy_equals_oracle_random(oracle, seed, n, extra, npairs, scramble):
+    rng = np.random.default_rng(seed)
+    edges, start, num, lst, p2r, r2p, pairs, n_slots = _random_graph_case(rng, n, extra, npairs, scramble)
+    orc = oracle.Mas(n, n_slots, start, num, lst, p2r, r2p)
+    orc.reorder(pairs)
+    h = native.hierarchy_host(n, n_slots, start, num, lst, p2r, r2p, pairs=pairs)
+    _compare(h, orc, orc.levels)
+
+
+@pytest.mark.parametrize("n,contacts,bank,levels", [(8, 0, 32, 0), (12, 300, 32, 0), (10, 100, 16, 0),
+                                                     (16, 500, 32, 4)])
+def test_library_hierarchy_equals_oracle_grids(oracle, n, contacts, bank, levels):
+    s = W.fem_grid(n, n_contacts=contacts, seed=n)
+    nv = s["n_vertices"]
+    p2r, r2p = oracle.morton_partition(s["positions"], bank)
+    st, nu, li = oracle.mesh_neighbors(nv, s["tets"])
+    orc = oracle.Mas(nv, len(p2r), st, nu, li, p2r, r2p, bank=bank, levels=levels)
+    orc.reorder(s["pairs"])
+    h = native.hierarchy_host(nv, len(p2r), st, nu, li, p2r, r2p, pairs=s["pairs"], bank=bank, levels=levels)
+    _compare(h, orc, orc.levels)
+
+
+def test_library_rejects_bad_input_with_an_error():
+    with pytest.raises(RuntimeError):
+        native.hierarchy_host(10, 5, np.zeros(10), np.zeros(10), np.zeros(1), np.arange(5), np.arange(10))
