@@ -40,7 +40,7 @@ def build_masx(jobs: int = 4):
             cmd = [HIPCC, f"--offload-arch={ARCH}", *common, "-Xclang", "-target-feature", "-Xclang",
                    "-packed-fp32-ops", "-c", os.path.join(CSRC, src), "-o", obj]
         else:
-            cmd = [HIPCC, *common, "-c", os.path.join(CSRC, src), "-o", obj]
+            cmd = [HIPCC, f"--offload-arch={ARCH}", *common, "-c", os.path.join(CSRC, src), "-o", obj]
         print("+", " ".join(cmd), flush=True)
         procs.append(subprocess.Popen(cmd))
         if len(procs) >= jobs:

@@ -34,17 +34,17 @@ enum Precision : int
 };
 
 // One per system; 128-byte aligned so systems never share a cache line.
+// Within one kernel launch no field is both read and written: each kernel's
+// first workgroup of a system writes only fields that kernel does not read.
 struct alignas(128) SysState
 {
-    double   rz, rz0, pAp, alpha, beta, rr, bb, true_rr;
-    double   tol;
-    int      k, status, failure, rule;
-    int      max_iter, k_check, check_every, beta_zero;
-    int      need_check, restarts, n_true_checks, pad0;
-    unsigned counter[6];
+    double rz_prev;   // r.z of the previous iteration (beta denominator): written by update
+    double rz_cur;    // r.z of this iteration: written by the SpMV kernel
+    double rz0, pAp, alpha, beta, rr, bb, true_rr, tol;
+    int    k, status, failure, rule;
+    int    max_iter, k_check, check_every, beta_zero;
+    int    need_check, restarts, n_true_checks, frozen;
 };
-
-enum Counter : int { kCntPre = 0, kCntSpmv = 1, kCntUpd = 2, kCntTrue = 3, kCntLoad = 4, kCntAux = 5 };
 
 enum WgType : int { kWgPlain = 0, kWgFemMas = 1, kWgAbd = 2 };
 
@@ -76,15 +76,18 @@ struct DevData
     float*  Rc;             // coarse restriction, float3 per coarse node
     float*  Zc;
     // MAS
-    const void*     minv;   // n_banks * nblk * 9 values of the storage precision
+    const void*     minv;   // n_banks banks in diagonal-major layout of the storage precision
+    const float*    minv_scale;  // per bank (half-precision storage is scaled to [-1, 1])
     const int*      bank_vec;
     const int*      bank_sys;
+    const int*      bank_kind;   // 0 level 0, 1 level 1, 2 level >= 2
     const int*      comp_ptr;
     const uint32_t* comp_mask;
     const int*      comp_parent;
-    const int*      coarse_parent;
-    const int*      sys_level_ptr;    // n_sys+1 into level ranges
-    const int*      sys_level_banks;  // pairs [b0, b1) of global bank ids
+    const int*      desc_ptr;    // level >= 2 coarse node -> level-1 descendants
+    const int*      desc_list;
+    const int*      row_parent1; // internal row -> level-1 coarse node or -1
+    const int*      anc;         // 8 per coarse node: itself and ancestors
     // ABD and diagonal block-Jacobi
     const float* abd_inv;   // 144 per body, row-major
     const int*   body_row;
@@ -97,7 +100,10 @@ struct DevData
     const int*  wg_pre_range;   // per system [begin,end) of its workgroups in wg_pre (2 ints)
     const int*  wg_spmv_range;
     const int*  wg_upd_range;
-    double*     partial;        // per workgroup partial sums (sized for the largest grid)
+    double*     part_pre;       // per-workgroup partial sums, consumed by the next kernel
+    double*     part_spmv;
+    double*     part_upd;
+    double*     part_tr;
     SysState*   state;
 };
 

@@ -2,17 +2,24 @@
 //
 // Conventions
 //  * vectors are float3 per internal block row (12 bytes, array of structs);
-//  * matrix values for the SpMV are nine float planes (entry-major SoA) so that
-//    lanes reading consecutive entries issue fully coalesced dword loads;
-//  * every reduction is deterministic: per-workgroup partial sums in FP64 are
-//    combined in a fixed order by the last workgroup of each system (detected
-//    with a per-system counter), never with floating point atomics;
+//  * SpMV matrix values are nine float planes (entry-major SoA) so that lanes
+//    reading consecutive entries issue fully coalesced dword loads;
+//  * bank inverses are stored diagonal-major: for block diagonal d (blocks
+//    (I, I+d)), value c of every block is contiguous over I, so the lanes of
+//    a bank read each diagonal with coalesced loads;
+//  * reductions are deterministic and need no fences or atomics: every
+//    workgroup writes its FP64 partial sum, and the consuming kernel (the next
+//    launch) sums a system's partials in a fixed order in each workgroup;
+//    the first workgroup of a system in a kernel writes the scalars that kernel
+//    does not read;
 //  * kernels of a finished system return at once, so a captured chunk of
 //    iterations can run past convergence at negligible cost.
 #include "kernels.hpp"
 
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
+
+#include <type_traits>
 
 namespace masx
 {
@@ -21,17 +28,51 @@ namespace
 
 __device__ __forceinline__ int packed_index(int bank, int r, int c) { return bank * r - r * (r + 1) / 2 + c; }
 
-template <typename T> __device__ __forceinline__ float to_f(T v);
-template <> __device__ __forceinline__ float to_f<float>(float v) { return v; }
-template <> __device__ __forceinline__ float to_f<__half>(__half v) { return __half2float(v); }
-template <> __device__ __forceinline__ float to_f<__hip_bfloat16>(__hip_bfloat16 v) { return __bfloat162float(v); }
+// blocks stored before block diagonal d of a bank
+__host__ __device__ __forceinline__ constexpr int diag_off(int bank, int d) { return d * bank - d * (d - 1) / 2; }
 
-template <typename T> __device__ __forceinline__ T from_d(double v);
-template <> __device__ __forceinline__ float from_d<float>(double v) { return (float)v; }
-template <> __device__ __forceinline__ __half from_d<__half>(double v) { return __float2half((float)v); }
-template <> __device__ __forceinline__ __hip_bfloat16 from_d<__hip_bfloat16>(double v)
+template <typename T> struct Store { static constexpr int kWords = 5; };  // value pairs (0,1)..(6,7),(8,-)
+template <> struct Store<float> { static constexpr int kWords = 9; };
+
+__device__ __forceinline__ float lo_half(uint32_t w, __half*) { return __half2float(__ushort_as_half((unsigned short)(w & 0xffffu))); }
+__device__ __forceinline__ float hi_half(uint32_t w, __half*) { return __half2float(__ushort_as_half((unsigned short)(w >> 16))); }
+__device__ __forceinline__ float lo_half(uint32_t w, __hip_bfloat16*) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float hi_half(uint32_t w, __hip_bfloat16*) { return __uint_as_float(w & 0xffff0000u); }
+
+__device__ __forceinline__ uint32_t half_bits(double v, __half*)
 {
-    return __float2bfloat16((float)v);
+    return (uint32_t)__half_as_ushort(__float2half_rn((float)v));
+}
+__device__ __forceinline__ uint32_t half_bits(double v, __hip_bfloat16*)
+{
+    const __hip_bfloat16 h = __float2bfloat16((float)v);
+    unsigned short u;
+    __builtin_memcpy(&u, &h, 2);
+    return (uint32_t)u;
+}
+
+// Loads the nine values of block (I, I+d) of one bank (row-major within the block).
+template <int BANK>
+__device__ __forceinline__ void load_block(const float* M, int d, int I, float v[9])
+{
+    const int len = BANK - d;
+    const float* p = M + 9 * diag_off(BANK, d) + I;
+#pragma unroll
+    for(int c = 0; c < 9; ++c) v[c] = p[c * len];
+}
+template <int BANK, typename H>
+__device__ __forceinline__ void load_block_h(const uint32_t* M, int d, int I, float v[9])
+{
+    const int len = BANK - d;
+    const uint32_t* p = M + 5 * diag_off(BANK, d) + I;
+#pragma unroll
+    for(int q = 0; q < 4; ++q)
+    {
+        const uint32_t w = p[q * len];
+        v[2 * q]     = lo_half(w, (H*)nullptr);
+        v[2 * q + 1] = hi_half(w, (H*)nullptr);
+    }
+    v[8] = lo_half(p[4 * len], (H*)nullptr);
 }
 
 // wave64 butterfly sum
@@ -42,7 +83,7 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// Block-wide FP64 sum in a fixed order; result valid in thread 0.
+// Block-wide FP64 sum in a fixed order; the result is returned to every thread.
 __device__ double block_sum(double v, double* smem)
 {
     v = wave_sum(v);
@@ -50,43 +91,19 @@ __device__ double block_sum(double v, double* smem)
     if(lane == 0) smem[wid] = v;
     __syncthreads();
     double t = 0.0;
-    if(threadIdx.x == 0)
-        for(int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) t += smem[w];
+    for(int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) t += smem[w];
     __syncthreads();
     return t;
 }
 
-// Publishes this workgroup's partial and reports whether it is the last
-// workgroup of system `s` for counter `cnt`.  In the last workgroup every
-// thread has executed an acquire fence, so other workgroups' global writes
-// are visible.
-__device__ bool publish_partial(const DevData& d, int s, int cnt, double v, int nwg, bool* flag)
+// Sum of the partials of system s's workgroups of the producing kernel
+// (range = per-system [begin, end) pairs).  Identical in every workgroup.
+__device__ double sys_sum(const double* part, const int* range, int s, double* smem)
 {
-    if(threadIdx.x == 0)
-    {
-        d.partial[blockIdx.x] = v;
-        __threadfence();
-        const unsigned prev = atomicAdd(&d.state[s].counter[cnt], 1u);
-        *flag = prev == (unsigned)(nwg - 1);
-    }
-    __syncthreads();
-    const bool last = *flag;
-    if(last) __threadfence();
-    return last;
-}
-
-// Sum of the partials of workgroups [b, e) in a fixed order (whole block
-// participates; result valid in all threads).
-__device__ double sum_partials(const DevData& d, int b, int e, double* smem)
-{
+    const int b = range[2 * s], e = range[2 * s + 1];
     double v = 0.0;
-    for(int i = b + (int)threadIdx.x; i < e; i += blockDim.x) v += d.partial[i];
-    v = block_sum(v, smem);
-    if(threadIdx.x == 0) smem[8] = v;
-    __syncthreads();
-    v = smem[8];
-    __syncthreads();
-    return v;
+    for(int i = b + (int)threadIdx.x; i < e; i += blockDim.x) v += part[i];
+    return block_sum(v, smem);
 }
 
 __device__ __forceinline__ bool finite_d(double v) { return isfinite(v); }
@@ -159,19 +176,21 @@ __global__ void __launch_bounds__(256) k_gal_gather(const double* fine, double* 
 
 // Dense local assembly, FP64 Cholesky with a pivot floor
 // (pro/.../TraditionalMAS32Preconditioner.cu:1382-1439), triangular inverse
-// and A^-1 = L^-T L^-1, stored as packed upper 3x3 blocks (block row-major)
-// in the storage precision T.  One workgroup per bank; the symmetric matrix
-// lives in LDS as a packed lower triangle.
+// and A^-1 = L^-T L^-1, written diagonal-major in the storage precision T.
+// One workgroup per bank; the symmetric matrix lives in LDS as a packed lower
+// triangle.  Half-precision storage is scaled by the largest diagonal entry of
+// the inverse (for an SPD inverse, the largest entry in magnitude).
 template <int BANK, typename T>
 __global__ void __launch_bounds__(256) k_mas_factor(DevData d, const double* E, const int* ent_ptr, const int* ent_pos,
-                                                    const int* ent_src, T* out, int* status)
+                                                    const int* ent_src, void* out, float* scale_out, int* status)
 {
     constexpr int DIM  = 3 * BANK;
     constexpr int NL   = DIM * (DIM + 1) / 2;
     constexpr int NBLK = BANK * (BANK + 1) / 2;
+    constexpr bool kHalf = !std::is_same<T, float>::value;
     __shared__ double L[NL];
     __shared__ double colbuf[DIM];
-    __shared__ double red[2];
+    __shared__ double red[8];
     const int bank = blockIdx.x;
     const int tid  = threadIdx.x;
     auto at = [](int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; };
@@ -254,18 +273,54 @@ __global__ void __launch_bounds__(256) k_mas_factor(DevData d, const double* E, 
         __syncthreads();
     }
     // A^-1(p,q) = sum_{k >= max(p,q)} X(k,p) X(k,q), X = L^-1 (lower)
-    T* dst = out + (size_t)bank * NBLK * 9;
-    for(int t = tid; t < NBLK * 9; t += blockDim.x)
-    {
-        const int blk = t / 9, e = t % 9;
-        int I = 0;
-        while(I + 1 < BANK && packed_index(BANK, I + 1, I + 1) <= blk) ++I;
-        const int J = blk - packed_index(BANK, I, I) + I;
-        const int p = 3 * I + e / 3, q = 3 * J + e % 3;
+    auto inv_at = [&](int p, int q) {
         const int k0 = p > q ? p : q;
         double s = 0.0;
         for(int k = k0; k < DIM; ++k) s += L[at(k, p)] * L[at(k, q)];
-        dst[t] = from_d<T>(s);
+        return s;
+    };
+    double scale = 1.0;
+    if(kHalf)
+    {
+        double m = 0.0;
+        for(int p = tid; p < DIM; p += blockDim.x) m = fmax(m, inv_at(p, p));
+#pragma unroll
+        for(int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+        if((tid & 63) == 0) red[1 + (tid >> 6)] = m;
+        __syncthreads();
+        m = 0.0;
+        for(int w = 0; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, red[1 + w]);
+        scale = m > 0.0 ? m : 1.0;
+    }
+    if(tid == 0) scale_out[bank] = (float)scale;
+    const double inv_scale = 1.0 / scale;
+    if constexpr(!kHalf)
+    {
+        float* dst = reinterpret_cast<float*>(out) + (size_t)bank * NBLK * 9;
+        for(int o = tid; o < NBLK * 9; o += blockDim.x)
+        {
+            int dd = 0;
+            while(dd + 1 < BANK && 9 * diag_off(BANK, dd + 1) <= o) ++dd;
+            const int len = BANK - dd, rem = o - 9 * diag_off(BANK, dd);
+            const int c = rem / len, I = rem % len;
+            dst[o] = (float)inv_at(3 * I + c / 3, 3 * (I + dd) + c % 3);
+        }
+    }
+    else
+    {
+        uint32_t* dst = reinterpret_cast<uint32_t*>(out) + (size_t)bank * NBLK * 5;
+        for(int o = tid; o < NBLK * 5; o += blockDim.x)
+        {
+            int dd = 0;
+            while(dd + 1 < BANK && 5 * diag_off(BANK, dd + 1) <= o) ++dd;
+            const int len = BANK - dd, rem = o - 5 * diag_off(BANK, dd);
+            const int q = rem / len, I = rem % len;
+            const int c0 = 2 * q, c1 = 2 * q + 1;
+            const uint32_t lo = half_bits(inv_at(3 * I + c0 / 3, 3 * (I + dd) + c0 % 3) * inv_scale, (T*)nullptr);
+            const uint32_t hi =
+                c1 < 9 ? half_bits(inv_at(3 * I + c1 / 3, 3 * (I + dd) + c1 % 3) * inv_scale, (T*)nullptr) : 0u;
+            dst[o] = lo | (hi << 16);
+        }
     }
     if(tid == 0 && bad && status) atomicAdd(status, bad);
 }
@@ -388,12 +443,12 @@ __global__ void __launch_bounds__(256) k_store_x(DevData d, const int* ext_to_in
     for(int c = 0; c < 3; ++c) x[3 * (long long)i + c] = (double)d.x[3 * (long long)R + c];
 }
 
-__global__ void k_init_state(DevData d, int rule, double tol, const int* max_iter, int check_every)
+__global__ void k_init_state(DevData d, int rule, double tol, const int* max_iter, int check_every, int frozen)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if(s >= d.n_sys) return;
     SysState& st = d.state[s];
-    st.rz = st.rz0 = st.pAp = st.alpha = st.beta = st.rr = st.bb = st.true_rr = 0.0;
+    st.rz_prev = st.rz_cur = st.rz0 = st.pAp = st.alpha = st.beta = st.rr = st.bb = st.true_rr = 0.0;
     st.tol = tol;
     st.k = 0;
     st.status = kInit;
@@ -406,117 +461,253 @@ __global__ void k_init_state(DevData d, int rule, double tol, const int* max_ite
     st.need_check = 0;
     st.restarts = 0;
     st.n_true_checks = 0;
-    for(int c = 0; c < 6; ++c) st.counter[c] = 0u;
+    st.frozen = frozen;
+}
+
+// Completes the init / restart phase: bb (init), status -> running.
+__global__ void __launch_bounds__(256) k_phase_finish(DevData d)
+{
+    __shared__ double smem[8];
+    const int s = blockIdx.x;
+    SysState& st = d.state[s];
+    if(st.status != kInit && st.status != kRestarting) return;
+    const double v = sys_sum(d.part_upd, d.wg_upd_range, s, smem);
+    if(threadIdx.x == 0)
+    {
+        if(st.status == kInit)
+        {
+            st.bb = v;
+            st.rr = v;
+            st.status = v > 0.0 ? kRunning : kDone;
+        }
+        else
+        {
+            st.rr = v;
+            st.status = kRunning;
+        }
+    }
 }
 
 // ========================================================= iteration kernels
 
-// Coarse restriction of levels >= 2 for system s, by one workgroup:
-// R(parent) = sum of the R of the component's lanes in the bank below.
-__device__ void restrict_coarse_levels(const DevData& d, int s)
-{
-    const int lvl_b = d.sys_level_ptr[s], lvl_e = d.sys_level_ptr[s + 1];
-    const int group = threadIdx.x >> 5, lane = threadIdx.x & 31, ngroups = blockDim.x >> 5;
-    for(int lv = lvl_b; lv + 1 < lvl_e; ++lv)  // banks of level lv feed level lv+1
-    {
-        const int b0 = d.sys_level_banks[2 * lv], b1 = d.sys_level_banks[2 * lv + 1];
-        for(int b = b0 + group; b < b1; b += ngroups)
-        {
-            const int base = d.bank_vec[b] + lane;
-            const float rx = d.Rc[3 * base + 0], ry = d.Rc[3 * base + 1], rz = d.Rc[3 * base + 2];
-            for(int c = d.comp_ptr[b]; c < d.comp_ptr[b + 1]; ++c)
-            {
-                const bool in = (d.comp_mask[c] >> lane) & 1u;
-                float sx = in ? rx : 0.f, sy = in ? ry : 0.f, sz = in ? rz : 0.f;
-#pragma unroll
-                for(int o = 16; o > 0; o >>= 1)
-                {
-                    sx += __shfl_xor(sx, o, 32);
-                    sy += __shfl_xor(sy, o, 32);
-                    sz += __shfl_xor(sz, o, 32);
-                }
-                if(lane == 0)
-                {
-                    const int pc = d.comp_parent[c];
-                    d.Rc[3 * pc + 0] = sx;
-                    d.Rc[3 * pc + 1] = sy;
-                    d.Rc[3 * pc + 2] = sz;
-                }
-            }
-        }
-        __threadfence_block();
-        __syncthreads();
-    }
-}
-
-// MAS local solve: Z = Minv * R for every bank of every level.  Each
-// workgroup stages its banks' packed inverses and residuals in LDS with
-// 16-byte loads, then thread t of a bank forms scalar row t of the product,
-// using the stored upper blocks directly and transposed for the lower part.
+// MAS local solve: Z = Minv * R for every bank of every level.  A workgroup
+// holds 64/BANK banks; each bank is spread over BANK lanes (lane I = block
+// row I) and the four waves take block diagonals d = wave (mod 4).  For the
+// stored block (I, I+d) lane I adds B r_{I+d} to its own row and sends
+// B^T r_I to lane I+d by a shuffle; the four partial results are summed in a
+// fixed order.  Residuals of level >= 2 banks are formed here from the level-1
+// residuals of their descendants.
 template <int BANK, typename T>
-__global__ void __launch_bounds__(192) k_mas_local(DevData d, int force)
+__global__ void __launch_bounds__(256) k_mas_local(DevData d, int force)
 {
-    constexpr int DIM  = 3 * BANK;
-    constexpr int NBLK = BANK * (BANK + 1) / 2;
-    constexpr int BPW  = 192 / DIM;                  // banks per workgroup
-    constexpr int VALS = NBLK * 9;                   // values per bank
-    constexpr int VEC16 = VALS * (int)sizeof(T) / 16;
-    static_assert(VALS * sizeof(T) % 16 == 0, "bank storage must be a multiple of 16 bytes");
-    __shared__ uint4 smat[BPW * VEC16];
-    __shared__ float sR[BPW * DIM];
+    constexpr int BPW   = 64 / BANK;
+    constexpr int NBLK  = BANK * (BANK + 1) / 2;
+    constexpr int WORDS = Store<T>::kWords;
+    constexpr int SHORT_LIST = 64;
+    __shared__ float  sR[64 * 3];
+    __shared__ float  sZ[3][64 * 3];
+    __shared__ int    s_live[BPW], s_kind[BPW], s_vec[BPW];
+    __shared__ int    s_dbeg[64], s_dend[64];
+    __shared__ double smem[8];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int lb = lane / BANK, I = lane % BANK;
     const int bank0 = blockIdx.x * BPW;
-    const int nb = min(BPW, d.n_banks - bank0);
-    // a workgroup's banks may belong to different systems
-    bool any_live = force != 0;
-    for(int i = 0; i < nb && !any_live; ++i) any_live = d.state[d.bank_sys[bank0 + i]].status == kRunning;
-    if(!any_live) return;
-    const uint4* g = reinterpret_cast<const uint4*>(reinterpret_cast<const T*>(d.minv) + (size_t)bank0 * VALS);
-    for(int i = threadIdx.x; i < nb * VEC16; i += blockDim.x) smat[i] = g[i];
-    for(int i = threadIdx.x; i < nb * DIM; i += blockDim.x)
+    if(tid < BPW)
     {
-        const int b = bank0 + i / DIM;
-        const float* src = b < d.n_fine_banks ? d.r : d.Rc;
-        sR[i] = src[3 * (long long)d.bank_vec[b] + i % DIM];
+        const int b = bank0 + tid;
+        int live = 0, kind = 0, vec = 0;
+        if(b < d.n_banks)
+        {
+            live = force || d.state[d.bank_sys[b]].status == kRunning;
+            kind = d.bank_kind[b];
+            vec  = d.bank_vec[b];
+        }
+        s_live[tid] = live;
+        s_kind[tid] = kind;
+        s_vec[tid]  = vec;
     }
     __syncthreads();
-    const int lb = threadIdx.x / DIM;
-    if(lb >= nb) return;
-    const int b = bank0 + lb;
-    if(!force && d.state[d.bank_sys[b]].status != kRunning) return;
-    const int t = threadIdx.x % DIM;
-    const int I = t / 3, a = t % 3;
-    const T*     M = reinterpret_cast<const T*>(smat) + (size_t)lb * VALS;
-    const float* R = sR + lb * DIM;
-    float acc = 0.f;
-    int   pd  = packed_index(BANK, I, I);  // block (I, J) for J >= I is pd + (J - I)
-    int   pt  = I;                          // block (J, I) for J < I, advanced per J
-#pragma unroll 4
-    for(int J = 0; J < BANK; ++J)
+    bool any = false, deep = false;
+#pragma unroll
+    for(int k = 0; k < BPW; ++k)
     {
-        const bool up  = J >= I;
-        const int  blk = up ? pd + (J - I) : pt;
-        const int  off = up ? blk * 9 + 3 * a : blk * 9 + a;
-        const int  st  = up ? 1 : 3;
-        acc += to_f(M[off]) * R[3 * J] + to_f(M[off + st]) * R[3 * J + 1] + to_f(M[off + 2 * st]) * R[3 * J + 2];
-        if(!up) pt += BANK - (J + 1);
+        any |= s_live[k] != 0;
+        deep |= s_live[k] && s_kind[k] == 2;
     }
-    float* dst = b < d.n_fine_banks ? d.z : d.Zc;
-    dst[3 * (long long)d.bank_vec[b] + t] = acc;
+    if(!any) return;
+    // residuals of level-0 and level-1 banks
+    for(int i = tid; i < 64 * 3; i += blockDim.x)
+    {
+        const int k = i / (BANK * 3);
+        float v = 0.f;
+        if(s_live[k] && s_kind[k] < 2)
+            v = (s_kind[k] == 0 ? d.r : d.Rc)[3 * (long long)s_vec[k] + i % (BANK * 3)];
+        sR[i] = v;
+    }
+    if(deep)
+    {
+        // level >= 2: R(node) = sum of its level-1 descendants' residuals
+        if(tid < 64)
+        {
+            const int k = tid / BANK;
+            int b0 = 0, e0 = 0;
+            if(s_live[k] && s_kind[k] == 2)
+            {
+                const int c = s_vec[k] + tid % BANK;
+                b0 = d.desc_ptr[c];
+                e0 = d.desc_ptr[c + 1];
+            }
+            s_dbeg[tid] = b0;
+            s_dend[tid] = e0;
+        }
+        __syncthreads();
+        {
+            const int node = tid >> 2, q = tid & 3;
+            const int k = node / BANK;
+            const int b0 = s_dbeg[node], e0 = s_dend[node];
+            const bool mine = s_live[k] && s_kind[k] == 2 && e0 - b0 <= SHORT_LIST;
+            float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+            if(mine)
+                for(int e = b0 + q; e < e0; e += 4)
+                {
+                    const long long c1 = 3 * (long long)d.desc_list[e];
+                    a0 += d.Rc[c1];
+                    a1 += d.Rc[c1 + 1];
+                    a2 += d.Rc[c1 + 2];
+                }
+            a0 += __shfl_xor(a0, 1, 64); a1 += __shfl_xor(a1, 1, 64); a2 += __shfl_xor(a2, 1, 64);
+            a0 += __shfl_xor(a0, 2, 64); a1 += __shfl_xor(a1, 2, 64); a2 += __shfl_xor(a2, 2, 64);
+            if(mine && q == 0)
+            {
+                sR[3 * node] = a0; sR[3 * node + 1] = a1; sR[3 * node + 2] = a2;
+            }
+        }
+        // long lists (the top levels): the whole workgroup per node
+        for(int node = 0; node < 64; ++node)
+        {
+            const int k = node / BANK;
+            const int b0 = s_dbeg[node], e0 = s_dend[node];
+            if(!(s_live[k] && s_kind[k] == 2) || e0 - b0 <= SHORT_LIST) continue;
+            double a[3] = {0.0, 0.0, 0.0};
+            for(int e = b0 + tid; e < e0; e += blockDim.x)
+            {
+                const long long c1 = 3 * (long long)d.desc_list[e];
+                a[0] += d.Rc[c1]; a[1] += d.Rc[c1 + 1]; a[2] += d.Rc[c1 + 2];
+            }
+            for(int c = 0; c < 3; ++c)
+            {
+                const double t = block_sum(a[c], smem);
+                if(tid == 0) sR[3 * node + c] = (float)t;
+            }
+        }
+    }
+    __syncthreads();
+    const int b = bank0 + lb;
+    const bool live = b < d.n_banks && s_live[lb];
+    const float r0 = sR[3 * lane], r1 = sR[3 * lane + 1], r2 = sR[3 * lane + 2];
+    float z0 = 0.f, z1 = 0.f, z2 = 0.f;
+    if(b < d.n_banks)
+    {
+#pragma unroll
+        for(int kk = 0; kk < BANK / 4; ++kk)
+        {
+            const int dd  = wave + 4 * kk;
+            const int len = BANK - dd;
+            const bool act = live && I < len;
+            float v[9];
+            if(act)
+            {
+                if constexpr(WORDS == 9)
+                    load_block<BANK>(reinterpret_cast<const float*>(d.minv) + (size_t)b * NBLK * 9, dd, I, v);
+                else
+                    load_block_h<BANK, T>(reinterpret_cast<const uint32_t*>(d.minv) + (size_t)b * NBLK * 5, dd, I, v);
+            }
+            else
+            {
+#pragma unroll
+                for(int c = 0; c < 9; ++c) v[c] = 0.f;
+            }
+            const int j = act ? lane + dd : lane;
+            const float q0 = sR[3 * j], q1 = sR[3 * j + 1], q2 = sR[3 * j + 2];
+            z0 += v[0] * q0 + v[1] * q1 + v[2] * q2;
+            z1 += v[3] * q0 + v[4] * q1 + v[5] * q2;
+            z2 += v[6] * q0 + v[7] * q1 + v[8] * q2;
+            // transposed contribution to row I + dd
+            float t0 = v[0] * r0 + v[3] * r1 + v[6] * r2;
+            float t1 = v[1] * r0 + v[4] * r1 + v[7] * r2;
+            float t2 = v[2] * r0 + v[5] * r1 + v[8] * r2;
+            t0 = __shfl_up(t0, dd, BANK);
+            t1 = __shfl_up(t1, dd, BANK);
+            t2 = __shfl_up(t2, dd, BANK);
+            if(dd > 0 && I >= dd)
+            {
+                z0 += t0; z1 += t1; z2 += t2;
+            }
+        }
+    }
+    if(wave > 0)
+    {
+        sZ[wave - 1][3 * lane] = z0; sZ[wave - 1][3 * lane + 1] = z1; sZ[wave - 1][3 * lane + 2] = z2;
+    }
+    __syncthreads();
+    if(wave == 0 && live)
+    {
+#pragma unroll
+        for(int w = 0; w < 3; ++w)
+        {
+            z0 += sZ[w][3 * lane]; z1 += sZ[w][3 * lane + 1]; z2 += sZ[w][3 * lane + 2];
+        }
+        if(WORDS != 9)
+        {
+            const float sc = d.minv_scale[b];
+            z0 *= sc; z1 *= sc; z2 *= sc;
+        }
+        float* dst = (s_kind[lb] == 0 ? d.z : d.Zc) + 3 * (long long)(s_vec[lb] + I);
+        dst[0] = z0; dst[1] = z1; dst[2] = z2;
+    }
 }
 
-// Finish the preconditioner: prolong coarse corrections into the level-0
+// Convergence decision after the previous update (or init / restart), then
+// finish the preconditioner: prolong the coarse corrections into the level-0
 // result (MAS rows), 12x12 block-Jacobi (ABD bodies), identity or 3x3
-// block-Jacobi (plain rows); then r.z and beta.
+// block-Jacobi (plain rows); partial r.z.
 __global__ void __launch_bounds__(256) k_precond_finish(DevData d, int force)
 {
-    __shared__ double smem[16];
-    __shared__ bool   flag;
+    __shared__ double smem[8];
     const int4 w = d.wg_pre[blockIdx.x];
     const int  s = w.x;
     SysState&  st = d.state[s];
-    if(!force && st.status != kRunning) return;
-    const int type = w.y >= 0 ? (w.y >> 28) : 0;
-    const int aux  = w.y & 0x0fffffff;
+    if(!force)
+    {
+        if(st.status != kRunning) return;
+        const double rr = sys_sum(d.part_upd, d.wg_upd_range, s, smem);
+        int ns = kRunning, fail = kOk, chk = 0;
+        if(!st.frozen)
+        {
+            if(!finite_d(rr)) { ns = kDone; fail = kNonFinite; }
+            else if(st.rule == kRuleRefRtz)
+            {
+                // pcg_solver.cu:277: the gate tests the r.z of the previous iteration
+                if(st.k >= 1 && fabs(st.rz_cur) <= st.tol * st.rz0) ns = kDone;
+                else if(st.k >= 1 && st.k + 1 >= st.max_iter) { ns = kDone; fail = kIterationCap; }
+            }
+            else
+            {
+                if(rr <= st.tol * st.tol * st.bb) { ns = kCandidate; chk = 1; }
+                else if(st.k >= st.max_iter) { ns = kDone; fail = kIterationCap; }
+                else if(st.check_every > 0 && st.k - st.k_check >= st.check_every) chk = 1;
+            }
+        }
+        if(blockIdx.x == d.wg_pre_range[2 * s] && threadIdx.x == 0)
+        {
+            st.rr = rr;
+            if(ns != kRunning) st.status = ns;
+            if(fail != kOk) st.failure = fail;
+            if(chk) st.need_check = 1;
+        }
+        if(ns != kRunning) return;
+    }
+    const int type = w.y >> 28;
     double part = 0.0;
     if(type == kWgAbd)
     {
@@ -544,17 +735,20 @@ __global__ void __launch_bounds__(256) k_precond_finish(DevData d, int force)
             if(type == kWgFemMas)
             {
                 z0 = d.z[o]; z1 = d.z[o + 1]; z2 = d.z[o + 2];
-                const int gb = aux + (int)(threadIdx.x >> 5);
-                const int lane = threadIdx.x & 31;
-                int c = -1;
-                for(int k = d.comp_ptr[gb]; k < d.comp_ptr[gb + 1]; ++k)
-                    if((d.comp_mask[k] >> lane) & 1u) { c = d.comp_parent[k]; break; }
-                while(c >= 0)
+                const int p1 = d.row_parent1[row];
+                if(p1 >= 0)
                 {
-                    z0 += d.Zc[3 * c];
-                    z1 += d.Zc[3 * c + 1];
-                    z2 += d.Zc[3 * c + 2];
-                    c = d.coarse_parent[c];
+                    const int4 a = reinterpret_cast<const int4*>(d.anc)[2 * p1];
+                    const int4 c = reinterpret_cast<const int4*>(d.anc)[2 * p1 + 1];
+                    const int ids[8] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w};
+#pragma unroll
+                    for(int k = 0; k < 8; ++k)
+                        if(ids[k] >= 0)
+                        {
+                            z0 += d.Zc[3 * (long long)ids[k]];
+                            z1 += d.Zc[3 * (long long)ids[k] + 1];
+                            z2 += d.Zc[3 * (long long)ids[k] + 2];
+                        }
                 }
             }
             else if(d.row_mode[row])
@@ -574,44 +768,56 @@ __global__ void __launch_bounds__(256) k_precond_finish(DevData d, int force)
     }
     if(force) return;
     part = block_sum(part, smem);
-    const int* rg = d.wg_pre_range + 2 * s;
-    if(!publish_partial(d, s, kCntPre, part, rg[1] - rg[0], &flag)) return;
-    const double rz = sum_partials(d, rg[0], rg[1], smem);
-    if(threadIdx.x == 0)
-    {
-        st.counter[kCntPre] = 0u;
-        if(st.k == 0 && st.restarts == 0) st.rz0 = rz;
-        if(!finite_d(rz) || (st.rule == kRuleTrueRel && rz <= 0.0))
-        {
-            st.status = kDone;
-            st.failure = kBadRz;
-        }
-        st.beta = st.beta_zero ? 0.0 : rz / st.rz;
-        st.beta_zero = 0;
-        st.rz = rz;
-    }
+    if(threadIdx.x == 0) d.part_pre[blockIdx.x] = part;
 }
 
-// Ap = A p_new with p_new = z + beta * p_old formed on the fly (the p update
-// of the previous iteration is fused here), p.Ap, alpha.
-template <int G>
+// beta from the r.z partials, then Ap = A p_new with p_new = z + beta p_old
+// formed on the fly (the p update is fused here); partial p.Ap.  G lanes per
+// block row, P row passes per workgroup with their loads interleaved.
+template <int G, int P>
 __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __restrict__ p_old,
                                                     float* __restrict__ p_new)
 {
-    __shared__ double smem[16];
-    __shared__ bool   flag;
+    __shared__ double smem[8];
     const int4 w = d.wg_spmv[blockIdx.x];
     const int  s = w.x;
     SysState&  st = d.state[s];
     if(st.status != kRunning) return;
-    const float beta = (float)st.beta;
+    const double rz = sys_sum(d.part_pre, d.wg_pre_range, s, smem);
+    const bool bad = !finite_d(rz) || (st.rule == kRuleTrueRel && rz <= 0.0);
+    if(blockIdx.x == d.wg_spmv_range[2 * s] && threadIdx.x == 0)
+    {
+        if(bad && !st.frozen)
+        {
+            st.status = kDone;
+            st.failure = kBadRz;
+        }
+        st.rz_cur = rz;
+        st.beta = st.beta_zero ? 0.0 : rz / st.rz_prev;
+        if(st.k == 0 && st.restarts == 0) st.rz0 = rz;
+    }
+    if(bad && !st.frozen) return;
+    const float beta = st.beta_zero ? 0.f : (float)(rz / st.rz_prev);
     const int   sub = threadIdx.x % G;
-    double      part = 0.0;
-    for(int row = w.z + (int)threadIdx.x / G; row < w.w; row += 256 / G)
+    constexpr int RPP = 256 / G;  // rows per pass
+    double part = 0.0;
+    int rows[P], eb[P], ee[P];
+#pragma unroll
+    for(int q = 0; q < P; ++q)
+    {
+        rows[q] = w.z + q * RPP + (int)threadIdx.x / G;
+        eb[q] = ee[q] = 0;
+        if(rows[q] < w.w)
+        {
+            eb[q] = d.row_ptr[rows[q]];
+            ee[q] = d.row_ptr[rows[q] + 1];
+        }
+    }
+#pragma unroll
+    for(int q = 0; q < P; ++q)
     {
         float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-        const int e_end = d.row_ptr[row + 1];
-        for(int e = d.row_ptr[row] + sub; e < e_end; e += G)
+        for(int e = eb[q] + sub; e < ee[q]; e += G)
         {
             const long long c = 3 * (long long)d.col[e];
             const float x0 = d.z[c] + beta * p_old[c];
@@ -629,53 +835,64 @@ __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __re
             a1 += __shfl_xor(a1, o, G);
             a2 += __shfl_xor(a2, o, G);
         }
-        if(sub == 0)
+        if(sub == 0 && rows[q] < w.w)
         {
-            const long long o = 3 * (long long)row;
-            const float q0 = d.z[o] + beta * p_old[o];
-            const float q1 = d.z[o + 1] + beta * p_old[o + 1];
-            const float q2 = d.z[o + 2] + beta * p_old[o + 2];
-            p_new[o] = q0; p_new[o + 1] = q1; p_new[o + 2] = q2;
+            const long long o = 3 * (long long)rows[q];
+            const float y0 = d.z[o] + beta * p_old[o];
+            const float y1 = d.z[o + 1] + beta * p_old[o + 1];
+            const float y2 = d.z[o + 2] + beta * p_old[o + 2];
+            p_new[o] = y0; p_new[o + 1] = y1; p_new[o + 2] = y2;
             d.Ap[o] = a0; d.Ap[o + 1] = a1; d.Ap[o + 2] = a2;
-            part += (double)q0 * a0 + (double)q1 * a1 + (double)q2 * a2;
+            part += (double)y0 * a0 + (double)y1 * a1 + (double)y2 * a2;
         }
     }
     part = block_sum(part, smem);
-    const int* rg = d.wg_spmv_range + 2 * s;
-    if(!publish_partial(d, s, kCntSpmv, part, rg[1] - rg[0], &flag)) return;
-    const double pAp = sum_partials(d, rg[0], rg[1], smem);
-    if(threadIdx.x == 0)
-    {
-        st.counter[kCntSpmv] = 0u;
-        st.pAp = pAp;
-        st.k += 1;
-        if(!finite_d(pAp) || (st.rule == kRuleTrueRel && pAp <= 0.0))
-        {
-            st.status = kDone;
-            st.failure = kBadCurvature;
-            st.alpha = 0.0;
-        }
-        else
-            st.alpha = st.rz / pAp;
-    }
+    if(threadIdx.x == 0) d.part_spmv[blockIdx.x] = part;
 }
 
-// x += alpha p, r -= alpha Ap (mode 0), or r := rt (restart / init, mode 1);
-// level-1 restriction of the MAS rows; r.r; convergence decision; coarse
-// restriction of levels >= 2 by the last workgroup of each system.
+// mode 0: alpha from the p.Ap partials, x += alpha p, r -= alpha Ap;
+// mode 1 (init / restart): r := rt.  Then the level-1 restriction of the MAS
+// rows and the partial r.r (b.b in FP64 at init).
 __global__ void __launch_bounds__(256) k_update(DevData d, const float* __restrict__ p, int mode)
 {
-    __shared__ double smem[16];
-    __shared__ bool   flag;
+    __shared__ double smem[8];
     const int4 w = d.wg_upd[blockIdx.x];
     const int  s = w.x;
     SysState&  st = d.state[s];
-    const int want = mode == 0 ? kRunning : -1;
-    if(mode == 0 ? st.status != want : !(st.status == kRestarting || st.status == kInit)) return;
-    const int   type  = w.y >= 0 ? (w.y >> 28) : 0;
-    const int   aux   = w.y & 0x0fffffff;
-    const float alpha = (float)st.alpha;
-    const int   row   = w.z + threadIdx.x;
+    float alpha = 0.f;
+    bool init = false;
+    if(mode == 0)
+    {
+        if(st.status != kRunning) return;
+        const double pAp = sys_sum(d.part_spmv, d.wg_spmv_range, s, smem);
+        const bool bad = !finite_d(pAp) || (st.rule == kRuleTrueRel && pAp <= 0.0);
+        if(blockIdx.x == d.wg_upd_range[2 * s] && threadIdx.x == 0)
+        {
+            if(bad && !st.frozen)
+            {
+                st.status = kDone;
+                st.failure = kBadCurvature;
+            }
+            else
+            {
+                st.pAp = pAp;
+                st.alpha = st.rz_cur / pAp;
+                st.k = st.k + 1;
+                st.rz_prev = st.rz_cur;
+                st.beta_zero = 0;
+            }
+        }
+        if(bad && !st.frozen) return;
+        alpha = (float)(st.rz_cur / pAp);
+    }
+    else
+    {
+        if(st.status != kRestarting && st.status != kInit) return;
+        init = st.status == kInit;
+    }
+    const int type = w.y >> 28;
+    const int aux  = w.y & 0x0fffffff;
+    const int row  = w.z + threadIdx.x;
     float r0 = 0.f, r1 = 0.f, r2 = 0.f;
     double part = 0.0;
     if(row < w.w)
@@ -695,90 +912,47 @@ __global__ void __launch_bounds__(256) k_update(DevData d, const float* __restri
             r0 = d.rt[o]; r1 = d.rt[o + 1]; r2 = d.rt[o + 2];
         }
         d.r[o] = r0; d.r[o + 1] = r1; d.r[o + 2] = r2;
-        if(st.status == kInit)
+        if(init)
             part = d.b64[o] * d.b64[o] + d.b64[o + 1] * d.b64[o + 1] + d.b64[o + 2] * d.b64[o + 2];
         else
             part = (double)r0 * r0 + (double)r1 * r1 + (double)r2 * r2;
     }
-    if(type == kWgFemMas)
+    // level-1 restriction: the bank's lanes (a power of two, <= 64) sum the
+    // residual rows of each of its components
+    const int bank_shift = __ffs(d.bank) - 1;
+    if(type == kWgFemMas && w.z + (int)((threadIdx.x >> bank_shift) << bank_shift) < w.w)
     {
-        const int gb = aux + (int)(threadIdx.x >> 5);
-        const int lane = threadIdx.x & 31;
-        if(w.z + (int)(threadIdx.x & ~31) < w.w)
-            for(int c = d.comp_ptr[gb]; c < d.comp_ptr[gb + 1]; ++c)
+        const int gb = aux + (int)(threadIdx.x >> bank_shift);
+        const int lane = threadIdx.x & (d.bank - 1);
+        for(int c = d.comp_ptr[gb]; c < d.comp_ptr[gb + 1]; ++c)
+        {
+            const bool in = (d.comp_mask[c] >> lane) & 1u;
+            float sx = in ? r0 : 0.f, sy = in ? r1 : 0.f, sz = in ? r2 : 0.f;
+            for(int o = d.bank >> 1; o > 0; o >>= 1)
             {
-                const bool in = (d.comp_mask[c] >> lane) & 1u;
-                float sx = in ? r0 : 0.f, sy = in ? r1 : 0.f, sz = in ? r2 : 0.f;
-#pragma unroll
-                for(int o = 16; o > 0; o >>= 1)
-                {
-                    sx += __shfl_xor(sx, o, 32);
-                    sy += __shfl_xor(sy, o, 32);
-                    sz += __shfl_xor(sz, o, 32);
-                }
-                if(lane == 0)
-                {
-                    const int pc = d.comp_parent[c];
-                    d.Rc[3 * pc] = sx; d.Rc[3 * pc + 1] = sy; d.Rc[3 * pc + 2] = sz;
-                }
+                sx += __shfl_xor(sx, o, 64);
+                sy += __shfl_xor(sy, o, 64);
+                sz += __shfl_xor(sz, o, 64);
             }
+            if(lane == 0)
+            {
+                const long long pc = 3 * (long long)d.comp_parent[c];
+                d.Rc[pc] = sx; d.Rc[pc + 1] = sy; d.Rc[pc + 2] = sz;
+            }
+        }
     }
     part = block_sum(part, smem);
-    const int* rg = d.wg_upd_range + 2 * s;
-    if(!publish_partial(d, s, kCntUpd, part, rg[1] - rg[0], &flag)) return;
-    const double rr = sum_partials(d, rg[0], rg[1], smem);
-    __shared__ int next_status;
-    if(threadIdx.x == 0)
-    {
-        st.counter[kCntUpd] = 0u;
-        int ns = kRunning;
-        if(st.status == kInit)
-        {
-            st.bb = rr;
-            st.rr = rr;
-            if(rr == 0.0) ns = kDone;  // zero right-hand side: x = 0
-        }
-        else if(st.status == kRestarting)
-        {
-            st.rr = rr;
-        }
-        else
-        {
-            st.rr = rr;
-            if(!finite_d(rr))
-            {
-                ns = kDone;
-                st.failure = kNonFinite;
-            }
-            else if(st.rule == kRuleRefRtz)
-            {
-                // pcg_solver.cu:277: the gate tests the r.z of the previous iteration
-                if(fabs(st.rz) <= st.tol * st.rz0) ns = kDone;
-                else if(st.k + 1 >= st.max_iter) { ns = kDone; st.failure = kIterationCap; }
-            }
-            else
-            {
-                if(rr <= st.tol * st.tol * st.bb) { ns = kCandidate; st.need_check = 1; }
-                else if(st.k >= st.max_iter) { ns = kDone; st.failure = kIterationCap; }
-                else if(st.check_every > 0 && st.k - st.k_check >= st.check_every) st.need_check = 1;
-            }
-        }
-        st.status   = ns;
-        next_status = ns;
-    }
-    __syncthreads();
-    if(next_status == kRunning) restrict_coarse_levels(d, s);
+    if(threadIdx.x == 0) d.part_upd[blockIdx.x] = part;
 }
 
 // r_true = b - A x in FP64 from the FP64 input blocks (rows of the SpMV
-// tiling) for systems that asked for a check; decides convergence / restart.
+// tiling) for systems that asked for a check (or all, when reporting).
 __global__ void __launch_bounds__(256) k_true_residual(DevData d, int report_only)
 {
-    __shared__ double smem[16];
-    __shared__ bool   flag;
+    __shared__ double smem[8];
     const int4 w = d.wg_spmv[blockIdx.x];
     const int  s = w.x;
-    SysState&  st = d.state[s];
+    const SysState& st = d.state[s];
     if(!report_only && !st.need_check) return;
     double part = 0.0;
     for(int row = w.z + (int)threadIdx.x; row < w.w; row += blockDim.x)
@@ -809,42 +983,47 @@ __global__ void __launch_bounds__(256) k_true_residual(DevData d, int report_onl
         part += t0 * t0 + t1 * t1 + t2 * t2;
     }
     part = block_sum(part, smem);
-    const int* rg = d.wg_spmv_range + 2 * s;
-    if(!publish_partial(d, s, kCntTrue, part, rg[1] - rg[0], &flag)) return;
-    const double tr = sum_partials(d, rg[0], rg[1], smem);
-    if(threadIdx.x == 0)
+    if(threadIdx.x == 0) d.part_tr[blockIdx.x] = part;
+}
+
+// One workgroup per system: true residual decision (done / restart on drift).
+__global__ void __launch_bounds__(256) k_check_finish(DevData d, int report_only)
+{
+    __shared__ double smem[8];
+    const int s = blockIdx.x;
+    SysState& st = d.state[s];
+    if(!report_only && !st.need_check) return;
+    const double tr = sys_sum(d.part_tr, d.wg_spmv_range, s, smem);
+    if(threadIdx.x != 0) return;
+    st.true_rr = tr;
+    if(report_only) return;
+    st.n_true_checks += 1;
+    st.need_check = 0;
+    st.k_check = st.k;
+    const double tol2 = st.tol * st.tol * st.bb;
+    bool restart = false;
+    if(st.status == kCandidate)
     {
-        st.counter[kCntTrue] = 0u;
-        st.true_rr = tr;
-        if(report_only) return;
-        st.n_true_checks += 1;
-        st.need_check = 0;
-        st.k_check = st.k;
-        const double tol2 = st.tol * st.tol * st.bb;
-        if(st.status == kCandidate)
-        {
-            if(tr <= tol2) st.status = kDone;
-            else
-            {
-                st.status = kRestarting;
-                st.beta_zero = 1;
-                st.restarts += 1;
-            }
-        }
-        else if(st.status == kRunning)
-        {
-            const double drift = fabs(sqrt(tr) - sqrt(st.rr));
-            if(drift > 10.0 * st.tol * sqrt(st.bb))
-            {
-                st.status = kRestarting;
-                st.beta_zero = 1;
-                st.restarts += 1;
-            }
-        }
-        if(st.status == kRestarting && st.k >= st.max_iter)
+        if(tr <= tol2) st.status = kDone;
+        else restart = true;
+    }
+    else if(st.status == kRunning)
+    {
+        const double drift = fabs(sqrt(tr) - sqrt(st.rr));
+        restart = drift > 10.0 * st.tol * sqrt(st.bb);
+    }
+    if(restart)
+    {
+        if(st.k >= st.max_iter)
         {
             st.status = kDone;
             st.failure = kIterationCap;
+        }
+        else
+        {
+            st.status = kRestarting;
+            st.beta_zero = 1;
+            st.restarts += 1;
         }
     }
 }
@@ -903,14 +1082,6 @@ __global__ void __launch_bounds__(256) k_axpby_int_to_ext(const int* ext_to_int,
     }
 }
 
-__global__ void __launch_bounds__(256) k_ext_to_r32(const int* ext_to_int, const double* src_ext, float* r, int n_ext)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if(i >= n_ext) return;
-    const long long R = ext_to_int[i];
-    for(int c = 0; c < 3; ++c) r[3 * R + c] = (float)src_ext[3 * (long long)i + c];
-}
-
 __global__ void __launch_bounds__(256) k_z32_to_ext(const int* ext_to_int, const float* z, double* dst_ext, int n_ext)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -954,12 +1125,19 @@ hipError_t launch_gal_gather(const double* fine, double* E, const int* ptr, cons
     return hipSuccess;
 }
 
+size_t mas_bank_bytes(int bank, int precision)
+{
+    const size_t nblk = (size_t)bank * (bank + 1) / 2;
+    return nblk * (precision == kPrecFp32 ? 9 : 5) * 4;
+}
+
 hipError_t launch_mas_factor(const DevData& d, const double* E, const int* ent_ptr, const int* ent_pos,
-                             const int* ent_src, void* out, int precision, int* status, hipStream_t st)
+                             const int* ent_src, void* out, float* scale, int precision, int* status, hipStream_t st)
 {
     if(d.n_banks <= 0) return hipSuccess;
     const dim3 g(d.n_banks), b(256);
-#define MASX_FACTOR(BK, T) hipLaunchKernelGGL((k_mas_factor<BK, T>), g, b, 0, st, d, E, ent_ptr, ent_pos, ent_src, (T*)out, status)
+#define MASX_FACTOR(BK, T) \
+    hipLaunchKernelGGL((k_mas_factor<BK, T>), g, b, 0, st, d, E, ent_ptr, ent_pos, ent_src, out, scale, status)
     if(d.bank == 32)
     {
         if(precision == kPrecFp16) MASX_FACTOR(32, __half);
@@ -1011,11 +1189,18 @@ hipError_t launch_store_x(const DevData& d, const int* e2i, double* x, int n_ext
     return hipSuccess;
 }
 
-hipError_t launch_init_state(const DevData& d, int rule, double tol, const int* max_iter, int check_every,
+hipError_t launch_init_state(const DevData& d, int rule, double tol, const int* max_iter, int check_every, int frozen,
                              hipStream_t st)
 {
     hipLaunchKernelGGL(k_init_state, dim3(blocks_for(d.n_sys, 64)), dim3(64), 0, st, d, rule, tol, max_iter,
-                       check_every);
+                       check_every, frozen);
+    MASX_CHECK_LAUNCH();
+    return hipSuccess;
+}
+
+hipError_t launch_phase_finish(const DevData& d, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_phase_finish, dim3(d.n_sys), dim3(256), 0, st, d);
     MASX_CHECK_LAUNCH();
     return hipSuccess;
 }
@@ -1023,8 +1208,8 @@ hipError_t launch_init_state(const DevData& d, int rule, double tol, const int* 
 hipError_t launch_mas_local(const DevData& d, int precision, int force, hipStream_t st)
 {
     if(d.n_banks <= 0) return hipSuccess;
-    const int bpw = 192 / (3 * d.bank);
-    const dim3 g(blocks_for(d.n_banks, bpw)), b(192);
+    const int bpw = 64 / d.bank;
+    const dim3 g(blocks_for(d.n_banks, bpw)), b(256);
 #define MASX_LOCAL(BK, T) hipLaunchKernelGGL((k_mas_local<BK, T>), g, b, 0, st, d, force)
     if(d.bank == 32)
     {
@@ -1053,18 +1238,36 @@ hipError_t launch_precond_finish(const DevData& d, int n_wg, int force, hipStrea
     return hipSuccess;
 }
 
-hipError_t launch_spmv_fused(const DevData& d, int n_wg, int lanes_per_row, const float* p_old, float* p_new,
-                             hipStream_t st)
+bool spmv_tiling_supported(int lanes_per_row, int passes)
+{
+    switch(lanes_per_row * 16 + passes)
+    {
+        case 4 * 16 + 1: case 4 * 16 + 2: case 8 * 16 + 1: case 8 * 16 + 2: case 8 * 16 + 4: case 16 * 16 + 1:
+        case 16 * 16 + 2: case 16 * 16 + 4: case 32 * 16 + 1: case 32 * 16 + 2: return true;
+        default: return false;
+    }
+}
+
+hipError_t launch_spmv_fused(const DevData& d, int n_wg, int lanes_per_row, int passes, const float* p_old,
+                             float* p_new, hipStream_t st)
 {
     if(n_wg <= 0) return hipSuccess;
-    switch(lanes_per_row)
+#define MASX_SPMV(G, P) hipLaunchKernelGGL((k_spmv_fused<G, P>), dim3(n_wg), dim3(256), 0, st, d, p_old, p_new)
+    switch(lanes_per_row * 16 + passes)
     {
-        case 4: hipLaunchKernelGGL((k_spmv_fused<4>), dim3(n_wg), dim3(256), 0, st, d, p_old, p_new); break;
-        case 8: hipLaunchKernelGGL((k_spmv_fused<8>), dim3(n_wg), dim3(256), 0, st, d, p_old, p_new); break;
-        case 16: hipLaunchKernelGGL((k_spmv_fused<16>), dim3(n_wg), dim3(256), 0, st, d, p_old, p_new); break;
-        case 32: hipLaunchKernelGGL((k_spmv_fused<32>), dim3(n_wg), dim3(256), 0, st, d, p_old, p_new); break;
+        case 4 * 16 + 1: MASX_SPMV(4, 1); break;
+        case 4 * 16 + 2: MASX_SPMV(4, 2); break;
+        case 8 * 16 + 1: MASX_SPMV(8, 1); break;
+        case 8 * 16 + 2: MASX_SPMV(8, 2); break;
+        case 8 * 16 + 4: MASX_SPMV(8, 4); break;
+        case 16 * 16 + 1: MASX_SPMV(16, 1); break;
+        case 16 * 16 + 2: MASX_SPMV(16, 2); break;
+        case 16 * 16 + 4: MASX_SPMV(16, 4); break;
+        case 32 * 16 + 1: MASX_SPMV(32, 1); break;
+        case 32 * 16 + 2: MASX_SPMV(32, 2); break;
         default: return hipErrorInvalidValue;
     }
+#undef MASX_SPMV
     MASX_CHECK_LAUNCH();
     return hipSuccess;
 }
@@ -1081,6 +1284,8 @@ hipError_t launch_true_residual(const DevData& d, int n_wg, int report_only, hip
 {
     if(n_wg <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_true_residual, dim3(n_wg), dim3(256), 0, st, d, report_only);
+    MASX_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_check_finish, dim3(d.n_sys), dim3(256), 0, st, d, report_only);
     MASX_CHECK_LAUNCH();
     return hipSuccess;
 }
@@ -1107,14 +1312,6 @@ hipError_t launch_axpby_int_to_ext(const int* e2i, const double* y_int, double* 
     if(n_ext <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_axpby_int_to_ext, dim3(blocks_for(n_ext, 256)), dim3(256), 0, st, e2i, y_int, y_ext, a, b,
                        n_ext);
-    MASX_CHECK_LAUNCH();
-    return hipSuccess;
-}
-
-hipError_t launch_ext_to_r32(const int* e2i, const double* src, float* r, int n_ext, hipStream_t st)
-{
-    if(n_ext <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_ext_to_r32, dim3(blocks_for(n_ext, 256)), dim3(256), 0, st, e2i, src, r, n_ext);
     MASX_CHECK_LAUNCH();
     return hipSuccess;
 }

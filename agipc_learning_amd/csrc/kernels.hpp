@@ -4,31 +4,33 @@
 
 namespace masx
 {
+// bytes of one bank inverse in the diagonal-major layout of `precision`
+size_t mas_bank_bytes(int bank, int precision);
+bool   spmv_tiling_supported(int lanes_per_row, int passes);
+
 hipError_t launch_pack_values(const DevData& d, const int* dest1, const int* dest2, long long nb, hipStream_t st);
 hipError_t launch_gal_gather(const double* fine, double* E, const int* ptr, const int* src, int e0, int e1,
                              int src_is_fine, hipStream_t st);
 hipError_t launch_mas_factor(const DevData& d, const double* E, const int* ent_ptr, const int* ent_pos,
-                             const int* ent_src, void* out, int precision, int* status, hipStream_t st);
+                             const int* ent_src, void* out, float* scale, int precision, int* status, hipStream_t st);
 hipError_t launch_abd_factor(const DevData& d, const int* body_src, float* inv, int n_bodies, hipStream_t st);
 hipError_t launch_diag_factor(const DevData& d, const int* diag_src, const int* needs, float* inv, hipStream_t st);
 hipError_t launch_load_rhs(const DevData& d, const int* e2i, const double* b, int n_ext, hipStream_t st);
 hipError_t launch_store_x(const DevData& d, const int* e2i, double* x, int n_ext, hipStream_t st);
-hipError_t launch_init_state(const DevData& d, int rule, double tol, const int* max_iter, int check_every,
+hipError_t launch_init_state(const DevData& d, int rule, double tol, const int* max_iter, int check_every, int frozen,
                              hipStream_t st);
+hipError_t launch_phase_finish(const DevData& d, hipStream_t st);
 hipError_t launch_mas_local(const DevData& d, int precision, int force, hipStream_t st);
 hipError_t launch_precond_finish(const DevData& d, int n_wg, int force, hipStream_t st);
-hipError_t launch_spmv_fused(const DevData& d, int n_wg, int lanes_per_row, const float* p_old, float* p_new,
-                             hipStream_t st);
+hipError_t launch_spmv_fused(const DevData& d, int n_wg, int lanes_per_row, int passes, const float* p_old,
+                             float* p_new, hipStream_t st);
 hipError_t launch_update(const DevData& d, int n_wg, const float* p, int mode, hipStream_t st);
+// true residual of the checked (or, report_only, all) systems and its decision
 hipError_t launch_true_residual(const DevData& d, int n_wg, int report_only, hipStream_t st);
 hipError_t launch_spmv64(const DevData& d, const double* x, double* y, hipStream_t st);
 hipError_t launch_gather_ext64(const int* e2i, const double* src, double* dst, int n_ext, hipStream_t st);
 hipError_t launch_axpby_int_to_ext(const int* e2i, const double* y_int, double* y_ext, double a, double b, int n_ext,
                                    hipStream_t st);
-hipError_t launch_ext_to_r32(const int* e2i, const double* src, float* r, int n_ext, hipStream_t st);
 hipError_t launch_z32_to_ext(const int* e2i, const float* z, double* dst, int n_ext, hipStream_t st);
-}  // namespace masx
-namespace masx
-{
 hipError_t launch_gather_blocks(const double* src, const uint32_t* idx, double* dst, long long n, hipStream_t st);
-}
+}  // namespace masx

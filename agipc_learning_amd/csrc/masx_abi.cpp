@@ -185,6 +185,14 @@ int masx_set_contacts(masx_solver* s, int sys, const int* pairs, int n_pairs, in
     });
 }
 
+int masx_analyze(masx_solver* s)
+{
+    return guard([&] {
+        require(s != nullptr, "null solver");
+        s->impl.analyze();
+    });
+}
+
 int masx_setup(masx_solver* s)
 {
     return guard([&] {
@@ -287,7 +295,7 @@ int masx_time_kernel(masx_solver* s, int which, int reps, double* ms)
 int masx_plan_info_get(masx_solver* s, masx_plan_info* info)
 {
     return guard([&] {
-        require(s && info && s->impl.is_setup(), "solver not set up");
+        require(s && info && s->impl.is_analyzed(), "solver not analyzed");
         const auto& P       = s->impl.plan();
         info->n_rows        = P.n_rows;
         info->n_ext         = P.n_ext;
@@ -300,7 +308,7 @@ int masx_plan_info_get(masx_solver* s, masx_plan_info* info)
         info->n_coarse      = P.n_coarse;
         info->n_bodies      = P.n_bodies;
         info->n_gal         = P.n_gal;
-        info->inverse_bytes = (long long)s->impl.inverse_bytes();
+        info->inverse_bytes = s->impl.is_setup() ? (long long)s->impl.inverse_bytes() : 0;
     });
 }
 
@@ -308,7 +316,7 @@ int masx_get_hierarchy(masx_solver* s, int sys, int* levels, int* total, int* le
                        uint32_t* fine_mask, int* prefix, int* going_next, int* coarse_table)
 {
     return guard([&] {
-        require(s && s->impl.is_setup() && sys >= 0 && sys < s->impl.n_systems(), "bad system index");
+        require(s && s->impl.is_analyzed() && sys >= 0 && sys < s->impl.n_systems(), "bad system index");
         const auto& h = s->impl.system(sys).hier;
         require(s->impl.system(sys).use_mas, "system has no MAS hierarchy");
         if(levels) *levels = h.levels;
@@ -535,8 +543,8 @@ int masx_mas_set_preconditioner_bcoo(masx_mas* m, const double* d_values, const 
                     throw std::runtime_error("allocation failed");
                 m->d_sel_n = triplet_num;
             }
-            if(hipMemcpyAsync(m->d_sel, idx.data(), sizeof(uint32_t) * triplet_num, hipMemcpyHostToDevice, st)
-               != hipSuccess)
+            // blocking copy: the gather below runs on the library's stream
+            if(hipMemcpy(m->d_sel, idx.data(), sizeof(uint32_t) * triplet_num, hipMemcpyHostToDevice) != hipSuccess)
                 throw std::runtime_error("index upload failed");
             m->solver->impl.gather_values(0, d_values, m->d_sel, st);
         }

@@ -503,6 +503,70 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
         }
     }
     for(int b = 0; b < P.n_banks; ++b) P.comp_ptr[b + 1] += P.comp_ptr[b];
+    // level of each bank, level-1 descendants of deeper nodes, ancestors of
+    // level-1 nodes, level-1 parent of each fine row
+    P.bank_kind.assign(P.n_banks, 0);
+    P.anc.assign(8 * (size_t)std::max(n_coarse, 1), -1);
+    P.row_parent1.assign(P.n_rows, -1);
+    std::vector<std::vector<int>> desc(n_coarse);
+    for(int s = 0; s < S; ++s)
+    {
+        auto& d = sys[s];
+        if(!d.use_mas) continue;
+        const auto& h = d.hier;
+        for(int l = 1; l < h.levels; ++l)
+        {
+            const int b0 = gbank_of_slot(s, h.level_begin[l]);
+            const int nb = pad_to(h.level_count[l], bank) / bank;
+            for(int b = b0; b < b0 + nb; ++b) P.bank_kind[b] = l == 1 ? 1 : 2;
+        }
+        for(int c = 0; h.levels > 1 && c < h.level_count[1]; ++c)
+        {
+            const int slot1 = h.level_begin[1] + c;
+            const int g1    = gcoarse(s, slot1);
+            int       cur   = slot1, k = 0;
+            const int top_begin = h.level_begin[h.levels - 1];
+            while(cur >= 0 && k < 8)
+            {
+                P.anc[8 * (size_t)g1 + k++] = gcoarse(s, cur);
+                if(cur != slot1) desc[gcoarse(s, cur)].push_back(g1);
+                if(cur >= top_begin) break;
+                cur = h.going_next[cur];
+            }
+        }
+        for(int slot = 0; slot < h.n_slots; ++slot)
+        {
+            const int v = d.part_to_real[slot];
+            if(v >= 0 && h.levels > 1) P.row_parent1[P.fem_row0[s] + slot] = gcoarse(s, h.going_next[v]);
+        }
+    }
+    P.desc_ptr.assign(n_coarse + 1, 0);
+    for(int c = 0; c < n_coarse; ++c) P.desc_ptr[c + 1] = P.desc_ptr[c] + (int)desc[c].size();
+    P.desc_list.resize(P.desc_ptr[n_coarse]);
+    for(int c = 0; c < n_coarse; ++c) std::copy(desc[c].begin(), desc[c].end(), P.desc_list.begin() + P.desc_ptr[c]);
+    // self-check: every level-1 node lists itself first and appears exactly
+    // once among the descendants of each deeper level
+    for(int s = 0; s < S; ++s)
+    {
+        const auto& d = sys[s];
+        if(!d.use_mas || d.hier.levels < 2) continue;
+        const auto& h = d.hier;
+        for(int c = 0; c < h.level_count[1]; ++c)
+        {
+            const int g1 = gcoarse(s, h.level_begin[1] + c);
+            if(P.anc[8 * (size_t)g1] != g1) throw std::logic_error("ancestor list of a level-1 node is wrong");
+        }
+        for(int l = 2; l < h.levels; ++l)
+        {
+            long long n = 0;
+            for(int c = 0; c < h.level_count[l]; ++c)
+            {
+                const int g = gcoarse(s, h.level_begin[l] + c);
+                n += P.desc_ptr[g + 1] - P.desc_ptr[g];
+            }
+            if(n != h.level_count[1]) throw std::logic_error("descendant lists do not cover level 1");
+        }
+    }
     P.comp_mask.resize(P.comp_ptr[P.n_banks]);
     P.comp_parent.resize(P.comp_ptr[P.n_banks]);
     P.sys_level_ptr.assign(S + 1, 0);
@@ -642,6 +706,71 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
             P.bank_ent_pos[P.bank_ent_ptr[b] + k] = be[b][k].first;
             P.bank_ent_src[P.bank_ent_ptr[b] + k] = be[b][k].second;
         }
+}
+
+
+void validate_plan(const Plan& P)
+{
+    auto fail = [](const char* what) { throw std::logic_error(std::string("plan check failed: ") + what); };
+    const long long nb = P.nb_total;
+    const int nblk = P.bank * (P.bank + 1) / 2;
+    if((int)P.row_ptr.size() != P.n_rows + 1 || P.row_ptr[0] != 0 || P.row_ptr.back() != P.nnz) fail("row_ptr");
+    for(int r = 0; r < P.n_rows; ++r)
+        if(P.row_ptr[r + 1] < P.row_ptr[r]) fail("row_ptr order");
+    for(long long e = 0; e < P.nnz; ++e)
+    {
+        if(P.col[e] < 0 || P.col[e] >= P.n_rows) fail("col");
+        if((P.entry_src[e] & 0x7fffffff) >= nb) fail("entry_src");
+    }
+    for(long long g = 0; g < nb; ++g)
+    {
+        if(P.dest1[g] < 0 || P.dest1[g] >= P.nnz) fail("dest1");
+        if(P.dest2[g] < -1 || P.dest2[g] >= P.nnz) fail("dest2");
+    }
+    for(int v : P.ext_to_int)
+        if(v < 0 || v >= P.n_rows) fail("ext_to_int");
+    if((int)P.bank_vec.size() != P.n_banks || (int)P.bank_kind.size() != P.n_banks) fail("bank arrays");
+    for(int b = 0; b < P.n_banks; ++b)
+    {
+        const bool fine = b < P.n_fine_banks;
+        if(fine != (P.bank_kind[b] == 0)) fail("bank_kind");
+        if(P.bank_vec[b] < 0 || P.bank_vec[b] + P.bank > (fine ? P.n_rows : P.n_coarse)) fail("bank_vec");
+        if(P.bank_sys[b] < 0 || P.bank_sys[b] >= P.n_sys) fail("bank_sys");
+        if(P.comp_ptr[b + 1] < P.comp_ptr[b]) fail("comp_ptr");
+        for(int k = P.comp_ptr[b]; k < P.comp_ptr[b + 1]; ++k)
+            if(P.comp_parent[k] < 0 || P.comp_parent[k] >= P.n_coarse) fail("comp_parent");
+        if(P.bank_ent_ptr[b + 1] < P.bank_ent_ptr[b]) fail("bank_ent_ptr");
+        for(int k = P.bank_ent_ptr[b]; k < P.bank_ent_ptr[b + 1]; ++k)
+        {
+            if((P.bank_ent_pos[k] & 0xffff) >= nblk) fail("bank_ent_pos");
+            if(P.bank_ent_src[k] < 0 || P.bank_ent_src[k] >= (fine ? nb : P.n_gal)) fail("bank_ent_src");
+        }
+    }
+    if((int)P.desc_ptr.size() != P.n_coarse + 1) fail("desc_ptr");
+    for(int c = 0; c < P.n_coarse; ++c)
+        if(P.desc_ptr[c + 1] < P.desc_ptr[c]) fail("desc_ptr order");
+    for(int v : P.desc_list)
+        if(v < 0 || v >= P.n_coarse) fail("desc_list");
+    for(int v : P.anc)
+        if(v < -1 || v >= P.n_coarse) fail("anc");
+    if((int)P.row_parent1.size() != P.n_rows) fail("row_parent1");
+    for(int v : P.row_parent1)
+        if(v < -1 || v >= P.n_coarse) fail("row_parent1 range");
+    for(size_t g = 0; g + 1 < P.gal_level_ptr.size(); ++g)
+        for(int e = P.gal_level_ptr[g]; e < P.gal_level_ptr[g + 1]; ++e)
+            for(int k = P.gal_ptr[e]; k < P.gal_ptr[e + 1]; ++k)
+            {
+                const int idx = P.gal_src[k] >> 2;
+                if(idx < 0 || idx >= (P.gal_src_is_fine[g] ? nb : P.gal_level_ptr[g])) fail("gal_src");
+            }
+    for(int b = 0; b < P.n_bodies; ++b)
+    {
+        if(P.body_row[b] < 0 || P.body_row[b] + 4 > P.n_rows) fail("body_row");
+        for(int k = 0; k < 16; ++k)
+            if(P.body_src[16 * b + k] != -1 && (P.body_src[16 * b + k] & 0x7fffffff) >= nb) fail("body_src");
+    }
+    for(int v : P.diag_src)
+        if(v < -1 || v >= nb) fail("diag_src");
 }
 
 }  // namespace masx

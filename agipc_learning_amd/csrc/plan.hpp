@@ -102,6 +102,11 @@ struct Plan
     std::vector<uint32_t> comp_mask;
     std::vector<int> comp_parent;    // global coarse index
     std::vector<int> coarse_parent;  // per global coarse node: parent global coarse index or -1
+    std::vector<int> bank_kind;      // 0 level 0 (R from r), 1 level 1 (R in Rc), 2 level >= 2 (R from level-1 sums)
+    std::vector<int> desc_ptr;       // per global coarse node (level >= 2): range of its level-1 descendants
+    std::vector<int> desc_list;
+    std::vector<int> row_parent1;    // per internal row: level-1 global coarse index, -1 if none
+    std::vector<int> anc;            // per global coarse node, 8 ints: itself and its ancestors, -1 padded
     // restriction schedule for levels >= 2 (per system, coarse banks grouped by level)
     std::vector<int> sys_level_banks;   // flattened: per system, per level l>=1, [bank begin, end)
     std::vector<int> sys_level_ptr;     // n_sys+1 offsets into sys_level_banks (pairs)
@@ -127,5 +132,7 @@ struct Plan
 };
 
 void build_plan(Plan& plan, std::vector<SystemDesc>& systems);
+// Bounds of every index array the kernels dereference; throws on violation.
+void validate_plan(const Plan& plan);
 
 }  // namespace masx

@@ -45,11 +45,21 @@ template struct DeviceArray<uint32_t>;
 template struct DeviceArray<int4>;
 template struct DeviceArray<SysState>;
 
-Solver::Solver()
+Solver::Solver() = default;
+
+void Solver::ensure_stream()
 {
+    if(work_) return;
     MASX_HIP(hipStreamCreateWithFlags(&work_, hipStreamNonBlocking));
     MASX_HIP(hipEventCreate(&ev_a_));
     MASX_HIP(hipEventCreate(&ev_b_));
+}
+
+void Solver::analyze()
+{
+    build_plan(plan_, systems_);
+    validate_plan(plan_);
+    analyzed_ = true;
 }
 
 Solver::~Solver()
@@ -61,10 +71,31 @@ Solver::~Solver()
     if(work_) (void)hipStreamDestroy(work_);
 }
 
+hipStream_t Solver::enter(hipStream_t user)
+{
+    // with no stream given, order after the legacy default stream (the
+    // library's stream is non-blocking and would not otherwise wait for it)
+    MASX_HIP(hipEventRecord(ev_a_, user));
+    MASX_HIP(hipStreamWaitEvent(work_, ev_a_, 0));
+    return work_;
+}
+
+void Solver::leave(hipStream_t user)
+{
+    if(user)
+    {
+        MASX_HIP(hipEventRecord(ev_b_, work_));
+        MASX_HIP(hipStreamWaitEvent(user, ev_b_, 0));
+    }
+    else
+        MASX_HIP(hipStreamSynchronize(work_));
+}
+
 int Solver::add_system(const SystemDesc& d)
 {
     systems_.push_back(d);
     setup_done_ = false;
+    analyzed_   = false;
     built_      = false;
     return (int)systems_.size() - 1;
 }
@@ -75,7 +106,10 @@ void Solver::set_spmv_tiling(int lanes_per_row, int rows_per_wg)
         throw std::invalid_argument("lanes per row must be 4, 8, 16 or 32");
     if(rows_per_wg < 256 / lanes_per_row || rows_per_wg % (256 / lanes_per_row))
         throw std::invalid_argument("rows per workgroup must be a multiple of 256 / lanes per row");
+    const int passes = rows_per_wg / (256 / lanes_per_row);
+    if(!spmv_tiling_supported(lanes_per_row, passes)) throw std::invalid_argument("unsupported SpMV tiling");
     lanes_       = lanes_per_row;
+    passes_      = passes;
     rows_per_wg_ = rows_per_wg;
     if(setup_done_)
     {
@@ -88,7 +122,8 @@ void Solver::set_spmv_tiling(int lanes_per_row, int rows_per_wg)
 
 void Solver::setup()
 {
-    build_plan(plan_, systems_);
+    if(!analyzed_) analyze();
+    ensure_stream();
     const Plan& P = plan_;
     hipStream_t st = work_;
     d_row_ptr_.upload(P.row_ptr, st);
@@ -113,12 +148,15 @@ void Solver::setup()
     MASX_HIP(hipMemsetAsync(d_Zc_.ptr, 0, d_Zc_.n * sizeof(float), st));
     d_bank_vec_.upload(P.bank_vec, st);
     d_bank_sys_.upload(P.bank_sys, st);
+    d_bank_kind_.upload(P.bank_kind, st);
     d_comp_ptr_.upload(P.comp_ptr, st);
     d_comp_mask_.upload(P.comp_mask, st);
     d_comp_parent_.upload(P.comp_parent, st);
-    d_coarse_parent_.upload(P.coarse_parent, st);
-    d_sys_level_ptr_.upload(P.sys_level_ptr, st);
-    d_sys_level_banks_.upload(P.sys_level_banks, st);
+    d_desc_ptr_.upload(P.desc_ptr, st);
+    d_desc_list_.upload(P.desc_list, st);
+    d_row_parent1_.upload(P.row_parent1, st);
+    d_anc_.upload(P.anc, st);
+    d_minv_scale_.alloc(std::max(P.n_banks, 1));
     d_gal_ptr_.upload(P.gal_ptr, st);
     d_gal_src_.upload(P.gal_src, st);
     d_ent_ptr_.upload(P.bank_ent_ptr, st);
@@ -205,7 +243,10 @@ void Solver::build_work_tables()
     n_wg_pre_  = (int)pre.size();
     n_wg_spmv_ = (int)spmv.size();
     n_wg_upd_  = (int)upd.size();
-    d_partial_.alloc(std::max({n_wg_pre_, n_wg_spmv_, n_wg_upd_}) + 1);
+    d_part_pre_.alloc(n_wg_pre_);
+    d_part_spmv_.alloc(n_wg_spmv_);
+    d_part_upd_.alloc(n_wg_upd_);
+    d_part_tr_.alloc(n_wg_spmv_);
     MASX_HIP(hipStreamSynchronize(st));
 }
 
@@ -237,14 +278,17 @@ void Solver::make_devdata()
     d.Rc = d_Rc_.ptr;
     d.Zc = d_Zc_.ptr;
     d.minv = d_minv_.ptr;
+    d.minv_scale = d_minv_scale_.ptr;
     d.bank_vec = d_bank_vec_.ptr;
     d.bank_sys = d_bank_sys_.ptr;
+    d.bank_kind = d_bank_kind_.ptr;
     d.comp_ptr = d_comp_ptr_.ptr;
     d.comp_mask = d_comp_mask_.ptr;
     d.comp_parent = d_comp_parent_.ptr;
-    d.coarse_parent = d_coarse_parent_.ptr;
-    d.sys_level_ptr = d_sys_level_ptr_.ptr;
-    d.sys_level_banks = d_sys_level_banks_.ptr;
+    d.desc_ptr = d_desc_ptr_.ptr;
+    d.desc_list = d_desc_list_.ptr;
+    d.row_parent1 = d_row_parent1_.ptr;
+    d.anc = d_anc_.ptr;
     d.abd_inv = d_abd_inv_.ptr;
     d.body_row = d_body_row_.ptr;
     d.diag_inv = d_diag_inv_.ptr;
@@ -255,7 +299,10 @@ void Solver::make_devdata()
     d.wg_pre_range = d_wg_pre_range_.ptr;
     d.wg_spmv_range = d_wg_spmv_range_.ptr;
     d.wg_upd_range = d_wg_upd_range_.ptr;
-    d.partial = d_partial_.ptr;
+    d.part_pre = d_part_pre_.ptr;
+    d.part_spmv = d_part_spmv_.ptr;
+    d.part_upd = d_part_upd_.ptr;
+    d.part_tr = d_part_tr_.ptr;
     d.state = d_state_.ptr;
 }
 
@@ -263,8 +310,10 @@ void Solver::set_values(int s, const double* d_values, hipStream_t st)
 {
     if(!setup_done_) throw std::runtime_error("setup() must precede set_values()");
     const long long nb = systems_.at(s).nb;
+    hipStream_t w = enter(st);
     if(nb) MASX_HIP(hipMemcpyAsync(d_vals64_.ptr + 9 * plan_.src_off[s], d_values, 9 * nb * sizeof(double),
-                                   hipMemcpyDeviceToDevice, st ? st : work_));
+                                   hipMemcpyDeviceToDevice, w));
+    leave(st);
     built_ = false;
 }
 
@@ -272,35 +321,33 @@ void Solver::set_values_host(int s, const double* h_values, hipStream_t st)
 {
     if(!setup_done_) throw std::runtime_error("setup() must precede set_values()");
     const long long nb = systems_.at(s).nb;
+    hipStream_t w = enter(st);
     if(nb) MASX_HIP(hipMemcpyAsync(d_vals64_.ptr + 9 * plan_.src_off[s], h_values, 9 * nb * sizeof(double),
-                                   hipMemcpyHostToDevice, st ? st : work_));
+                                   hipMemcpyHostToDevice, w));
+    MASX_HIP(hipStreamSynchronize(w));  // the host buffer may go away after return
+    leave(st);
     built_ = false;
 }
 
 void Solver::gather_values(int s, const double* d_values, const uint32_t* d_indices, hipStream_t st)
 {
     if(!setup_done_) throw std::runtime_error("setup() must precede set_values()");
-    MASX_HIP(launch_gather_blocks(d_values, d_indices, d_vals64_.ptr + 9 * plan_.src_off[s], systems_.at(s).nb,
-                                  st ? st : work_));
+    hipStream_t w = enter(st);
+    MASX_HIP(launch_gather_blocks(d_values, d_indices, d_vals64_.ptr + 9 * plan_.src_off[s], systems_.at(s).nb, w));
+    leave(st);
     built_ = false;
 }
 
 size_t Solver::inverse_bytes() const
 {
-    const size_t vals = (size_t)plan_.n_banks * plan_.bank * (plan_.bank + 1) / 2 * 9;
-    return vals * (precision_ == kPrecFp32 ? 4 : 2);
+    return (size_t)plan_.n_banks * mas_bank_bytes(plan_.bank, precision_);
 }
 
 void Solver::build(int precision, hipStream_t user)
 {
     if(!setup_done_) throw std::runtime_error("setup() must precede build()");
     if(precision < kPrecFp32 || precision > kPrecBf16) throw std::invalid_argument("unknown precision");
-    hipStream_t st = work_;
-    if(user)
-    {
-        MASX_HIP(hipEventRecord(ev_a_, user));
-        MASX_HIP(hipStreamWaitEvent(st, ev_a_, 0));
-    }
+    hipStream_t st = enter(user);
     if(precision != precision_ || !d_minv_.ptr)
     {
         precision_ = precision;
@@ -315,22 +362,20 @@ void Solver::build(int precision, hipStream_t user)
         MASX_HIP(launch_gal_gather(d_vals64_.ptr, d_E_.ptr, d_gal_ptr_.ptr, d_gal_src_.ptr, P.gal_level_ptr[g],
                                    P.gal_level_ptr[g + 1], P.gal_src_is_fine[g], st));
     MASX_HIP(hipMemsetAsync(d_factor_status_.ptr, 0, sizeof(int), st));
-    MASX_HIP(launch_mas_factor(dd_, d_E_.ptr, d_ent_ptr_.ptr, d_ent_pos_.ptr, d_ent_src_.ptr, d_minv_.ptr, precision_,
-                               d_factor_status_.ptr, st));
+    MASX_HIP(launch_mas_factor(dd_, d_E_.ptr, d_ent_ptr_.ptr, d_ent_pos_.ptr, d_ent_src_.ptr, d_minv_.ptr,
+                               d_minv_scale_.ptr, precision_, d_factor_status_.ptr, st));
     MASX_HIP(launch_abd_factor(dd_, d_body_src_.ptr, d_abd_inv_.ptr, P.n_bodies, st));
     MASX_HIP(launch_diag_factor(dd_, d_diag_src_.ptr, d_needs_diag_.ptr, d_diag_inv_.ptr, st));
-    if(user)
-    {
-        MASX_HIP(hipEventRecord(ev_b_, st));
-        MASX_HIP(hipStreamWaitEvent(user, ev_b_, 0));
-    }
+    leave(user);
     built_ = true;
 }
 
 long long Solver::factor_status() const
 {
     int v = 0;
-    MASX_HIP(hipMemcpy(&v, d_factor_status_.ptr, sizeof(int), hipMemcpyDeviceToHost));
+    MASX_HIP(hipStreamSynchronize(work_));
+    MASX_HIP(hipMemcpyAsync(&v, d_factor_status_.ptr, sizeof(int), hipMemcpyDeviceToHost, work_));
+    MASX_HIP(hipStreamSynchronize(work_));
     return v;
 }
 
@@ -340,7 +385,7 @@ void Solver::launch_iteration(hipStream_t st, int i)
     float*       p_new = (i & 1) ? d_p0_.ptr : d_p1_.ptr;
     MASX_HIP(launch_mas_local(dd_, precision_, 0, st));
     MASX_HIP(launch_precond_finish(dd_, n_wg_pre_, 0, st));
-    MASX_HIP(launch_spmv_fused(dd_, n_wg_spmv_, lanes_, p_old, p_new, st));
+    MASX_HIP(launch_spmv_fused(dd_, n_wg_spmv_, lanes_, passes_, p_old, p_new, st));
     MASX_HIP(launch_update(dd_, n_wg_upd_, p_new, 0, st));
 }
 
@@ -351,6 +396,7 @@ void Solver::run_chunk(hipStream_t st, int chunk, bool with_check)
     {
         MASX_HIP(launch_true_residual(dd_, n_wg_spmv_, 0, st));
         MASX_HIP(launch_update(dd_, n_wg_upd_, d_p0_.ptr, 1, st));
+        MASX_HIP(launch_phase_finish(dd_, st));
     }
 }
 
@@ -359,12 +405,7 @@ void Solver::solve(const double* d_b, double* d_x, const SolveParams& prm, std::
 {
     if(!built_) throw std::runtime_error("build() must precede solve()");
     const Plan& P = plan_;
-    hipStream_t st = work_;
-    if(user)
-    {
-        MASX_HIP(hipEventRecord(ev_a_, user));
-        MASX_HIP(hipStreamWaitEvent(st, ev_a_, 0));
-    }
+    hipStream_t st = enter(user);
     const int chunk = std::max(2, prm.chunk + (prm.chunk & 1));
     std::vector<int> max_iter(P.n_sys);
     for(int s = 0; s < P.n_sys; ++s)
@@ -374,9 +415,10 @@ void Solver::solve(const double* d_b, double* d_x, const SolveParams& prm, std::
     for(auto* a : {&d_x_, &d_r_, &d_z_, &d_p0_, &d_p1_, &d_Ap_, &d_rt_}) MASX_HIP(hipMemsetAsync(a->ptr, 0, nvb, st));
     MASX_HIP(hipMemsetAsync(d_b64_.ptr, 0, 3 * (size_t)P.n_rows * sizeof(double), st));
     MASX_HIP(launch_init_state(dd_, prm.rule, prm.tol, d_max_iter_.ptr, prm.rule == kRuleTrueRel ? prm.check_every : 0,
-                               st));
+                               0, st));
     MASX_HIP(launch_load_rhs(dd_, d_e2i_.ptr, d_b, P.n_ext, st));
     MASX_HIP(launch_update(dd_, n_wg_upd_, d_p0_.ptr, 1, st));
+    MASX_HIP(launch_phase_finish(dd_, st));
     const bool with_check = prm.rule == kRuleTrueRel;
     if(prm.use_graph && (!graph_exec_ || graph_chunk_ != chunk || graph_check_ != with_check))
     {
@@ -424,97 +466,120 @@ void Solver::solve(const double* d_b, double* d_x, const SolveParams& prm, std::
             o.restarts = h.restarts;
             o.true_checks = h.n_true_checks;
             o.rz0 = h.rz0;
-            o.rz = h.rz;
+            o.rz = h.rz_cur;
             o.rel_res_recursive = h.bb > 0 ? std::sqrt(h.rr / h.bb) : 0.0;
             o.rel_res_true = h.bb > 0 ? std::sqrt(h.true_rr / h.bb) : std::sqrt(h.true_rr);
         }
     }
-    if(user)
-    {
-        MASX_HIP(hipEventRecord(ev_b_, st));
-        MASX_HIP(hipStreamWaitEvent(user, ev_b_, 0));
-    }
+    leave(user);
 }
 
 void Solver::spmv(double a, const double* d_x, double b, double* d_y, hipStream_t user)
 {
     if(!setup_done_) throw std::runtime_error("setup() must precede spmv()");
-    hipStream_t st = user ? user : work_;
+    hipStream_t st = enter(user);
     MASX_HIP(hipMemsetAsync(d_tmp64a_.ptr, 0, 3 * (size_t)plan_.n_rows * sizeof(double), st));
     MASX_HIP(launch_gather_ext64(d_e2i_.ptr, d_x, d_tmp64a_.ptr, plan_.n_ext, st));
     MASX_HIP(launch_spmv64(dd_, d_tmp64a_.ptr, d_tmp64b_.ptr, st));
     MASX_HIP(launch_axpby_int_to_ext(d_e2i_.ptr, d_tmp64b_.ptr, d_y, a, b, plan_.n_ext, st));
-    if(!user) MASX_HIP(hipStreamSynchronize(st));
+    leave(user);
 }
 
 void Solver::apply_preconditioner(const double* d_r, double* d_z, hipStream_t user)
 {
     if(!built_) throw std::runtime_error("build() must precede apply_preconditioner()");
-    hipStream_t st = user ? user : work_;
+    hipStream_t st = enter(user);
     const size_t nvb = 3 * (size_t)plan_.n_rows * sizeof(float);
     MASX_HIP(hipMemsetAsync(d_rt_.ptr, 0, nvb, st));
     MASX_HIP(hipMemsetAsync(d_b64_.ptr, 0, 3 * (size_t)plan_.n_rows * sizeof(double), st));
     std::vector<int> ones(plan_.n_sys, 1);
     MASX_HIP(hipMemcpyAsync(d_max_iter_.ptr, ones.data(), sizeof(int) * plan_.n_sys, hipMemcpyHostToDevice, st));
-    MASX_HIP(launch_init_state(dd_, kRuleTrueRel, 0.0, d_max_iter_.ptr, 0, st));
-    MASX_HIP(launch_ext_to_r32(d_e2i_.ptr, d_r, d_rt_.ptr, plan_.n_ext, st));
+    MASX_HIP(launch_init_state(dd_, kRuleTrueRel, 0.0, d_max_iter_.ptr, 0, 1, st));
+    MASX_HIP(launch_load_rhs(dd_, d_e2i_.ptr, d_r, plan_.n_ext, st));
     MASX_HIP(launch_update(dd_, n_wg_upd_, d_p0_.ptr, 1, st));
+    MASX_HIP(launch_phase_finish(dd_, st));
     MASX_HIP(launch_mas_local(dd_, precision_, 1, st));
     MASX_HIP(launch_precond_finish(dd_, n_wg_pre_, 1, st));
     MASX_HIP(launch_z32_to_ext(d_e2i_.ptr, d_z_.ptr, d_z, plan_.n_ext, st));
-    MASX_HIP(hipStreamSynchronize(st));
+    leave(user);
 }
 
 void Solver::download_inverse(std::vector<float>& out)
 {
-    const size_t vals = (size_t)plan_.n_banks * plan_.bank * (plan_.bank + 1) / 2 * 9;
-    out.resize(vals);
-    if(precision_ == kPrecFp32)
-    {
-        MASX_HIP(hipMemcpy(out.data(), d_minv_.ptr, vals * sizeof(float), hipMemcpyDeviceToHost));
-        return;
-    }
-    std::vector<uint16_t> h(vals);
-    MASX_HIP(hipMemcpy(h.data(), d_minv_.ptr, vals * sizeof(uint16_t), hipMemcpyDeviceToHost));
-    for(size_t i = 0; i < vals; ++i)
-    {
-        const uint16_t v = h[i];
+    // to packed upper blocks (block row-major), unscaled
+    const int    B    = plan_.bank;
+    const size_t nblk = (size_t)B * (B + 1) / 2;
+    out.assign((size_t)plan_.n_banks * nblk * 9, 0.f);
+    MASX_HIP(hipStreamSynchronize(work_));
+    std::vector<float> scale(plan_.n_banks, 1.f);
+    if(plan_.n_banks)
+        MASX_HIP(hipMemcpy(scale.data(), d_minv_scale_.ptr, sizeof(float) * plan_.n_banks, hipMemcpyDeviceToHost));
+    const size_t bank_bytes = mas_bank_bytes(B, precision_);
+    std::vector<uint32_t> raw((size_t)plan_.n_banks * bank_bytes / 4);
+    if(!raw.empty()) MASX_HIP(hipMemcpy(raw.data(), d_minv_.ptr, raw.size() * 4, hipMemcpyDeviceToHost));
+    auto half_to_float = [&](uint32_t v16) {
         float f;
         if(precision_ == kPrecBf16)
         {
-            const uint32_t u = (uint32_t)v << 16;
+            const uint32_t u = v16 << 16;
             std::memcpy(&f, &u, 4);
+            return f;
         }
-        else
+        const uint32_t sgn = (v16 >> 15) & 1u, ex = (v16 >> 10) & 0x1fu, man = v16 & 0x3ffu;
+        float m;
+        if(ex == 0) m = std::ldexp((float)man, -24);
+        else if(ex == 31) m = man ? NAN : INFINITY;
+        else m = std::ldexp((float)(man | 0x400u), (int)ex - 25);
+        return sgn ? -m : m;
+    };
+    for(int b = 0; b < plan_.n_banks; ++b)
+    {
+        const uint32_t* bw = raw.data() + (size_t)b * bank_bytes / 4;
+        float* dst = out.data() + (size_t)b * nblk * 9;
+        for(int dd = 0; dd < B; ++dd)
         {
-            const uint32_t sgn = (v >> 15) & 1u, ex = (v >> 10) & 0x1fu, man = v & 0x3ffu;
-            float m;
-            if(ex == 0) m = std::ldexp((float)man, -24);
-            else if(ex == 31) m = man ? NAN : INFINITY;
-            else m = std::ldexp((float)(man | 0x400u), (int)ex - 25);
-            f = sgn ? -m : m;
+            const int len = B - dd, off = dd * B - dd * (dd - 1) / 2;
+            for(int I = 0; I < len; ++I)
+            {
+                const int blk = B * I - I * (I + 1) / 2 + I + dd;  // packed index of (I, I+dd)
+                for(int c = 0; c < 9; ++c)
+                {
+                    float v;
+                    if(precision_ == kPrecFp32)
+                    {
+                        std::memcpy(&v, &bw[9 * off + c * len + I], 4);
+                    }
+                    else
+                    {
+                        const uint32_t w = bw[5 * off + (c / 2) * len + I];
+                        v = half_to_float(c % 2 ? (w >> 16) : (w & 0xffffu)) * scale[b];
+                    }
+                    dst[9 * (size_t)blk + c] = v;
+                }
+            }
         }
-        out[i] = f;
     }
 }
 
 void Solver::download_coarse_entries(std::vector<double>& out)
 {
     out.resize(9 * (size_t)plan_.n_gal);
+    MASX_HIP(hipStreamSynchronize(work_));
     if(plan_.n_gal) MASX_HIP(hipMemcpy(out.data(), d_E_.ptr, out.size() * sizeof(double), hipMemcpyDeviceToHost));
 }
 
 double Solver::time_kernel(int which, int reps)
 {
     if(!built_) throw std::runtime_error("build() must precede time_kernel()");
-    hipStream_t st = work_;
+    hipStream_t st = enter(nullptr);
+    MASX_HIP(hipStreamSynchronize(st));
     // put every system in the running state with finite scalars
     std::vector<SysState> h(plan_.n_sys);
     MASX_HIP(hipMemcpy(h.data(), d_state_.ptr, sizeof(SysState) * plan_.n_sys, hipMemcpyDeviceToHost));
     for(auto& s : h)
     {
         s.status = kRunning;
-        s.rz = s.rz0 = 1.0;
+        s.rz0 = 1.0;
         s.beta = 0.5;
         s.alpha = 0.0;
         s.max_iter = 1 << 30;
@@ -522,7 +587,9 @@ double Solver::time_kernel(int which, int reps)
         s.need_check = 0;
         s.check_every = 0;
         s.tol = 0.0;
-        for(auto& c : s.counter) c = 0u;
+        s.beta_zero = 0;
+        s.rz_prev = s.rz_cur = 1.0;
+        s.frozen = 1;
     }
     MASX_HIP(hipMemcpy(d_state_.ptr, h.data(), sizeof(SysState) * plan_.n_sys, hipMemcpyHostToDevice));
     auto launch = [&]() {
@@ -530,7 +597,7 @@ double Solver::time_kernel(int which, int reps)
         {
             case 0: MASX_HIP(launch_mas_local(dd_, precision_, 0, st)); break;
             case 1: MASX_HIP(launch_precond_finish(dd_, n_wg_pre_, 0, st)); break;
-            case 2: MASX_HIP(launch_spmv_fused(dd_, n_wg_spmv_, lanes_, d_p0_.ptr, d_p1_.ptr, st)); break;
+            case 2: MASX_HIP(launch_spmv_fused(dd_, n_wg_spmv_, lanes_, passes_, d_p0_.ptr, d_p1_.ptr, st)); break;
             case 3: MASX_HIP(launch_update(dd_, n_wg_upd_, d_p1_.ptr, 0, st)); break;
             default: throw std::invalid_argument("unknown kernel id");
         }

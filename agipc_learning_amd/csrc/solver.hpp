@@ -63,8 +63,10 @@ class Solver
     int  add_system(const SystemDesc& d);
     SystemDesc& system(int s) { return systems_.at(s); }
     int  n_systems() const { return (int)systems_.size(); }
-    void setup();                      // structure analysis + allocation
+    void analyze();                    // host-only structure analysis (no device needed)
+    void setup();                      // analyze (if needed) + allocation and upload
     bool is_setup() const { return setup_done_; }
+    bool is_analyzed() const { return analyzed_; }
     void set_values(int s, const double* d_values, hipStream_t st);      // device FP64 col-major
     void set_values_host(int s, const double* h_values, hipStream_t st);
     void gather_values(int s, const double* d_values, const uint32_t* d_indices, hipStream_t st);
@@ -90,15 +92,20 @@ class Solver
   private:
     void build_work_tables();
     void make_devdata();
+    hipStream_t enter(hipStream_t user);  // work stream, ordered after `user`
+    void leave(hipStream_t user);         // orders `user` after the work stream, or waits
     void run_chunk(hipStream_t st, int chunk, bool with_check);
     void launch_iteration(hipStream_t st, int i);
 
     std::vector<SystemDesc> systems_;
     Plan plan_;
     bool setup_done_ = false;
+    bool analyzed_   = false;
+    void ensure_stream();
     int  precision_  = kPrecFp32;
     int  lanes_      = 16;
-    int  rows_per_wg_ = 64;
+    int  passes_     = 1;
+    int  rows_per_wg_ = 16;
     bool built_      = false;
 
     // device
@@ -107,15 +114,16 @@ class Solver
     DeviceArray<double>    d_vals64_, d_E_, d_b64_, d_tmp64a_, d_tmp64b_;
     DeviceArray<float>     d_x_, d_r_, d_z_, d_p0_, d_p1_, d_Ap_, d_rt_, d_Rc_, d_Zc_;
     DeviceArray<char>      d_minv_;
-    DeviceArray<int>       d_bank_vec_, d_bank_sys_, d_comp_ptr_, d_comp_parent_, d_coarse_parent_;
+    DeviceArray<float>     d_minv_scale_;
+    DeviceArray<int>       d_bank_vec_, d_bank_sys_, d_bank_kind_, d_comp_ptr_, d_comp_parent_;
     DeviceArray<uint32_t>  d_comp_mask_;
-    DeviceArray<int>       d_sys_level_ptr_, d_sys_level_banks_;
+    DeviceArray<int>       d_desc_ptr_, d_desc_list_, d_row_parent1_, d_anc_;
     DeviceArray<int>       d_gal_ptr_, d_gal_src_, d_ent_ptr_, d_ent_pos_, d_ent_src_;
     DeviceArray<float>     d_abd_inv_, d_diag_inv_;
     DeviceArray<int>       d_body_row_, d_body_src_, d_diag_src_, d_needs_diag_, d_row_mode_;
     DeviceArray<int4>      d_wg_pre_, d_wg_spmv_, d_wg_upd_;
     DeviceArray<int>       d_wg_pre_range_, d_wg_spmv_range_, d_wg_upd_range_;
-    DeviceArray<double>    d_partial_;
+    DeviceArray<double>    d_part_pre_, d_part_spmv_, d_part_upd_, d_part_tr_;
     DeviceArray<SysState>  d_state_;
     DeviceArray<int>       d_max_iter_, d_factor_status_;
     int n_wg_pre_ = 0, n_wg_spmv_ = 0, n_wg_upd_ = 0;

@@ -81,6 +81,7 @@ def load():
         "masx_set_topology": ([vp, C.c_int, up, up, up, C.c_int, ip, C.c_int, ip], C.c_int),
         "masx_set_topology_morton": ([vp, C.c_int, up, up, up, C.c_int, dp], C.c_int),
         "masx_set_contacts": ([vp, C.c_int, ip, C.c_int, C.c_int], C.c_int),
+        "masx_analyze": ([vp], C.c_int),
         "masx_setup": ([vp], C.c_int),
         "masx_set_values": ([vp, C.c_int, vp, vp], C.c_int),
         "masx_set_values_host": ([vp, C.c_int, dp, vp], C.c_int),
@@ -203,7 +204,7 @@ class Solver:
         d = SystemDesc(block_rows, len(rows), n_abd, n_fem, int(use_mas), int(global_diag), bank, levels)
         idx = C.c_int()
         check(self.L.masx_add_system(self.h, C.byref(d), _ptr(rows, C.c_int), _ptr(cols, C.c_int), C.byref(idx)))
-        self.systems.append(dict(block_rows=block_rows, nb=len(rows), n_abd=n_abd, n_fem=n_fem))
+        self.systems.append(dict(block_rows=block_rows, nb=len(rows), n_abd=n_abd, n_fem=n_fem, bank=bank))
         return idx.value
 
     def set_topology(self, sys, nbr_start, nbr_num, nbr_list, part_to_real, real_to_part):
@@ -222,6 +223,9 @@ class Solver:
         p = _i32(pairs).reshape(-1)
         check(self.L.masx_set_contacts(self.h, sys, _ptr(p, C.c_int) if len(p) else None, len(p) // 4,
                                        node_offset))
+
+    def analyze(self):
+        check(self.L.masx_analyze(self.h))
 
     def setup(self):
         check(self.L.masx_setup(self.h))
@@ -275,7 +279,7 @@ class Solver:
         Lv, T, n_slots = lv.value, tot.value, int(lb[1])
         lc = np.zeros(Lv + 1, np.int32)
         fm = np.zeros(d["n_fem"], np.uint32)
-        px = np.zeros(max(n_slots // 32, 1), np.int32)
+        px = np.zeros(max(n_slots // d["bank"], 1), np.int32)
         gn = np.zeros(T, np.int32)
         ct = np.zeros((d["n_fem"], 6), np.int32)
         check(self.L.masx_get_hierarchy(self.h, sys, C.byref(lv), C.byref(tot), _ptr(lc, C.c_int), _ptr(lb, C.c_int),

@@ -7,10 +7,8 @@ from . import native
 from . import workloads as W
 
 
-def solver_for(systems, *, use_mas=True, global_diag=False, bank=32, levels=0, device_values=True):
-    """One Solver holding every system of `systems` (a dict or a list of dicts)."""
-    import torch
-
+def describe(systems, *, use_mas=True, global_diag=False, bank=32, levels=0):
+    """Solver with the structure of every system added (no device work)."""
     if isinstance(systems, dict):
         systems = [systems]
     s = native.Solver()
@@ -23,6 +21,17 @@ def solver_for(systems, *, use_mas=True, global_diag=False, bank=32, levels=0, d
             s.set_topology_morton(i, start, num, lst, w["positions"])
             if len(w.get("pairs", [])):
                 s.set_contacts(i, w["pairs"])
+    return s
+
+
+def solver_for(systems, *, use_mas=True, global_diag=False, bank=32, levels=0, device_values=True):
+    """One Solver holding every system of `systems` (a dict or a list of dicts),
+    set up on the device with its values loaded."""
+    import torch
+
+    if isinstance(systems, dict):
+        systems = [systems]
+    s = describe(systems, use_mas=use_mas, global_diag=global_diag, bank=bank, levels=levels)
     s.setup()
     for i, w in enumerate(systems):
         if device_values:

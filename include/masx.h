@@ -101,8 +101,12 @@ int masx_set_topology_morton(masx_solver* s, int sys, const uint32_t* nbr_start,
  * TraditionalMAS32Preconditioner.cu:1550-1960.  Host array. */
 int masx_set_contacts(masx_solver* s, int sys, const int* pairs, int n_pairs, int node_offset);
 
-/* Structure analysis: hierarchy (ReorderRealtime, TraditionalMAS32Preconditioner.cu
- * :2189-2230), internal slot order, block layout, Galerkin maps; allocation. */
+/* Host-only structure analysis: hierarchy (ReorderRealtime,
+ * TraditionalMAS32Preconditioner.cu:2189-2230), internal slot order, block
+ * layout, Galerkin maps, with a bounds check of every kernel index.  Needs no
+ * device. */
+int masx_analyze(masx_solver* s);
+/* masx_analyze (if not done) + device allocation and upload. */
 int masx_setup(masx_solver* s);
 
 /* Block values of one system (device, FP64 col-major, same order as rows/cols). */
