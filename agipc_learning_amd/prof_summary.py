@@ -1,0 +1,114 @@
+"""Summaries of rocprofv3 CSV output for the solver's kernels.
+
+  python -m agipc_learning_amd.prof_summary stats  <dir>/<name>_kernel_stats.csv  [--cmd "..."]
+  python -m agipc_learning_amd.prof_summary traffic <fetch.csv> <write.csv> <copy_fetch.csv> <copy_write.csv>
+         [--cmd "..."] [--json out.json]
+
+`stats` prints a markdown table of the kernel-trace statistics.  `traffic`
+turns FETCH_SIZE / WRITE_SIZE (KiB per dispatch) into HBM bytes per launch,
+each scaled by what the same counter reports for a plain device copy of known
+size (the copy's true read and write bytes over its counter values), and
+prints a markdown table; `--json` also writes the per-kernel bytes.
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import csv
+import json
+import sys
+
+COPY_BYTES = 256 << 20  # bench.py --copy-calibration copies 256 MiB per launch
+
+
+def _short(name):
+    return name.split("(")[0].replace("void ", "").replace("masx::", "")
+
+
+def stats(path, cmd=None, top=25):
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    out = []
+    if cmd:
+        out.append(f"Command: `{cmd}`\n")
+    out.append("| kernel | calls | total ms | avg us | min us | max us | % |")
+    out.append("|---|---|---|---|---|---|---|")
+    for r in rows[:top]:
+        out.append(f"| {_short(r['Name'])} | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
+                   f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['MinNs']) / 1e3:.2f} | "
+                   f"{float(r['MaxNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
+    return "\n".join(out)
+
+
+def _per_kernel(path, counter):
+    d = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] == counter:
+            d[r["Kernel_Name"]].append(float(r["Counter_Value"]) * 1024.0)
+    return d
+
+
+def _copy_factor(path, counter):
+    d = _per_kernel(path, counter)
+    vals = d.get("__amd_rocclr_copyBuffer", [])
+    big = [v for v in vals if v > COPY_BYTES / 16]
+    if not big:
+        raise SystemExit(f"no copy kernel in {path}")
+    return COPY_BYTES / (sum(big) / len(big))
+
+
+def _busy(vals):
+    """Mean over the dispatches that did the work (the early-exit launches of
+    systems already converged read almost nothing)."""
+    m = max(vals)
+    busy = [v for v in vals if v >= 0.5 * m]
+    return sum(busy) / len(busy), len(busy), len(vals)
+
+
+def traffic(fetch, write, copy_fetch, copy_write, cmd=None, json_out=None):
+    fr = _copy_factor(copy_fetch, "FETCH_SIZE")
+    fw = _copy_factor(copy_write, "WRITE_SIZE")
+    F = _per_kernel(fetch, "FETCH_SIZE")
+    W = _per_kernel(write, "WRITE_SIZE")
+    out = []
+    if cmd:
+        out.append(f"Commands: {cmd}\n")
+    out.append(f"Copy calibration (256 MiB device copy): read factor {fr:.3f}, write factor {fw:.3f}.\n")
+    out.append("| kernel | dispatches (busy/all) | read MB | write MB | HBM MB per launch |")
+    out.append("|---|---|---|---|---|")
+    res = {}
+    for k in sorted(F, key=lambda k: -_busy(F[k])[0]):
+        if k not in W:
+            continue
+        r, nb, na = _busy(F[k])
+        w, _, _ = _busy(W[k])
+        r, w = r * fr, w * fw
+        res[_short(k)] = {"read_bytes": r, "write_bytes": w, "bytes": r + w}
+        out.append(f"| {_short(k)} | {nb}/{na} | {r / 1e6:.2f} | {w / 1e6:.2f} | {(r + w) / 1e6:.2f} |")
+    if json_out:
+        json.dump({"read_factor": fr, "write_factor": fw, "kernels": res}, open(json_out, "w"), indent=1)
+    return "\n".join(out)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="what", required=True)
+    a = sub.add_parser("stats")
+    a.add_argument("csv")
+    a.add_argument("--cmd")
+    b = sub.add_parser("traffic")
+    b.add_argument("fetch")
+    b.add_argument("write")
+    b.add_argument("copy_fetch")
+    b.add_argument("copy_write")
+    b.add_argument("--cmd")
+    b.add_argument("--json")
+    args = ap.parse_args(argv)
+    if args.what == "stats":
+        print(stats(args.csv, args.cmd))
+    else:
+        print(traffic(args.fetch, args.write, args.copy_fetch, args.copy_write, args.cmd, args.json))
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,261 @@
+"""Benchmark of the MAS-PCG Newton-step linear solve on one MI355X.
+
+Workload (default): 40x40x40 hex grid (68,921 vertices, 206,763 DOF, 384,000
+tets), stable Neo-Hookean Hessian with lognormal E (median 1e5 Pa, sigma 0.5),
+20,000 point-triangle contact pairs, 32-vertex Morton subdomains, 4 MAS
+levels, tolerance 1e-4 on the FP64 true relative residual.  Synthetic data,
+seed 0, generated in-process.
+
+One step = one Newton step's linear solve with fixed structure: the new
+matrix values (already in HBM, caller layout) are taken in, the MAS
+preconditioner is rebuilt from them (FP64 Galerkin + Cholesky, inverse stored
+in the chosen precision) and PCG runs to the tolerance.  `value` is the plain
+FP32 step time; the mixed-precision (FP16 / BF16 inverse storage) times and
+their speed-up are reported beside it.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=40, help="hex cells per edge")
+    ap.add_argument("--contacts", type=int, default=20000)
+    ap.add_argument("--levels", type=int, default=4)
+    ap.add_argument("--tol", type=float, default=1e-4)
+    ap.add_argument("--kernel-reps", type=int, default=200)
+    ap.add_argument("--precisions", default="fp32,fp16,bf16",
+                    help="comma list; fp32 is always measured (it is the headline value)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy (HBM counter calibration) and exit")
+    return ap.parse_args()
+
+
+def algorithmic_bytes(info, n_vertices, prec_bytes):
+    """Bytes one launch of each iteration kernel must move at minimum.
+
+    spmv: nb*(36+4) values+indices of the upper blocks, (n+1)*4 row pointers,
+          12 B/row x and 12 B/row y, plus the fused p = z + beta p update
+          (z and old p read instead of x: +12 B/row, p written: +12 B/row).
+    mas_local: every bank inverse (528 blocks x 9 x s) and the bank's R read
+          and Z written (32 slots x 12 B each way).
+    update: x, r, p, Ap read, x and r written (6 x 12 B/row); level-1
+          restriction output is folded into the bank term.
+    precond_finish: r and z read, z written (3 x 12 B/row)."""
+    n = n_vertices
+    nb = info["nb_total"]
+    banks = info["n_banks"]
+    bank = info["bank"]
+    nblk = bank * (bank + 1) // 2
+    return {
+        "spmv": nb * 40 + (n + 1) * 4 + n * 24 + n * 24,
+        "mas_local": banks * nblk * 9 * prec_bytes + banks * bank * 12 * 2,
+        "update": n * 12 * 6,
+        "precond_finish": n * 12 * 3,
+    }
+
+
+KERNEL_PREFIX = {"spmv": "k_spmv_fused", "mas_local": "k_mas_local<32, float>", "update": "k_update",
+                 "precond_finish": "k_precond_finish"}
+
+
+def measured_traffic(kernel, workload_is_default):
+    """HBM bytes per launch of `kernel` from the committed counter summary of
+    this workload (FETCH_SIZE + WRITE_SIZE runs, copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "fem40c_hbm_traffic.json")
+    if not workload_is_default or not os.path.exists(path):
+        return None
+    data = json.load(open(path))["kernels"]
+    for name, v in data.items():
+        if name.startswith(KERNEL_PREFIX[kernel]):
+            return int(v["bytes"])
+    return None
+
+
+def copy_calibration():
+    import torch
+
+    n = 256 << 20
+    a = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": 5}))
+
+
+def cpu_baseline(w, levels, tol):
+    """The FP64 oracle (a scalar C restatement of the reference algorithm, one
+    thread) on one step of the same workload: MAS assemble + Cholesky inverse +
+    PCG to the tolerance.  The hierarchy is built before the timed region, as
+    the GPU step reuses its structure analysis too."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+
+    O.build()
+    nv = w["n_vertices"]
+    p2r, r2p = O.morton_partition(w["positions"])
+    start, num, lst = O.mesh_neighbors(nv, w["tets"])
+    mas = O.Mas(nv, len(p2r), start, num, lst, p2r, r2p, levels=levels)
+    mas.reorder(w["pairs"])
+    t0 = time.perf_counter()
+    mas.assemble(w["vals"], w["rows"], w["cols"])
+    mas.invert(1)
+    _, so = O.pcg(w["block_rows"], w["vals"], w["rows"], w["cols"], w["rhs"], mas=mas, tol=tol, rule=1)
+    dt = time.perf_counter() - t0
+    return {"value": dt * 1e3, "unit": "ms", "cores": 1, "kind": "port",
+            "sample": f"one full step of the same workload (MAS build + PCG, {so['iterations']} iterations, "
+                      f"true rel res {so['true_rel_res']:.2e}), FP64, single thread"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path. "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
+
+    import numpy as np
+    import torch
+    from agipc_learning_amd import session
+    from agipc_learning_amd import workloads as W
+
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py needs a GPU")
+    log(f"step: generate {args.n}^3 grid, {args.contacts} contacts")
+    w = W.fem_grid(args.n, young_sigma=0.5, n_contacts=args.contacts, seed=0)
+    nv = w["n_vertices"]
+    dof = 3 * nv
+    log(f"step: analyze + setup (vertices {nv}, upper blocks {w['nb']})")
+    t0 = time.perf_counter()
+    s = session.solver_for(w, levels=args.levels)
+    setup_ms = (time.perf_counter() - t0) * 1e3
+    info = s.info()
+    vals = torch.from_numpy(np.ascontiguousarray(w["vals"])).cuda()
+    b = torch.from_numpy(session.rhs_of(w)).cuda()
+    x = torch.zeros_like(b)
+    torch.cuda.synchronize()
+
+    def step(prec):
+        s.set_values(0, vals)
+        s.build(prec)
+        return s.solve(b, x, rule=1, tol=args.tol)[0]
+
+    def run(prec, steps, warmup):
+        for _ in range(warmup):
+            step(prec)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        st = None
+        for _ in range(steps):
+            st = step(prec)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) * 1e3 / steps
+        return ms, st
+
+    precs = ["fp32"] + [p for p in args.precisions.split(",") if p and p != "fp32"]
+    results = {}
+    for prec in precs:
+        log(f"step: timed {prec} ({args.warmup} warmup + {args.steps} steps)")
+        ms, st = run(prec, args.steps if prec == "fp32" else max(3, args.steps // 2), args.warmup)
+        if st["failure"] != 0 or st["rel_res_true"] > args.tol * (1 + 1e-6):
+            raise SystemExit(f"{prec} solve did not reach the tolerance: {st}")
+        # split of the step: build alone, solve alone
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(5):
+            s.build(prec)
+        torch.cuda.synchronize()
+        build_ms = (time.perf_counter() - t) * 1e3 / 5
+        t = time.perf_counter()
+        for _ in range(5):
+            s.solve(b, x, rule=1, tol=args.tol)
+        torch.cuda.synchronize()
+        pcg_ms = (time.perf_counter() - t) * 1e3 / 5
+        # per-kernel device time (HIP events on the library's stream)
+        kern = {name: s.time_kernel(k, args.kernel_reps) * 1e3
+                for k, name in enumerate(["mas_local", "precond_finish", "spmv", "update"])}
+        s.build(prec)  # time_kernel leaves the iteration state frozen; rebuild
+        results[prec] = dict(ms=ms, st=st, build_ms=build_ms, pcg_ms=pcg_ms, kernels_us=kern)
+        log(f"  {prec}: {ms:.3f} ms/step (build {build_ms:.3f}, pcg {pcg_ms:.3f}), iterations {st['iterations']}, "
+            f"true rel res {st['rel_res_true']:.2e}, kernels us {kern}")
+
+    plain = results["fp32"]
+    alg = algorithmic_bytes(info, nv, 4)
+    kern = plain["kernels_us"]
+    dominant = max(kern, key=lambda k: kern[k])
+    achieved = alg[dominant] / (kern[dominant] * 1e-6) / 1e9
+    roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(dominant, (args.n, args.contacts, args.levels) == (40, 20000, 4)),
+                "algorithmic_bytes": alg[dominant], "avg_us": round(kern[dominant], 2),
+                "per_kernel": {k: {"us": round(kern[k], 2), "GB/s": round(alg[k] / (kern[k] * 1e-6) / 1e9, 1)}
+                               for k in kern}}
+
+    cpu = None
+    if not args.no_cpu:
+        log("step: cpu baseline (oracle, one thread)")
+        cpu = cpu_baseline(w, args.levels, args.tol)
+        log(f"  cpu: {cpu['value']:.1f} ms")
+
+    iters = plain["st"]["iterations"]
+    out = {
+        "metric": "MAS-PCG solve ms to 1e-4 rel-res, mixed vs plain FP32; % HBM roofline",
+        "value": round(plain["ms"], 4),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(plain["ms"], 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"fem{args.n}c: {args.n}^3 hex grid, {nv} vertices, {dof} DOF, "
+                               f"{len(w['tets'])} tets, {args.contacts} contact pairs, lognormal E, "
+                               f"32-vertex Morton banks, {args.levels} MAS levels, tol {args.tol:g}, "
+                               "step = values in + MAS build + PCG",
+                   "vertices": nv, "dof": dof, "upper_blocks": int(info["nb_total"]), "banks": int(info["n_banks"])},
+        "iterations": iters,
+        "rel_res_true": plain["st"]["rel_res_true"],
+        "solves_per_s": round(1e3 / plain["ms"], 2),
+        "dof_iterations_per_s": round(dof * iters / (plain["pcg_ms"] * 1e-3), 1),
+        "plain_fp32": {"ms": round(plain["ms"], 4), "build_ms": round(plain["build_ms"], 4),
+                       "pcg_ms": round(plain["pcg_ms"], 4), "iterations": iters},
+        "mixed": {p: {"ms": round(results[p]["ms"], 4), "build_ms": round(results[p]["build_ms"], 4),
+                      "pcg_ms": round(results[p]["pcg_ms"], 4), "iterations": results[p]["st"]["iterations"],
+                      "rel_res_true": results[p]["st"]["rel_res_true"],
+                      "speedup_vs_fp32": round(plain["ms"] / results[p]["ms"], 3)} for p in precs[1:]},
+        "setup_ms": round(setup_ms, 1),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
