@@ -86,8 +86,7 @@ struct DevData
     const int*      comp_parent;
     const int*      desc_ptr;    // level >= 2 coarse node -> level-1 descendants
     const int*      desc_list;
-    const int*      row_parent1; // internal row -> level-1 coarse node or -1
-    const int*      anc;         // 8 per coarse node: itself and ancestors
+    const int*      row_anc;     // 8 per internal row: its level-1 node and that node's ancestors, -1 padded
     // ABD and diagonal block-Jacobi
     const float* abd_inv;   // 144 per body, row-major
     const int*   body_row;

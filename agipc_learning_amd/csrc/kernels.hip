@@ -106,93 +106,147 @@ __device__ double sys_sum(const double* part, const int* range, int s, double* s
     return block_sum(v, smem);
 }
 
+// The loads of sys_sum without the reduction: this thread's share of the
+// partials of system s (eight loads in flight per pass).  Forming the total
+// with block_sum later lets the caller issue its other loads in between.
+__device__ __forceinline__ double partial_share(const double* part, const int* range, int s)
+{
+    const int b = range[2 * s], e = range[2 * s + 1];
+    double v = 0.0;
+#pragma unroll 1
+    for(int i = b + (int)threadIdx.x; i < e; i += 8 * 256)
+    {
+        double q[8];
+#pragma unroll
+        for(int j = 0; j < 8; ++j) q[j] = i + 256 * j < e ? part[i + 256 * j] : 0.0;
+        v += ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7]));
+    }
+    return v;
+}
+
+// Workgroup order for row tiles: the hardware deals workgroups to the eight
+// XCDs in turn, so this gives each XCD (and its L2) a contiguous run of tiles.
+__device__ __forceinline__ int xcd_tile(int bid, int n)
+{
+    const int xcd = bid & 7, k = bid >> 3;
+    const int per = n >> 3, rem = n & 7;
+    return xcd * per + (xcd < rem ? xcd : rem) + k;
+}
+
 __device__ __forceinline__ bool finite_d(double v) { return isfinite(v); }
 
 }  // namespace
 
 // ===================================================================== build
 
-// Upper FP64 blocks (col-major) -> FP32 SoA planes of the full block CSR.
-__global__ void __launch_bounds__(256) k_pack_values(DevData d, const int* dest1, const int* dest2, long long nb)
+// Upper FP64 blocks (col-major) -> FP32 SoA planes of the full block CSR, one
+// thread per full entry so the plane writes are contiguous; an entry marked
+// transposed takes its upper block's transpose.
+__global__ void __launch_bounds__(256) k_pack_values(DevData d)
 {
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if(g >= nb) return;
-    const double* B = d.vals64 + 9 * g;
-    float v[9];
-#pragma unroll
-    for(int i = 0; i < 3; ++i)
-#pragma unroll
-        for(int j = 0; j < 3; ++j) v[3 * i + j] = (float)B[i + 3 * j];
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if(e >= d.nnz) return;
+    const int code = d.entry_src[e];
+    const double* B = d.vals64 + 9 * (long long)(code & 0x7fffffff);
     float* planes = const_cast<float*>(d.vals32);
-    const int e1 = dest1[g], e2 = dest2[g];
-    if(e1 >= 0)
-#pragma unroll
-        for(int c = 0; c < 9; ++c) planes[c * d.plane + e1] = v[c];
-    if(e2 >= 0)
+    if(code >= 0)
+    {
 #pragma unroll
         for(int i = 0; i < 3; ++i)
 #pragma unroll
-            for(int j = 0; j < 3; ++j) planes[(3 * i + j) * d.plane + e2] = v[3 * j + i];
+            for(int j = 0; j < 3; ++j) planes[(3 * i + j) * d.plane + e] = (float)B[i + 3 * j];
+    }
+    else
+    {
+#pragma unroll
+        for(int i = 0; i < 3; ++i)
+#pragma unroll
+            for(int j = 0; j < 3; ++j) planes[(3 * i + j) * d.plane + e] = (float)B[j + 3 * i];
+    }
 }
 
-// Galerkin coarse entries of one level: E[e] = sum of mapped sources.
+// Galerkin coarse entries of one level: E[e] = sum of its mapped sources
+// (as is, transposed, or B + B^T).  G lanes per entry stride over its source
+// list; their sums meet in a fixed shuffle tree.
+template <int G>
 __global__ void __launch_bounds__(256) k_gal_gather(const double* fine, double* E, const int* ptr, const int* src,
                                                     int e0, int e1, int src_is_fine)
 {
-    const int e = e0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if(e >= e1) return;
+    const int e   = e0 + (int)blockIdx.x * (256 / G) + (int)threadIdx.x / G;
+    const int sub = threadIdx.x % G;
+    const bool valid = e < e1;  // uniform over each group of G lanes
     double acc[9];
 #pragma unroll
     for(int i = 0; i < 9; ++i) acc[i] = 0.0;
-    for(int k = ptr[e]; k < ptr[e + 1]; ++k)
+    if(valid)
     {
-        const int code = src[k];
-        const int idx = code >> 2, mode = code & 3;
-        double B[9];
-        if(src_is_fine)
+        const int k1 = ptr[e + 1];
+#pragma unroll 2
+        for(int k = ptr[e] + sub; k < k1; k += G)
         {
+            const int code = src[k];
+            const int idx = code >> 2, mode = code & 3;
+            double B[9];
+            if(src_is_fine)
+            {
+#pragma unroll
+                for(int i = 0; i < 3; ++i)
+#pragma unroll
+                    for(int j = 0; j < 3; ++j) B[3 * i + j] = fine[9 * (long long)idx + i + 3 * j];
+            }
+            else
+            {
+#pragma unroll
+                for(int i = 0; i < 9; ++i) B[i] = E[9 * (long long)idx + i];
+            }
 #pragma unroll
             for(int i = 0; i < 3; ++i)
 #pragma unroll
-                for(int j = 0; j < 3; ++j) B[3 * i + j] = fine[9 * (long long)idx + i + 3 * j];
+                for(int j = 0; j < 3; ++j)
+                {
+                    const double bij = B[3 * i + j], bji = B[3 * j + i];
+                    acc[3 * i + j] += mode == 0 ? bij : mode == 1 ? bji : bij + bji;
+                }
         }
-        else
-        {
-#pragma unroll
-            for(int i = 0; i < 9; ++i) B[i] = E[9 * (long long)idx + i];
-        }
-#pragma unroll
-        for(int i = 0; i < 3; ++i)
-#pragma unroll
-            for(int j = 0; j < 3; ++j)
-            {
-                const double bij = B[3 * i + j], bji = B[3 * j + i];
-                acc[3 * i + j] += mode == 0 ? bij : mode == 1 ? bji : bij + bji;
-            }
     }
 #pragma unroll
-    for(int i = 0; i < 9; ++i) E[9 * (long long)e + i] = acc[i];
+    for(int o = G / 2; o > 0; o >>= 1)
+#pragma unroll
+        for(int i = 0; i < 9; ++i) acc[i] += __shfl_xor(acc[i], o, 64);
+    if(valid && sub == 0)
+#pragma unroll
+        for(int i = 0; i < 9; ++i) E[9 * (long long)e + i] = acc[i];
 }
 
-// Dense local assembly, FP64 Cholesky with a pivot floor
-// (pro/.../TraditionalMAS32Preconditioner.cu:1382-1439), triangular inverse
-// and A^-1 = L^-T L^-1, written diagonal-major in the storage precision T.
-// One workgroup per bank; the symmetric matrix lives in LDS as a packed lower
-// triangle.  Half-precision storage is scaled by the largest diagonal entry of
-// the inverse (for an SPD inverse, the largest entry in magnitude).
+// Dense local assembly and FP64 inversion, written diagonal-major in the
+// storage precision T.  One workgroup per bank.  The symmetric matrix is
+// assembled in LDS as a packed lower triangle, then inverted in place by
+// Gauss-Jordan elimination without pivoting on register tiles: the DIM x DIM
+// matrix is spread over a 16 x 16 thread grid, thread (ty, tx) holding rows
+// ty + 16 r and columns tx + 16 c.  Step k eliminates with pivot k; its
+// pivot row and column pass through LDS (double-buffered, one barrier per
+// step).  A pivot -- the Schur-complement diagonal -- below
+// max(1e-12, 1e-12 max|diag|) is raised to that floor, which inverts the same
+// modified matrix as the pivot-floored Cholesky of
+// pro/.../TraditionalMAS32Preconditioner.cu:1382-1439.  Half-precision storage
+// is scaled by the largest diagonal entry of the inverse (for an SPD inverse,
+// the largest entry in magnitude).
 template <int BANK, typename T>
 __global__ void __launch_bounds__(256) k_mas_factor(DevData d, const double* E, const int* ent_ptr, const int* ent_pos,
                                                     const int* ent_src, void* out, float* scale_out, int* status)
 {
     constexpr int DIM  = 3 * BANK;
+    constexpr int R    = DIM / 16;  // rows (and columns) per thread
     constexpr int NL   = DIM * (DIM + 1) / 2;
     constexpr int NBLK = BANK * (BANK + 1) / 2;
     constexpr bool kHalf = !std::is_same<T, float>::value;
+    static_assert(DIM % 16 == 0, "bank size must make 3*bank a multiple of 16");
     __shared__ double L[NL];
-    __shared__ double colbuf[DIM];
+    __shared__ double rowk[2][DIM], colk[2][DIM];
     __shared__ double red[8];
     const int bank = blockIdx.x;
     const int tid  = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
     auto at = [](int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; };
     for(int i = tid; i < NL; i += blockDim.x) L[i] = 0.0;
     __syncthreads();
@@ -218,72 +272,83 @@ __global__ void __launch_bounds__(256) k_mas_factor(DevData d, const double* E, 
             }
     }
     __syncthreads();
-    // zero diagonal -> 1 (padding slots behave as identity), pivot floor
+    // zero diagonal -> 1 (padding slots behave as identity)
     if(tid < DIM && L[at(tid, tid)] == 0.0) L[at(tid, tid)] = 1.0;
     __syncthreads();
-    if(tid == 0)
+    if(tid < 64)
     {
-        double s = 0.0;
-        for(int i = 0; i < DIM; ++i) s = fmax(s, fabs(L[at(i, i)]));
-        red[0] = fmax(1.0e-12, 1.0e-12 * s);
+        double m = 0.0;
+        for(int i = tid; i < DIM; i += 64) m = fmax(m, fabs(L[at(i, i)]));
+#pragma unroll
+        for(int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+        if(tid == 0) red[0] = fmax(1.0e-12, 1.0e-12 * m);
     }
+    double a[R][R];
+#pragma unroll
+    for(int r = 0; r < R; ++r)
+#pragma unroll
+        for(int c = 0; c < R; ++c) a[r][c] = L[at(ty + 16 * r, tx + 16 * c)];
     __syncthreads();
     const double floor_value = red[0];
     int bad = 0;
-    for(int k = 0; k < DIM; ++k)
+#pragma unroll
+    for(int kr = 0; kr < R; ++kr)
     {
-        if(tid == 0)
+#pragma unroll 1
+        for(int kt = 0; kt < 16; ++kt)
         {
-            double v = L[at(k, k)];
-            if(!isfinite(v)) { v = floor_value; ++bad; }
-            if(v < floor_value) { v = floor_value; ++bad; }
-            L[at(k, k)] = sqrt(v);
+            const int k = kt + 16 * kr, buf = k & 1;
+            if(tx == kt)
+            {
+#pragma unroll
+                for(int r = 0; r < R; ++r) colk[buf][ty + 16 * r] = a[r][kr];
+            }
+            if(ty == kt)
+            {
+#pragma unroll
+                for(int c = 0; c < R; ++c) rowk[buf][tx + 16 * c] = a[kr][c];
+            }
+            __syncthreads();
+            double piv = rowk[buf][k];
+            if(!(piv >= floor_value) || isinf(piv))  // also catches NaN
+            {
+                piv = floor_value;
+                ++bad;
+            }
+            const double ip = 1.0 / piv;
+            double rk[R];
+#pragma unroll
+            for(int c = 0; c < R; ++c) rk[c] = (tx == kt && c == kr) ? ip : rowk[buf][tx + 16 * c] * ip;
+#pragma unroll
+            for(int r = 0; r < R; ++r)
+            {
+                const bool is_k_row = ty == kt && r == kr;
+                const double f = colk[buf][ty + 16 * r];
+#pragma unroll
+                for(int c = 0; c < R; ++c)
+                {
+                    const bool is_k_col = tx == kt && c == kr;
+                    const double base = is_k_col ? 0.0 : a[r][c];
+                    a[r][c] = is_k_row ? rk[c] : base - f * rk[c];
+                }
+            }
         }
-        __syncthreads();
-        const double piv = L[at(k, k)];
-        for(int i = k + 1 + tid; i < DIM; i += blockDim.x)
-        {
-            double v = L[at(i, k)] / piv;
-            if(!isfinite(v)) v = 0.0;
-            L[at(i, k)] = v;
-            colbuf[i] = v;
-        }
-        __syncthreads();
-        // trailing update of the lower triangle: 8 row groups x 32 column lanes
-        for(int i = k + 1 + (tid >> 5); i < DIM; i += (int)(blockDim.x >> 5))
-        {
-            const double ci = colbuf[i];
-            for(int j = k + 1 + (tid & 31); j <= i; j += 32) L[i * (i + 1) / 2 + j] -= ci * colbuf[j];
-        }
-        __syncthreads();
     }
-    // in-place inverse of the lower factor, columns from last to first
-    for(int j = DIM - 1; j >= 0; --j)
-    {
-        if(tid == 0) L[at(j, j)] = 1.0 / L[at(j, j)];
-        for(int i = j + 1 + tid; i < DIM; i += blockDim.x) colbuf[i] = L[at(i, j)];
-        __syncthreads();
-        const double ajj = -L[at(j, j)];
-        for(int i = j + 1 + tid; i < DIM; i += blockDim.x)
+    // the inverse is symmetric: keep its lower triangle in LDS
+#pragma unroll
+    for(int r = 0; r < R; ++r)
+#pragma unroll
+        for(int c = 0; c < R; ++c)
         {
-            double s = 0.0;
-            for(int k = j + 1; k <= i; ++k) s += L[at(i, k)] * colbuf[k];
-            L[at(i, j)] = ajj * s;
+            const int i = ty + 16 * r, j = tx + 16 * c;
+            if(i >= j) L[i * (i + 1) / 2 + j] = a[r][c];
         }
-        __syncthreads();
-    }
-    // A^-1(p,q) = sum_{k >= max(p,q)} X(k,p) X(k,q), X = L^-1 (lower)
-    auto inv_at = [&](int p, int q) {
-        const int k0 = p > q ? p : q;
-        double s = 0.0;
-        for(int k = k0; k < DIM; ++k) s += L[at(k, p)] * L[at(k, q)];
-        return s;
-    };
+    __syncthreads();
     double scale = 1.0;
     if(kHalf)
     {
         double m = 0.0;
-        for(int p = tid; p < DIM; p += blockDim.x) m = fmax(m, inv_at(p, p));
+        for(int p = tid; p < DIM; p += blockDim.x) m = fmax(m, L[at(p, p)]);
 #pragma unroll
         for(int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
         if((tid & 63) == 0) red[1 + (tid >> 6)] = m;
@@ -303,7 +368,7 @@ __global__ void __launch_bounds__(256) k_mas_factor(DevData d, const double* E, 
             while(dd + 1 < BANK && 9 * diag_off(BANK, dd + 1) <= o) ++dd;
             const int len = BANK - dd, rem = o - 9 * diag_off(BANK, dd);
             const int c = rem / len, I = rem % len;
-            dst[o] = (float)inv_at(3 * I + c / 3, 3 * (I + dd) + c % 3);
+            dst[o] = (float)L[at(3 * I + c / 3, 3 * (I + dd) + c % 3)];
         }
     }
     else
@@ -316,9 +381,9 @@ __global__ void __launch_bounds__(256) k_mas_factor(DevData d, const double* E, 
             const int len = BANK - dd, rem = o - 5 * diag_off(BANK, dd);
             const int q = rem / len, I = rem % len;
             const int c0 = 2 * q, c1 = 2 * q + 1;
-            const uint32_t lo = half_bits(inv_at(3 * I + c0 / 3, 3 * (I + dd) + c0 % 3) * inv_scale, (T*)nullptr);
+            const uint32_t lo = half_bits(L[at(3 * I + c0 / 3, 3 * (I + dd) + c0 % 3)] * inv_scale, (T*)nullptr);
             const uint32_t hi =
-                c1 < 9 ? half_bits(inv_at(3 * I + c1 / 3, 3 * (I + dd) + c1 % 3) * inv_scale, (T*)nullptr) : 0u;
+                c1 < 9 ? half_bits(L[at(3 * I + c1 / 3, 3 * (I + dd) + c1 % 3)] * inv_scale, (T*)nullptr) : 0u;
             dst[o] = lo | (hi << 16);
         }
     }
@@ -496,44 +561,57 @@ __global__ void __launch_bounds__(256) k_phase_finish(DevData d)
 // stored block (I, I+d) lane I adds B r_{I+d} to its own row and sends
 // B^T r_I to lane I+d by a shuffle; the four partial results are summed in a
 // fixed order.  Residuals of level >= 2 banks are formed here from the level-1
-// residuals of their descendants.
+// residuals of their descendants.  Workgroups run the coarse banks (the end
+// of the bank list) first, so their longer gathers overlap the fine banks.
 template <int BANK, typename T>
 __global__ void __launch_bounds__(256) k_mas_local(DevData d, int force)
 {
     constexpr int BPW   = 64 / BANK;
     constexpr int NBLK  = BANK * (BANK + 1) / 2;
     constexpr int WORDS = Store<T>::kWords;
-    constexpr int SHORT_LIST = 64;
-    __shared__ float  sR[64 * 3];
-    __shared__ float  sZ[3][64 * 3];
-    __shared__ int    s_live[BPW], s_kind[BPW], s_vec[BPW];
-    __shared__ int    s_dbeg[64], s_dend[64];
-    __shared__ double smem[8];
+    constexpr int ND    = BANK / 4;        // block diagonals per wave
+    constexpr int TPN   = 256 / BANK;      // threads per node in the descendant sums
+    constexpr int CHUNK = 1024;            // descendant entries staged per pass
+    __shared__ float sR[64 * 3];
+    __shared__ float sZ[3][64 * 3];
+    __shared__ float sD[CHUNK * 3];
+    __shared__ int   s_live[BPW], s_kind[BPW], s_vec[BPW];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int lb = lane / BANK, I = lane % BANK;
-    const int bank0 = blockIdx.x * BPW;
+    const int bank0 = (int)(gridDim.x - 1 - blockIdx.x) * BPW;
+    const int b  = bank0 + lb;
+    const int bl = b < d.n_banks ? b : d.n_banks - 1;  // in-bounds address for idle lanes
+    // this wave's inverse blocks: their addresses depend on the bank only, so
+    // every load is issued before anything else is known
+    uint32_t raw[ND][WORDS];
+#pragma unroll
+    for(int kk = 0; kk < ND; ++kk)
+    {
+        const int dd = wave + 4 * kk, len = BANK - dd;
+        const int Ic = I < len ? I : 0;
+        const uint32_t* M = reinterpret_cast<const uint32_t*>(d.minv) + (size_t)bl * NBLK * WORDS +
+                            WORDS * diag_off(BANK, dd) + Ic;
+#pragma unroll
+        for(int c = 0; c < WORDS; ++c) raw[kk][c] = M[c * len];
+    }
     if(tid < BPW)
     {
-        const int b = bank0 + tid;
+        const int bb = bank0 + tid;
         int live = 0, kind = 0, vec = 0;
-        if(b < d.n_banks)
+        if(bb < d.n_banks)
         {
-            live = force || d.state[d.bank_sys[b]].status == kRunning;
-            kind = d.bank_kind[b];
-            vec  = d.bank_vec[b];
+            live = force || d.state[d.bank_sys[bb]].status == kRunning;
+            kind = d.bank_kind[bb];
+            vec  = d.bank_vec[bb];
         }
         s_live[tid] = live;
         s_kind[tid] = kind;
         s_vec[tid]  = vec;
     }
     __syncthreads();
-    bool any = false, deep = false;
+    bool any = false;
 #pragma unroll
-    for(int k = 0; k < BPW; ++k)
-    {
-        any |= s_live[k] != 0;
-        deep |= s_live[k] && s_kind[k] == 2;
-    }
+    for(int k = 0; k < BPW; ++k) any |= s_live[k] != 0;
     if(!any) return;
     // residuals of level-0 and level-1 banks
     for(int i = tid; i < 64 * 3; i += blockDim.x)
@@ -544,105 +622,112 @@ __global__ void __launch_bounds__(256) k_mas_local(DevData d, int force)
             v = (s_kind[k] == 0 ? d.r : d.Rc)[3 * (long long)s_vec[k] + i % (BANK * 3)];
         sR[i] = v;
     }
-    if(deep)
+    // level >= 2: R(node) = sum of its level-1 descendants' residuals.  The
+    // lists of a bank's consecutive nodes are contiguous in desc_list; they
+    // pass through LDS in chunks with every load of a chunk in flight, then
+    // each node's TPN threads sum its entries in list order.
+#pragma unroll 1
+    for(int k = 0; k < BPW; ++k)
     {
-        // level >= 2: R(node) = sum of its level-1 descendants' residuals
-        if(tid < 64)
+        if(!(s_live[k] && s_kind[k] == 2)) continue;  // uniform over the workgroup
+        const int c0 = s_vec[k];
+        const int lo = d.desc_ptr[c0], hi = d.desc_ptr[c0 + BANK];
+        const int node = tid / TPN, sub = tid % TPN;
+        const int nb = d.desc_ptr[c0 + node], ne = d.desc_ptr[c0 + node + 1];
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll 1
+        for(int base = lo; base < hi; base += CHUNK)
         {
-            const int k = tid / BANK;
-            int b0 = 0, e0 = 0;
-            if(s_live[k] && s_kind[k] == 2)
+            const int n = min(CHUNK, hi - base);
+            int   src[CHUNK / 256];
+            float v[CHUNK / 256][3];
+#pragma unroll
+            for(int j = 0; j < CHUNK / 256; ++j)
             {
-                const int c = s_vec[k] + tid % BANK;
-                b0 = d.desc_ptr[c];
-                e0 = d.desc_ptr[c + 1];
+                const int e = tid + 256 * j;
+                src[j] = e < n ? d.desc_list[base + e] : -1;
             }
-            s_dbeg[tid] = b0;
-            s_dend[tid] = e0;
-        }
-        __syncthreads();
-        {
-            const int node = tid >> 2, q = tid & 3;
-            const int k = node / BANK;
-            const int b0 = s_dbeg[node], e0 = s_dend[node];
-            const bool mine = s_live[k] && s_kind[k] == 2 && e0 - b0 <= SHORT_LIST;
-            float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-            if(mine)
-                for(int e = b0 + q; e < e0; e += 4)
+#pragma unroll
+            for(int j = 0; j < CHUNK / 256; ++j)
+            {
+                if(src[j] >= 0)
                 {
-                    const long long c1 = 3 * (long long)d.desc_list[e];
-                    a0 += d.Rc[c1];
-                    a1 += d.Rc[c1 + 1];
-                    a2 += d.Rc[c1 + 2];
+                    const long long c1 = 3 * (long long)src[j];
+                    v[j][0] = d.Rc[c1]; v[j][1] = d.Rc[c1 + 1]; v[j][2] = d.Rc[c1 + 2];
                 }
-            a0 += __shfl_xor(a0, 1, 64); a1 += __shfl_xor(a1, 1, 64); a2 += __shfl_xor(a2, 1, 64);
-            a0 += __shfl_xor(a0, 2, 64); a1 += __shfl_xor(a1, 2, 64); a2 += __shfl_xor(a2, 2, 64);
-            if(mine && q == 0)
+            }
+            __syncthreads();  // the previous chunk has been consumed
+#pragma unroll
+            for(int j = 0; j < CHUNK / 256; ++j)
             {
-                sR[3 * node] = a0; sR[3 * node + 1] = a1; sR[3 * node + 2] = a2;
+                const int e = tid + 256 * j;
+                if(e < n)
+                {
+                    sD[3 * e] = v[j][0]; sD[3 * e + 1] = v[j][1]; sD[3 * e + 2] = v[j][2];
+                }
+            }
+            __syncthreads();
+            const int q0 = max(nb, base), q1 = min(ne, base + n);
+            for(int q = q0 + sub; q < q1; q += TPN)
+            {
+                a0 += sD[3 * (q - base)];
+                a1 += sD[3 * (q - base) + 1];
+                a2 += sD[3 * (q - base) + 2];
             }
         }
-        // long lists (the top levels): the whole workgroup per node
-        for(int node = 0; node < 64; ++node)
+#pragma unroll
+        for(int o = TPN / 2; o > 0; o >>= 1)
         {
-            const int k = node / BANK;
-            const int b0 = s_dbeg[node], e0 = s_dend[node];
-            if(!(s_live[k] && s_kind[k] == 2) || e0 - b0 <= SHORT_LIST) continue;
-            double a[3] = {0.0, 0.0, 0.0};
-            for(int e = b0 + tid; e < e0; e += blockDim.x)
-            {
-                const long long c1 = 3 * (long long)d.desc_list[e];
-                a[0] += d.Rc[c1]; a[1] += d.Rc[c1 + 1]; a[2] += d.Rc[c1 + 2];
-            }
-            for(int c = 0; c < 3; ++c)
-            {
-                const double t = block_sum(a[c], smem);
-                if(tid == 0) sR[3 * node + c] = (float)t;
-            }
+            a0 += __shfl_xor(a0, o, 64);
+            a1 += __shfl_xor(a1, o, 64);
+            a2 += __shfl_xor(a2, o, 64);
+        }
+        if(sub == 0)
+        {
+            const int i = k * BANK + node;
+            sR[3 * i] = a0; sR[3 * i + 1] = a1; sR[3 * i + 2] = a2;
         }
     }
     __syncthreads();
-    const int b = bank0 + lb;
     const bool live = b < d.n_banks && s_live[lb];
     const float r0 = sR[3 * lane], r1 = sR[3 * lane + 1], r2 = sR[3 * lane + 2];
     float z0 = 0.f, z1 = 0.f, z2 = 0.f;
-    if(b < d.n_banks)
+#pragma unroll
+    for(int kk = 0; kk < ND; ++kk)
     {
-#pragma unroll
-        for(int kk = 0; kk < BANK / 4; ++kk)
+        const int dd  = wave + 4 * kk;
+        const bool act = live && I < BANK - dd;
+        float v[9];
+        if constexpr(WORDS == 9)
         {
-            const int dd  = wave + 4 * kk;
-            const int len = BANK - dd;
-            const bool act = live && I < len;
-            float v[9];
-            if(act)
-            {
-                if constexpr(WORDS == 9)
-                    load_block<BANK>(reinterpret_cast<const float*>(d.minv) + (size_t)b * NBLK * 9, dd, I, v);
-                else
-                    load_block_h<BANK, T>(reinterpret_cast<const uint32_t*>(d.minv) + (size_t)b * NBLK * 5, dd, I, v);
-            }
-            else
-            {
 #pragma unroll
-                for(int c = 0; c < 9; ++c) v[c] = 0.f;
-            }
-            const int j = act ? lane + dd : lane;
-            const float q0 = sR[3 * j], q1 = sR[3 * j + 1], q2 = sR[3 * j + 2];
-            z0 += v[0] * q0 + v[1] * q1 + v[2] * q2;
-            z1 += v[3] * q0 + v[4] * q1 + v[5] * q2;
-            z2 += v[6] * q0 + v[7] * q1 + v[8] * q2;
-            // transposed contribution to row I + dd
-            float t0 = v[0] * r0 + v[3] * r1 + v[6] * r2;
-            float t1 = v[1] * r0 + v[4] * r1 + v[7] * r2;
-            float t2 = v[2] * r0 + v[5] * r1 + v[8] * r2;
-            t0 = __shfl_up(t0, dd, BANK);
-            t1 = __shfl_up(t1, dd, BANK);
-            t2 = __shfl_up(t2, dd, BANK);
-            if(dd > 0 && I >= dd)
+            for(int c = 0; c < 9; ++c) v[c] = act ? __uint_as_float(raw[kk][c]) : 0.f;
+        }
+        else
+        {
+#pragma unroll
+            for(int q = 0; q < 4; ++q)
             {
-                z0 += t0; z1 += t1; z2 += t2;
+                v[2 * q]     = act ? lo_half(raw[kk][q], (T*)nullptr) : 0.f;
+                v[2 * q + 1] = act ? hi_half(raw[kk][q], (T*)nullptr) : 0.f;
             }
+            v[8] = act ? lo_half(raw[kk][4], (T*)nullptr) : 0.f;
+        }
+        const int j = act ? lane + dd : lane;
+        const float q0 = sR[3 * j], q1 = sR[3 * j + 1], q2 = sR[3 * j + 2];
+        z0 += v[0] * q0 + v[1] * q1 + v[2] * q2;
+        z1 += v[3] * q0 + v[4] * q1 + v[5] * q2;
+        z2 += v[6] * q0 + v[7] * q1 + v[8] * q2;
+        // transposed contribution to row I + dd
+        float t0 = v[0] * r0 + v[3] * r1 + v[6] * r2;
+        float t1 = v[1] * r0 + v[4] * r1 + v[7] * r2;
+        float t2 = v[2] * r0 + v[5] * r1 + v[8] * r2;
+        t0 = __shfl_up(t0, dd, BANK);
+        t1 = __shfl_up(t1, dd, BANK);
+        t2 = __shfl_up(t2, dd, BANK);
+        if(dd > 0 && I >= dd)
+        {
+            z0 += t0; z1 += t1; z2 += t2;
         }
     }
     if(wave > 0)
@@ -670,17 +755,53 @@ __global__ void __launch_bounds__(256) k_mas_local(DevData d, int force)
 // Convergence decision after the previous update (or init / restart), then
 // finish the preconditioner: prolong the coarse corrections into the level-0
 // result (MAS rows), 12x12 block-Jacobi (ABD bodies), identity or 3x3
-// block-Jacobi (plain rows); partial r.z.
+// block-Jacobi (plain rows); partial r.z.  The row data (and the coarse
+// corrections they need) are loaded while the r.r partials are reduced.
 __global__ void __launch_bounds__(256) k_precond_finish(DevData d, int force)
 {
     __shared__ double smem[8];
     const int4 w = d.wg_pre[blockIdx.x];
     const int  s = w.x;
     SysState&  st = d.state[s];
+    if(!force && st.status != kRunning) return;
+    const double share = force ? 0.0 : partial_share(d.part_upd, d.wg_upd_range, s);
+    const int type = w.y >> 28;
+    // row loads, independent of the decision
+    const int row = w.z + threadIdx.x;
+    const bool row_ok = type != kWgAbd && row < w.w;
+    const long long o = 3 * (long long)(row_ok ? row : w.z);
+    float r0 = 0.f, r1 = 0.f, r2 = 0.f, z0 = 0.f, z1 = 0.f, z2 = 0.f;
+    float c0 = 0.f, c1 = 0.f, c2 = 0.f;
+    int mode = 0;
+    if(row_ok)
+    {
+        r0 = d.r[o]; r1 = d.r[o + 1]; r2 = d.r[o + 2];
+        if(type == kWgFemMas)
+        {
+            z0 = d.z[o]; z1 = d.z[o + 1]; z2 = d.z[o + 2];
+            const int4 a = reinterpret_cast<const int4*>(d.row_anc)[2 * (long long)row];
+            const int4 e = reinterpret_cast<const int4*>(d.row_anc)[2 * (long long)row + 1];
+            const int ids[8] = {a.x, a.y, a.z, a.w, e.x, e.y, e.z, e.w};
+            float g[8][3];
+#pragma unroll
+            for(int k = 0; k < 8; ++k)
+            {
+                const long long gi = 3 * (long long)(ids[k] >= 0 ? ids[k] : 0);
+                g[k][0] = d.Zc[gi]; g[k][1] = d.Zc[gi + 1]; g[k][2] = d.Zc[gi + 2];
+            }
+#pragma unroll
+            for(int k = 0; k < 8; ++k)
+                if(ids[k] >= 0)
+                {
+                    c0 += g[k][0]; c1 += g[k][1]; c2 += g[k][2];
+                }
+        }
+        else
+            mode = d.row_mode[row];
+    }
     if(!force)
     {
-        if(st.status != kRunning) return;
-        const double rr = sys_sum(d.part_upd, d.wg_upd_range, s, smem);
+        const double rr = block_sum(share, smem);
         int ns = kRunning, fail = kOk, chk = 0;
         if(!st.frozen)
         {
@@ -707,7 +828,6 @@ __global__ void __launch_bounds__(256) k_precond_finish(DevData d, int force)
         }
         if(ns != kRunning) return;
     }
-    const int type = w.y >> 28;
     double part = 0.0;
     if(type == kWgAbd)
     {
@@ -724,47 +844,25 @@ __global__ void __launch_bounds__(256) k_precond_finish(DevData d, int force)
             part = (double)d.r[base + dof] * (double)acc;
         }
     }
-    else
+    else if(row_ok)
     {
-        const int row = w.z + threadIdx.x;
-        if(row < w.w)
+        if(type == kWgFemMas)
         {
-            const long long o = 3 * (long long)row;
-            const float r0 = d.r[o], r1 = d.r[o + 1], r2 = d.r[o + 2];
-            float z0, z1, z2;
-            if(type == kWgFemMas)
-            {
-                z0 = d.z[o]; z1 = d.z[o + 1]; z2 = d.z[o + 2];
-                const int p1 = d.row_parent1[row];
-                if(p1 >= 0)
-                {
-                    const int4 a = reinterpret_cast<const int4*>(d.anc)[2 * p1];
-                    const int4 c = reinterpret_cast<const int4*>(d.anc)[2 * p1 + 1];
-                    const int ids[8] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w};
-#pragma unroll
-                    for(int k = 0; k < 8; ++k)
-                        if(ids[k] >= 0)
-                        {
-                            z0 += d.Zc[3 * (long long)ids[k]];
-                            z1 += d.Zc[3 * (long long)ids[k] + 1];
-                            z2 += d.Zc[3 * (long long)ids[k] + 2];
-                        }
-                }
-            }
-            else if(d.row_mode[row])
-            {
-                const float* D = d.diag_inv + 9 * (long long)row;
-                z0 = D[0] * r0 + D[1] * r1 + D[2] * r2;
-                z1 = D[3] * r0 + D[4] * r1 + D[5] * r2;
-                z2 = D[6] * r0 + D[7] * r1 + D[8] * r2;
-            }
-            else
-            {
-                z0 = r0; z1 = r1; z2 = r2;
-            }
-            d.z[o] = z0; d.z[o + 1] = z1; d.z[o + 2] = z2;
-            part = (double)r0 * z0 + (double)r1 * z1 + (double)r2 * z2;
+            z0 += c0; z1 += c1; z2 += c2;
         }
+        else if(mode)
+        {
+            const float* D = d.diag_inv + 9 * (long long)row;
+            z0 = D[0] * r0 + D[1] * r1 + D[2] * r2;
+            z1 = D[3] * r0 + D[4] * r1 + D[5] * r2;
+            z2 = D[6] * r0 + D[7] * r1 + D[8] * r2;
+        }
+        else
+        {
+            z0 = r0; z1 = r1; z2 = r2;
+        }
+        d.z[o] = z0; d.z[o + 1] = z1; d.z[o + 2] = z2;
+        part = (double)r0 * z0 + (double)r1 * z1 + (double)r2 * z2;
     }
     if(force) return;
     part = block_sum(part, smem);
@@ -773,19 +871,72 @@ __global__ void __launch_bounds__(256) k_precond_finish(DevData d, int force)
 
 // beta from the r.z partials, then Ap = A p_new with p_new = z + beta p_old
 // formed on the fly (the p update is fused here); partial p.Ap.  G lanes per
-// block row, P row passes per workgroup with their loads interleaved.
-template <int G, int P>
+// block row, P row passes per workgroup; the first E entries of every lane
+// (matrix values, column indices and the z / old-p gathers) of all passes are
+// loaded before the r.z partials are reduced, longer rows finish in a loop.
+template <int G, int P, int E>
 __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __restrict__ p_old,
                                                     float* __restrict__ p_new)
 {
     __shared__ double smem[8];
-    const int4 w = d.wg_spmv[blockIdx.x];
-    const int  s = w.x;
+    const int  wg = xcd_tile(blockIdx.x, gridDim.x);
+    const int4 w  = d.wg_spmv[wg];
+    const int  s  = w.x;
     SysState&  st = d.state[s];
     if(st.status != kRunning) return;
-    const double rz = sys_sum(d.part_pre, d.wg_pre_range, s, smem);
+    const double share = partial_share(d.part_pre, d.wg_pre_range, s);
+    constexpr int RPP = 256 / G;  // rows per pass
+    const int sub = threadIdx.x % G;
+    int rows[P], eb[P], ee[P];
+#pragma unroll
+    for(int q = 0; q < P; ++q)
+    {
+        rows[q] = w.z + q * RPP + (int)threadIdx.x / G;
+        const int rc = rows[q] < w.w ? rows[q] : w.z;
+        eb[q] = d.row_ptr[rc];
+        ee[q] = rows[q] < w.w ? d.row_ptr[rc + 1] : eb[q];
+    }
+    float own[P][6];
+#pragma unroll
+    for(int q = 0; q < P; ++q)
+    {
+        const long long oc = 3 * (long long)(rows[q] < w.w ? rows[q] : w.z);
+        own[q][0] = d.z[oc]; own[q][1] = d.z[oc + 1]; own[q][2] = d.z[oc + 2];
+        own[q][3] = p_old[oc]; own[q][4] = p_old[oc + 1]; own[q][5] = p_old[oc + 2];
+    }
+    int   cix[P][E];
+    float val[P][E][9];
+    float gx[P][E][6];
+#pragma unroll
+    for(int q = 0; q < P; ++q)
+#pragma unroll
+        for(int j = 0; j < E; ++j)
+        {
+            const int e = eb[q] + sub + j * G;
+            cix[q][j] = e < ee[q] ? d.col[e] : -1;
+        }
+#pragma unroll
+    for(int q = 0; q < P; ++q)
+#pragma unroll
+        for(int j = 0; j < E; ++j)
+        {
+            const int e  = eb[q] + sub + j * G;
+            const int ec = e < ee[q] ? e : 0;
+#pragma unroll
+            for(int c = 0; c < 9; ++c) val[q][j][c] = d.vals32[c * d.plane + ec];
+        }
+#pragma unroll
+    for(int q = 0; q < P; ++q)
+#pragma unroll
+        for(int j = 0; j < E; ++j)
+        {
+            const long long c = 3 * (long long)(cix[q][j] >= 0 ? cix[q][j] : 0);
+            gx[q][j][0] = d.z[c]; gx[q][j][1] = d.z[c + 1]; gx[q][j][2] = d.z[c + 2];
+            gx[q][j][3] = p_old[c]; gx[q][j][4] = p_old[c + 1]; gx[q][j][5] = p_old[c + 2];
+        }
+    const double rz = block_sum(share, smem);
     const bool bad = !finite_d(rz) || (st.rule == kRuleTrueRel && rz <= 0.0);
-    if(blockIdx.x == d.wg_spmv_range[2 * s] && threadIdx.x == 0)
+    if(wg == d.wg_spmv_range[2 * s] && threadIdx.x == 0)
     {
         if(bad && !st.frozen)
         {
@@ -798,26 +949,24 @@ __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __re
     }
     if(bad && !st.frozen) return;
     const float beta = st.beta_zero ? 0.f : (float)(rz / st.rz_prev);
-    const int   sub = threadIdx.x % G;
-    constexpr int RPP = 256 / G;  // rows per pass
     double part = 0.0;
-    int rows[P], eb[P], ee[P];
-#pragma unroll
-    for(int q = 0; q < P; ++q)
-    {
-        rows[q] = w.z + q * RPP + (int)threadIdx.x / G;
-        eb[q] = ee[q] = 0;
-        if(rows[q] < w.w)
-        {
-            eb[q] = d.row_ptr[rows[q]];
-            ee[q] = d.row_ptr[rows[q] + 1];
-        }
-    }
 #pragma unroll
     for(int q = 0; q < P; ++q)
     {
         float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-        for(int e = eb[q] + sub; e < ee[q]; e += G)
+#pragma unroll
+        for(int j = 0; j < E; ++j)
+        {
+            const bool ok = cix[q][j] >= 0;
+            const float x0 = ok ? gx[q][j][0] + beta * gx[q][j][3] : 0.f;
+            const float x1 = ok ? gx[q][j][1] + beta * gx[q][j][4] : 0.f;
+            const float x2 = ok ? gx[q][j][2] + beta * gx[q][j][5] : 0.f;
+            const float* v = val[q][j];
+            a0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
+            a1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
+            a2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
+        }
+        for(int e = eb[q] + sub + E * G; e < ee[q]; e += G)
         {
             const long long c = 3 * (long long)d.col[e];
             const float x0 = d.z[c] + beta * p_old[c];
@@ -838,33 +987,79 @@ __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __re
         if(sub == 0 && rows[q] < w.w)
         {
             const long long o = 3 * (long long)rows[q];
-            const float y0 = d.z[o] + beta * p_old[o];
-            const float y1 = d.z[o + 1] + beta * p_old[o + 1];
-            const float y2 = d.z[o + 2] + beta * p_old[o + 2];
+            const float y0 = own[q][0] + beta * own[q][3];
+            const float y1 = own[q][1] + beta * own[q][4];
+            const float y2 = own[q][2] + beta * own[q][5];
             p_new[o] = y0; p_new[o + 1] = y1; p_new[o + 2] = y2;
             d.Ap[o] = a0; d.Ap[o + 1] = a1; d.Ap[o + 2] = a2;
             part += (double)y0 * a0 + (double)y1 * a1 + (double)y2 * a2;
         }
     }
     part = block_sum(part, smem);
-    if(threadIdx.x == 0) d.part_spmv[blockIdx.x] = part;
+    if(threadIdx.x == 0) d.part_spmv[wg] = part;
 }
 
 // mode 0: alpha from the p.Ap partials, x += alpha p, r -= alpha Ap;
 // mode 1 (init / restart): r := rt.  Then the level-1 restriction of the MAS
-// rows and the partial r.r (b.b in FP64 at init).
+// rows and the partial r.r (b.b in FP64 at init).  The row vectors and the
+// component lists are loaded while the p.Ap partials are reduced.
 __global__ void __launch_bounds__(256) k_update(DevData d, const float* __restrict__ p, int mode)
 {
     __shared__ double smem[8];
     const int4 w = d.wg_upd[blockIdx.x];
     const int  s = w.x;
     SysState&  st = d.state[s];
-    float alpha = 0.f;
     bool init = false;
     if(mode == 0)
     {
         if(st.status != kRunning) return;
-        const double pAp = sys_sum(d.part_spmv, d.wg_spmv_range, s, smem);
+    }
+    else
+    {
+        if(st.status != kRestarting && st.status != kInit) return;
+        init = st.status == kInit;
+    }
+    const double share = mode == 0 ? partial_share(d.part_spmv, d.wg_spmv_range, s) : 0.0;
+    const int type = w.y >> 28;
+    const int aux  = w.y & 0x0fffffff;
+    const int row  = w.z + threadIdx.x;
+    const bool row_ok = row < w.w;
+    const long long o = 3 * (long long)(row_ok ? row : w.z);
+    float x[3] = {0.f, 0.f, 0.f}, pv[3] = {0.f, 0.f, 0.f}, rv[3] = {0.f, 0.f, 0.f}, ap[3] = {0.f, 0.f, 0.f};
+    double bv[3] = {0.0, 0.0, 0.0};
+    if(row_ok)
+    {
+        if(mode == 0)
+        {
+#pragma unroll
+            for(int c = 0; c < 3; ++c)
+            {
+                x[c] = d.x[o + c]; pv[c] = p[o + c]; rv[c] = d.r[o + c]; ap[c] = d.Ap[o + c];
+            }
+        }
+        else
+        {
+#pragma unroll
+            for(int c = 0; c < 3; ++c) rv[c] = d.rt[o + c];
+            if(init)
+#pragma unroll
+                for(int c = 0; c < 3; ++c) bv[c] = d.b64[o + c];
+        }
+    }
+    // level-1 restriction set-up: the bank's lanes (a power of two, <= 64)
+    const int bank_shift = __ffs(d.bank) - 1;
+    const bool restrict_rows = type == kWgFemMas && w.z + (int)((threadIdx.x >> bank_shift) << bank_shift) < w.w;
+    const int gb = aux + (int)(threadIdx.x >> bank_shift);
+    int cb = 0, ce = 0;
+    if(restrict_rows)
+    {
+        cb = d.comp_ptr[gb];
+        ce = d.comp_ptr[gb + 1];
+    }
+    float alpha = 0.f;
+    if(mode == 0)
+    {
+        const double pAp = block_sum(share, smem);
         const bool bad = !finite_d(pAp) || (st.rule == kRuleTrueRel && pAp <= 0.0);
         if(blockIdx.x == d.wg_upd_range[2 * s] && threadIdx.x == 0)
         {
@@ -885,54 +1080,42 @@ __global__ void __launch_bounds__(256) k_update(DevData d, const float* __restri
         if(bad && !st.frozen) return;
         alpha = (float)(st.rz_cur / pAp);
     }
-    else
-    {
-        if(st.status != kRestarting && st.status != kInit) return;
-        init = st.status == kInit;
-    }
-    const int type = w.y >> 28;
-    const int aux  = w.y & 0x0fffffff;
-    const int row  = w.z + threadIdx.x;
     float r0 = 0.f, r1 = 0.f, r2 = 0.f;
     double part = 0.0;
-    if(row < w.w)
+    if(row_ok)
     {
-        const long long o = 3 * (long long)row;
         if(mode == 0)
         {
-            d.x[o]     += alpha * p[o];
-            d.x[o + 1] += alpha * p[o + 1];
-            d.x[o + 2] += alpha * p[o + 2];
-            r0 = d.r[o] - alpha * d.Ap[o];
-            r1 = d.r[o + 1] - alpha * d.Ap[o + 1];
-            r2 = d.r[o + 2] - alpha * d.Ap[o + 2];
+            d.x[o] = x[0] + alpha * pv[0];
+            d.x[o + 1] = x[1] + alpha * pv[1];
+            d.x[o + 2] = x[2] + alpha * pv[2];
+            r0 = rv[0] - alpha * ap[0];
+            r1 = rv[1] - alpha * ap[1];
+            r2 = rv[2] - alpha * ap[2];
         }
         else
         {
-            r0 = d.rt[o]; r1 = d.rt[o + 1]; r2 = d.rt[o + 2];
+            r0 = rv[0]; r1 = rv[1]; r2 = rv[2];
         }
         d.r[o] = r0; d.r[o + 1] = r1; d.r[o + 2] = r2;
         if(init)
-            part = d.b64[o] * d.b64[o] + d.b64[o + 1] * d.b64[o + 1] + d.b64[o + 2] * d.b64[o + 2];
+            part = bv[0] * bv[0] + bv[1] * bv[1] + bv[2] * bv[2];
         else
             part = (double)r0 * r0 + (double)r1 * r1 + (double)r2 * r2;
     }
-    // level-1 restriction: the bank's lanes (a power of two, <= 64) sum the
-    // residual rows of each of its components
-    const int bank_shift = __ffs(d.bank) - 1;
-    if(type == kWgFemMas && w.z + (int)((threadIdx.x >> bank_shift) << bank_shift) < w.w)
+    // level-1 restriction: sum the residual rows of each component of the bank
+    if(restrict_rows)
     {
-        const int gb = aux + (int)(threadIdx.x >> bank_shift);
         const int lane = threadIdx.x & (d.bank - 1);
-        for(int c = d.comp_ptr[gb]; c < d.comp_ptr[gb + 1]; ++c)
+        for(int c = cb; c < ce; ++c)
         {
             const bool in = (d.comp_mask[c] >> lane) & 1u;
             float sx = in ? r0 : 0.f, sy = in ? r1 : 0.f, sz = in ? r2 : 0.f;
-            for(int o = d.bank >> 1; o > 0; o >>= 1)
+            for(int off = d.bank >> 1; off > 0; off >>= 1)
             {
-                sx += __shfl_xor(sx, o, 64);
-                sy += __shfl_xor(sy, o, 64);
-                sz += __shfl_xor(sz, o, 64);
+                sx += __shfl_xor(sx, off, 64);
+                sy += __shfl_xor(sy, off, 64);
+                sz += __shfl_xor(sz, off, 64);
             }
             if(lane == 0)
             {
@@ -1107,20 +1290,26 @@ __global__ void __launch_bounds__(256) k_gather_blocks(const double* src, const 
 
 static inline unsigned blocks_for(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
-hipError_t launch_pack_values(const DevData& d, const int* dest1, const int* dest2, long long nb, hipStream_t st)
+hipError_t launch_pack_values(const DevData& d, hipStream_t st)
 {
-    if(nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_pack_values, dim3(blocks_for(nb, 256)), dim3(256), 0, st, d, dest1, dest2, nb);
+    if(d.nnz <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_values, dim3(blocks_for(d.nnz, 256)), dim3(256), 0, st, d);
     MASX_CHECK_LAUNCH();
     return hipSuccess;
 }
 
 hipError_t launch_gal_gather(const double* fine, double* E, const int* ptr, const int* src, int e0, int e1,
-                             int src_is_fine, hipStream_t st)
+                             int src_is_fine, int lanes, hipStream_t st)
 {
     if(e1 <= e0) return hipSuccess;
-    hipLaunchKernelGGL(k_gal_gather, dim3(blocks_for(e1 - e0, 256)), dim3(256), 0, st, fine, E, ptr, src, e0, e1,
-                       src_is_fine);
+    if(lanes == 64)
+        hipLaunchKernelGGL(k_gal_gather<64>, dim3(blocks_for(e1 - e0, 4)), dim3(256), 0, st, fine, E, ptr, src, e0,
+                           e1, src_is_fine);
+    else if(lanes == 16)
+        hipLaunchKernelGGL(k_gal_gather<16>, dim3(blocks_for(e1 - e0, 16)), dim3(256), 0, st, fine, E, ptr, src, e0,
+                           e1, src_is_fine);
+    else
+        return hipErrorInvalidValue;
     MASX_CHECK_LAUNCH();
     return hipSuccess;
 }
@@ -1238,33 +1427,44 @@ hipError_t launch_precond_finish(const DevData& d, int n_wg, int force, hipStrea
     return hipSuccess;
 }
 
-bool spmv_tiling_supported(int lanes_per_row, int passes)
+bool spmv_tiling_supported(int lanes_per_row, int passes, int entries_per_lane)
 {
-    switch(lanes_per_row * 16 + passes)
+    switch(lanes_per_row * 256 + passes * 16 + entries_per_lane)
     {
-        case 4 * 16 + 1: case 4 * 16 + 2: case 8 * 16 + 1: case 8 * 16 + 2: case 8 * 16 + 4: case 16 * 16 + 1:
-        case 16 * 16 + 2: case 16 * 16 + 4: case 32 * 16 + 1: case 32 * 16 + 2: return true;
+        case 4 * 256 + 4 * 16 + 4:
+        case 8 * 256 + 2 * 16 + 2:
+        case 8 * 256 + 4 * 16 + 2:
+        case 8 * 256 + 2 * 16 + 3:
+        case 16 * 256 + 1 * 16 + 1:
+        case 16 * 256 + 2 * 16 + 1:
+        case 16 * 256 + 4 * 16 + 1:
+        case 16 * 256 + 1 * 16 + 2:
+        case 16 * 256 + 2 * 16 + 2:
+        case 32 * 256 + 2 * 16 + 1:
+        case 32 * 256 + 4 * 16 + 1: return true;
         default: return false;
     }
 }
 
-hipError_t launch_spmv_fused(const DevData& d, int n_wg, int lanes_per_row, int passes, const float* p_old,
-                             float* p_new, hipStream_t st)
+hipError_t launch_spmv_fused(const DevData& d, int n_wg, int lanes_per_row, int passes, int entries_per_lane,
+                             const float* p_old, float* p_new, hipStream_t st)
 {
     if(n_wg <= 0) return hipSuccess;
-#define MASX_SPMV(G, P) hipLaunchKernelGGL((k_spmv_fused<G, P>), dim3(n_wg), dim3(256), 0, st, d, p_old, p_new)
-    switch(lanes_per_row * 16 + passes)
+#define MASX_SPMV(G, P, E) \
+    hipLaunchKernelGGL((k_spmv_fused<G, P, E>), dim3(n_wg), dim3(256), 0, st, d, p_old, p_new)
+    switch(lanes_per_row * 256 + passes * 16 + entries_per_lane)
     {
-        case 4 * 16 + 1: MASX_SPMV(4, 1); break;
-        case 4 * 16 + 2: MASX_SPMV(4, 2); break;
-        case 8 * 16 + 1: MASX_SPMV(8, 1); break;
-        case 8 * 16 + 2: MASX_SPMV(8, 2); break;
-        case 8 * 16 + 4: MASX_SPMV(8, 4); break;
-        case 16 * 16 + 1: MASX_SPMV(16, 1); break;
-        case 16 * 16 + 2: MASX_SPMV(16, 2); break;
-        case 16 * 16 + 4: MASX_SPMV(16, 4); break;
-        case 32 * 16 + 1: MASX_SPMV(32, 1); break;
-        case 32 * 16 + 2: MASX_SPMV(32, 2); break;
+        case 4 * 256 + 4 * 16 + 4: MASX_SPMV(4, 4, 4); break;
+        case 8 * 256 + 2 * 16 + 2: MASX_SPMV(8, 2, 2); break;
+        case 8 * 256 + 4 * 16 + 2: MASX_SPMV(8, 4, 2); break;
+        case 8 * 256 + 2 * 16 + 3: MASX_SPMV(8, 2, 3); break;
+        case 16 * 256 + 1 * 16 + 1: MASX_SPMV(16, 1, 1); break;
+        case 16 * 256 + 2 * 16 + 1: MASX_SPMV(16, 2, 1); break;
+        case 16 * 256 + 4 * 16 + 1: MASX_SPMV(16, 4, 1); break;
+        case 16 * 256 + 1 * 16 + 2: MASX_SPMV(16, 1, 2); break;
+        case 16 * 256 + 2 * 16 + 2: MASX_SPMV(16, 2, 2); break;
+        case 32 * 256 + 2 * 16 + 1: MASX_SPMV(32, 2, 1); break;
+        case 32 * 256 + 4 * 16 + 1: MASX_SPMV(32, 4, 1); break;
         default: return hipErrorInvalidValue;
     }
 #undef MASX_SPMV

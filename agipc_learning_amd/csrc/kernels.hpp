@@ -6,11 +6,12 @@ namespace masx
 {
 // bytes of one bank inverse in the diagonal-major layout of `precision`
 size_t mas_bank_bytes(int bank, int precision);
-bool   spmv_tiling_supported(int lanes_per_row, int passes);
+bool   spmv_tiling_supported(int lanes_per_row, int passes, int entries_per_lane);
 
-hipError_t launch_pack_values(const DevData& d, const int* dest1, const int* dest2, long long nb, hipStream_t st);
+hipError_t launch_pack_values(const DevData& d, hipStream_t st);
+// lanes per coarse entry: 16 or 64
 hipError_t launch_gal_gather(const double* fine, double* E, const int* ptr, const int* src, int e0, int e1,
-                             int src_is_fine, hipStream_t st);
+                             int src_is_fine, int lanes, hipStream_t st);
 hipError_t launch_mas_factor(const DevData& d, const double* E, const int* ent_ptr, const int* ent_pos,
                              const int* ent_src, void* out, float* scale, int precision, int* status, hipStream_t st);
 hipError_t launch_abd_factor(const DevData& d, const int* body_src, float* inv, int n_bodies, hipStream_t st);
@@ -22,8 +23,8 @@ hipError_t launch_init_state(const DevData& d, int rule, double tol, const int* 
 hipError_t launch_phase_finish(const DevData& d, hipStream_t st);
 hipError_t launch_mas_local(const DevData& d, int precision, int force, hipStream_t st);
 hipError_t launch_precond_finish(const DevData& d, int n_wg, int force, hipStream_t st);
-hipError_t launch_spmv_fused(const DevData& d, int n_wg, int lanes_per_row, int passes, const float* p_old,
-                             float* p_new, hipStream_t st);
+hipError_t launch_spmv_fused(const DevData& d, int n_wg, int lanes_per_row, int passes, int entries_per_lane,
+                             const float* p_old, float* p_new, hipStream_t st);
 hipError_t launch_update(const DevData& d, int n_wg, const float* p, int mode, hipStream_t st);
 // true residual of the checked (or, report_only, all) systems and its decision
 hipError_t launch_true_residual(const DevData& d, int n_wg, int report_only, hipStream_t st);

@@ -343,8 +343,6 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
     }
     // ---- full block CSR (both triangles)
     std::vector<int> cnt(P.n_rows + 1, 0);
-    P.dest1.assign(P.nb_total, -1);
-    P.dest2.assign(P.nb_total, -1);
     for(int s = 0; s < S; ++s)
     {
         auto& d = sys[s];
@@ -399,16 +397,6 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
                 P.col[i]       = tmp[i - b].first;
                 P.entry_src[i] = tmp[i - b].second;
             }
-        }
-        for(long long e = 0; e < P.nnz; ++e)
-        {
-            const int src = P.entry_src[e] & 0x7fffffff;
-            if(P.entry_src[e] < 0)
-                P.dest2[src] = (int)e;
-            else if(P.dest1[src] < 0)
-                P.dest1[src] = (int)e;
-            else
-                P.dest2[src] = (int)e;  // duplicate diagonal input blocks cannot occur: rows==cols gives one entry
         }
     }
     // ---- global 3x3 block-Jacobi and ABD bodies
@@ -507,7 +495,7 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
     // level-1 nodes, level-1 parent of each fine row
     P.bank_kind.assign(P.n_banks, 0);
     P.anc.assign(8 * (size_t)std::max(n_coarse, 1), -1);
-    P.row_parent1.assign(P.n_rows, -1);
+    P.row_anc.assign(8 * (size_t)P.n_rows, -1);
     std::vector<std::vector<int>> desc(n_coarse);
     for(int s = 0; s < S; ++s)
     {
@@ -537,7 +525,10 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
         for(int slot = 0; slot < h.n_slots; ++slot)
         {
             const int v = d.part_to_real[slot];
-            if(v >= 0 && h.levels > 1) P.row_parent1[P.fem_row0[s] + slot] = gcoarse(s, h.going_next[v]);
+            if(v < 0 || h.levels < 2) continue;
+            const int g1 = gcoarse(s, h.going_next[v]);
+            std::copy(P.anc.begin() + 8 * (size_t)g1, P.anc.begin() + 8 * (size_t)g1 + 8,
+                      P.row_anc.begin() + 8 * (size_t)(P.fem_row0[s] + slot));
         }
     }
     P.desc_ptr.assign(n_coarse + 1, 0);
@@ -722,11 +713,6 @@ void validate_plan(const Plan& P)
         if(P.col[e] < 0 || P.col[e] >= P.n_rows) fail("col");
         if((P.entry_src[e] & 0x7fffffff) >= nb) fail("entry_src");
     }
-    for(long long g = 0; g < nb; ++g)
-    {
-        if(P.dest1[g] < 0 || P.dest1[g] >= P.nnz) fail("dest1");
-        if(P.dest2[g] < -1 || P.dest2[g] >= P.nnz) fail("dest2");
-    }
     for(int v : P.ext_to_int)
         if(v < 0 || v >= P.n_rows) fail("ext_to_int");
     if((int)P.bank_vec.size() != P.n_banks || (int)P.bank_kind.size() != P.n_banks) fail("bank arrays");
@@ -753,9 +739,9 @@ void validate_plan(const Plan& P)
         if(v < 0 || v >= P.n_coarse) fail("desc_list");
     for(int v : P.anc)
         if(v < -1 || v >= P.n_coarse) fail("anc");
-    if((int)P.row_parent1.size() != P.n_rows) fail("row_parent1");
-    for(int v : P.row_parent1)
-        if(v < -1 || v >= P.n_coarse) fail("row_parent1 range");
+    if(P.row_anc.size() != 8 * (size_t)P.n_rows) fail("row_anc");
+    for(int v : P.row_anc)
+        if(v < -1 || v >= P.n_coarse) fail("row_anc range");
     for(size_t g = 0; g + 1 < P.gal_level_ptr.size(); ++g)
         for(int e = P.gal_level_ptr[g]; e < P.gal_level_ptr[g + 1]; ++e)
             for(int k = P.gal_ptr[e]; k < P.gal_ptr[e + 1]; ++k)

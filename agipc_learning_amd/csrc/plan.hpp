@@ -90,7 +90,6 @@ struct Plan
     std::vector<int> col;            // nnz
     std::vector<long long> src_off;  // n_sys+1: offset of each system's upper blocks in the
                                      // concatenated value array
-    std::vector<int> dest1, dest2;   // per upper block (global index): full entries (dest2 -1 if diag)
     std::vector<int> entry_src;      // per full entry: global upper block | transposed flag (bit 31)
     long long nnz = 0, nb_total = 0;
     // MAS banks: fine banks first (all systems), then coarse banks
@@ -105,8 +104,9 @@ struct Plan
     std::vector<int> bank_kind;      // 0 level 0 (R from r), 1 level 1 (R in Rc), 2 level >= 2 (R from level-1 sums)
     std::vector<int> desc_ptr;       // per global coarse node (level >= 2): range of its level-1 descendants
     std::vector<int> desc_list;
-    std::vector<int> row_parent1;    // per internal row: level-1 global coarse index, -1 if none
     std::vector<int> anc;            // per global coarse node, 8 ints: itself and its ancestors, -1 padded
+    std::vector<int> row_anc;        // per internal row, 8 ints: its level-1 node and that node's
+                                     // ancestors (global coarse indices), -1 padded
     // restriction schedule for levels >= 2 (per system, coarse banks grouped by level)
     std::vector<int> sys_level_banks;   // flattened: per system, per level l>=1, [bank begin, end)
     std::vector<int> sys_level_ptr;     // n_sys+1 offsets into sys_level_banks (pairs)

@@ -53,6 +53,7 @@ void Solver::ensure_stream()
     MASX_HIP(hipStreamCreateWithFlags(&work_, hipStreamNonBlocking));
     MASX_HIP(hipEventCreate(&ev_a_));
     MASX_HIP(hipEventCreate(&ev_b_));
+    for(auto& e : chunk_ev_) MASX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
 
 void Solver::analyze()
@@ -68,6 +69,8 @@ Solver::~Solver()
     if(h_state_) (void)hipHostFree(h_state_);
     if(ev_a_) (void)hipEventDestroy(ev_a_);
     if(ev_b_) (void)hipEventDestroy(ev_b_);
+    for(auto e : chunk_ev_)
+        if(e) (void)hipEventDestroy(e);
     if(work_) (void)hipStreamDestroy(work_);
 }
 
@@ -100,16 +103,18 @@ int Solver::add_system(const SystemDesc& d)
     return (int)systems_.size() - 1;
 }
 
-void Solver::set_spmv_tiling(int lanes_per_row, int rows_per_wg)
+void Solver::set_spmv_tiling(int lanes_per_row, int rows_per_wg, int entries_per_lane)
 {
     if(lanes_per_row != 4 && lanes_per_row != 8 && lanes_per_row != 16 && lanes_per_row != 32)
         throw std::invalid_argument("lanes per row must be 4, 8, 16 or 32");
     if(rows_per_wg < 256 / lanes_per_row || rows_per_wg % (256 / lanes_per_row))
         throw std::invalid_argument("rows per workgroup must be a multiple of 256 / lanes per row");
     const int passes = rows_per_wg / (256 / lanes_per_row);
-    if(!spmv_tiling_supported(lanes_per_row, passes)) throw std::invalid_argument("unsupported SpMV tiling");
+    if(!spmv_tiling_supported(lanes_per_row, passes, entries_per_lane))
+        throw std::invalid_argument("unsupported SpMV tiling");
     lanes_       = lanes_per_row;
     passes_      = passes;
+    entries_     = entries_per_lane;
     rows_per_wg_ = rows_per_wg;
     if(setup_done_)
     {
@@ -118,6 +123,22 @@ void Solver::set_spmv_tiling(int lanes_per_row, int rows_per_wg)
         if(graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
         graph_exec_ = nullptr;
     }
+}
+
+Solver::Tiling Solver::autotune_spmv(int reps)
+{
+    if(!built_) throw std::runtime_error("build() must precede autotune");
+    static const int kCandidates[][3] = {{16, 16, 1}, {16, 32, 1}, {16, 64, 1}, {16, 16, 2}, {16, 32, 2}, {8, 64, 2},
+                                         {8, 128, 2}, {8, 64, 3}, {4, 256, 4}, {32, 16, 1}, {32, 32, 1}};
+    Tiling best{lanes_, rows_per_wg_, entries_, 1e30};
+    for(const auto& c : kCandidates)
+    {
+        set_spmv_tiling(c[0], c[1], c[2]);
+        const double ms = time_kernel(2, reps);
+        if(ms < best.ms) best = Tiling{c[0], c[1], c[2], ms};
+    }
+    set_spmv_tiling(best.lanes, best.rows_per_wg, best.entries);
+    return best;
 }
 
 void Solver::setup()
@@ -130,10 +151,8 @@ void Solver::setup()
     d_col_.upload(P.col, st);
     d_entry_src_.upload(P.entry_src, st);
     d_row_sys_.upload(P.row_sys, st);
-    d_dest1_.upload(P.dest1, st);
-    d_dest2_.upload(P.dest2, st);
     d_e2i_.upload(P.ext_to_int, st);
-    const long long plane = (P.nnz + 63) / 64 * 64;
+    const long long plane = std::max<long long>(64, (P.nnz + 63) / 64 * 64);
     d_vals32_.alloc(9 * (size_t)plane);
     d_vals64_.alloc(9 * (size_t)P.nb_total);
     d_E_.alloc(9 * (size_t)std::max<long long>(P.n_gal, 1));
@@ -154,8 +173,7 @@ void Solver::setup()
     d_comp_parent_.upload(P.comp_parent, st);
     d_desc_ptr_.upload(P.desc_ptr, st);
     d_desc_list_.upload(P.desc_list, st);
-    d_row_parent1_.upload(P.row_parent1, st);
-    d_anc_.upload(P.anc, st);
+    d_row_anc_.upload(P.row_anc, st);
     d_minv_scale_.alloc(std::max(P.n_banks, 1));
     d_gal_ptr_.upload(P.gal_ptr, st);
     d_gal_src_.upload(P.gal_src, st);
@@ -174,7 +192,7 @@ void Solver::setup()
     d_factor_status_.alloc(1);
     MASX_HIP(hipMemsetAsync(d_state_.ptr, 0, d_state_.n * sizeof(SysState), st));
     if(h_state_) (void)hipHostFree(h_state_);
-    MASX_HIP(hipHostMalloc((void**)&h_state_, sizeof(SysState) * P.n_sys, hipHostMallocDefault));
+    MASX_HIP(hipHostMalloc((void**)&h_state_, sizeof(SysState) * P.n_sys * (kDepth + 1), hipHostMallocDefault));
     build_work_tables();
     make_devdata();
     dd_.plane = plane;
@@ -260,7 +278,7 @@ void Solver::make_devdata()
     d.n_banks = P.n_banks;
     d.n_fine_banks = P.n_fine_banks;
     d.nnz = P.nnz;
-    d.plane = (P.nnz + 63) / 64 * 64;
+    d.plane = std::max<long long>(64, (P.nnz + 63) / 64 * 64);
     d.row_ptr = d_row_ptr_.ptr;
     d.col = d_col_.ptr;
     d.entry_src = d_entry_src_.ptr;
@@ -287,8 +305,7 @@ void Solver::make_devdata()
     d.comp_parent = d_comp_parent_.ptr;
     d.desc_ptr = d_desc_ptr_.ptr;
     d.desc_list = d_desc_list_.ptr;
-    d.row_parent1 = d_row_parent1_.ptr;
-    d.anc = d_anc_.ptr;
+    d.row_anc = d_row_anc_.ptr;
     d.abd_inv = d_abd_inv_.ptr;
     d.body_row = d_body_row_.ptr;
     d.diag_inv = d_diag_inv_.ptr;
@@ -357,10 +374,16 @@ void Solver::build(int precision, hipStream_t user)
         graph_exec_ = nullptr;
     }
     const Plan& P = plan_;
-    MASX_HIP(launch_pack_values(dd_, d_dest1_.ptr, d_dest2_.ptr, P.nb_total, st));
+    MASX_HIP(launch_pack_values(dd_, st));
     for(size_t g = 0; g + 1 < P.gal_level_ptr.size(); ++g)
-        MASX_HIP(launch_gal_gather(d_vals64_.ptr, d_E_.ptr, d_gal_ptr_.ptr, d_gal_src_.ptr, P.gal_level_ptr[g],
-                                   P.gal_level_ptr[g + 1], P.gal_src_is_fine[g], st));
+    {
+        // a whole wave per entry where the lists are long (the top levels)
+        const int e0 = P.gal_level_ptr[g], e1 = P.gal_level_ptr[g + 1];
+        const long long srcs = e1 > e0 ? (long long)P.gal_ptr[e1] - P.gal_ptr[e0] : 0;
+        const int lanes = srcs > 32ll * (e1 - e0) ? 64 : 16;
+        MASX_HIP(launch_gal_gather(d_vals64_.ptr, d_E_.ptr, d_gal_ptr_.ptr, d_gal_src_.ptr, e0, e1,
+                                   P.gal_src_is_fine[g], lanes, st));
+    }
     MASX_HIP(hipMemsetAsync(d_factor_status_.ptr, 0, sizeof(int), st));
     MASX_HIP(launch_mas_factor(dd_, d_E_.ptr, d_ent_ptr_.ptr, d_ent_pos_.ptr, d_ent_src_.ptr, d_minv_.ptr,
                                d_minv_scale_.ptr, precision_, d_factor_status_.ptr, st));
@@ -385,7 +408,7 @@ void Solver::launch_iteration(hipStream_t st, int i)
     float*       p_new = (i & 1) ? d_p0_.ptr : d_p1_.ptr;
     MASX_HIP(launch_mas_local(dd_, precision_, 0, st));
     MASX_HIP(launch_precond_finish(dd_, n_wg_pre_, 0, st));
-    MASX_HIP(launch_spmv_fused(dd_, n_wg_spmv_, lanes_, passes_, p_old, p_new, st));
+    MASX_HIP(launch_spmv_fused(dd_, n_wg_spmv_, lanes_, passes_, entries_, p_old, p_new, st));
     MASX_HIP(launch_update(dd_, n_wg_upd_, p_new, 0, st));
 }
 
@@ -436,28 +459,49 @@ void Solver::solve(const double* d_b, double* d_x, const SolveParams& prm, std::
     int max_total = 0;
     for(int m : max_iter) max_total = std::max(max_total, m);
     const int max_chunks = max_total / chunk + 8 + (with_check ? max_total / 10 : 0);
-    for(int c = 0; c < max_chunks; ++c)
-    {
+    // Chunks are launched ahead of the status checks: kDepth in flight, and
+    // up to 80% of the previous solve's iterations before the first check.
+    // A chunk launched after convergence costs only its early-exit launches.
+    const size_t state_bytes = sizeof(SysState) * P.n_sys;
+    int launched = 0, checked = 0;
+    auto launch_one = [&]() {
         if(prm.use_graph)
             MASX_HIP(hipGraphLaunch(graph_exec_, st));
         else
             run_chunk(st, chunk, with_check);
-        MASX_HIP(hipMemcpyAsync(h_state_, d_state_.ptr, sizeof(SysState) * P.n_sys, hipMemcpyDeviceToHost, st));
-        MASX_HIP(hipStreamSynchronize(st));
+        const int slot = launched % kDepth;
+        MASX_HIP(hipMemcpyAsync(h_state_ + (size_t)slot * P.n_sys, d_state_.ptr, state_bytes, hipMemcpyDeviceToHost,
+                                st));
+        MASX_HIP(hipEventRecord(chunk_ev_[slot], st));
+        ++launched;
+    };
+    const int ahead = std::min(max_chunks, std::max(kDepth, (int)(0.8 * last_iters_) / chunk));
+    while(launched < ahead) launch_one();
+    checked = std::max(0, launched - kDepth);
+    for(;;)
+    {
+        while(launched < max_chunks && launched - checked < kDepth) launch_one();
+        if(checked >= launched) break;
+        const int slot = checked % kDepth;
+        MASX_HIP(hipEventSynchronize(chunk_ev_[slot]));
         bool all_done = true;
-        for(int s = 0; s < P.n_sys; ++s) all_done &= h_state_[s].status == kDone;
+        for(int s = 0; s < P.n_sys; ++s) all_done &= h_state_[(size_t)slot * P.n_sys + s].status == kDone;
         if(all_done) break;
+        ++checked;
     }
     MASX_HIP(launch_store_x(dd_, d_e2i_.ptr, d_x, P.n_ext, st));
     MASX_HIP(launch_true_residual(dd_, n_wg_spmv_, 1, st));
-    MASX_HIP(hipMemcpyAsync(h_state_, d_state_.ptr, sizeof(SysState) * P.n_sys, hipMemcpyDeviceToHost, st));
+    SysState* fin = h_state_ + (size_t)kDepth * P.n_sys;
+    MASX_HIP(hipMemcpyAsync(fin, d_state_.ptr, state_bytes, hipMemcpyDeviceToHost, st));
     MASX_HIP(hipStreamSynchronize(st));
+    last_iters_ = 0;
+    for(int s = 0; s < P.n_sys; ++s) last_iters_ = std::max(last_iters_, fin[s].k);
     if(stats)
     {
         stats->resize(P.n_sys);
         for(int s = 0; s < P.n_sys; ++s)
         {
-            const SysState& h = h_state_[s];
+            const SysState& h = fin[s];
             SolveStats& o = (*stats)[s];
             o.iterations = h.k;
             if(h.rule == kRuleRefRtz && h.failure == kIterationCap) o.iterations = h.max_iter;
@@ -597,7 +641,9 @@ double Solver::time_kernel(int which, int reps)
         {
             case 0: MASX_HIP(launch_mas_local(dd_, precision_, 0, st)); break;
             case 1: MASX_HIP(launch_precond_finish(dd_, n_wg_pre_, 0, st)); break;
-            case 2: MASX_HIP(launch_spmv_fused(dd_, n_wg_spmv_, lanes_, passes_, d_p0_.ptr, d_p1_.ptr, st)); break;
+            case 2:
+                MASX_HIP(launch_spmv_fused(dd_, n_wg_spmv_, lanes_, passes_, entries_, d_p0_.ptr, d_p1_.ptr, st));
+                break;
             case 3: MASX_HIP(launch_update(dd_, n_wg_upd_, d_p1_.ptr, 0, st)); break;
             default: throw std::invalid_argument("unknown kernel id");
         }

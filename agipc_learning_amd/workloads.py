@@ -148,7 +148,7 @@ def batch64_sizes(seed=0, count=64, lo=5000, hi=80000):
     """Cells per axis of each cube: vertex count U[lo,hi], n = round(cbrt(V)) - 1."""
     rng = np.random.default_rng(seed)
     verts = rng.integers(lo, hi + 1, size=count)
-    return [max(2, int(round(math.cbrt(v))) - 1) for v in verts]
+    return [max(2, int(round(float(v) ** (1.0 / 3.0))) - 1) for v in verts]
 
 
 def batch64(seed=0, count=64, lo=5000, hi=80000, contact_fraction=0.05):

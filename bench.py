@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--precisions", default="fp32,fp16,bf16",
                     help="comma list; fp32 is always measured (it is the headline value)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 64-system measurement")
+    ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -128,6 +130,48 @@ def cpu_baseline(w, levels, tol):
                       f"true rel res {so['true_rel_res']:.2e}), FP64, single thread"}
 
 
+def batch64(args):
+    """64 independent cube systems (vertex counts U[5k, 80k], seed 0; 5% of
+    vertices in contact pairs) solved together in one batched launch
+    sequence: plain FP32 vs FP16 inverse storage.  One step = values in +
+    build + solve of all 64 systems."""
+    import numpy as np
+    import torch
+    from agipc_learning_amd import session
+    from agipc_learning_amd import workloads as W
+
+    log("step: batch64 generate")
+    ws = W.batch64(seed=0)
+    nv = sum(w["n_vertices"] for w in ws)
+    log(f"step: batch64 analyze + setup ({nv} vertices, {sum(w['nb'] for w in ws)} upper blocks)")
+    s = session.solver_for(ws)
+    vals = [torch.from_numpy(np.ascontiguousarray(w["vals"])).cuda() for w in ws]
+    b = torch.from_numpy(session.rhs_of(ws)).cuda()
+    x = torch.zeros_like(b)
+    out = {"systems": len(ws), "vertices": nv, "dof": 3 * nv, "upper_blocks": int(s.info()["nb_total"])}
+    for prec in ("fp32", "fp16"):
+        def step():
+            for i, v in enumerate(vals):
+                s.set_values(i, v)
+            s.build(prec)
+            return s.solve(b, x, rule=1, tol=args.tol)
+        step()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.batch_steps):
+            st = step()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) * 1e3 / args.batch_steps
+        worst = max(x["rel_res_true"] for x in st)
+        fails = sum(1 for x in st if x["failure"] != 0)
+        its = [x["iterations"] for x in st]
+        out[prec] = {"ms": round(ms, 3), "max_iterations": max(its), "mean_iterations": round(sum(its) / len(its), 1),
+                     "worst_rel_res_true": worst, "failures": fails}
+        log(f"  batch64 {prec}: {ms:.2f} ms/step, iterations max {max(its)}, worst true rel res {worst:.2e}")
+    out["fp16_speedup_vs_fp32"] = round(out["fp32"]["ms"] / out["fp16"]["ms"], 3)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -176,6 +220,10 @@ def main():
         ms = (time.perf_counter() - t) * 1e3 / steps
         return ms, st
 
+    log("step: autotune SpMV tiling")
+    s.build("fp32")
+    tiling = s.autotune(50)
+    log(f"  tiling {tiling}")
     precs = ["fp32"] + [p for p in args.precisions.split(",") if p and p != "fp32"]
     results = {}
     for prec in precs:
@@ -221,6 +269,12 @@ def main():
         cpu = cpu_baseline(w, args.levels, args.tol)
         log(f"  cpu: {cpu['value']:.1f} ms")
 
+    batch = None
+    if not args.no_batch:
+        del s, vals, b, x
+        torch.cuda.empty_cache()
+        batch = batch64(args)
+
     iters = plain["st"]["iterations"]
     out = {
         "metric": "MAS-PCG solve ms to 1e-4 rel-res, mixed vs plain FP32; % HBM roofline",
@@ -251,8 +305,10 @@ def main():
                       "rel_res_true": results[p]["st"]["rel_res_true"],
                       "speedup_vs_fp32": round(plain["ms"] / results[p]["ms"], 3)} for p in precs[1:]},
         "setup_ms": round(setup_ms, 1),
+        "spmv_tiling": tiling,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batch64": batch,
     }
     print(json.dumps(out), flush=True)
 

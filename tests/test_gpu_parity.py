@@ -255,3 +255,32 @@ def test_mas_object_api_matches_oracle(oracle):
     m.preconditioning(dev(R), Z)
     want = mas.apply(R)
     assert np.linalg.norm(host(Z) - want) <= 1e-4 * np.linalg.norm(want)
+
+
+TILINGS = [(16, 16, 1), (16, 32, 1), (16, 64, 1), (16, 16, 2), (16, 32, 2), (8, 64, 2), (8, 128, 2), (8, 64, 3),
+           (4, 256, 4), (32, 16, 1), (32, 32, 1)]
+
+
+def test_spmv_tilings_agree_and_autotune():
+    """Every SpMV tiling gives the same solve (rows with more entries than the
+    unrolled part go through the remainder loop: contact rows here)."""
+    w = W.fem_grid(10, n_contacts=600, seed=5)
+    s = session.solver_for(w)
+    s.build("fp32")
+    b = dev(session.rhs_of(w))
+    ref = None
+    for t in TILINGS:
+        s.set_spmv_tiling(*t)
+        x = torch.zeros_like(b)
+        st = s.solve(b, x, rule=1, tol=1e-6)[0]
+        assert st["failure"] == 0 and st["rel_res_true"] <= 1e-6 * (1 + 1e-6), (t, st)
+        if ref is None:
+            ref = (st["iterations"], host(x))
+        else:
+            assert abs(st["iterations"] - ref[0]) <= 1, (t, st["iterations"], ref[0])
+            assert np.linalg.norm(host(x) - ref[1]) <= 1e-5 * np.linalg.norm(ref[1]), t
+    best = s.autotune(5)
+    assert (best["lanes_per_row"], best["rows_per_wg"], best["entries_per_lane"]) in TILINGS
+    x = torch.zeros_like(b)
+    st = s.solve(b, x, rule=1, tol=1e-6)[0]
+    assert st["failure"] == 0 and abs(st["iterations"] - ref[0]) <= 1
