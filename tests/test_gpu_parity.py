@@ -284,3 +284,22 @@ def test_spmv_tilings_agree_and_autotune():
     x = torch.zeros_like(b)
     st = s.solve(b, x, rule=1, tol=1e-6)[0]
     assert st["failure"] == 0 and abs(st["iterations"] - ref[0]) <= 1
+
+
+def test_frozen_dump_replay_on_gpu(tmp_path):
+    """A frozen system written in the reference's dump layout and read back
+    solves with the reference-held counts (MAS32 over the identity partition)."""
+    from agipc_learning_amd import frozen
+
+    d = padded_chain()
+    frozen.write_frozen(tmp_path, d["vals"], d["rows"], d["cols"], d["rhs"])
+    r = frozen.read_frozen(tmp_path)
+    s = native.Solver()
+    s.add_system(block_rows=r["block_rows"], rows=r["rows"], cols=r["cols"], use_mas=True)
+    s.set_topology(0, d["start"], d["num"], d["list"], d["part_to_real"], d["real_to_part"])
+    s.setup()
+    s.set_values(0, dev(r["vals"]))
+    s.build("fp32")
+    x = torch.zeros(3 * r["block_rows"], dtype=torch.float64, device="cuda")
+    st = s.solve(dev(r["rhs"]), x, rule=1, tol=1e-3, max_iter=512)[0]
+    assert st["iterations"] == REFERENCE_ITERATIONS["mas32"], st
