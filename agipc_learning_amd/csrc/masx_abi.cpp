@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "converter.hpp"
 #include "solver.hpp"
 
 namespace
@@ -575,5 +576,34 @@ int masx_mas_preconditioning(masx_mas* m, const double* d_R, double* d_Z, void* 
 }
 
 masx_solver* masx_mas_solver(masx_mas* m) { return m ? m->solver.get() : nullptr; }
+
+struct masx_converter
+{
+    masx::Converter impl;
+};
+
+int masx_converter_create(masx_converter** out)
+{
+    return guard([&] {
+        require(out != nullptr, "null output");
+        *out = new masx_converter();
+    });
+}
+
+int masx_converter_destroy(masx_converter* c)
+{
+    return guard([&] { delete c; });
+}
+
+int masx_convert_triplets(masx_converter* c, int n, int n_block_rows, const int* d_rows, const int* d_cols,
+                          const double* d_vals, int drop_lower, int* d_out_rows, int* d_out_cols, double* d_out_vals,
+                          int* n_unique, void* stream)
+{
+    return guard([&] {
+        require(c != nullptr && n_unique != nullptr, "null argument");
+        *n_unique = c->impl.convert(n, n_block_rows, d_rows, d_cols, d_vals, drop_lower, d_out_rows, d_out_cols,
+                                    d_out_vals, as_stream(stream));
+    });
+}
 
 }  // extern "C"

@@ -90,6 +90,9 @@ def load():
         "masx_spmv": ([vp, C.c_double, vp, C.c_double, vp, vp], C.c_int),
         "masx_apply_preconditioner": ([vp, vp, vp, vp], C.c_int),
         "masx_set_spmv_tiling": ([vp, C.c_int, C.c_int, C.c_int], C.c_int),
+        "masx_converter_create": ([C.POINTER(vp)], C.c_int),
+        "masx_converter_destroy": ([vp], C.c_int),
+        "masx_convert_triplets": ([vp, C.c_int, C.c_int, vp, vp, vp, C.c_int, vp, vp, vp, ip, vp], C.c_int),
         "masx_autotune": ([vp, C.c_int, ip, ip, ip, dp], C.c_int),
         "masx_time_kernel": ([vp, C.c_int, C.c_int, dp], C.c_int),
         "masx_plan_info_get": ([vp, C.POINTER(PlanInfo)], C.c_int),
@@ -310,6 +313,40 @@ class Solver:
         c = C.c_longlong()
         check(self.L.masx_factor_status(self.h, C.byref(c)))
         return c.value
+
+
+class Converter:
+    """Triplets -> unique BCOO sorted by (row, col) on the GPU
+    (Converter::convert, linear_system/utils/converter.cu:137-284)."""
+
+    def __init__(self):
+        self.L = load()
+        h = C.c_void_p()
+        check(self.L.masx_converter_create(C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h is not None and self.h.value:
+                self.L.masx_converter_destroy(self.h)
+        except Exception:
+            pass
+
+    def convert(self, rows, cols, vals, n_block_rows, drop_lower=False, stream=None):
+        """rows, cols: int32 CUDA tensors; vals: (n, 9) float64 CUDA tensor.
+        Returns (rows, cols, vals) CUDA tensors of the unique blocks."""
+        import torch
+
+        n = int(rows.numel())
+        out_r = torch.empty(max(n, 1), dtype=torch.int32, device=rows.device)
+        out_c = torch.empty_like(out_r)
+        out_v = torch.empty((max(n, 1), 9), dtype=torch.float64, device=rows.device)
+        m = C.c_int()
+        check(self.L.masx_convert_triplets(self.h, n, n_block_rows, _dev(rows), _dev(cols), _dev(vals),
+                                           int(drop_lower), _dev(out_r), _dev(out_c), _dev(out_v), C.byref(m),
+                                           stream))
+        k = m.value
+        return out_r[:k], out_c[:k], out_v[:k]
 
 
 class MasPreconditioner:

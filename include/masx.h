@@ -182,6 +182,22 @@ int masx_hierarchy_host(int n_fem, int n_slots, int bank, int levels, const uint
                         int* out_total, int* level_count, int* level_begin, uint32_t* fine_mask, int* prefix,
                         int* going_next, long long going_next_cap, int* coarse_table);
 
+/* ---- triplet -> BCOO conversion ---------------------------------------
+ * Converter::convert (linear_system/utils/converter.cu:137-284): n triplets
+ * (device int32 rows, int32 cols, col-major Matrix3d values) become unique
+ * blocks sorted by (row, col) with duplicates summed in input order;
+ * drop_lower != 0 also removes blocks with row > col (Converter::ge2sym,
+ * :286-336).  Outputs are device arrays with capacity n; *n_unique receives
+ * the count (the call synchronizes `stream`, as the reference does to read
+ * h_unique_key_number).  Indices outside [0, n_block_rows) are an error.
+ * Scratch is kept in the converter object and grows as needed. */
+typedef struct masx_converter masx_converter;
+int masx_converter_create(masx_converter** out);
+int masx_converter_destroy(masx_converter* c);
+int masx_convert_triplets(masx_converter* c, int n, int n_block_rows, const int* d_rows, const int* d_cols,
+                          const double* d_vals, int drop_lower, int* d_out_rows, int* d_out_cols,
+                          double* d_out_vals, int* n_unique, void* stream);
+
 /* ---------------------------------------------------------------------------
  * MAS preconditioner object (TraditionalMAS32Preconditioner /
  * MASPreconditioner API, TraditionalMAS32Preconditioner.cuh:78-116), for a

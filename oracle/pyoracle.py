@@ -235,3 +235,32 @@ def pcg(block_rows, vals, rows, cols, b, *, global_diag=False, n_abd=0, abd_inv=
                   mas.h if mas is not None else None, fem_offset, tol, rule, max_iter, C.byref(st))
     return x, dict(iterations=st.iterations, failure=st.failure, rz0=st.rz0, rz_last=st.rz_last,
                    rec_rel_res=st.rec_rel_res, true_rel_res=st.true_rel_res)
+
+
+def convert_triplets(rows, cols, vals, drop_lower=False):
+    """Triplets -> unique blocks sorted by (row, col), duplicates summed in
+    input order (Converter::convert, linear_system/utils/converter.cu:137-284:
+    stable sort on (row << 32) + col, then a segmented sum); drop_lower keeps
+    row <= col only (Converter::ge2sym, :286-336).  Values: (n, 9) col-major.
+    Returns (rows, cols, vals) of the unique blocks."""
+    rows = np.asarray(rows, np.int64)
+    cols = np.asarray(cols, np.int64)
+    vals = np.asarray(vals, np.float64).reshape(-1, 9)
+    keep = np.ones(len(rows), bool) if not drop_lower else rows <= cols
+    idx = np.nonzero(keep)[0]
+    key = (rows[idx] << 32) + cols[idx]
+    order = idx[np.argsort(key, kind="stable")]
+    k = (rows[order] << 32) + cols[order]
+    if len(order) == 0:
+        return np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros((0, 9))
+    head = np.ones(len(order), bool)
+    head[1:] = k[1:] != k[:-1]
+    starts = np.nonzero(head)[0]
+    ends = np.append(starts[1:], len(order))
+    out = np.zeros((len(starts), 9))
+    for u, (s0, s1) in enumerate(zip(starts, ends)):
+        acc = np.zeros(9)
+        for t in order[s0:s1]:  # sequential, input order
+            acc = acc + vals[t]
+        out[u] = acc
+    return rows[order[starts]].astype(np.int32), cols[order[starts]].astype(np.int32), out

@@ -303,3 +303,67 @@ def test_frozen_dump_replay_on_gpu(tmp_path):
     x = torch.zeros(3 * r["block_rows"], dtype=torch.float64, device="cuda")
     st = s.solve(dev(r["rhs"]), x, rule=1, tol=1e-3, max_iter=512)[0]
     assert st["iterations"] == REFERENCE_ITERATIONS["mas32"], st
+
+
+@pytest.mark.parametrize("n,nr,drop_lower,long_row", [(5000, 50, False, False), (20000, 400, True, False),
+                                                      (3000, 30, False, True), (1, 1, False, False)])
+def test_converter_matches_oracle(oracle, n, nr, drop_lower, long_row):
+    """Triplets -> BCOO on the GPU equals the oracle bit for bit (same sort
+    order, duplicates summed in input order); a row of >1024 triplets takes
+    the global-scratch sort."""
+    rng = np.random.default_rng(n + nr)
+    r = rng.integers(0, nr, n).astype(np.int32)
+    c = rng.integers(0, nr, n).astype(np.int32)
+    if long_row:
+        r[: n // 2] = 7  # one row with 1500 triplets
+    v = rng.standard_normal((n, 9))
+    conv = native.Converter()
+    ur, uc, uv = conv.convert(torch.from_numpy(r).cuda(), torch.from_numpy(c).cuda(), dev(v), nr, drop_lower)
+    er, ec, ev = oracle.convert_triplets(r, c, v, drop_lower)
+    assert np.array_equal(host(ur), er) and np.array_equal(host(uc), ec)
+    assert np.array_equal(host(uv), ev)
+    # a second call with other sizes reuses (and grows) the scratch
+    ur2, _, _ = conv.convert(torch.from_numpy(r[:n // 3]).cuda(), torch.from_numpy(c[:n // 3]).cuda(),
+                             dev(v[:n // 3]), nr, drop_lower)
+    assert np.array_equal(host(ur2), oracle.convert_triplets(r[:n // 3], c[:n // 3], v[:n // 3], drop_lower)[0])
+
+
+def test_converter_rejects_bad_index():
+    conv = native.Converter()
+    r = torch.tensor([0, 5], dtype=torch.int32, device="cuda")
+    c = torch.tensor([0, 1], dtype=torch.int32, device="cuda")
+    with pytest.raises(RuntimeError):
+        conv.convert(r, c, dev(np.zeros((2, 9))), 3)
+
+
+def test_converter_feeds_the_solver(oracle):
+    """Per-element triplets of a FEM system (every block split in random
+    parts, shuffled) converted on the GPU solve like the assembled system."""
+    w = W.fem_grid(8, n_contacts=100, seed=3)
+    rng = np.random.default_rng(5)
+    parts = rng.integers(1, 4, len(w["rows"]))
+    idx = np.repeat(np.arange(len(w["rows"])), parts)
+    frac = rng.random(len(idx))
+    vals = w["vals"][idx] * frac[:, None]
+    # make the parts sum exactly to the block: the last part takes the rest
+    last = np.r_[np.nonzero(np.diff(idx))[0], len(idx) - 1]
+    sums = np.zeros_like(w["vals"])
+    np.add.at(sums, idx, vals)
+    vals[last] += w["vals"] - sums
+    perm = rng.permutation(len(idx))
+    r, c, v = w["rows"][idx][perm].astype(np.int32), w["cols"][idx][perm].astype(np.int32), vals[perm]
+    conv = native.Converter()
+    ur, uc, uv = conv.convert(torch.from_numpy(r).cuda(), torch.from_numpy(c).cuda(), dev(v), w["block_rows"])
+    er, ec, ev = oracle.convert_triplets(r, c, v)
+    assert np.array_equal(host(uv), ev)
+    s = native.Solver()
+    s.add_system(block_rows=w["block_rows"], rows=host(ur), cols=host(uc), use_mas=False, global_diag=True)
+    s.setup()
+    s.set_values(0, uv.contiguous())
+    s.build("fp32")
+    b = session.rhs_of(w)
+    x = torch.zeros(len(b), dtype=torch.float64, device="cuda")
+    st = s.solve(dev(b), x, rule=1, tol=1e-6)[0]
+    assert st["failure"] == 0 and st["rel_res_true"] <= 1e-6 * (1 + 1e-6)
+    xo, so = oracle.pcg(w["block_rows"], w["vals"], w["rows"], w["cols"], b, global_diag=True, tol=1e-6, rule=1)
+    assert np.linalg.norm(host(x) - xo) <= 1e-4 * np.linalg.norm(xo)

@@ -452,3 +452,39 @@ def test_pcg_diagonal_system_exact(oracle):
                            max_iter=10)
         assert st["iterations"] == 1 and st["failure"] == 0
         np.testing.assert_allclose(x, b / d, rtol=1e-14)
+
+
+def _random_triplets(rng, n, nr, dup=3):
+    r = rng.integers(0, nr, n // dup)
+    c = rng.integers(0, nr, n // dup)
+    pick = rng.integers(0, len(r), n)
+    return r[pick].astype(np.int32), c[pick].astype(np.int32), rng.standard_normal((n, 9))
+
+
+@pytest.mark.parametrize("drop_lower", [False, True])
+def test_converter_oracle_invariants(oracle, drop_lower):
+    """Pinning the converter restatement: the dense matrix of the unique
+    blocks equals the dense sum of the kept triplets; keys are strictly
+    increasing in (row, col); nothing kept is lost."""
+    rng = np.random.default_rng(11)
+    r, c, v = _random_triplets(rng, 3000, 40)
+    ur, uc, uv = oracle.convert_triplets(r, c, v, drop_lower)
+    key = ur.astype(np.int64) * 2**32 + uc
+    assert np.all(np.diff(key) > 0)
+    dense = np.zeros((120, 120))
+    want = np.zeros((120, 120))
+    for rr, cc, vv in zip(ur, uc, uv):
+        dense[3 * rr:3 * rr + 3, 3 * cc:3 * cc + 3] += vv.reshape(3, 3).T
+    for rr, cc, vv in zip(r, c, v):
+        if drop_lower and rr > cc:
+            continue
+        want[3 * rr:3 * rr + 3, 3 * cc:3 * cc + 3] += vv.reshape(3, 3).T
+    assert np.allclose(dense, want, rtol=0, atol=1e-12)
+    kept = {(a, b) for a, b in zip(r, c) if not (drop_lower and a > b)}
+    assert len(ur) == len(kept)
+    # a pair appearing once passes through untouched
+    once = [(i, a, b) for i, (a, b) in enumerate(zip(r, c)) if np.sum((r == a) & (c == b)) == 1
+            and not (drop_lower and a > b)]
+    for i, a, b in once[:20]:
+        j = np.nonzero((ur == a) & (uc == b))[0][0]
+        assert np.array_equal(uv[j], v[i])
