@@ -84,8 +84,12 @@ struct DevData
     const int*      comp_ptr;
     const uint32_t* comp_mask;
     const int*      comp_parent;
-    const int*      desc_ptr;    // level >= 2 coarse node -> level-1 descendants
-    const int*      desc_list;
+    const int*      deep_ptr;    // level >= 2 coarse node -> its update-group partial slots
+    const int*      deep_list;
+    const int*      bank_ugroup; // fine bank -> update group
+    const int4*     ugroup;      // {slot base, slot count, component begin, component end}
+    const int*      comp_dslot;  // 4 per fine-bank component: group-local slot per deep level, -1 none
+    float*          part_deep;   // float3 per slot, written by the update kernel
     const int*      row_anc;     // 8 per internal row: its level-1 node and that node's ancestors, -1 padded
     // ABD and diagonal block-Jacobi
     const float* abd_inv;   // 144 per body, row-major

@@ -124,6 +124,16 @@ __device__ __forceinline__ double partial_share(const double* part, const int* r
     return v;
 }
 
+// a[k] for a small register array without dynamic indexing (no scratch)
+template <int N>
+__device__ __forceinline__ int pick(const int (&a)[N], int k)
+{
+    int v = a[0];
+#pragma unroll
+    for(int j = 1; j < N; ++j) v = k == j ? a[j] : v;
+    return v;
+}
+
 // Workgroup order for row tiles: the hardware deals workgroups to the eight
 // XCDs in turn, so this gives each XCD (and its L2) a contiguous run of tiles.
 __device__ __forceinline__ int xcd_tile(int bid, int n)
@@ -508,7 +518,53 @@ __global__ void __launch_bounds__(256) k_store_x(DevData d, const int* ext_to_in
     for(int c = 0; c < 3; ++c) x[3 * (long long)i + c] = (double)d.x[3 * (long long)R + c];
 }
 
-__global__ void k_init_state(DevData d, int rule, double tol, const int* max_iter, int check_every, int frozen)
+// Zeroes the iteration vectors and loads b (FP64 and FP32 copies) in the
+// internal row order; padding rows get b = 0.
+__global__ void __launch_bounds__(256) k_solve_prepare(DevData d, const int* int_to_ext, const double* b)
+{
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if(row >= d.n_rows) return;
+    const int e = int_to_ext[row];
+    const long long o = 3 * (long long)row;
+#pragma unroll
+    for(int c = 0; c < 3; ++c)
+    {
+        const double v = e >= 0 ? b[3 * (long long)e + c] : 0.0;
+        d.b64[o + c] = v;
+        d.rt[o + c]  = (float)v;
+        d.x[o + c] = 0.f; d.r[o + c] = 0.f; d.z[o + c] = 0.f;
+        d.p0[o + c] = 0.f; d.p1[o + c] = 0.f; d.Ap[o + c] = 0.f;
+    }
+}
+
+// Number of systems not yet done, written to host-visible memory with a
+// sequence number (one workgroup, last node of every captured chunk):
+// host[0] = count, then host[1] = ++(*seq).
+__global__ void __launch_bounds__(256) k_export_status(DevData d, int* host, int* seq)
+{
+    __shared__ int s[256];
+    int n = 0;
+    for(int i = threadIdx.x; i < d.n_sys; i += blockDim.x) n += d.state[i].status != kDone;
+    s[threadIdx.x] = n;
+    __syncthreads();
+    for(int off = 128; off > 0; off >>= 1)
+    {
+        if((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+    }
+    if(threadIdx.x == 0)
+    {
+        const int q = *seq + 1;
+        *seq = q;
+        *(volatile int*)host = s[0];
+        __threadfence_system();
+        *(volatile int*)(host + 1) = q;
+        __threadfence_system();
+    }
+}
+
+__global__ void k_init_state(DevData d, int rule, double tol, const int* max_iter, int max_override, int check_every,
+                             int frozen)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if(s >= d.n_sys) return;
@@ -519,7 +575,7 @@ __global__ void k_init_state(DevData d, int rule, double tol, const int* max_ite
     st.status = kInit;
     st.failure = kOk;
     st.rule = rule;
-    st.max_iter = max_iter[s];
+    st.max_iter = max_override > 0 ? max_override : max_iter[s];
     st.k_check = 0;
     st.check_every = check_every;
     st.beta_zero = 1;
@@ -575,14 +631,33 @@ __global__ void __launch_bounds__(256) k_mas_local(DevData d, int force)
     __shared__ float sR[64 * 3];
     __shared__ float sZ[3][64 * 3];
     __shared__ float sD[CHUNK * 3];
-    __shared__ int   s_live[BPW], s_kind[BPW], s_vec[BPW];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int lb = lane / BANK, I = lane % BANK;
     const int bank0 = (int)(gridDim.x - 1 - blockIdx.x) * BPW;
     const int b  = bank0 + lb;
     const int bl = b < d.n_banks ? b : d.n_banks - 1;  // in-bounds address for idle lanes
+    // per-bank metadata: uniform over the workgroup (scalar loads)
+    int s_live[BPW], s_kind[BPW], s_vec[BPW];
+#pragma unroll
+    for(int k = 0; k < BPW; ++k)
+    {
+        const int bb = bank0 + k;
+        s_live[k] = s_kind[k] = s_vec[k] = 0;
+        if(bb < d.n_banks)
+        {
+            s_kind[k] = d.bank_kind[bb];
+            s_vec[k]  = d.bank_vec[bb];
+            s_live[k] = force || d.state[d.bank_sys[bb]].status == kRunning;
+        }
+    }
+    {
+        bool any = false;
+#pragma unroll
+        for(int k = 0; k < BPW; ++k) any |= s_live[k] != 0;
+        if(!any) return;  // systems already done: leave before streaming the inverse
+    }
     // this wave's inverse blocks: their addresses depend on the bank only, so
-    // every load is issued before anything else is known
+    // every load is issued before the residuals are known
     uint32_t raw[ND][WORDS];
 #pragma unroll
     for(int kk = 0; kk < ND; ++kk)
@@ -594,46 +669,28 @@ __global__ void __launch_bounds__(256) k_mas_local(DevData d, int force)
 #pragma unroll
         for(int c = 0; c < WORDS; ++c) raw[kk][c] = M[c * len];
     }
-    if(tid < BPW)
-    {
-        const int bb = bank0 + tid;
-        int live = 0, kind = 0, vec = 0;
-        if(bb < d.n_banks)
-        {
-            live = force || d.state[d.bank_sys[bb]].status == kRunning;
-            kind = d.bank_kind[bb];
-            vec  = d.bank_vec[bb];
-        }
-        s_live[tid] = live;
-        s_kind[tid] = kind;
-        s_vec[tid]  = vec;
-    }
-    __syncthreads();
-    bool any = false;
-#pragma unroll
-    for(int k = 0; k < BPW; ++k) any |= s_live[k] != 0;
-    if(!any) return;
     // residuals of level-0 and level-1 banks
     for(int i = tid; i < 64 * 3; i += blockDim.x)
     {
         const int k = i / (BANK * 3);
         float v = 0.f;
-        if(s_live[k] && s_kind[k] < 2)
-            v = (s_kind[k] == 0 ? d.r : d.Rc)[3 * (long long)s_vec[k] + i % (BANK * 3)];
+        if(pick(s_live, k) && pick(s_kind, k) < 2)
+            v = (pick(s_kind, k) == 0 ? d.r : d.Rc)[3 * (long long)pick(s_vec, k) + i % (BANK * 3)];
         sR[i] = v;
     }
-    // level >= 2: R(node) = sum of its level-1 descendants' residuals.  The
-    // lists of a bank's consecutive nodes are contiguous in desc_list; they
+    // level >= 2: R(node) = sum of the partials the update kernel formed for
+    // it, one per 256-row group holding some of its level-1 descendants.  The
+    // lists of a bank's consecutive nodes are contiguous in deep_list; they
     // pass through LDS in chunks with every load of a chunk in flight, then
     // each node's TPN threads sum its entries in list order.
 #pragma unroll 1
     for(int k = 0; k < BPW; ++k)
     {
-        if(!(s_live[k] && s_kind[k] == 2)) continue;  // uniform over the workgroup
-        const int c0 = s_vec[k];
-        const int lo = d.desc_ptr[c0], hi = d.desc_ptr[c0 + BANK];
+        if(!(pick(s_live, k) && pick(s_kind, k) == 2)) continue;  // uniform over the workgroup
+        const int c0 = pick(s_vec, k);
+        const int lo = d.deep_ptr[c0], hi = d.deep_ptr[c0 + BANK];
         const int node = tid / TPN, sub = tid % TPN;
-        const int nb = d.desc_ptr[c0 + node], ne = d.desc_ptr[c0 + node + 1];
+        const int nb = d.deep_ptr[c0 + node], ne = d.deep_ptr[c0 + node + 1];
         float a0 = 0.f, a1 = 0.f, a2 = 0.f;
 #pragma unroll 1
         for(int base = lo; base < hi; base += CHUNK)
@@ -645,7 +702,7 @@ __global__ void __launch_bounds__(256) k_mas_local(DevData d, int force)
             for(int j = 0; j < CHUNK / 256; ++j)
             {
                 const int e = tid + 256 * j;
-                src[j] = e < n ? d.desc_list[base + e] : -1;
+                src[j] = e < n ? d.deep_list[base + e] : -1;
             }
 #pragma unroll
             for(int j = 0; j < CHUNK / 256; ++j)
@@ -653,7 +710,7 @@ __global__ void __launch_bounds__(256) k_mas_local(DevData d, int force)
                 if(src[j] >= 0)
                 {
                     const long long c1 = 3 * (long long)src[j];
-                    v[j][0] = d.Rc[c1]; v[j][1] = d.Rc[c1 + 1]; v[j][2] = d.Rc[c1 + 2];
+                    v[j][0] = d.part_deep[c1]; v[j][1] = d.part_deep[c1 + 1]; v[j][2] = d.part_deep[c1 + 2];
                 }
             }
             __syncthreads();  // the previous chunk has been consumed
@@ -689,7 +746,7 @@ __global__ void __launch_bounds__(256) k_mas_local(DevData d, int force)
         }
     }
     __syncthreads();
-    const bool live = b < d.n_banks && s_live[lb];
+    const bool live = b < d.n_banks && pick(s_live, lb);
     const float r0 = sR[3 * lane], r1 = sR[3 * lane + 1], r2 = sR[3 * lane + 2];
     float z0 = 0.f, z1 = 0.f, z2 = 0.f;
 #pragma unroll
@@ -747,7 +804,7 @@ __global__ void __launch_bounds__(256) k_mas_local(DevData d, int force)
             const float sc = d.minv_scale[b];
             z0 *= sc; z1 *= sc; z2 *= sc;
         }
-        float* dst = (s_kind[lb] == 0 ? d.z : d.Zc) + 3 * (long long)(s_vec[lb] + I);
+        float* dst = (pick(s_kind, lb) == 0 ? d.z : d.Zc) + 3 * (long long)(pick(s_vec, lb) + I);
         dst[0] = z0; dst[1] = z1; dst[2] = z2;
     }
 }
@@ -870,14 +927,85 @@ __global__ void __launch_bounds__(256) k_precond_finish(DevData d, int force)
 }
 
 // beta from the r.z partials, then Ap = A p_new with p_new = z + beta p_old
-// formed on the fly (the p update is fused here); partial p.Ap.  G lanes per
-// block row, P row passes per workgroup; the first E entries of every lane
-// (matrix values, column indices and the z / old-p gathers) of all passes are
-// loaded before the r.z partials are reduced, longer rows finish in a loop.
-template <int G, int P, int E>
+// formed on the fly (the p update is fused here); partial p.Ap.
+//
+// Row-aligned tiles: a workgroup owns whole rows [r0, r1) and their entries
+// [e0, e1) of the full block CSR.  Thread t takes entries e0 + t + 256 j
+// (j < U), so the column indices, the nine value planes and the z / old-p
+// gathers of a chunk are contiguous and all in flight before the r.z
+// partials are reduced.  Each entry's 3-vector product is staged in LDS and
+// thread t then sums row r0 + t's products in entry order.
+template <int U>
+struct SpmvChunk
+{
+    int   cix[U];
+    float v[U][9];
+    float g[U][6];
+};
+
+template <int U>
+__device__ __forceinline__ void spmv_load_chunk(const DevData& d, const float* __restrict__ p_old, int base, int n,
+                                                SpmvChunk<U>& c)
+{
+#pragma unroll
+    for(int j = 0; j < U; ++j)
+    {
+        const int  k  = (int)threadIdx.x + 256 * j;
+        const bool ok = k < n;
+        const int  e  = base + (ok ? k : 0);
+        c.cix[j] = ok ? d.col[e] : -1;
+#pragma unroll
+        for(int q = 0; q < 9; ++q) c.v[j][q] = d.vals32[q * d.plane + e];
+    }
+#pragma unroll
+    for(int j = 0; j < U; ++j)
+    {
+        const long long x = 3 * (long long)(c.cix[j] >= 0 ? c.cix[j] : 0);
+        c.g[j][0] = d.z[x]; c.g[j][1] = d.z[x + 1]; c.g[j][2] = d.z[x + 2];
+        c.g[j][3] = p_old[x]; c.g[j][4] = p_old[x + 1]; c.g[j][5] = p_old[x + 2];
+    }
+}
+
+// products of a loaded chunk -> LDS, then the owning rows add them in order
+template <int U>
+__device__ __forceinline__ void spmv_reduce_chunk(const SpmvChunk<U>& c, float beta, int base, int n, bool has_row,
+                                                  int rb, int re, float* sP, float acc[3])
+{
+#pragma unroll
+    for(int j = 0; j < U; ++j)
+    {
+        const int k = (int)threadIdx.x + 256 * j;
+        if(k < n)
+        {
+            const float x0 = c.g[j][0] + beta * c.g[j][3];
+            const float x1 = c.g[j][1] + beta * c.g[j][4];
+            const float x2 = c.g[j][2] + beta * c.g[j][5];
+            const float* v = c.v[j];
+            sP[3 * k]     = v[0] * x0 + v[1] * x1 + v[2] * x2;
+            sP[3 * k + 1] = v[3] * x0 + v[4] * x1 + v[5] * x2;
+            sP[3 * k + 2] = v[6] * x0 + v[7] * x1 + v[8] * x2;
+        }
+    }
+    __syncthreads();
+    if(has_row)
+    {
+        const int q0 = max(rb, base), q1 = min(re, base + n);
+        for(int q = q0; q < q1; ++q)
+        {
+            acc[0] += sP[3 * (q - base)];
+            acc[1] += sP[3 * (q - base) + 1];
+            acc[2] += sP[3 * (q - base) + 2];
+        }
+    }
+    __syncthreads();
+}
+
+template <int U>
 __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __restrict__ p_old,
                                                     float* __restrict__ p_new)
 {
+    constexpr int CH = 256 * U;
+    __shared__ float  sP[CH * 3];
     __shared__ double smem[8];
     const int  wg = xcd_tile(blockIdx.x, gridDim.x);
     const int4 w  = d.wg_spmv[wg];
@@ -885,55 +1013,23 @@ __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __re
     SysState&  st = d.state[s];
     if(st.status != kRunning) return;
     const double share = partial_share(d.part_pre, d.wg_pre_range, s);
-    constexpr int RPP = 256 / G;  // rows per pass
-    const int sub = threadIdx.x % G;
-    int rows[P], eb[P], ee[P];
-#pragma unroll
-    for(int q = 0; q < P; ++q)
+    const int r0 = w.z, r1 = w.w;
+    const int e0 = d.row_ptr[r0], e1 = d.row_ptr[r1];
+    const int row = r0 + (int)threadIdx.x;
+    const bool has_row = row < r1;
+    int rb = 0, re = 0;
+    float own[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if(has_row)
     {
-        rows[q] = w.z + q * RPP + (int)threadIdx.x / G;
-        const int rc = rows[q] < w.w ? rows[q] : w.z;
-        eb[q] = d.row_ptr[rc];
-        ee[q] = rows[q] < w.w ? d.row_ptr[rc + 1] : eb[q];
+        rb = d.row_ptr[row];
+        re = d.row_ptr[row + 1];
+        const long long o = 3 * (long long)row;
+        own[0] = d.z[o]; own[1] = d.z[o + 1]; own[2] = d.z[o + 2];
+        own[3] = p_old[o]; own[4] = p_old[o + 1]; own[5] = p_old[o + 2];
     }
-    float own[P][6];
-#pragma unroll
-    for(int q = 0; q < P; ++q)
-    {
-        const long long oc = 3 * (long long)(rows[q] < w.w ? rows[q] : w.z);
-        own[q][0] = d.z[oc]; own[q][1] = d.z[oc + 1]; own[q][2] = d.z[oc + 2];
-        own[q][3] = p_old[oc]; own[q][4] = p_old[oc + 1]; own[q][5] = p_old[oc + 2];
-    }
-    int   cix[P][E];
-    float val[P][E][9];
-    float gx[P][E][6];
-#pragma unroll
-    for(int q = 0; q < P; ++q)
-#pragma unroll
-        for(int j = 0; j < E; ++j)
-        {
-            const int e = eb[q] + sub + j * G;
-            cix[q][j] = e < ee[q] ? d.col[e] : -1;
-        }
-#pragma unroll
-    for(int q = 0; q < P; ++q)
-#pragma unroll
-        for(int j = 0; j < E; ++j)
-        {
-            const int e  = eb[q] + sub + j * G;
-            const int ec = e < ee[q] ? e : 0;
-#pragma unroll
-            for(int c = 0; c < 9; ++c) val[q][j][c] = d.vals32[c * d.plane + ec];
-        }
-#pragma unroll
-    for(int q = 0; q < P; ++q)
-#pragma unroll
-        for(int j = 0; j < E; ++j)
-        {
-            const long long c = 3 * (long long)(cix[q][j] >= 0 ? cix[q][j] : 0);
-            gx[q][j][0] = d.z[c]; gx[q][j][1] = d.z[c + 1]; gx[q][j][2] = d.z[c + 2];
-            gx[q][j][3] = p_old[c]; gx[q][j][4] = p_old[c + 1]; gx[q][j][5] = p_old[c + 2];
-        }
+    SpmvChunk<U> c;
+    const int n0 = min(CH, e1 - e0);
+    if(n0 > 0) spmv_load_chunk<U>(d, p_old, e0, n0, c);
     const double rz = block_sum(share, smem);
     const bool bad = !finite_d(rz) || (st.rule == kRuleTrueRel && rz <= 0.0);
     if(wg == d.wg_spmv_range[2 * s] && threadIdx.x == 0)
@@ -949,51 +1045,26 @@ __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __re
     }
     if(bad && !st.frozen) return;
     const float beta = st.beta_zero ? 0.f : (float)(rz / st.rz_prev);
-    double part = 0.0;
-#pragma unroll
-    for(int q = 0; q < P; ++q)
+    float acc[3] = {0.f, 0.f, 0.f};
+    if(n0 > 0) spmv_reduce_chunk<U>(c, beta, e0, n0, has_row, rb, re, sP, acc);
+    // tiles holding a row longer than one chunk
+#pragma unroll 1
+    for(int base = e0 + CH; base < e1; base += CH)
     {
-        float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-#pragma unroll
-        for(int j = 0; j < E; ++j)
-        {
-            const bool ok = cix[q][j] >= 0;
-            const float x0 = ok ? gx[q][j][0] + beta * gx[q][j][3] : 0.f;
-            const float x1 = ok ? gx[q][j][1] + beta * gx[q][j][4] : 0.f;
-            const float x2 = ok ? gx[q][j][2] + beta * gx[q][j][5] : 0.f;
-            const float* v = val[q][j];
-            a0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
-            a1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
-            a2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
-        }
-        for(int e = eb[q] + sub + E * G; e < ee[q]; e += G)
-        {
-            const long long c = 3 * (long long)d.col[e];
-            const float x0 = d.z[c] + beta * p_old[c];
-            const float x1 = d.z[c + 1] + beta * p_old[c + 1];
-            const float x2 = d.z[c + 2] + beta * p_old[c + 2];
-            const float* v = d.vals32 + e;
-            a0 += v[0] * x0 + v[d.plane] * x1 + v[2 * d.plane] * x2;
-            a1 += v[3 * d.plane] * x0 + v[4 * d.plane] * x1 + v[5 * d.plane] * x2;
-            a2 += v[6 * d.plane] * x0 + v[7 * d.plane] * x1 + v[8 * d.plane] * x2;
-        }
-#pragma unroll
-        for(int o = G / 2; o > 0; o >>= 1)
-        {
-            a0 += __shfl_xor(a0, o, G);
-            a1 += __shfl_xor(a1, o, G);
-            a2 += __shfl_xor(a2, o, G);
-        }
-        if(sub == 0 && rows[q] < w.w)
-        {
-            const long long o = 3 * (long long)rows[q];
-            const float y0 = own[q][0] + beta * own[q][3];
-            const float y1 = own[q][1] + beta * own[q][4];
-            const float y2 = own[q][2] + beta * own[q][5];
-            p_new[o] = y0; p_new[o + 1] = y1; p_new[o + 2] = y2;
-            d.Ap[o] = a0; d.Ap[o + 1] = a1; d.Ap[o + 2] = a2;
-            part += (double)y0 * a0 + (double)y1 * a1 + (double)y2 * a2;
-        }
+        const int n = min(CH, e1 - base);
+        spmv_load_chunk<U>(d, p_old, base, n, c);
+        spmv_reduce_chunk<U>(c, beta, base, n, has_row, rb, re, sP, acc);
+    }
+    double part = 0.0;
+    if(has_row)
+    {
+        const long long o = 3 * (long long)row;
+        const float y0 = own[0] + beta * own[3];
+        const float y1 = own[1] + beta * own[4];
+        const float y2 = own[2] + beta * own[5];
+        p_new[o] = y0; p_new[o + 1] = y1; p_new[o + 2] = y2;
+        d.Ap[o] = acc[0]; d.Ap[o + 1] = acc[1]; d.Ap[o + 2] = acc[2];
+        part = (double)y0 * acc[0] + (double)y1 * acc[1] + (double)y2 * acc[2];
     }
     part = block_sum(part, smem);
     if(threadIdx.x == 0) d.part_spmv[wg] = part;
@@ -1056,6 +1127,18 @@ __global__ void __launch_bounds__(256) k_update(DevData d, const float* __restri
         cb = d.comp_ptr[gb];
         ce = d.comp_ptr[gb + 1];
     }
+    // lane j of a bank holds the bank's component cb + j (at most one per lane)
+    const int blane = threadIdx.x & (d.bank - 1);
+    const bool has_comp = restrict_rows && cb + blane < ce;
+    uint32_t my_mask = 0;
+    int      my_parent = 0;
+    int4     my_ds = make_int4(-1, -1, -1, -1);
+    if(has_comp)
+    {
+        my_mask   = d.comp_mask[cb + blane];
+        my_parent = d.comp_parent[cb + blane];
+        my_ds     = reinterpret_cast<const int4*>(d.comp_dslot)[cb + blane];
+    }
     float alpha = 0.f;
     if(mode == 0)
     {
@@ -1104,12 +1187,19 @@ __global__ void __launch_bounds__(256) k_update(DevData d, const float* __restri
             part = (double)r0 * r0 + (double)r1 * r1 + (double)r2 * r2;
     }
     // level-1 restriction: sum the residual rows of each component of the bank
+    __shared__ float sC[256 * 3];
+    __shared__ int4  sDs[256];
+    int4 grp = make_int4(0, 0, 0, 0);
+    if(type == kWgFemMas) grp = d.ugroup[d.bank_ugroup[aux]];  // uniform over the workgroup
     if(restrict_rows)
     {
-        const int lane = threadIdx.x & (d.bank - 1);
+        const int lane  = blane;
+        const int gbase = (int)(threadIdx.x & 63) & ~(d.bank - 1);  // the bank's first lane in the wave
         for(int c = cb; c < ce; ++c)
         {
-            const bool in = (d.comp_mask[c] >> lane) & 1u;
+            const uint32_t mask   = (uint32_t)__shfl((int)my_mask, gbase + (c - cb), 64);
+            const int      parent = __shfl(my_parent, gbase + (c - cb), 64);
+            const bool in = (mask >> lane) & 1u;
             float sx = in ? r0 : 0.f, sy = in ? r1 : 0.f, sz = in ? r2 : 0.f;
             for(int off = d.bank >> 1; off > 0; off >>= 1)
             {
@@ -1119,9 +1209,33 @@ __global__ void __launch_bounds__(256) k_update(DevData d, const float* __restri
             }
             if(lane == 0)
             {
-                const long long pc = 3 * (long long)d.comp_parent[c];
+                const long long pc = 3 * (long long)parent;
                 d.Rc[pc] = sx; d.Rc[pc + 1] = sy; d.Rc[pc + 2] = sz;
+                const int j = c - grp.z;
+                sC[3 * j] = sx; sC[3 * j + 1] = sy; sC[3 * j + 2] = sz;
             }
+        }
+        if(has_comp && grp.y > 0) sDs[cb + blane - grp.z] = my_ds;
+    }
+    // one partial per deeper ancestor of the group's components (levels >= 2),
+    // summed over the components in order
+    if(grp.y > 0)
+    {
+        __syncthreads();
+        for(int t = threadIdx.x; t < grp.y; t += blockDim.x)
+        {
+            float ax = 0.f, ay = 0.f, az = 0.f;
+            for(int c = grp.z; c < grp.w; ++c)
+            {
+                const int4 ds = sDs[c - grp.z];
+                if(ds.x == t || ds.y == t || ds.z == t || ds.w == t)
+                {
+                    const int j = c - grp.z;
+                    ax += sC[3 * j]; ay += sC[3 * j + 1]; az += sC[3 * j + 2];
+                }
+            }
+            const long long o = 3 * (long long)(grp.x + t);
+            d.part_deep[o] = ax; d.part_deep[o + 1] = ay; d.part_deep[o + 2] = az;
         }
     }
     part = block_sum(part, smem);
@@ -1378,11 +1492,25 @@ hipError_t launch_store_x(const DevData& d, const int* e2i, double* x, int n_ext
     return hipSuccess;
 }
 
-hipError_t launch_init_state(const DevData& d, int rule, double tol, const int* max_iter, int check_every, int frozen,
-                             hipStream_t st)
+hipError_t launch_init_state(const DevData& d, int rule, double tol, const int* max_iter, int max_override,
+                             int check_every, int frozen, hipStream_t st)
 {
     hipLaunchKernelGGL(k_init_state, dim3(blocks_for(d.n_sys, 64)), dim3(64), 0, st, d, rule, tol, max_iter,
-                       check_every, frozen);
+                       max_override, check_every, frozen);
+    MASX_CHECK_LAUNCH();
+    return hipSuccess;
+}
+
+hipError_t launch_solve_prepare(const DevData& d, const int* int_to_ext, const double* b, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_solve_prepare, dim3(blocks_for(d.n_rows, 256)), dim3(256), 0, st, d, int_to_ext, b);
+    MASX_CHECK_LAUNCH();
+    return hipSuccess;
+}
+
+hipError_t launch_export_status(const DevData& d, int* host, int* seq, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_export_status, dim3(1), dim3(256), 0, st, d, host, seq);
     MASX_CHECK_LAUNCH();
     return hipSuccess;
 }
@@ -1427,47 +1555,22 @@ hipError_t launch_precond_finish(const DevData& d, int n_wg, int force, hipStrea
     return hipSuccess;
 }
 
-bool spmv_tiling_supported(int lanes_per_row, int passes, int entries_per_lane)
+bool spmv_tiling_supported(int entries_per_thread)
 {
-    switch(lanes_per_row * 256 + passes * 16 + entries_per_lane)
-    {
-        case 4 * 256 + 4 * 16 + 4:
-        case 8 * 256 + 2 * 16 + 2:
-        case 8 * 256 + 4 * 16 + 2:
-        case 8 * 256 + 2 * 16 + 3:
-        case 16 * 256 + 1 * 16 + 1:
-        case 16 * 256 + 2 * 16 + 1:
-        case 16 * 256 + 4 * 16 + 1:
-        case 16 * 256 + 1 * 16 + 2:
-        case 16 * 256 + 2 * 16 + 2:
-        case 32 * 256 + 2 * 16 + 1:
-        case 32 * 256 + 4 * 16 + 1: return true;
-        default: return false;
-    }
+    return entries_per_thread == 2 || entries_per_thread == 4 || entries_per_thread == 8;
 }
 
-hipError_t launch_spmv_fused(const DevData& d, int n_wg, int lanes_per_row, int passes, int entries_per_lane,
-                             const float* p_old, float* p_new, hipStream_t st)
+hipError_t launch_spmv_fused(const DevData& d, int n_wg, int entries_per_thread, const float* p_old, float* p_new,
+                             hipStream_t st)
 {
     if(n_wg <= 0) return hipSuccess;
-#define MASX_SPMV(G, P, E) \
-    hipLaunchKernelGGL((k_spmv_fused<G, P, E>), dim3(n_wg), dim3(256), 0, st, d, p_old, p_new)
-    switch(lanes_per_row * 256 + passes * 16 + entries_per_lane)
+    switch(entries_per_thread)
     {
-        case 4 * 256 + 4 * 16 + 4: MASX_SPMV(4, 4, 4); break;
-        case 8 * 256 + 2 * 16 + 2: MASX_SPMV(8, 2, 2); break;
-        case 8 * 256 + 4 * 16 + 2: MASX_SPMV(8, 4, 2); break;
-        case 8 * 256 + 2 * 16 + 3: MASX_SPMV(8, 2, 3); break;
-        case 16 * 256 + 1 * 16 + 1: MASX_SPMV(16, 1, 1); break;
-        case 16 * 256 + 2 * 16 + 1: MASX_SPMV(16, 2, 1); break;
-        case 16 * 256 + 4 * 16 + 1: MASX_SPMV(16, 4, 1); break;
-        case 16 * 256 + 1 * 16 + 2: MASX_SPMV(16, 1, 2); break;
-        case 16 * 256 + 2 * 16 + 2: MASX_SPMV(16, 2, 2); break;
-        case 32 * 256 + 2 * 16 + 1: MASX_SPMV(32, 2, 1); break;
-        case 32 * 256 + 4 * 16 + 1: MASX_SPMV(32, 4, 1); break;
+        case 2: hipLaunchKernelGGL(k_spmv_fused<2>, dim3(n_wg), dim3(256), 0, st, d, p_old, p_new); break;
+        case 4: hipLaunchKernelGGL(k_spmv_fused<4>, dim3(n_wg), dim3(256), 0, st, d, p_old, p_new); break;
+        case 8: hipLaunchKernelGGL(k_spmv_fused<8>, dim3(n_wg), dim3(256), 0, st, d, p_old, p_new); break;
         default: return hipErrorInvalidValue;
     }
-#undef MASX_SPMV
     MASX_CHECK_LAUNCH();
     return hipSuccess;
 }

@@ -6,7 +6,7 @@ namespace masx
 {
 // bytes of one bank inverse in the diagonal-major layout of `precision`
 size_t mas_bank_bytes(int bank, int precision);
-bool   spmv_tiling_supported(int lanes_per_row, int passes, int entries_per_lane);
+bool   spmv_tiling_supported(int entries_per_thread);
 
 hipError_t launch_pack_values(const DevData& d, hipStream_t st);
 // lanes per coarse entry: 16 or 64
@@ -18,13 +18,17 @@ hipError_t launch_abd_factor(const DevData& d, const int* body_src, float* inv, 
 hipError_t launch_diag_factor(const DevData& d, const int* diag_src, const int* needs, float* inv, hipStream_t st);
 hipError_t launch_load_rhs(const DevData& d, const int* e2i, const double* b, int n_ext, hipStream_t st);
 hipError_t launch_store_x(const DevData& d, const int* e2i, double* x, int n_ext, hipStream_t st);
-hipError_t launch_init_state(const DevData& d, int rule, double tol, const int* max_iter, int check_every, int frozen,
-                             hipStream_t st);
+hipError_t launch_init_state(const DevData& d, int rule, double tol, const int* max_iter, int max_override,
+                             int check_every, int frozen, hipStream_t st);
+hipError_t launch_solve_prepare(const DevData& d, const int* int_to_ext, const double* b, hipStream_t st);
+// host[0] = systems not done, then host[1] = ++*seq (device counter)
+hipError_t launch_export_status(const DevData& d, int* host, int* seq, hipStream_t st);
 hipError_t launch_phase_finish(const DevData& d, hipStream_t st);
 hipError_t launch_mas_local(const DevData& d, int precision, int force, hipStream_t st);
 hipError_t launch_precond_finish(const DevData& d, int n_wg, int force, hipStream_t st);
-hipError_t launch_spmv_fused(const DevData& d, int n_wg, int lanes_per_row, int passes, int entries_per_lane,
-                             const float* p_old, float* p_new, hipStream_t st);
+// row-aligned tiles of at most 256 rows and 256 * entries_per_thread entries (2, 4 or 8)
+hipError_t launch_spmv_fused(const DevData& d, int n_wg, int entries_per_thread, const float* p_old, float* p_new,
+                             hipStream_t st);
 hipError_t launch_update(const DevData& d, int n_wg, const float* p, int mode, hipStream_t st);
 // true residual of the checked (or, report_only, all) systems and its decision
 hipError_t launch_true_residual(const DevData& d, int n_wg, int report_only, hipStream_t st);

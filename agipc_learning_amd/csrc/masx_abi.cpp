@@ -277,23 +277,20 @@ int masx_apply_preconditioner(masx_solver* s, const double* d_r, double* d_z, vo
     });
 }
 
-int masx_set_spmv_tiling(masx_solver* s, int lanes_per_row, int rows_per_wg, int entries_per_lane)
+int masx_set_spmv_tiling(masx_solver* s, int entries_per_thread)
 {
     return guard([&] {
         require(s != nullptr, "null solver");
-        s->impl.set_spmv_tiling(lanes_per_row, rows_per_wg, entries_per_lane);
+        s->impl.set_spmv_tiling(entries_per_thread);
     });
 }
 
-int masx_autotune(masx_solver* s, int reps, int* lanes_per_row, int* rows_per_wg, int* entries_per_lane,
-                  double* ms)
+int masx_autotune(masx_solver* s, int reps, int* entries_per_thread, double* ms)
 {
     return guard([&] {
         require(s != nullptr && reps > 0, "bad argument");
         const auto t = s->impl.autotune_spmv(reps);
-        if(lanes_per_row) *lanes_per_row = t.lanes;
-        if(rows_per_wg) *rows_per_wg = t.rows_per_wg;
-        if(entries_per_lane) *entries_per_lane = t.entries;
+        if(entries_per_thread) *entries_per_thread = t.entries;
         if(ms) *ms = t.ms;
     });
 }

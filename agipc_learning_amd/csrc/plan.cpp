@@ -588,6 +588,71 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
         P.sys_level_ptr[s + 1] = (int)P.sys_level_banks.size() / 2;
     }
 
+    // Residuals of levels >= 2, pre-reduced per update group: the update
+    // kernel treats 256 rows (256/bank fine banks) of one system per workgroup
+    // and sums the level-1 component residuals of those banks.  Each group
+    // also emits one partial per distinct deeper ancestor of its components
+    // (slots), so a level >= 2 node gathers its groups' partials instead of all
+    // its level-1 descendants.
+    {
+        const int per_group = 256 / bank;
+        P.bank_ugroup.assign(std::max(n_fine_banks, 1), -1);
+        P.comp_dslot.assign(4 * (size_t)std::max(P.comp_ptr[n_fine_banks], 1), -1);
+        P.ugroup.clear();
+        std::vector<std::vector<int>> deep(n_coarse);
+        int slots = 0;
+        std::vector<long long> covered(n_coarse, 0);
+        for(int s = 0; s < S; ++s)
+        {
+            const auto& d = sys[s];
+            if(!d.use_mas) continue;
+            const int fb0 = fine_bank_base[s], nfb = d.hier.n_slots / bank;
+            for(int q = 0; q < nfb; q += per_group)
+            {
+                const int g = (int)P.ugroup.size() / 4;
+                const int b0 = fb0 + q, b1 = fb0 + std::min(nfb, q + per_group);
+                std::vector<int> nodes;  // distinct deep ancestors, first appearance order
+                for(int b = b0; b < b1; ++b)
+                {
+                    P.bank_ugroup[b] = g;
+                    for(int c = P.comp_ptr[b]; c < P.comp_ptr[b + 1]; ++c)
+                    {
+                        const int g1 = P.comp_parent[c];
+                        for(int k = 1; k < 8; ++k)
+                        {
+                            const int a = P.anc[8 * (size_t)g1 + k];
+                            if(a < 0) break;
+                            if(k > 4) throw std::invalid_argument("more than 6 MAS levels");
+                            int slot = -1;
+                            for(size_t t = 0; t < nodes.size(); ++t)
+                                if(nodes[t] == a) slot = (int)t;
+                            if(slot < 0)
+                            {
+                                slot = (int)nodes.size();
+                                nodes.push_back(a);
+                            }
+                            P.comp_dslot[4 * (size_t)c + k - 1] = slot;
+                            ++covered[a];
+                        }
+                    }
+                }
+                if(P.comp_ptr[b1] - P.comp_ptr[b0] > 256) throw std::logic_error("update group with > 256 components");
+                P.ugroup.insert(P.ugroup.end(), {slots, (int)nodes.size(), P.comp_ptr[b0], P.comp_ptr[b1]});
+                for(size_t t = 0; t < nodes.size(); ++t) deep[nodes[t]].push_back(slots + (int)t);
+                slots += (int)nodes.size();
+            }
+        }
+        P.n_deep_slots = slots;
+        P.deep_ptr.assign(n_coarse + 1, 0);
+        for(int c = 0; c < n_coarse; ++c) P.deep_ptr[c + 1] = P.deep_ptr[c] + (int)deep[c].size();
+        P.deep_list.clear();
+        for(int c = 0; c < n_coarse; ++c) P.deep_list.insert(P.deep_list.end(), deep[c].begin(), deep[c].end());
+        // self-check: every level-1 descendant of a deep node is counted once
+        for(int c = 0; c < n_coarse; ++c)
+            if(covered[c] != P.desc_ptr[c + 1] - P.desc_ptr[c])
+                throw std::logic_error("update-group partials do not cover the level-1 descendants");
+    }
+
     // Galerkin coarse operators A_l = P_l^T A P_l (all entries, not only in-bank),
     // built level by level from the previous level; group g holds level g+1 of
     // every system.
@@ -732,6 +797,23 @@ void validate_plan(const Plan& P)
             if(P.bank_ent_src[k] < 0 || P.bank_ent_src[k] >= (fine ? nb : P.n_gal)) fail("bank_ent_src");
         }
     }
+    if((int)P.deep_ptr.size() != P.n_coarse + 1 || P.deep_ptr.back() != (int)P.deep_list.size()) fail("deep_ptr");
+    for(int c = 0; c < P.n_coarse; ++c)
+        if(P.deep_ptr[c + 1] < P.deep_ptr[c]) fail("deep_ptr order");
+    for(int v : P.deep_list)
+        if(v < 0 || v >= P.n_deep_slots) fail("deep_list");
+    for(size_t g = 0; g < P.ugroup.size() / 4; ++g)
+    {
+        const int* u = &P.ugroup[4 * g];
+        if(u[0] < 0 || u[0] + u[1] > P.n_deep_slots || u[2] < 0 || u[3] < u[2] || u[3] - u[2] > 256 ||
+           u[3] > P.comp_ptr[P.n_fine_banks])
+            fail("ugroup");
+        for(int c = u[2]; c < u[3]; ++c)
+            for(int k = 0; k < 4; ++k)
+                if(P.comp_dslot[4 * (size_t)c + k] < -1 || P.comp_dslot[4 * (size_t)c + k] >= u[1]) fail("comp_dslot");
+    }
+    for(int b = 0; b < P.n_fine_banks; ++b)
+        if(P.bank_ugroup[b] < -1 || P.bank_ugroup[b] >= (int)P.ugroup.size() / 4) fail("bank_ugroup");
     if((int)P.desc_ptr.size() != P.n_coarse + 1) fail("desc_ptr");
     for(int c = 0; c < P.n_coarse; ++c)
         if(P.desc_ptr[c + 1] < P.desc_ptr[c]) fail("desc_ptr order");

@@ -107,6 +107,14 @@ struct Plan
     std::vector<int> anc;            // per global coarse node, 8 ints: itself and its ancestors, -1 padded
     std::vector<int> row_anc;        // per internal row, 8 ints: its level-1 node and that node's
                                      // ancestors (global coarse indices), -1 padded
+    // level >= 2 residuals pre-reduced per update group (256 rows of one system)
+    std::vector<int> bank_ugroup;    // per fine bank: its update group (-1 if none)
+    std::vector<int> ugroup;         // per group, 4 ints: slot base, slot count, component begin, end
+    std::vector<int> comp_dslot;     // per fine-bank component, 4 ints: group-local slot of its ancestor
+                                     // at levels 2..5, -1 if none
+    std::vector<int> deep_ptr;       // per global coarse node: range of its partial slots in deep_list
+    std::vector<int> deep_list;
+    int n_deep_slots = 0;
     // restriction schedule for levels >= 2 (per system, coarse banks grouped by level)
     std::vector<int> sys_level_banks;   // flattened: per system, per level l>=1, [bank begin, end)
     std::vector<int> sys_level_ptr;     // n_sys+1 offsets into sys_level_banks (pairs)

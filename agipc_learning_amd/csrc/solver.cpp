@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
+#include <thread>
 
 #include "kernels.hpp"
 
@@ -67,6 +68,7 @@ Solver::~Solver()
 {
     if(graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if(h_state_) (void)hipHostFree(h_state_);
+    if(h_export_) (void)hipHostFree(h_export_);
     if(ev_a_) (void)hipEventDestroy(ev_a_);
     if(ev_b_) (void)hipEventDestroy(ev_b_);
     for(auto e : chunk_ev_)
@@ -103,19 +105,10 @@ int Solver::add_system(const SystemDesc& d)
     return (int)systems_.size() - 1;
 }
 
-void Solver::set_spmv_tiling(int lanes_per_row, int rows_per_wg, int entries_per_lane)
+void Solver::set_spmv_tiling(int entries_per_thread)
 {
-    if(lanes_per_row != 4 && lanes_per_row != 8 && lanes_per_row != 16 && lanes_per_row != 32)
-        throw std::invalid_argument("lanes per row must be 4, 8, 16 or 32");
-    if(rows_per_wg < 256 / lanes_per_row || rows_per_wg % (256 / lanes_per_row))
-        throw std::invalid_argument("rows per workgroup must be a multiple of 256 / lanes per row");
-    const int passes = rows_per_wg / (256 / lanes_per_row);
-    if(!spmv_tiling_supported(lanes_per_row, passes, entries_per_lane))
-        throw std::invalid_argument("unsupported SpMV tiling");
-    lanes_       = lanes_per_row;
-    passes_      = passes;
-    entries_     = entries_per_lane;
-    rows_per_wg_ = rows_per_wg;
+    if(!spmv_tiling_supported(entries_per_thread)) throw std::invalid_argument("SpMV entries per thread must be 2, 4 or 8");
+    entries_ = entries_per_thread;
     if(setup_done_)
     {
         build_work_tables();
@@ -128,16 +121,14 @@ void Solver::set_spmv_tiling(int lanes_per_row, int rows_per_wg, int entries_per
 Solver::Tiling Solver::autotune_spmv(int reps)
 {
     if(!built_) throw std::runtime_error("build() must precede autotune");
-    static const int kCandidates[][3] = {{16, 16, 1}, {16, 32, 1}, {16, 64, 1}, {16, 16, 2}, {16, 32, 2}, {8, 64, 2},
-                                         {8, 128, 2}, {8, 64, 3}, {4, 256, 4}, {32, 16, 1}, {32, 32, 1}};
-    Tiling best{lanes_, rows_per_wg_, entries_, 1e30};
-    for(const auto& c : kCandidates)
+    Tiling best{entries_, 1e30};
+    for(const int u : {2, 4, 8})
     {
-        set_spmv_tiling(c[0], c[1], c[2]);
+        set_spmv_tiling(u);
         const double ms = time_kernel(2, reps);
-        if(ms < best.ms) best = Tiling{c[0], c[1], c[2], ms};
+        if(ms < best.ms) best = Tiling{u, ms};
     }
-    set_spmv_tiling(best.lanes, best.rows_per_wg, best.entries);
+    set_spmv_tiling(best.entries);
     return best;
 }
 
@@ -171,8 +162,12 @@ void Solver::setup()
     d_comp_ptr_.upload(P.comp_ptr, st);
     d_comp_mask_.upload(P.comp_mask, st);
     d_comp_parent_.upload(P.comp_parent, st);
-    d_desc_ptr_.upload(P.desc_ptr, st);
-    d_desc_list_.upload(P.desc_list, st);
+    d_deep_ptr_.upload(P.deep_ptr, st);
+    d_deep_list_.upload(P.deep_list, st);
+    d_bank_ugroup_.upload(P.bank_ugroup, st);
+    d_ugroup_.upload(P.ugroup, st);
+    d_comp_dslot_.upload(P.comp_dslot, st);
+    d_part_deep_.alloc(3 * (size_t)std::max(P.n_deep_slots, 1));
     d_row_anc_.upload(P.row_anc, st);
     d_minv_scale_.alloc(std::max(P.n_banks, 1));
     d_gal_ptr_.upload(P.gal_ptr, st);
@@ -188,7 +183,22 @@ void Solver::setup()
     d_abd_inv_.alloc(144 * (size_t)std::max(P.n_bodies, 1));
     d_diag_inv_.alloc(9 * (size_t)P.n_rows);
     d_state_.alloc(P.n_sys);
-    d_max_iter_.alloc(P.n_sys);
+    {
+        std::vector<int> mi(P.n_sys), i2e(P.n_rows, -1);
+        for(int q = 0; q < P.n_sys; ++q) mi[q] = std::max(1, (int)(0.3 * 3.0 * systems_[q].block_rows));
+        for(int e = 0; e < P.n_ext; ++e) i2e[P.ext_to_int[e]] = e;
+        d_max_iter_.upload(mi, st);
+        d_i2e_.upload(i2e, st);
+    }
+    if(!h_export_)
+    {
+        MASX_HIP(hipHostMalloc((void**)&h_export_, 64, hipHostMallocCoherent | hipHostMallocMapped));
+        MASX_HIP(hipHostGetDevicePointer((void**)&d_export_, h_export_, 0));
+    }
+    h_export_[0] = h_export_[1] = 0;
+    exports_ = 0;
+    d_export_seq_.alloc(1);
+    MASX_HIP(hipMemsetAsync(d_export_seq_.ptr, 0, sizeof(int), st));
     d_factor_status_.alloc(1);
     MASX_HIP(hipMemsetAsync(d_state_.ptr, 0, d_state_.n * sizeof(SysState), st));
     if(h_state_) (void)hipHostFree(h_state_);
@@ -244,12 +254,32 @@ void Solver::build_work_tables()
         rows_chunks(pre, s, P.fem_row0[s], P.row_off[s + 1], fem_type, kBlock, fine_bank_base[s]);
         pre_r[2 * s + 1] = (int)pre.size();
         body_base += d.n_abd;
-        // SpMV
+        // SpMV: row-aligned tiles of at most 256 rows and 256 * entries_ entries
+        // (a row longer than that gets a tile of its own)
         spmv_r[2 * s] = (int)spmv.size();
-        rows_chunks(spmv, s, P.row_off[s], P.row_off[s + 1], kWgPlain, rows_per_wg_, 0);
+        {
+            const int cap = 256 * entries_;
+            int r0 = P.row_off[s];
+            while(r0 < P.row_off[s + 1])
+            {
+                int r1 = r0 + 1;
+                while(r1 < P.row_off[s + 1] && r1 - r0 < 256 && P.row_ptr[r1 + 1] - P.row_ptr[r0] <= cap) ++r1;
+                spmv.push_back(make_int4(s, kWgPlain << 28, r0, r1));
+                r0 = r1;
+            }
+        }
         spmv_r[2 * s + 1] = (int)spmv.size();
         if(upd_r[2 * s] == upd_r[2 * s + 1] || pre_r[2 * s] == pre_r[2 * s + 1] || spmv_r[2 * s] == spmv_r[2 * s + 1])
             throw std::runtime_error("empty system");
+    }
+    // the update workgroups of MAS rows must be exactly the plan's update groups
+    for(const int4& w : upd)
+    {
+        if((w.y >> 28) != kWgFemMas) continue;
+        const int aux = w.y & 0x0fffffff, nbw = (w.w - w.z) / P.bank;
+        const int g = aux < (int)P.bank_ugroup.size() ? P.bank_ugroup[aux] : -1;
+        if(g < 0 || P.ugroup[4 * g + 2] != P.comp_ptr[aux] || P.ugroup[4 * g + 3] != P.comp_ptr[aux + nbw])
+            throw std::logic_error("update workgroups do not match the plan's update groups");
     }
     hipStream_t st = work_;
     d_wg_pre_.upload(pre, st);
@@ -303,8 +333,12 @@ void Solver::make_devdata()
     d.comp_ptr = d_comp_ptr_.ptr;
     d.comp_mask = d_comp_mask_.ptr;
     d.comp_parent = d_comp_parent_.ptr;
-    d.desc_ptr = d_desc_ptr_.ptr;
-    d.desc_list = d_desc_list_.ptr;
+    d.deep_ptr = d_deep_ptr_.ptr;
+    d.deep_list = d_deep_list_.ptr;
+    d.bank_ugroup = d_bank_ugroup_.ptr;
+    d.ugroup = reinterpret_cast<const int4*>(d_ugroup_.ptr);
+    d.comp_dslot = d_comp_dslot_.ptr;
+    d.part_deep = d_part_deep_.ptr;
     d.row_anc = d_row_anc_.ptr;
     d.abd_inv = d_abd_inv_.ptr;
     d.body_row = d_body_row_.ptr;
@@ -408,7 +442,7 @@ void Solver::launch_iteration(hipStream_t st, int i)
     float*       p_new = (i & 1) ? d_p0_.ptr : d_p1_.ptr;
     MASX_HIP(launch_mas_local(dd_, precision_, 0, st));
     MASX_HIP(launch_precond_finish(dd_, n_wg_pre_, 0, st));
-    MASX_HIP(launch_spmv_fused(dd_, n_wg_spmv_, lanes_, passes_, entries_, p_old, p_new, st));
+    MASX_HIP(launch_spmv_fused(dd_, n_wg_spmv_, entries_, p_old, p_new, st));
     MASX_HIP(launch_update(dd_, n_wg_upd_, p_new, 0, st));
 }
 
@@ -421,6 +455,7 @@ void Solver::run_chunk(hipStream_t st, int chunk, bool with_check)
         MASX_HIP(launch_update(dd_, n_wg_upd_, d_p0_.ptr, 1, st));
         MASX_HIP(launch_phase_finish(dd_, st));
     }
+    MASX_HIP(launch_export_status(dd_, d_export_, d_export_seq_.ptr, st));
 }
 
 void Solver::solve(const double* d_b, double* d_x, const SolveParams& prm, std::vector<SolveStats>* stats,
@@ -430,16 +465,13 @@ void Solver::solve(const double* d_b, double* d_x, const SolveParams& prm, std::
     const Plan& P = plan_;
     hipStream_t st = enter(user);
     const int chunk = std::max(2, prm.chunk + (prm.chunk & 1));
-    std::vector<int> max_iter(P.n_sys);
+    int max_total = 0;
     for(int s = 0; s < P.n_sys; ++s)
-        max_iter[s] = prm.max_iter > 0 ? prm.max_iter : std::max(1, (int)(0.3 * 3.0 * systems_[s].block_rows));
-    MASX_HIP(hipMemcpyAsync(d_max_iter_.ptr, max_iter.data(), sizeof(int) * P.n_sys, hipMemcpyHostToDevice, st));
-    const size_t nvb = 3 * (size_t)P.n_rows * sizeof(float);
-    for(auto* a : {&d_x_, &d_r_, &d_z_, &d_p0_, &d_p1_, &d_Ap_, &d_rt_}) MASX_HIP(hipMemsetAsync(a->ptr, 0, nvb, st));
-    MASX_HIP(hipMemsetAsync(d_b64_.ptr, 0, 3 * (size_t)P.n_rows * sizeof(double), st));
-    MASX_HIP(launch_init_state(dd_, prm.rule, prm.tol, d_max_iter_.ptr, prm.rule == kRuleTrueRel ? prm.check_every : 0,
-                               0, st));
-    MASX_HIP(launch_load_rhs(dd_, d_e2i_.ptr, d_b, P.n_ext, st));
+        max_total = std::max(max_total, prm.max_iter > 0 ? prm.max_iter
+                                                         : std::max(1, (int)(0.3 * 3.0 * systems_[s].block_rows)));
+    MASX_HIP(launch_init_state(dd_, prm.rule, prm.tol, d_max_iter_.ptr, prm.max_iter,
+                               prm.rule == kRuleTrueRel ? prm.check_every : 0, 0, st));
+    MASX_HIP(launch_solve_prepare(dd_, d_i2e_.ptr, d_b, st));
     MASX_HIP(launch_update(dd_, n_wg_upd_, d_p0_.ptr, 1, st));
     MASX_HIP(launch_phase_finish(dd_, st));
     const bool with_check = prm.rule == kRuleTrueRel;
@@ -456,24 +488,39 @@ void Solver::solve(const double* d_b, double* d_x, const SolveParams& prm, std::
         graph_chunk_ = chunk;
         graph_check_ = with_check;
     }
-    int max_total = 0;
-    for(int m : max_iter) max_total = std::max(max_total, m);
     const int max_chunks = max_total / chunk + 8 + (with_check ? max_total / 10 : 0);
     // Chunks are launched ahead of the status checks: kDepth in flight, and
     // up to 80% of the previous solve's iterations before the first check.
     // A chunk launched after convergence costs only its early-exit launches.
     const size_t state_bytes = sizeof(SysState) * P.n_sys;
+    // Every chunk ends by writing (unfinished count, sequence number) to host
+    // memory; the host polls that instead of a per-chunk event or copy.
+    const long long seq0 = exports_;
     int launched = 0, checked = 0;
     auto launch_one = [&]() {
         if(prm.use_graph)
             MASX_HIP(hipGraphLaunch(graph_exec_, st));
         else
             run_chunk(st, chunk, with_check);
-        const int slot = launched % kDepth;
-        MASX_HIP(hipMemcpyAsync(h_state_ + (size_t)slot * P.n_sys, d_state_.ptr, state_bytes, hipMemcpyDeviceToHost,
-                                st));
-        MASX_HIP(hipEventRecord(chunk_ev_[slot], st));
         ++launched;
+        ++exports_;
+    };
+    auto wait_export = [&](long long target) {
+        for(unsigned spins = 1;; ++spins)
+        {
+            if((long long)*(volatile int*)(h_export_ + 1) >= target) return;
+            if((spins & 255) == 0)
+            {
+                const hipError_t q = hipStreamQuery(st);
+                if(q == hipSuccess)
+                {
+                    if((long long)*(volatile int*)(h_export_ + 1) >= target) return;
+                    throw std::runtime_error("solve: chunk status was not exported");
+                }
+                if(q != hipErrorNotReady) MASX_HIP(q);
+                std::this_thread::yield();
+            }
+        }
     };
     const int ahead = std::min(max_chunks, std::max(kDepth, (int)(0.8 * last_iters_) / chunk));
     while(launched < ahead) launch_one();
@@ -482,11 +529,9 @@ void Solver::solve(const double* d_b, double* d_x, const SolveParams& prm, std::
     {
         while(launched < max_chunks && launched - checked < kDepth) launch_one();
         if(checked >= launched) break;
-        const int slot = checked % kDepth;
-        MASX_HIP(hipEventSynchronize(chunk_ev_[slot]));
-        bool all_done = true;
-        for(int s = 0; s < P.n_sys; ++s) all_done &= h_state_[(size_t)slot * P.n_sys + s].status == kDone;
-        if(all_done) break;
+        // a later chunk may already have overwritten the count, which only moves it on
+        wait_export(seq0 + checked + 1);
+        if(*(volatile int*)h_export_ == 0) break;
         ++checked;
     }
     MASX_HIP(launch_store_x(dd_, d_e2i_.ptr, d_x, P.n_ext, st));
@@ -536,9 +581,7 @@ void Solver::apply_preconditioner(const double* d_r, double* d_z, hipStream_t us
     const size_t nvb = 3 * (size_t)plan_.n_rows * sizeof(float);
     MASX_HIP(hipMemsetAsync(d_rt_.ptr, 0, nvb, st));
     MASX_HIP(hipMemsetAsync(d_b64_.ptr, 0, 3 * (size_t)plan_.n_rows * sizeof(double), st));
-    std::vector<int> ones(plan_.n_sys, 1);
-    MASX_HIP(hipMemcpyAsync(d_max_iter_.ptr, ones.data(), sizeof(int) * plan_.n_sys, hipMemcpyHostToDevice, st));
-    MASX_HIP(launch_init_state(dd_, kRuleTrueRel, 0.0, d_max_iter_.ptr, 0, 1, st));
+    MASX_HIP(launch_init_state(dd_, kRuleTrueRel, 0.0, d_max_iter_.ptr, 1, 0, 1, st));
     MASX_HIP(launch_load_rhs(dd_, d_e2i_.ptr, d_r, plan_.n_ext, st));
     MASX_HIP(launch_update(dd_, n_wg_upd_, d_p0_.ptr, 1, st));
     MASX_HIP(launch_phase_finish(dd_, st));
@@ -642,7 +685,7 @@ double Solver::time_kernel(int which, int reps)
             case 0: MASX_HIP(launch_mas_local(dd_, precision_, 0, st)); break;
             case 1: MASX_HIP(launch_precond_finish(dd_, n_wg_pre_, 0, st)); break;
             case 2:
-                MASX_HIP(launch_spmv_fused(dd_, n_wg_spmv_, lanes_, passes_, entries_, d_p0_.ptr, d_p1_.ptr, st));
+                MASX_HIP(launch_spmv_fused(dd_, n_wg_spmv_, entries_, d_p0_.ptr, d_p1_.ptr, st));
                 break;
             case 3: MASX_HIP(launch_update(dd_, n_wg_upd_, d_p1_.ptr, 0, st)); break;
             default: throw std::invalid_argument("unknown kernel id");

@@ -82,11 +82,10 @@ class Solver
     void download_inverse(std::vector<float>& out);  // FP32 view of the packed bank inverses
     void download_coarse_entries(std::vector<double>& out);
     size_t inverse_bytes() const;
-    int  spmv_lanes() const { return lanes_; }
-    void set_spmv_tiling(int lanes_per_row, int rows_per_wg, int entries_per_lane);
+    void set_spmv_tiling(int entries_per_thread);
     struct Tiling
     {
-        int    lanes, rows_per_wg, entries;
+        int    entries;  // SpMV entries per thread per chunk
         double ms;
     };
     // times every supported SpMV tiling on the built system and keeps the fastest
@@ -110,22 +109,19 @@ class Solver
     bool analyzed_   = false;
     void ensure_stream();
     int  precision_  = kPrecFp32;
-    int  lanes_      = 16;
-    int  passes_     = 2;
-    int  entries_    = 1;
-    int  rows_per_wg_ = 32;
+    int  entries_    = 4;
     bool built_      = false;
 
     // device
-    DeviceArray<int>       d_row_ptr_, d_col_, d_entry_src_, d_row_sys_, d_e2i_;
+    DeviceArray<int>       d_row_ptr_, d_col_, d_entry_src_, d_row_sys_, d_e2i_, d_i2e_;
     DeviceArray<float>     d_vals32_;
     DeviceArray<double>    d_vals64_, d_E_, d_b64_, d_tmp64a_, d_tmp64b_;
-    DeviceArray<float>     d_x_, d_r_, d_z_, d_p0_, d_p1_, d_Ap_, d_rt_, d_Rc_, d_Zc_;
+    DeviceArray<float>     d_x_, d_r_, d_z_, d_p0_, d_p1_, d_Ap_, d_rt_, d_Rc_, d_Zc_, d_part_deep_;
     DeviceArray<char>      d_minv_;
     DeviceArray<float>     d_minv_scale_;
     DeviceArray<int>       d_bank_vec_, d_bank_sys_, d_bank_kind_, d_comp_ptr_, d_comp_parent_;
     DeviceArray<uint32_t>  d_comp_mask_;
-    DeviceArray<int>       d_desc_ptr_, d_desc_list_, d_row_anc_;
+    DeviceArray<int>       d_deep_ptr_, d_deep_list_, d_row_anc_, d_bank_ugroup_, d_ugroup_, d_comp_dslot_;
     DeviceArray<int>       d_gal_ptr_, d_gal_src_, d_ent_ptr_, d_ent_pos_, d_ent_src_;
     DeviceArray<float>     d_abd_inv_, d_diag_inv_;
     DeviceArray<int>       d_body_row_, d_body_src_, d_diag_src_, d_needs_diag_, d_row_mode_;
@@ -142,6 +138,10 @@ class Solver
     static constexpr int kDepth = 2;       // graph chunks in flight during a solve
     hipEvent_t  chunk_ev_[kDepth] = {};
     SysState*   h_state_ = nullptr;        // pinned: kDepth + 1 slots of n_sys states
+    int*        h_export_ = nullptr;       // coherent pinned: unfinished-system count after each chunk
+    int*        d_export_ = nullptr;       // its device address
+    long long   exports_ = 0;              // status exports launched so far (host mirror of the sequence)
+    DeviceArray<int> d_export_seq_;        // device sequence counter of the exports
     int         last_iters_ = 0;           // iterations of the previous solve (launch-ahead hint)
     hipGraphExec_t graph_exec_ = nullptr;
     int graph_chunk_ = 0;

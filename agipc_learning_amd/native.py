@@ -89,11 +89,11 @@ def load():
         "masx_solve": ([vp, vp, vp, C.POINTER(SolveParams), C.POINTER(SolveStats), vp], C.c_int),
         "masx_spmv": ([vp, C.c_double, vp, C.c_double, vp, vp], C.c_int),
         "masx_apply_preconditioner": ([vp, vp, vp, vp], C.c_int),
-        "masx_set_spmv_tiling": ([vp, C.c_int, C.c_int, C.c_int], C.c_int),
+        "masx_set_spmv_tiling": ([vp, C.c_int], C.c_int),
         "masx_converter_create": ([C.POINTER(vp)], C.c_int),
         "masx_converter_destroy": ([vp], C.c_int),
         "masx_convert_triplets": ([vp, C.c_int, C.c_int, vp, vp, vp, C.c_int, vp, vp, vp, ip, vp], C.c_int),
-        "masx_autotune": ([vp, C.c_int, ip, ip, ip, dp], C.c_int),
+        "masx_autotune": ([vp, C.c_int, ip, dp], C.c_int),
         "masx_time_kernel": ([vp, C.c_int, C.c_int, dp], C.c_int),
         "masx_plan_info_get": ([vp, C.POINTER(PlanInfo)], C.c_int),
         "masx_get_hierarchy": ([vp, C.c_int, ip, ip, ip, ip, up, ip, ip, ip], C.c_int),
@@ -261,14 +261,14 @@ class Solver:
     def apply_preconditioner(self, r, z, stream=None):
         check(self.L.masx_apply_preconditioner(self.h, _dev(r), _dev(z), stream))
 
-    def set_spmv_tiling(self, lanes_per_row, rows_per_wg, entries_per_lane=1):
-        check(self.L.masx_set_spmv_tiling(self.h, lanes_per_row, rows_per_wg, entries_per_lane))
+    def set_spmv_tiling(self, entries_per_thread):
+        check(self.L.masx_set_spmv_tiling(self.h, entries_per_thread))
 
     def autotune(self, reps=50):
         """Times every supported SpMV tiling and keeps the fastest."""
-        g, r, e, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
-        check(self.L.masx_autotune(self.h, reps, C.byref(g), C.byref(r), C.byref(e), C.byref(ms)))
-        return dict(lanes_per_row=g.value, rows_per_wg=r.value, entries_per_lane=e.value, ms=ms.value)
+        e, ms = C.c_int(), C.c_double()
+        check(self.L.masx_autotune(self.h, reps, C.byref(e), C.byref(ms)))
+        return dict(entries_per_thread=e.value, ms=ms.value)
 
     def time_kernel(self, which, reps=20):
         ms = C.c_double()

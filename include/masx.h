@@ -134,15 +134,14 @@ int masx_spmv(masx_solver* s, double a, const double* d_x, double b, double* d_y
  * -> GlobalLinearSystem::apply_preconditioner (global_linear_system.cu:293-306). */
 int masx_apply_preconditioner(masx_solver* s, const double* d_r, double* d_z, void* stream);
 
-/* SpMV tiling: lanes per block row (4, 8, 16, 32), rows per workgroup and
- * block entries per lane loaded ahead of the r.z reduction.  Supported
- * combinations are those masx_autotune tries. */
-int masx_set_spmv_tiling(masx_solver* s, int lanes_per_row, int rows_per_wg, int entries_per_lane);
+/* SpMV tiling: a workgroup owns whole rows (at most 256) holding at most
+ * 256 * entries_per_thread block entries (2, 4 or 8); a longer row gets a
+ * workgroup of its own. */
+int masx_set_spmv_tiling(masx_solver* s, int entries_per_thread);
 /* Times every supported SpMV tiling on the built system (`reps` launches
  * each) and keeps the fastest; reports it and its time per launch (ms).
  * Leaves the iteration state frozen: call masx_solve afterwards as usual. */
-int masx_autotune(masx_solver* s, int reps, int* lanes_per_row, int* rows_per_wg, int* entries_per_lane,
-                  double* ms);
+int masx_autotune(masx_solver* s, int reps, int* entries_per_thread, double* ms);
 /* Average device time of one kernel (0 MAS local, 1 preconditioner finish,
  * 2 fused SpMV, 3 update) over `reps` back-to-back launches, milliseconds. */
 int masx_time_kernel(masx_solver* s, int which, int reps, double* ms);
