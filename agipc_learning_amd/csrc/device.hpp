@@ -100,6 +100,8 @@ struct DevData
     const int4* wg_pre;
     const int4* wg_spmv;
     const int4* wg_upd;
+    const int2* wg_spmv_ent;    // per SpMV tile: its entry range [e0, e1)
+    const int4* wg_upd_grp;     // per update workgroup: its update group (zeros for non-MAS rows)
     const int*  wg_pre_range;   // per system [begin,end) of its workgroups in wg_pre (2 ints)
     const int*  wg_spmv_range;
     const int*  wg_upd_range;

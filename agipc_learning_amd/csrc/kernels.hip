@@ -326,21 +326,28 @@ __global__ void __launch_bounds__(256) k_mas_factor(DevData d, const double* E, 
                 ++bad;
             }
             const double ip = 1.0 / piv;
-            double rk[R];
+            double rk[R], f[R];
 #pragma unroll
-            for(int c = 0; c < R; ++c) rk[c] = (tx == kt && c == kr) ? ip : rowk[buf][tx + 16 * c] * ip;
+            for(int c = 0; c < R; ++c) rk[c] = rowk[buf][tx + 16 * c] * ip;
+#pragma unroll
+            for(int r = 0; r < R; ++r) f[r] = colk[buf][ty + 16 * r];
+            // rank-1 update of every element, then the pivot column
+            // (a_ik := -a_ik / p), the pivot row (a_kj := a_kj / p) and the
+            // pivot itself (1 / p) are overwritten by the threads holding them
 #pragma unroll
             for(int r = 0; r < R; ++r)
-            {
-                const bool is_k_row = ty == kt && r == kr;
-                const double f = colk[buf][ty + 16 * r];
 #pragma unroll
-                for(int c = 0; c < R; ++c)
-                {
-                    const bool is_k_col = tx == kt && c == kr;
-                    const double base = is_k_col ? 0.0 : a[r][c];
-                    a[r][c] = is_k_row ? rk[c] : base - f * rk[c];
-                }
+                for(int c = 0; c < R; ++c) a[r][c] -= f[r] * rk[c];
+            if(tx == kt)
+            {
+#pragma unroll
+                for(int r = 0; r < R; ++r) a[r][kr] = -(f[r] * ip);
+            }
+            if(ty == kt)
+            {
+#pragma unroll
+                for(int c = 0; c < R; ++c) a[kr][c] = rk[c];
+                if(tx == kt) a[kr][kr] = ip;
             }
         }
     }
@@ -620,7 +627,8 @@ __global__ void __launch_bounds__(256) k_phase_finish(DevData d)
 // residuals of their descendants.  Workgroups run the coarse banks (the end
 // of the bank list) first, so their longer gathers overlap the fine banks.
 template <int BANK, typename T>
-__global__ void __launch_bounds__(256) k_mas_local(DevData d, int force)
+// four waves per SIMD: the FP32 variant fits 128 VGPRs without spilling
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) k_mas_local(DevData d, int force)
 {
     constexpr int BPW   = 64 / BANK;
     constexpr int NBLK  = BANK * (BANK + 1) / 2;
@@ -1014,7 +1022,8 @@ __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __re
     if(st.status != kRunning) return;
     const double share = partial_share(d.part_pre, d.wg_pre_range, s);
     const int r0 = w.z, r1 = w.w;
-    const int e0 = d.row_ptr[r0], e1 = d.row_ptr[r1];
+    const int2 er = d.wg_spmv_ent[wg];
+    const int e0 = er.x, e1 = er.y;
     const int row = r0 + (int)threadIdx.x;
     const bool has_row = row < r1;
     int rb = 0, re = 0;
@@ -1189,8 +1198,7 @@ __global__ void __launch_bounds__(256) k_update(DevData d, const float* __restri
     // level-1 restriction: sum the residual rows of each component of the bank
     __shared__ float sC[256 * 3];
     __shared__ int4  sDs[256];
-    int4 grp = make_int4(0, 0, 0, 0);
-    if(type == kWgFemMas) grp = d.ugroup[d.bank_ugroup[aux]];  // uniform over the workgroup
+    const int4 grp = d.wg_upd_grp[blockIdx.x];  // uniform over the workgroup
     if(restrict_rows)
     {
         const int lane  = blane;

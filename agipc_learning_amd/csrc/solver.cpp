@@ -44,6 +44,7 @@ template struct DeviceArray<double>;
 template struct DeviceArray<char>;
 template struct DeviceArray<uint32_t>;
 template struct DeviceArray<int4>;
+template struct DeviceArray<int2>;
 template struct DeviceArray<SysState>;
 
 Solver::Solver() = default;
@@ -273,18 +274,25 @@ void Solver::build_work_tables()
             throw std::runtime_error("empty system");
     }
     // the update workgroups of MAS rows must be exactly the plan's update groups
-    for(const int4& w : upd)
+    std::vector<int4> upd_grp(upd.size(), make_int4(0, 0, 0, 0));
+    for(size_t k = 0; k < upd.size(); ++k)
     {
+        const int4& w = upd[k];
         if((w.y >> 28) != kWgFemMas) continue;
         const int aux = w.y & 0x0fffffff, nbw = (w.w - w.z) / P.bank;
         const int g = aux < (int)P.bank_ugroup.size() ? P.bank_ugroup[aux] : -1;
         if(g < 0 || P.ugroup[4 * g + 2] != P.comp_ptr[aux] || P.ugroup[4 * g + 3] != P.comp_ptr[aux + nbw])
             throw std::logic_error("update workgroups do not match the plan's update groups");
+        upd_grp[k] = make_int4(P.ugroup[4 * g], P.ugroup[4 * g + 1], P.ugroup[4 * g + 2], P.ugroup[4 * g + 3]);
     }
+    std::vector<int2> spmv_ent(spmv.size());
+    for(size_t k = 0; k < spmv.size(); ++k) spmv_ent[k] = make_int2(P.row_ptr[spmv[k].z], P.row_ptr[spmv[k].w]);
     hipStream_t st = work_;
     d_wg_pre_.upload(pre, st);
     d_wg_spmv_.upload(spmv, st);
     d_wg_upd_.upload(upd, st);
+    d_wg_upd_grp_.upload(upd_grp, st);
+    d_wg_spmv_ent_.upload(spmv_ent, st);
     d_wg_pre_range_.upload(pre_r, st);
     d_wg_spmv_range_.upload(spmv_r, st);
     d_wg_upd_range_.upload(upd_r, st);
@@ -347,6 +355,8 @@ void Solver::make_devdata()
     d.wg_pre = d_wg_pre_.ptr;
     d.wg_spmv = d_wg_spmv_.ptr;
     d.wg_upd = d_wg_upd_.ptr;
+    d.wg_upd_grp = d_wg_upd_grp_.ptr;
+    d.wg_spmv_ent = d_wg_spmv_ent_.ptr;
     d.wg_pre_range = d_wg_pre_range_.ptr;
     d.wg_spmv_range = d_wg_spmv_range_.ptr;
     d.wg_upd_range = d_wg_upd_range_.ptr;

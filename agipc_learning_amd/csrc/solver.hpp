@@ -125,7 +125,8 @@ class Solver
     DeviceArray<int>       d_gal_ptr_, d_gal_src_, d_ent_ptr_, d_ent_pos_, d_ent_src_;
     DeviceArray<float>     d_abd_inv_, d_diag_inv_;
     DeviceArray<int>       d_body_row_, d_body_src_, d_diag_src_, d_needs_diag_, d_row_mode_;
-    DeviceArray<int4>      d_wg_pre_, d_wg_spmv_, d_wg_upd_;
+    DeviceArray<int4>      d_wg_pre_, d_wg_spmv_, d_wg_upd_, d_wg_upd_grp_;
+    DeviceArray<int2>      d_wg_spmv_ent_;
     DeviceArray<int>       d_wg_pre_range_, d_wg_spmv_range_, d_wg_upd_range_;
     DeviceArray<double>    d_part_pre_, d_part_spmv_, d_part_upd_, d_part_tr_;
     DeviceArray<SysState>  d_state_;

@@ -167,7 +167,13 @@ def batch64(args):
         its = [x["iterations"] for x in st]
         out[prec] = {"ms": round(ms, 3), "max_iterations": max(its), "mean_iterations": round(sum(its) / len(its), 1),
                      "worst_rel_res_true": worst, "failures": fails}
-        log(f"  batch64 {prec}: {ms:.2f} ms/step, iterations max {max(its)}, worst true rel res {worst:.2e}")
+        kern = {name: round(s.time_kernel(k, 20) * 1e3, 2)
+                for k, name in enumerate(["mas_local", "precond_finish", "spmv", "update"])}
+        alg = algorithmic_bytes(s.info(), nv, 4 if prec == "fp32" else 2)
+        out[prec]["kernels_us"] = kern
+        out[prec]["kernels_GBs"] = {k: round(alg[k] / (kern[k] * 1e-6) / 1e9, 1) for k in kern}
+        log(f"  batch64 {prec}: {ms:.2f} ms/step, iterations max {max(its)}, worst true rel res {worst:.2e}, "
+            f"kernels us {kern}, GB/s {out[prec]['kernels_GBs']}")
     out["fp16_speedup_vs_fp32"] = round(out["fp32"]["ms"] / out["fp16"]["ms"], 3)
     return out
 
