@@ -81,15 +81,16 @@ KERNEL_PREFIX = {"spmv": "k_spmv_fused", "mas_local": "k_mas_local<32, float>", 
                  "precond_finish": "k_precond_finish"}
 
 
-def measured_traffic(kernel, workload_is_default):
+def measured_traffic(kernel, workload_is_default, spmv_entries):
     """HBM bytes per launch of `kernel` from the committed counter summary of
     this workload (FETCH_SIZE + WRITE_SIZE runs, copy-calibrated), or None."""
     path = os.path.join(ROOT, "profiles", "fem40c_hbm_traffic.json")
     if not workload_is_default or not os.path.exists(path):
         return None
     data = json.load(open(path))["kernels"]
+    want = f"k_spmv_fused<{spmv_entries}>" if kernel == "spmv" else KERNEL_PREFIX[kernel]
     for name, v in data.items():
-        if name.startswith(KERNEL_PREFIX[kernel]):
+        if name.startswith(want):
             return int(v["bytes"])
     return None
 
@@ -264,7 +265,8 @@ def main():
     achieved = alg[dominant] / (kern[dominant] * 1e-6) / 1e9
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(dominant, (args.n, args.contacts, args.levels) == (40, 20000, 4)),
+                "traffic": measured_traffic(dominant, (args.n, args.contacts, args.levels) == (40, 20000, 4),
+                                            tiling["entries_per_thread"]),
                 "algorithmic_bytes": alg[dominant], "avg_us": round(kern[dominant], 2),
                 "per_kernel": {k: {"us": round(kern[k], 2), "GB/s": round(alg[k] / (kern[k] * 1e-6) / 1e9, 1)}
                                for k in kern}}
