@@ -179,6 +179,67 @@ def batch64(args):
     return out
 
 
+def hybrid(args):
+    """ABD + FEM system: 2,000 affine bodies (dense SPD 12x12 diagonal blocks),
+    30,000 body-pair couplings, a 20^3 FEM block (9,261 vertices) coupled to
+    200 bodies; ABD 12x12 block-Jacobi + MAS on the FEM part.  Plain FP32 vs
+    FP16 inverse storage; the solution is compared with the FP64 oracle's."""
+    import numpy as np
+    import torch
+    from agipc_learning_amd import session
+    from agipc_learning_amd import workloads as W
+
+    log("step: hybrid generate + setup")
+    w = W.hybrid(seed=0)
+    s = session.solver_for(w)
+    vals = torch.from_numpy(np.ascontiguousarray(w["vals"])).cuda()
+    b = torch.from_numpy(session.rhs_of(w)).cuda()
+    x = torch.zeros_like(b)
+    out = {"bodies": int(w["n_bodies"]), "fem_vertices": int(w["n_vertices"]), "dof": int(len(b)),
+           "upper_blocks": int(s.info()["nb_total"])}
+    xs = {}
+    for prec in ("fp32", "fp16"):
+        def step():
+            s.set_values(0, vals)
+            s.build(prec)
+            return s.solve(b, x, rule=1, tol=args.tol)[0]
+        step()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.batch_steps):
+            st = step()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) * 1e3 / args.batch_steps
+        xs[prec] = x.cpu().numpy().copy()
+        out[prec] = {"ms": round(ms, 4), "iterations": st["iterations"], "rel_res_true": st["rel_res_true"],
+                     "failure": st["failure"]}
+        log(f"  hybrid {prec}: {ms:.3f} ms/step, iterations {st['iterations']}, true rel res {st['rel_res_true']:.2e}")
+    out["fp16_speedup_vs_fp32"] = round(out["fp32"]["ms"] / out["fp16"]["ms"], 3)
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+
+        O.build()
+        nv = w["n_vertices"]
+        p2r, r2p = O.morton_partition(w["positions"])
+        st_, nu_, li_ = O.mesh_neighbors(nv, w["tets"])
+        mas = O.Mas(nv, len(p2r), st_, nu_, li_, p2r, r2p)
+        mas.reorder(w["pairs"])
+        fem = (w["rows"] >= w["fem_offset"]) & (w["cols"] >= w["fem_offset"])
+        mas.assemble(w["vals"], w["rows"], w["cols"], indices=np.nonzero(fem)[0].astype(np.uint32),
+                     offset=w["fem_offset"])
+        mas.invert(1)
+        abd = O.abd_block_inverse(w["n_bodies"], w["vals"], w["rows"], w["cols"])
+        xo, so = O.pcg(w["block_rows"], w["vals"], w["rows"], w["cols"], session.rhs_of(w), n_abd=w["n_bodies"],
+                       abd_inv=abd, mas=mas, fem_offset=w["fem_offset"], tol=args.tol, rule=1)
+        for prec in xs:
+            out[prec]["solution_rel_err_vs_oracle"] = float(np.linalg.norm(xs[prec] - xo) / np.linalg.norm(xo))
+        out["oracle_iterations"] = so["iterations"]
+        log(f"  hybrid oracle: {so['iterations']} iterations, errors "
+            f"{[out[p]['solution_rel_err_vs_oracle'] for p in xs]}")
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -277,6 +338,7 @@ def main():
         cpu = cpu_baseline(w, args.levels, args.tol)
         log(f"  cpu: {cpu['value']:.1f} ms")
 
+    hyb = None if args.no_batch else hybrid(args)
     batch = None
     if not args.no_batch:
         del s, vals, b, x
@@ -316,6 +378,7 @@ def main():
         "spmv_tiling": tiling,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "hybrid": hyb,
         "batch64": batch,
     }
     print(json.dumps(out), flush=True)

@@ -368,17 +368,20 @@ def test_converter_feeds_the_solver(oracle):
 
 
 def test_spmv_row_longer_than_a_chunk(oracle):
-    """A star: block row 0 couples to 1500 others, more entries than one SpMV
-    chunk holds for every tiling (the tile of that row loops over chunks)."""
-    n = 1501
+    """Two hubs: block row 0 couples to 1500 rows after it (upper entries) and
+    the last row to 1500 rows before it (lower entries), more than one SpMV
+    chunk holds for every tiling (those rows' tiles loop over chunks)."""
+    n = 1502
     rng = np.random.default_rng(2)
     rows, cols, vals = [], [], []
     for i in range(n):
         rows.append(i); cols.append(i)
-        vals.append((np.eye(3) * (3000.0 if i == 0 else 4.0)).T.reshape(9))
-    for j in range(1, n):
-        B = rng.standard_normal((3, 3)) * 0.5
-        rows.append(0); cols.append(j); vals.append(B.T.reshape(9))
+        vals.append((np.eye(3) * (3000.0 if i in (0, n - 1) else 8.0)).T.reshape(9))
+    for j in range(1, n - 1):
+        for hub in (0, n - 1):
+            B = rng.standard_normal((3, 3)) * 0.5
+            r, c = (hub, j) if hub < j else (j, hub)
+            rows.append(r); cols.append(c); vals.append(B.T.reshape(9))
     rows, cols, vals = np.array(rows, np.int32), np.array(cols, np.int32), np.array(vals)
     b = rng.standard_normal(3 * n)
     xo, so = oracle.pcg(n, vals, rows, cols, b, global_diag=True, tol=1e-6, rule=1)
