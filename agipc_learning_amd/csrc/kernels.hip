@@ -96,6 +96,20 @@ __device__ double block_sum(double v, double* smem)
     return t;
 }
 
+// Block-wide FP64 sum in the same fixed order as block_sum, valid in thread
+// 0 only; one barrier (for the last reduction of a kernel).
+__device__ double block_sum_last(double v, double* smem)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if(lane == 0) smem[wid] = v;
+    __syncthreads();
+    double t = 0.0;
+    if(threadIdx.x == 0)
+        for(int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) t += smem[w];
+    return t;
+}
+
 // Sum of the partials of system s's workgroups of the producing kernel
 // (range = per-system [begin, end) pairs).  Identical in every workgroup.
 __device__ double sys_sum(const double* part, const int* range, int s, double* smem)
@@ -837,7 +851,24 @@ __global__ void __launch_bounds__(256) k_precond_finish(DevData d, int force)
     const long long o = 3 * (long long)(row_ok ? row : w.z);
     float r0 = 0.f, r1 = 0.f, r2 = 0.f, z0 = 0.f, z1 = 0.f, z2 = 0.f;
     float c0 = 0.f, c1 = 0.f, c2 = 0.f;
+    float dg[9];
     int mode = 0;
+    // ABD bodies: 12 threads per body, thread `dof` holds row dof of its 12x12 inverse
+    const int  abd_body = w.z + (int)threadIdx.x / 12, abd_dof = (int)threadIdx.x % 12;
+    const bool abd_ok = type == kWgAbd && (int)threadIdx.x < 12 * kBodiesPerWg && abd_body < w.w;
+    float abd_p[12], abd_r[12];
+    long long abd_base = 0;
+    if(abd_ok)
+    {
+        const float* P = d.abd_inv + 144 * (long long)abd_body + 12 * abd_dof;
+        abd_base = 3 * (long long)d.body_row[abd_body];
+#pragma unroll
+        for(int j = 0; j < 12; ++j)
+        {
+            abd_p[j] = P[j];
+            abd_r[j] = d.r[abd_base + j];
+        }
+    }
     if(row_ok)
     {
         r0 = d.r[o]; r1 = d.r[o + 1]; r2 = d.r[o + 2];
@@ -862,7 +893,15 @@ __global__ void __launch_bounds__(256) k_precond_finish(DevData d, int force)
                 }
         }
         else
+        {
             mode = d.row_mode[row];
+            if(mode)
+            {
+                const float* D = d.diag_inv + 9 * (long long)row;
+#pragma unroll
+                for(int q = 0; q < 9; ++q) dg[q] = D[q];
+            }
+        }
     }
     if(!force)
     {
@@ -896,17 +935,13 @@ __global__ void __launch_bounds__(256) k_precond_finish(DevData d, int force)
     double part = 0.0;
     if(type == kWgAbd)
     {
-        const int t = threadIdx.x;
-        const int body = w.z + t / 12, dof = t % 12;
-        if(t < 12 * kBodiesPerWg && body < w.w)
+        if(abd_ok)
         {
-            const float* P  = d.abd_inv + 144 * (long long)body + 12 * dof;
-            const long long base = 3 * (long long)d.body_row[body];
             float acc = 0.f;
 #pragma unroll
-            for(int j = 0; j < 12; ++j) acc += P[j] * d.r[base + j];
-            d.z[base + dof] = acc;
-            part = (double)d.r[base + dof] * (double)acc;
+            for(int j = 0; j < 12; ++j) acc += abd_p[j] * abd_r[j];
+            d.z[abd_base + abd_dof] = acc;
+            part = (double)abd_r[abd_dof] * (double)acc;
         }
     }
     else if(row_ok)
@@ -917,10 +952,9 @@ __global__ void __launch_bounds__(256) k_precond_finish(DevData d, int force)
         }
         else if(mode)
         {
-            const float* D = d.diag_inv + 9 * (long long)row;
-            z0 = D[0] * r0 + D[1] * r1 + D[2] * r2;
-            z1 = D[3] * r0 + D[4] * r1 + D[5] * r2;
-            z2 = D[6] * r0 + D[7] * r1 + D[8] * r2;
+            z0 = dg[0] * r0 + dg[1] * r1 + dg[2] * r2;
+            z1 = dg[3] * r0 + dg[4] * r1 + dg[5] * r2;
+            z2 = dg[6] * r0 + dg[7] * r1 + dg[8] * r2;
         }
         else
         {
@@ -930,7 +964,7 @@ __global__ void __launch_bounds__(256) k_precond_finish(DevData d, int force)
         part = (double)r0 * z0 + (double)r1 * z1 + (double)r2 * z2;
     }
     if(force) return;
-    part = block_sum(part, smem);
+    part = block_sum_last(part, smem);
     if(threadIdx.x == 0) d.part_pre[blockIdx.x] = part;
 }
 
@@ -1020,6 +1054,10 @@ __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __re
     const int  s  = w.x;
     SysState&  st = d.state[s];
     if(st.status != kRunning) return;
+    // state read before any store of this kernel (scalar loads)
+    const int    st_rule = st.rule, st_frozen = st.frozen, st_beta_zero = st.beta_zero, st_k = st.k,
+                 st_restarts = st.restarts;
+    const double st_rz_prev = st.rz_prev;
     const double share = partial_share(d.part_pre, d.wg_pre_range, s);
     const int r0 = w.z, r1 = w.w;
     const int2 er = d.wg_spmv_ent[wg];
@@ -1040,20 +1078,20 @@ __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __re
     const int n0 = min(CH, e1 - e0);
     if(n0 > 0) spmv_load_chunk<U>(d, p_old, e0, n0, c);
     const double rz = block_sum(share, smem);
-    const bool bad = !finite_d(rz) || (st.rule == kRuleTrueRel && rz <= 0.0);
+    const bool bad = !finite_d(rz) || (st_rule == kRuleTrueRel && rz <= 0.0);
     if(wg == d.wg_spmv_range[2 * s] && threadIdx.x == 0)
     {
-        if(bad && !st.frozen)
+        if(bad && !st_frozen)
         {
             st.status = kDone;
             st.failure = kBadRz;
         }
         st.rz_cur = rz;
-        st.beta = st.beta_zero ? 0.0 : rz / st.rz_prev;
-        if(st.k == 0 && st.restarts == 0) st.rz0 = rz;
+        st.beta = st_beta_zero ? 0.0 : rz / st_rz_prev;
+        if(st_k == 0 && st_restarts == 0) st.rz0 = rz;
     }
-    if(bad && !st.frozen) return;
-    const float beta = st.beta_zero ? 0.f : (float)(rz / st.rz_prev);
+    if(bad && !st_frozen) return;
+    const float beta = st_beta_zero ? 0.f : (float)(rz / st_rz_prev);
     float acc[3] = {0.f, 0.f, 0.f};
     if(n0 > 0) spmv_reduce_chunk<U>(c, beta, e0, n0, has_row, rb, re, sP, acc);
     // tiles holding a row longer than one chunk
@@ -1075,7 +1113,7 @@ __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __re
         d.Ap[o] = acc[0]; d.Ap[o + 1] = acc[1]; d.Ap[o + 2] = acc[2];
         part = (double)y0 * acc[0] + (double)y1 * acc[1] + (double)y2 * acc[2];
     }
-    part = block_sum(part, smem);
+    part = block_sum_last(part, smem);
     if(threadIdx.x == 0) d.part_spmv[wg] = part;
 }
 
@@ -1099,6 +1137,8 @@ __global__ void __launch_bounds__(256) k_update(DevData d, const float* __restri
         if(st.status != kRestarting && st.status != kInit) return;
         init = st.status == kInit;
     }
+    // read before this kernel's first store, so it stays a scalar load
+    const int4 grp = d.wg_upd_grp[blockIdx.x];  // uniform over the workgroup
     const double share = mode == 0 ? partial_share(d.part_spmv, d.wg_spmv_range, s) : 0.0;
     const int type = w.y >> 28;
     const int aux  = w.y & 0x0fffffff;
@@ -1198,7 +1238,7 @@ __global__ void __launch_bounds__(256) k_update(DevData d, const float* __restri
     // level-1 restriction: sum the residual rows of each component of the bank
     __shared__ float sC[256 * 3];
     __shared__ int4  sDs[256];
-    const int4 grp = d.wg_upd_grp[blockIdx.x];  // uniform over the workgroup
+
     if(restrict_rows)
     {
         const int lane  = blane;
@@ -1225,11 +1265,15 @@ __global__ void __launch_bounds__(256) k_update(DevData d, const float* __restri
         }
         if(has_comp && grp.y > 0) sDs[cb + blane - grp.z] = my_ds;
     }
+    // the r.r partial of each wave goes to LDS ahead of the barrier the deeper
+    // partials need; thread 0 adds the four after it
+    part = wave_sum(part);
+    if((threadIdx.x & 63) == 0) smem[threadIdx.x >> 6] = part;
+    __syncthreads();
     // one partial per deeper ancestor of the group's components (levels >= 2),
     // summed over the components in order
     if(grp.y > 0)
     {
-        __syncthreads();
         for(int t = threadIdx.x; t < grp.y; t += blockDim.x)
         {
             float ax = 0.f, ay = 0.f, az = 0.f;
@@ -1246,8 +1290,12 @@ __global__ void __launch_bounds__(256) k_update(DevData d, const float* __restri
             d.part_deep[o] = ax; d.part_deep[o + 1] = ay; d.part_deep[o + 2] = az;
         }
     }
-    part = block_sum(part, smem);
-    if(threadIdx.x == 0) d.part_upd[blockIdx.x] = part;
+    if(threadIdx.x == 0)
+    {
+        double t = 0.0;
+        for(int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) t += smem[w];
+        d.part_upd[blockIdx.x] = t;
+    }
 }
 
 // r_true = b - A x in FP64 from the FP64 input blocks (rows of the SpMV
@@ -1287,7 +1335,7 @@ __global__ void __launch_bounds__(256) k_true_residual(DevData d, int report_onl
         d.rt[o] = (float)t0; d.rt[o + 1] = (float)t1; d.rt[o + 2] = (float)t2;
         part += t0 * t0 + t1 * t1 + t2 * t2;
     }
-    part = block_sum(part, smem);
+    part = block_sum_last(part, smem);
     if(threadIdx.x == 0) d.part_tr[blockIdx.x] = part;
 }
 

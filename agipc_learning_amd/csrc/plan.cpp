@@ -292,6 +292,24 @@ void build_hierarchy(Hierarchy& h, int n_fem, int n_slots, int bank, int levels,
 
 // ----------------------------------------------------------------- plan ----
 
+// Stable LSD radix sort of records with a non-negative 64-bit `key` below
+// key_bound (12-bit digits, as many passes as the bound needs).
+template <typename T>
+static void radix_sort_stable(std::vector<T>& a, unsigned long long key_bound)
+{
+    constexpr int kBits = 12, kBuckets = 1 << kBits;
+    std::vector<T> tmp(a.size());
+    std::vector<size_t> cnt(kBuckets + 1);
+    for(int shift = 0; shift < 64 && (key_bound >> shift) != 0; shift += kBits)
+    {
+        std::fill(cnt.begin(), cnt.end(), 0);
+        for(const T& x : a) ++cnt[((unsigned long long)x.key >> shift & (kBuckets - 1)) + 1];
+        for(int k = 0; k < kBuckets; ++k) cnt[k + 1] += cnt[k];
+        for(const T& x : a) tmp[cnt[(unsigned long long)x.key >> shift & (kBuckets - 1)]++] = x;
+        a.swap(tmp);
+    }
+}
+
 void build_plan(Plan& P, std::vector<SystemDesc>& sys)
 {
     const int S = (int)sys.size();
@@ -391,7 +409,18 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
             const int b = P.row_ptr[r], e = P.row_ptr[r + 1];
             tmp.resize(e - b);
             for(int i = b; i < e; ++i) tmp[i - b] = {P.col[i], P.entry_src[i]};
-            std::stable_sort(tmp.begin(), tmp.end(), [](auto& x, auto& y) { return x.first < y.first; });
+            if(tmp.size() <= 64)  // rows are short: stable insertion sort, no allocation
+            {
+                for(size_t i = 1; i < tmp.size(); ++i)
+                {
+                    const auto v = tmp[i];
+                    size_t j = i;
+                    for(; j > 0 && tmp[j - 1].first > v.first; --j) tmp[j] = tmp[j - 1];
+                    tmp[j] = v;
+                }
+            }
+            else
+                std::stable_sort(tmp.begin(), tmp.end(), [](auto& x, auto& y) { return x.first < y.first; });
             for(int i = b; i < e; ++i)
             {
                 P.col[i]       = tmp[i - b].first;
@@ -700,7 +729,7 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
                     push(h.going_next[I], h.going_next[J], prev_idx[s][e], I == J);
                 }
             }
-            std::stable_sort(ents.begin(), ents.end(), [](const Ent& a, const Ent& b) { return a.key < b.key; });
+            radix_sort_stable(ents, (unsigned long long)h.total * (unsigned long long)h.total);
             prev_ij[s].clear();
             prev_idx[s].clear();
             for(size_t i = 0; i < ents.size();)

@@ -332,6 +332,34 @@ def main():
                 "per_kernel": {k: {"us": round(kern[k], 2), "GB/s": round(alg[k] / (kern[k] * 1e-6) / 1e9, 1)}
                                for k in kern}}
 
+    # A Newton step whose contact set changed needs a new structure: the host
+    # analysis (hierarchy, layouts, Galerkin maps) and device set-up.  Timed
+    # on the same mesh with another contact set (mesh neighbours are per mesh).
+    log("step: structure change (new contact set)")
+    from agipc_learning_amd import native
+
+    w2 = W.fem_grid(args.n, young_sigma=0.5, n_contacts=args.contacts, seed=1)
+    nbr = W.neighbors(w2["n_vertices"], w2["tets"])
+    vals2 = torch.from_numpy(np.ascontiguousarray(w2["vals"])).cuda()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s2 = native.Solver()
+    s2.add_system(block_rows=w2["block_rows"], rows=w2["rows"], cols=w2["cols"], use_mas=True, levels=args.levels)
+    s2.set_topology_morton(0, *nbr, w2["positions"])
+    s2.set_contacts(0, w2["pairs"])
+    t1 = time.perf_counter()
+    s2.analyze()
+    t2 = time.perf_counter()
+    s2.setup()
+    s2.set_values(0, vals2)
+    s2.build("fp32")
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    restructure = {"ms": round((t3 - t0) * 1e3, 2), "host_analysis_ms": round((t2 - t1) * 1e3, 2),
+                   "device_setup_and_build_ms": round((t3 - t2) * 1e3, 2)}
+    log(f"  structure change: {restructure}")
+    del s2, vals2
+
     cpu = None
     if not args.no_cpu:
         log("step: cpu baseline (oracle, one thread)")
@@ -375,6 +403,7 @@ def main():
                       "rel_res_true": results[p]["st"]["rel_res_true"],
                       "speedup_vs_fp32": round(plain["ms"] / results[p]["ms"], 3)} for p in precs[1:]},
         "setup_ms": round(setup_ms, 1),
+        "structure_change": restructure,
         "spmv_tiling": tiling,
         "roofline": roofline,
         "cpu_baseline": cpu,
