@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "converter.hpp"
+#include "hierarchy.hpp"
 #include "solver.hpp"
 
 namespace
@@ -172,6 +173,20 @@ int masx_set_topology_morton(masx_solver* s, int sys, const uint32_t* nbr_start,
         if(masx_set_topology(s, sys, nbr_start, nbr_num, nbr_list, list_len, p2r.data(), (int)p2r.size(),
                              r2p.data()))
             throw std::runtime_error(g_error);
+    });
+}
+
+int masx_morton_partition(int n, const double* positions, int bank, int* part_to_real, int part_cap,
+                          int* real_to_part)
+{
+    return guard([&] {
+        require(n >= 1 && positions && part_to_real && real_to_part, "bad argument");
+        require(bank == 32 || bank == 16, "bank size must be 32 or 16");
+        std::vector<int> p2r, r2p;
+        masx::morton_partition(n, positions, bank, p2r, r2p);
+        require(part_cap >= (int)p2r.size(), "part_to_real buffer too small");
+        std::memcpy(part_to_real, p2r.data(), sizeof(int) * p2r.size());
+        std::memcpy(real_to_part, r2p.data(), sizeof(int) * r2p.size());
     });
 }
 
@@ -600,6 +615,66 @@ int masx_convert_triplets(masx_converter* c, int n, int n_block_rows, const int*
         require(c != nullptr && n_unique != nullptr, "null argument");
         *n_unique = c->impl.convert(n, n_block_rows, d_rows, d_cols, d_vals, drop_lower, d_out_rows, d_out_cols,
                                     d_out_vals, as_stream(stream));
+    });
+}
+
+struct masx_hier
+{
+    masx::DeviceHierarchy impl;
+};
+
+int masx_hier_create(masx_hier** out)
+{
+    return guard([&] {
+        require(out != nullptr, "null output");
+        *out = new masx_hier();
+    });
+}
+
+int masx_hier_destroy(masx_hier* h)
+{
+    return guard([&] { delete h; });
+}
+
+int masx_hier_build(masx_hier* h, int n_fem, int n_slots, int bank, int levels, const uint32_t* d_nbr_start,
+                    const uint32_t* d_nbr_num, const uint32_t* d_nbr_list, long long list_len,
+                    const int* d_part_to_real, const int* d_real_to_part, const int* d_pairs, int n_pairs,
+                    int node_offset, int* out_levels, int* out_total, int* out_components, void* stream)
+{
+    return guard([&] {
+        require(h != nullptr, "null hierarchy");
+        h->impl.build(n_fem, n_slots, bank, levels, d_nbr_start, d_nbr_num, d_nbr_list, list_len, d_part_to_real,
+                      d_real_to_part, d_pairs, n_pairs, node_offset, as_stream(stream));
+        if(out_levels) *out_levels = h->impl.levels();
+        if(out_total) *out_total = h->impl.total();
+        if(out_components) *out_components = h->impl.n_components();
+    });
+}
+
+int masx_hier_download(masx_hier* h, int* level_count, int* level_begin, uint32_t* fine_mask, int* prefix,
+                       int* going_next, int* coarse_table, int* comp_ptr, uint32_t* comp_mask, int* comp_parent,
+                       void* stream)
+{
+    return guard([&] {
+        require(h != nullptr, "null hierarchy");
+        h->impl.download(level_count, level_begin, fine_mask, prefix, going_next, coarse_table, comp_ptr, comp_mask,
+                         comp_parent, as_stream(stream));
+        if(level_count) level_count[0] = 0;
+    });
+}
+
+int masx_hier_device(masx_hier* h, masx_hier_arrays* out)
+{
+    return guard([&] {
+        require(h != nullptr && out != nullptr, "null argument");
+        require(h->impl.levels() > 0, "hierarchy not built");
+        out->going_next   = h->impl.d_going_next();
+        out->coarse_table = h->impl.d_coarse_table();
+        out->fine_mask    = h->impl.d_fine_mask();
+        out->prefix       = h->impl.d_prefix();
+        out->comp_ptr     = h->impl.d_comp_ptr();
+        out->comp_mask    = h->impl.d_comp_mask();
+        out->comp_parent  = h->impl.d_comp_parent();
     });
 }
 

@@ -1,5 +1,6 @@
 // Host-side structure analysis (see plan.hpp).
 #include "plan.hpp"
+#include "pair_code.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -30,39 +31,7 @@ int reference_level_count(int vert_num, int part_map_size, int bank)
     return std::min(levels + 1, 6);
 }
 
-int decode_pair(const int* p, int v[4])
-{
-    // sign encoding of the collision pair list (TraditionalMAS32Preconditioner.cu:1550-1960)
-    const int x = p[0], y = p[1], z = p[2], w = p[3];
-    if(x >= 0)
-    {
-        v[0] = x; v[1] = y; v[2] = z; v[3] = w < 0 ? -w - 1 : w;
-        return 4;
-    }
-    v[0] = -x - 1;
-    if(z < 0)
-    {
-        if(y < 0)
-        {
-            v[1] = -y - 1; v[2] = -z - 1; v[3] = -w - 1;
-            return 4;
-        }
-        v[1] = y;
-        return 2;
-    }
-    if(w < 0)
-    {
-        if(y < 0)
-        {
-            v[1] = -y - 1; v[2] = z; v[3] = -w - 1;
-            return 4;
-        }
-        v[1] = y; v[2] = z;
-        return 3;
-    }
-    v[1] = y; v[2] = z; v[3] = w;
-    return 4;
-}
+int decode_pair(const int* p, int v[4]) { return decode_pair_code(p, v); }
 
 void morton_partition(int n, const double* pos, int bank, std::vector<int>& p2r, std::vector<int>& r2p)
 {

@@ -51,6 +51,12 @@ class PlanInfo(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class HierArrays(C.Structure):
+    _fields_ = [("going_next", C.c_void_p), ("coarse_table", C.c_void_p), ("fine_mask", C.c_void_p),
+                ("prefix", C.c_void_p), ("comp_ptr", C.c_void_p), ("comp_mask", C.c_void_p),
+                ("comp_parent", C.c_void_p)]
+
+
 _lib = None
 
 
@@ -81,6 +87,7 @@ def load():
         "masx_set_topology": ([vp, C.c_int, up, up, up, C.c_int, ip, C.c_int, ip], C.c_int),
         "masx_set_topology_morton": ([vp, C.c_int, up, up, up, C.c_int, dp], C.c_int),
         "masx_set_contacts": ([vp, C.c_int, ip, C.c_int, C.c_int], C.c_int),
+        "masx_morton_partition": ([C.c_int, dp, C.c_int, ip, C.c_int, ip], C.c_int),
         "masx_analyze": ([vp], C.c_int),
         "masx_setup": ([vp], C.c_int),
         "masx_set_values": ([vp, C.c_int, vp, vp], C.c_int),
@@ -102,6 +109,12 @@ def load():
         "masx_factor_status": ([vp, C.POINTER(C.c_longlong)], C.c_int),
         "masx_hierarchy_host": ([C.c_int, C.c_int, C.c_int, C.c_int, up, up, up, ip, ip, ip, C.c_int, C.c_int,
                                  ip, ip, ip, ip, up, ip, ip, C.c_longlong, ip], C.c_int),
+        "masx_hier_create": ([C.POINTER(vp)], C.c_int),
+        "masx_hier_destroy": ([vp], C.c_int),
+        "masx_hier_build": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_longlong, vp, vp, vp, C.c_int,
+                             C.c_int, ip, ip, ip, vp], C.c_int),
+        "masx_hier_download": ([vp, ip, ip, up, ip, ip, ip, ip, up, ip, vp], C.c_int),
+        "masx_hier_device": ([vp, C.POINTER(HierArrays)], C.c_int),
         "masx_mas_create": ([C.POINTER(vp), C.c_int], C.c_int),
         "masx_mas_destroy": ([vp], C.c_int),
         "masx_mas_init_neighbor": ([vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
@@ -148,6 +161,18 @@ def _dev(t):
     if not t.is_contiguous():
         raise ValueError("expected a contiguous tensor")
     return C.c_void_p(t.data_ptr())
+
+
+def morton_partition(positions, bank=32):
+    """Vertex partition by Morton order (the one masx_set_topology_morton uses):
+    (part_to_real padded to the bank with -1, real_to_part)."""
+    L = load()
+    pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+    n = len(pos)
+    p2r = np.zeros((n + bank - 1) // bank * bank, np.int32)
+    r2p = np.zeros(n, np.int32)
+    check(L.masx_morton_partition(n, _ptr(pos, C.c_double), bank, _ptr(p2r, C.c_int), len(p2r), _ptr(r2p, C.c_int)))
+    return p2r, r2p
 
 
 def hierarchy_host(n_fem, n_slots, nbr_start, nbr_num, nbr_list, part_to_real, real_to_part, *, pairs=None,
@@ -347,6 +372,66 @@ class Converter:
                                            stream))
         k = m.value
         return out_r[:k], out_c[:k], out_v[:k]
+
+
+class DeviceHierarchy:
+    """MAS hierarchy rebuilt on the GPU (ReorderRealtime,
+    TraditionalMAS32Preconditioner.cu:2189-2230); same arrays as
+    hierarchy_host, plus the per-bank component lists."""
+
+    def __init__(self):
+        self.L = load()
+        h = C.c_void_p()
+        check(self.L.masx_hier_create(C.byref(h)))
+        self.h = h
+        self.n_fem = self.n_slots = self.bank = 0
+        self.levels = self.total = self.n_components = 0
+
+    def __del__(self):
+        try:
+            if self.h is not None and self.h.value:
+                self.L.masx_hier_destroy(self.h)
+        except Exception:
+            pass
+
+    def build(self, n_fem, n_slots, nbr_start, nbr_num, nbr_list, part_to_real, real_to_part, *, pairs=None,
+              node_offset=0, bank=32, levels=0, stream=None):
+        """Topology, partition and pairs are CUDA tensors (uint32 / int32,
+        pairs (n, 4) int32 or None).  Returns (levels, total, components)."""
+        lv, tot, nc = C.c_int(), C.c_int(), C.c_int()
+        n_pairs = 0 if pairs is None else int(pairs.numel()) // 4
+        check(self.L.masx_hier_build(self.h, n_fem, n_slots, bank, levels, _dev(nbr_start), _dev(nbr_num),
+                                     _dev(nbr_list) if nbr_list.numel() else None, int(nbr_list.numel()),
+                                     _dev(part_to_real), _dev(real_to_part), _dev(pairs) if n_pairs else None,
+                                     n_pairs, node_offset, C.byref(lv), C.byref(tot), C.byref(nc), stream))
+        self.n_fem, self.n_slots, self.bank = n_fem, n_slots, bank
+        self.levels, self.total, self.n_components = lv.value, tot.value, nc.value
+        return self.levels, self.total, self.n_components
+
+    def download(self, stream=None):
+        """Host copies, the keys of hierarchy_host plus comp_ptr, comp_mask, comp_parent."""
+        Lv, T = self.levels, self.total
+        lc = np.zeros(Lv + 1, np.int32)
+        lb = np.zeros(Lv + 1, np.int32)
+        fm = np.zeros(self.n_fem, np.uint32)
+        px = np.zeros(self.n_slots // self.bank, np.int32)
+        gn = np.zeros(max(T, self.n_fem), np.int32)
+        ct = np.zeros((self.n_fem, 6), np.int32)
+        cp = np.zeros(T // self.bank + 1, np.int32)
+        cm = np.zeros(max(self.n_components, 1), np.uint32)
+        cq = np.zeros(max(self.n_components, 1), np.int32)
+        check(self.L.masx_hier_download(self.h, _ptr(lc, C.c_int), _ptr(lb, C.c_int), _ptr(fm, C.c_uint32),
+                                        _ptr(px, C.c_int), _ptr(gn, C.c_int), _ptr(ct, C.c_int), _ptr(cp, C.c_int),
+                                        _ptr(cm, C.c_uint32), _ptr(cq, C.c_int), stream))
+        k = self.n_components
+        return dict(levels=Lv, total=T, level_count=lc, level_begin=lb, fine_mask=fm, prefix=px, going_next=gn,
+                    coarse_table=ct, comp_ptr=cp, comp_mask=cm[:k], comp_parent=cq[:k])
+
+    def device_arrays(self):
+        """Raw device addresses of the result arrays (valid until the next build)."""
+        a = HierArrays()
+        check(self.L.masx_hier_device(self.h, C.byref(a)))
+        return {name: getattr(a, name) for name, _ in HierArrays._fields_}
 
 
 class MasPreconditioner:

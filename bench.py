@@ -357,8 +357,35 @@ def main():
     t3 = time.perf_counter()
     restructure = {"ms": round((t3 - t0) * 1e3, 2), "host_analysis_ms": round((t2 - t1) * 1e3, 2),
                    "device_setup_and_build_ms": round((t3 - t2) * 1e3, 2)}
-    log(f"  structure change: {restructure}")
     del s2, vals2
+    # the MAS hierarchy alone (the reference's per-step ReorderRealtime): host
+    # build vs the device build, inputs resident where each one runs
+    p2r, r2p = native.morton_partition(w2["positions"])
+    n2, slots2 = w2["n_vertices"], len(p2r)
+    host_ms = []
+    for _ in range(3):
+        t = time.perf_counter()
+        hh = native.hierarchy_host(n2, slots2, *nbr, p2r, r2p, pairs=w2["pairs"])
+        host_ms.append((time.perf_counter() - t) * 1e3)
+    cu = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).cuda()
+    d_in = [cu(nbr[0], np.uint32), cu(nbr[1], np.uint32), cu(nbr[2], np.uint32), cu(p2r, np.int32),
+            cu(r2p, np.int32)]
+    d_pairs = cu(np.asarray(w2["pairs"]).reshape(-1, 4), np.int32)
+    hier = native.DeviceHierarchy()
+    dev_ms = []
+    for _ in range(6):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        hier.build(n2, slots2, *d_in, pairs=d_pairs)
+        dev_ms.append((time.perf_counter() - t) * 1e3)
+    dh = hier.download()
+    same = all(np.array_equal(dh[k], hh[k]) for k in ("fine_mask", "prefix", "coarse_table", "level_begin"))
+    if not same:
+        raise SystemExit("device hierarchy differs from the host build")
+    restructure["hierarchy_host_ms"] = round(sorted(host_ms)[1], 3)
+    restructure["hierarchy_device_ms"] = round(sorted(dev_ms[1:])[2], 3)
+    log(f"  structure change: {restructure}")
+    del hier, d_in, d_pairs
 
     cpu = None
     if not args.no_cpu:

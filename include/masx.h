@@ -96,6 +96,11 @@ int masx_set_topology(masx_solver* s, int sys, const uint32_t* nbr_start, const 
 /* Same, with the partition computed from vertex positions by Morton order. */
 int masx_set_topology_morton(masx_solver* s, int sys, const uint32_t* nbr_start, const uint32_t* nbr_num,
                              const uint32_t* nbr_list, int list_len, const double* positions);
+/* The Morton partition masx_set_topology_morton uses (gl_main.cu:1176-1243),
+ * host arrays: part_to_real (part_cap >= n rounded up to the bank, -1 in
+ * padding slots), real_to_part (n). */
+int masx_morton_partition(int n, const double* positions, int bank, int* part_to_real, int part_cap,
+                          int* real_to_part);
 /* Contact pairs (int4 per pair, sign-encoded types, global vertex ids;
  * node_offset is subtracted), the _collisonPairs of GIPC.cuh:49-51 consumed by
  * TraditionalMAS32Preconditioner.cu:1550-1960.  Host array. */
@@ -180,6 +185,49 @@ int masx_hierarchy_host(int n_fem, int n_slots, int bank, int levels, const uint
                         const int* real_to_part, const int* pairs, int n_pairs, int node_offset, int* out_levels,
                         int* out_total, int* level_count, int* level_begin, uint32_t* fine_mask, int* prefix,
                         int* going_next, long long going_next_cap, int* coarse_table);
+
+/* ---- MAS hierarchy on the device ---------------------------------------
+ * TraditionalMAS32Preconditioner::ReorderRealtime
+ * (TraditionalMAS32Preconditioner.cu:2189-2230: BuildConnectMaskL0 :1963,
+ * PreparePrefixSumL0 :1991, BuildLevel1 :2014, BuildConnectMaskLx :2055,
+ * NextLevelCluster :2091, ComputeNextLevel :2101, PrefixSumLx :2112,
+ * AggregationKernel :2130, BuildCollisionConnection :2163; kernels :463-1057
+ * and :1550-1960) run on the device.  Inputs are device arrays: nbr_start and
+ * nbr_num (n_fem), nbr_list (list_len), part_to_real (n_slots), real_to_part
+ * (n_fem), pairs (4 * n_pairs ints, sign-encoded; node_offset subtracted).
+ * The result equals masx_hierarchy_host bit for bit and stays on the device
+ * (masx_hier_device); masx_hier_download copies it out.  The build
+ * synchronizes `stream` once, at the end, to return the level sizes.  An
+ * inconsistent partition or an out-of-range neighbour id is an error (no
+ * kernel then writes the outputs).  Device scratch grows as needed and is
+ * kept in the object. */
+typedef struct masx_hier masx_hier;
+int masx_hier_create(masx_hier** out);
+int masx_hier_destroy(masx_hier* h);
+int masx_hier_build(masx_hier* h, int n_fem, int n_slots, int bank, int levels, const uint32_t* d_nbr_start,
+                    const uint32_t* d_nbr_num, const uint32_t* d_nbr_list, long long list_len,
+                    const int* d_part_to_real, const int* d_real_to_part, const int* d_pairs, int n_pairs,
+                    int node_offset, int* out_levels, int* out_total, int* out_components, void* stream);
+/* Host copies; any pointer may be NULL.  level_count (levels+1, [0] = 0 as in
+ * masx_hierarchy_host), level_begin (levels+1), fine_mask (n_fem), prefix
+ * (n_slots/bank), going_next (max(total, n_fem)), coarse_table (n_fem*6), and
+ * the per-bank component lists of levels 0..levels-2: comp_ptr (total/bank + 1), comp_mask and comp_parent
+ * (*out_components each; masks in leader-lane order, parent = next-level slot). */
+int masx_hier_download(masx_hier* h, int* level_count, int* level_begin, uint32_t* fine_mask, int* prefix,
+                       int* going_next, int* coarse_table, int* comp_ptr, uint32_t* comp_mask, int* comp_parent,
+                       void* stream);
+/* Device views of the result (valid until the next build or destroy). */
+typedef struct masx_hier_arrays
+{
+    const int*      going_next;
+    const int*      coarse_table;
+    const uint32_t* fine_mask;
+    const int*      prefix;
+    const int*      comp_ptr;
+    const uint32_t* comp_mask;
+    const int*      comp_parent;
+} masx_hier_arrays;
+int masx_hier_device(masx_hier* h, masx_hier_arrays* out);
 
 /* ---- triplet -> BCOO conversion ---------------------------------------
  * Converter::convert (linear_system/utils/converter.cu:137-284): n triplets

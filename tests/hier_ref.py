@@ -140,3 +140,38 @@ def build(n_fem, n_slots, edges, part_to_real, real_to_part, levels, pairs=(), n
             coarse_table[v, l] = cur
     return dict(level_count=np.array(level_count), level_begin=np.array(level_begin), fine_mask=fine_mask,
                 prefix=prefix, going_next=going_next, coarse_table=coarse_table, total=total)
+
+
+def components_from_hierarchy(h, r2p, bank):
+    """Per-bank component lists (comp_ptr, comp_mask, comp_parent) implied by
+    a hierarchy's going_next, fine masks and level layout: the members of a
+    component share one parent, so grouping a bank's nodes by parent gives its
+    components, and parents are numbered in (bank, leader lane) order, so
+    sorting by parent gives the leader order.  The top level lists none."""
+    L, T = int(h["levels"]), int(h["total"])
+    lb, lc = h["level_begin"], h["level_count"]
+    n_slots = int(lb[1])
+    per_bank = [[] for _ in range(T // bank)]
+    if L >= 2:
+        seen = {}
+        for v, s in enumerate(r2p):
+            p = int(h["going_next"][v])
+            seen.setdefault((int(s) // bank, p), int(h["fine_mask"][v]))
+        for (b, p), m in seen.items():
+            per_bank[b].append((p, m))
+    for lvl in range(1, L - 1):
+        groups = {}
+        for s in range(int(lb[lvl]), int(lb[lvl]) + int(lc[lvl])):
+            p = int(h["going_next"][s])
+            groups[(s // bank, p)] = groups.get((s // bank, p), 0) | (1 << (s % bank))
+        for (b, p), m in groups.items():
+            per_bank[b].append((p, m))
+    ptr = np.zeros(T // bank + 1, np.int32)
+    masks, parents = [], []
+    for b, lst in enumerate(per_bank):
+        lst.sort()
+        ptr[b + 1] = ptr[b] + len(lst)
+        parents.extend(p for p, _ in lst)
+        masks.extend(m for _, m in lst)
+    assert n_slots % bank == 0
+    return ptr, np.array(masks, np.uint32), np.array(parents, np.int32)

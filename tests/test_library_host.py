@@ -6,6 +6,7 @@ import pytest
 
 from agipc_learning_amd import native
 from agipc_learning_amd import workloads as W
+from hier_ref import components_from_hierarchy
 from test_oracle import _random_graph_case, kat35
 
 
@@ -43,6 +44,19 @@ def test_hierarchy_reference_kat_library(oracle):
     assert np.array_equal(h["going_next"][:valid], padded + np.array([0 if v < 32 else 1 for v in range(valid)]))
 
 
+def test_component_lists_from_hierarchy_kat():
+    # 35 vertices over two level-0 banks: one component each (all 32 lanes,
+    # then lanes 0-2), whose level-1 nodes are slots 64 and 65; those two are
+    # separate components of the level-1 bank; the top level lists nothing
+    valid, padded, start, num, lst, p2r, r2p = kat35()
+    h = native.hierarchy_host(valid, padded, start, num, lst, p2r, r2p)
+    ptr, masks, parents = components_from_hierarchy(h, r2p, 32)
+    assert h["levels"] == 3 and h["total"] == 128
+    assert list(ptr) == [0, 1, 2, 4, 4]
+    assert list(masks) == [0xFFFFFFFF, 0x7, 0x1, 0x2]
+    assert list(parents) == [64, 65, 96, 97]
+
+
 @pytest.mark.parametrize("seed,n,extra,npairs,scramble", [
     (0, 70, 10, 0, False), (1, 200, 60, 15, False), (2, 300, 400, 40, True), (3, 1000, 100, 30, False),
     (4, 129, 5, 8, True), (5, 2100, 300, 50, False), (6, 5000, 3000, 200, True)])
@@ -66,6 +80,14 @@ def test_library_hierarchy_equals_oracle_grids(oracle, n, contacts, bank, levels
     orc.reorder(s["pairs"])
     h = native.hierarchy_host(nv, len(p2r), st, nu, li, p2r, r2p, pairs=s["pairs"], bank=bank, levels=levels)
     _compare(h, orc, orc.levels)
+
+
+@pytest.mark.parametrize("n,bank", [(6, 32), (13, 16), (20, 32)])
+def test_library_morton_partition_equals_oracle(oracle, n, bank):
+    s = W.fem_grid(n, seed=n)
+    p2r, r2p = native.morton_partition(s["positions"], bank)
+    q2r, s2p = oracle.morton_partition(s["positions"], bank)
+    assert np.array_equal(p2r, q2r) and np.array_equal(r2p, s2p)
 
 
 def test_library_rejects_bad_input_with_an_error():
