@@ -24,7 +24,7 @@ namespace
 {
 constexpr int kMaxLevels = 6;
 
-enum HierFlag : int { kBadPartition = 1, kBadNeighbor = 2, kBadList = 4 };
+enum HierFlag : int { kBadPartition = 1, kBadNeighbor = 2, kBadList = 4, kAsymmetric = 8 };
 
 // Device level table, 32 ints: count of level l at [l], first slot at
 // [8 + l], error flags at [16], listed components at [17].
@@ -82,6 +82,16 @@ __device__ int block_scan_256x8(const int* in, int n, int* out)
     return carry;
 }
 
+// Level table (level 0 = n_slots slots at 0), going_next = -1, bank counts = 0.
+__global__ void __launch_bounds__(256) k_h_init(Lv lv, int n_slots, long long max_slots, long long max_banks,
+                                                int* going_next, int* ncomp_all)
+{
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if(t < 32) lv.v[t] = t == 0 ? n_slots : 0;
+    if(t < max_slots) going_next[t] = -1;
+    if(t < max_banks) ncomp_all[t] = 0;
+}
+
 // Input checks, the level-0 adjacency (own bit for slots holding a vertex) and
 // the starting node of every vertex.
 __global__ void __launch_bounds__(256) k_h_validate(Lv lv, int n_fem, int n_slots, int bank, const uint32_t* start,
@@ -93,7 +103,7 @@ __global__ void __launch_bounds__(256) k_h_validate(Lv lv, int n_fem, int n_slot
     if(t < n_slots)
     {
         const int r = p2r[t];
-        if(r < -1 || r >= n_fem) flag |= kBadPartition;
+        if(r < -1 || r >= n_fem || (r >= 0 && r2p[r] != t)) flag |= kBadPartition;
         adj[t] = r >= 0 ? (1u << (t % bank)) : 0u;
     }
     if(t < n_fem)
@@ -106,50 +116,145 @@ __global__ void __launch_bounds__(256) k_h_validate(Lv lv, int n_fem, int n_slot
             flag |= kBadList;
         else
             for(unsigned long long i = 0; i < c; ++i)
-                if(list[b + i] >= (uint32_t)n_fem) flag |= kBadNeighbor;
+            {
+                const uint32_t u = list[b + i];
+                if(u >= (uint32_t)n_fem)
+                {
+                    flag |= kBadNeighbor;
+                    continue;
+                }
+                // t must appear in u's list (u's own range is checked by u's thread)
+                const unsigned long long bu = start[u], cu = num[u];
+                if(bu + cu > (unsigned long long)list_len) continue;
+                bool back = false;
+                for(unsigned long long k = 0; k < cu && !back; ++k) back = list[bu + k] == (uint32_t)t;
+                if(!back) flag |= kAsymmetric;
+            }
     }
     if(flag) atomicOr(&lv.v[16], flag);
 }
 
-__device__ __forceinline__ void h_link(int a, int b, int bank, uint32_t* adj)
+// Levels with at most kLdsNodes nodes pool link masks per workgroup in LDS
+// (fast shared-memory atomics) and flush each touched node word once.
+constexpr int kLdsNodes = 4096;
+
+__device__ __forceinline__ void lds_or_begin(uint32_t* acc, int count)
 {
-    if(a != b && a / bank == b / bank)
+    for(int i = threadIdx.x; i < count; i += blockDim.x) acc[i] = 0u;
+    __syncthreads();
+}
+
+__device__ __forceinline__ void lds_or_flush(const uint32_t* acc, int count, uint32_t* dst)
+{
+    __syncthreads();
+    for(int i = threadIdx.x; i < count; i += blockDim.x)
     {
-        atomicOr(&adj[a], 1u << (b % bank));
-        atomicOr(&adj[b], 1u << (a % bank));
+        const uint32_t m = acc[i];
+        if(m) atomicOr(&dst[i], m);
     }
 }
 
-// Links at the current level: mesh edges (u > v) and every pair of distinct
-// in-range vertices of each contact, between the nodes holding them.  Slot
-// offsets of every level are bank multiples, so global slots share a bank
-// exactly when their level-local indices do.
-__global__ void __launch_bounds__(256) k_h_links(Lv lv, int n_fem, const uint32_t* start, const uint32_t* num,
-                                                 const uint32_t* list, int n_pairs, const int* pairs,
-                                                 int node_offset, const int* id, int bank, uint32_t* adj)
+// ORs `m` into adj[a] once per distinct `a` of the wavefront: lanes sharing a
+// target pool their bits and one lane issues the atomic (coarse levels map
+// many fine vertices onto few nodes, and per-edge atomics on one word would
+// serialize).  Lanes with m == 0 take no part.  Wave-uniform loop.
+__device__ __forceinline__ void or_into_dedup(int a, uint32_t m, uint32_t* adj)
+{
+    const int lane = threadIdx.x & 63;
+    bool act = m != 0u;
+    while(true)
+    {
+        const unsigned long long bal = __ballot(act);
+        if(bal == 0ull) break;
+        const int leader = __ffsll((long long)bal) - 1;
+        const int target = __shfl(a, leader, 64);
+        const bool match = act && a == target;
+        uint32_t v = match ? m : 0u;
+        for(int o = 32; o >= 1; o >>= 1) v |= (uint32_t)__shfl_xor((int)v, o, 64);
+        if(lane == leader) atomicOr(&adj[target], v);
+        act = act && !match;
+    }
+}
+
+// Mesh links at the current level, one thread per level-0 slot: the vertex
+// ORs the lanes of its neighbours' nodes that share its node's bank (a
+// symmetric list makes every edge count from both ends, as an undirected
+// link), then the masks are pooled per node.  Slot offsets of every level are
+// bank multiples, so global slots share a bank exactly when their
+// level-local indices do.
+__global__ void __launch_bounds__(256) k_h_mesh_links(Lv lv, int level, int n_slots, const int* p2r,
+                                                      const uint32_t* start, const uint32_t* num, const uint32_t* list,
+                                                      const int* id, int bank, uint32_t* adj)
+{
+    if(lv.failed()) return;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int a = -1;
+    uint32_t m = 0u;
+    const int v = s < n_slots ? p2r[s] : -1;
+    if(v >= 0)
+    {
+        a = id[v];
+        const uint32_t b0 = start[v], n = num[v];
+        for(uint32_t i = 0; i < n; ++i)
+        {
+            const int b = id[list[b0 + i]];
+            if(b != a && b / bank == a / bank) m |= 1u << (b % bank);
+        }
+    }
+    const int count = lv.count(level), begin = lv.begin(level);
+    if(level == 0)
+    {
+        if(m) atomicOr(&adj[a], m);  // one vertex per slot: no two lanes share a target
+    }
+    else if(count <= kLdsNodes)
+    {
+        __shared__ uint32_t acc[kLdsNodes];
+        lds_or_begin(acc, count);
+        if(m) atomicOr(&acc[a - begin], m);
+        lds_or_flush(acc, count, adj + begin);
+    }
+    else
+        or_into_dedup(a, m, adj);
+}
+
+// Contact links: every pair of distinct in-range vertices of a contact joins
+// their nodes when those share a bank (both directions).
+__global__ void __launch_bounds__(256) k_h_pair_links(Lv lv, int level, int n_pairs, const int* pairs, int node_offset, int n_fem,
+                                                      const int* id, int bank, uint32_t* adj)
 {
     if(lv.failed()) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if(t < n_fem)
-    {
-        const int a = id[t];
-        const uint32_t s = start[t], n = num[t];
-        for(uint32_t i = 0; i < n; ++i)
-        {
-            const int u = (int)list[s + i];
-            if(u > t) h_link(a, id[u], bank, adj);
-        }
-    }
+    int cp[4] = {-1, -1, -1, -1};
+    uint32_t msk[4] = {0u, 0u, 0u, 0u};
     if(t < n_pairs)
     {
         int v[4];
         const int cnt = decode_pair_code(pairs + 4 * (long long)t, v);
-        for(int i = 0; i < cnt; ++i) v[i] -= node_offset;
-        for(int i = 0; i < cnt; ++i)
-            for(int j = i + 1; j < cnt; ++j)
-                if(v[i] >= 0 && v[j] >= 0 && v[i] < n_fem && v[j] < n_fem && v[i] != v[j])
-                    h_link(id[v[i]], id[v[j]], bank, adj);
+        for(int i = 0; i < 4; ++i)
+        {
+            const int u = i < cnt ? v[i] - node_offset : -1;
+            cp[i] = u >= 0 && u < n_fem ? id[u] : -1;
+        }
+        for(int i = 0; i < 4; ++i)
+            for(int j = i + 1; j < 4; ++j)
+                if(cp[i] >= 0 && cp[j] >= 0 && cp[i] != cp[j] && cp[i] / bank == cp[j] / bank)
+                {
+                    msk[i] |= 1u << (cp[j] % bank);
+                    msk[j] |= 1u << (cp[i] % bank);
+                }
     }
+    const int count = lv.count(level), begin = lv.begin(level);
+    if(count <= kLdsNodes)
+    {
+        __shared__ uint32_t acc[kLdsNodes];
+        lds_or_begin(acc, count);
+        for(int i = 0; i < 4; ++i)
+            if(msk[i]) atomicOr(&acc[cp[i] - begin], msk[i]);
+        lds_or_flush(acc, count, adj + begin);
+    }
+    else
+        for(int i = 0; i < 4; ++i)
+            if(msk[i]) atomicOr(&adj[cp[i]], msk[i]);
 }
 
 // One wavefront per 64/bank banks: transitive closure of the adjacency masks
@@ -174,7 +279,9 @@ __global__ void __launch_bounds__(64) k_h_components(Lv lv, int level, int bank,
             const uint32_t mj = (uint32_t)__shfl((int)m, gbase + j, 64);
             if((m >> j) & 1u) r |= mj;
         }
+        const bool grew = r != m;
         m = r;
+        if(__ballot(grew) == 0ull) break;  // closed: a further pass changes nothing
     }
     if(in) compm[begin + c] = m;
     const bool leader = m != 0u && (int)__builtin_ctz(m) == l;
@@ -366,21 +473,20 @@ void DeviceHierarchy::build(int n_fem, int n_slots, int bank, int levels, const 
     grow(table_, table_cap_, (size_t)kMaxLevels * n_fem);
     if(!h_lv_) hier_check(hipHostMalloc((void**)&h_lv_, 32 * sizeof(int), hipHostMallocDefault));
 
-    hier_check(hipMemsetAsync(compm_, 0, max_slots * sizeof(uint32_t), st));
-    hier_check(hipMemsetAsync(going_next_, 0xff, max_slots * sizeof(int), st));
-    hier_check(hipMemsetAsync(ncomp_all_, 0, max_banks * sizeof(int), st));
-    for(int i = 0; i < 32; ++i) h_lv_[i] = 0;
-    h_lv_[0] = n_slots;
-    hier_check(hipMemcpyAsync(lv_, h_lv_, 32 * sizeof(int), hipMemcpyHostToDevice, st));
     const Lv lv{lv_};
     const int per = 64 / bank;
+    hipLaunchKernelGGL(k_h_init, dim3(blocks((long long)max_slots, 256)), dim3(256), 0, st, lv, n_slots,
+                       (long long)max_slots, (long long)max_banks, going_next_, ncomp_all_);
     hipLaunchKernelGGL(k_h_validate, dim3(blocks(std::max(n_fem, n_slots), 256)), dim3(256), 0, st, lv, n_fem,
                        n_slots, bank, start, num, list, list_len, p2r, r2p, id_, adj_);
     // grids are sized by the bound n_slots; kernels read the level sizes
     for(int level = 0; level < L; ++level)
     {
-        hipLaunchKernelGGL(k_h_links, dim3(blocks(std::max(n_fem, n_pairs), 256)), dim3(256), 0, st, lv, n_fem,
-                           start, num, list, n_pairs, pairs, node_offset, id_, bank, adj_);
+        hipLaunchKernelGGL(k_h_mesh_links, dim3(blocks(n_slots, 256)), dim3(256), 0, st, lv, level, n_slots, p2r,
+                           start, num, list, id_, bank, adj_);
+        if(n_pairs > 0)
+            hipLaunchKernelGGL(k_h_pair_links, dim3(blocks(n_pairs, 256)), dim3(256), 0, st, lv, level, n_pairs, pairs,
+                               node_offset, n_fem, id_, bank, adj_);
         hipLaunchKernelGGL(k_h_components, dim3(blocks(n_slots / bank, per)), dim3(64), 0, st, lv, level, bank,
                            adj_, compm_, ncomp_);
         hipLaunchKernelGGL(k_h_scan_level, dim3(1), dim3(256), 0, st, lv, level, bank, n_slots, ncomp_, off_);
@@ -402,6 +508,7 @@ void DeviceHierarchy::build(int n_fem, int n_slots, int bank, int levels, const 
     if(h_lv_[16] & kBadPartition) throw std::invalid_argument("part_to_real and real_to_part disagree");
     if(h_lv_[16] & kBadList) throw std::invalid_argument("neighbor range past the end of the list");
     if(h_lv_[16] & kBadNeighbor) throw std::invalid_argument("neighbor id out of range");
+    if(h_lv_[16] & kAsymmetric) throw std::invalid_argument("neighbor lists are not symmetric");
     levels_  = L;
     n_fem_   = n_fem;
     n_slots_ = n_slots;

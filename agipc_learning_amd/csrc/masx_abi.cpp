@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "converter.hpp"
+#include "fem.hpp"
 #include "hierarchy.hpp"
 #include "solver.hpp"
 
@@ -615,6 +616,46 @@ int masx_convert_triplets(masx_converter* c, int n, int n_block_rows, const int*
         require(c != nullptr && n_unique != nullptr, "null argument");
         *n_unique = c->impl.convert(n, n_block_rows, d_rows, d_cols, d_vals, drop_lower, d_out_rows, d_out_cols,
                                     d_out_vals, as_stream(stream));
+    });
+}
+
+struct masx_fem
+{
+    masx::FemHessian impl;
+};
+
+int masx_fem_create(masx_fem** out, int n_tets, const int* tets, long long nb, const int* rows, const int* cols)
+{
+    return guard([&] {
+        require(out != nullptr, "null output");
+        auto f = std::make_unique<masx_fem>();
+        f->impl.setup(n_tets, tets, nb, rows, cols);
+        *out = f.release();
+    });
+}
+
+int masx_fem_destroy(masx_fem* f)
+{
+    return guard([&] { delete f; });
+}
+
+int masx_fem_assemble(masx_fem* f, const double* d_dminv, const double* d_vol, const double* d_F, const double* d_mu,
+                      const double* d_lambda, const double* d_base, double* d_vals, void* stream)
+{
+    return guard([&] {
+        require(f != nullptr, "null fem object");
+        f->impl.assemble(d_dminv, d_vol, d_F, d_mu, d_lambda, d_base, d_vals, as_stream(stream));
+    });
+}
+
+int masx_fem_param_grad(masx_fem* f, const double* d_dminv, const double* d_vol, const double* d_F,
+                        const double* d_mu, const double* d_lambda, const double* d_x, const double* d_y,
+                        double* d_grad_mu, double* d_grad_lambda, void* stream)
+{
+    return guard([&] {
+        require(f != nullptr, "null fem object");
+        f->impl.param_grad(d_dminv, d_vol, d_F, d_mu, d_lambda, d_x, d_y, d_grad_mu, d_grad_lambda,
+                           as_stream(stream));
     });
 }
 

@@ -109,6 +109,10 @@ def load():
         "masx_factor_status": ([vp, C.POINTER(C.c_longlong)], C.c_int),
         "masx_hierarchy_host": ([C.c_int, C.c_int, C.c_int, C.c_int, up, up, up, ip, ip, ip, C.c_int, C.c_int,
                                  ip, ip, ip, ip, up, ip, ip, C.c_longlong, ip], C.c_int),
+        "masx_fem_create": ([C.POINTER(vp), C.c_int, ip, C.c_longlong, ip, ip], C.c_int),
+        "masx_fem_destroy": ([vp], C.c_int),
+        "masx_fem_assemble": ([vp, vp, vp, vp, vp, vp, vp, vp, vp], C.c_int),
+        "masx_fem_param_grad": ([vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp], C.c_int),
         "masx_hier_create": ([C.POINTER(vp)], C.c_int),
         "masx_hier_destroy": ([vp], C.c_int),
         "masx_hier_build": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_longlong, vp, vp, vp, C.c_int,
@@ -150,6 +154,12 @@ def _i32(a):
 
 def _u32(a):
     return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def torch_float64():
+    import torch
+
+    return torch.float64
 
 
 def _dev(t):
@@ -372,6 +382,65 @@ class Converter:
                                            stream))
         k = m.value
         return out_r[:k], out_c[:k], out_v[:k]
+
+
+class FemHessian:
+    """Stable Neo-Hookean element Hessians assembled on the GPU and the
+    parameter gradient of an adjoint solve (femEnergy.cu:432-459, 1641-1855)."""
+
+    def __init__(self, tets, rows, cols):
+        self.L = load()
+        t = _i32(tets).reshape(-1)
+        r, c = _i32(rows), _i32(cols)
+        h = C.c_void_p()
+        check(self.L.masx_fem_create(C.byref(h), len(t) // 4, _ptr(t, C.c_int), len(r), _ptr(r, C.c_int),
+                                     _ptr(c, C.c_int)))
+        self.h = h
+        self.n_tets, self.nb = len(t) // 4, len(r)
+        self.n_rows = int(c.max()) + 1 if len(c) else 0
+
+    def _check_elements(self, dminv, vol, F, mu, lam):
+        n = self.n_tets
+        for name, a, per in (("dminv", dminv, 9), ("vol", vol, 1), ("F", F, 9), ("mu", mu, 1), ("lam", lam, 1)):
+            if a.numel() != n * per or a.dtype != torch_float64():
+                raise ValueError(f"{name}: expected {n * per} float64 values")
+
+    def __del__(self):
+        try:
+            if self.h is not None and self.h.value:
+                self.L.masx_fem_destroy(self.h)
+        except Exception:
+            pass
+
+    def assemble(self, dminv, vol, F, mu, lam, base=None, out=None, stream=None):
+        """CUDA float64 tensors per tet (dminv (n, 9), vol (n,), F (n, 9), mu,
+        lam (n,)); returns the (nb, 9) block values base + sum of K_e."""
+        import torch
+
+        self._check_elements(dminv, vol, F, mu, lam)
+        if out is None:
+            out = torch.empty((self.nb, 9), dtype=torch.float64, device=mu.device)
+        for name, a in (("base", base), ("out", out)):
+            if a is not None and (a.numel() != 9 * self.nb or a.dtype != torch.float64):
+                raise ValueError(f"{name}: expected {9 * self.nb} float64 values")
+        check(self.L.masx_fem_assemble(self.h, _dev(dminv), _dev(vol), _dev(F), _dev(mu), _dev(lam),
+                                       _dev(base) if base is not None else None, _dev(out), stream))
+        return out
+
+    def param_grad(self, dminv, vol, F, mu, lam, x, y, stream=None):
+        """(-y_e^T dK_e/dmu x_e, -y_e^T dK_e/dlambda x_e) per tet; x, y CUDA
+        float64 block vectors."""
+        import torch
+
+        self._check_elements(dminv, vol, F, mu, lam)
+        for name, a in (("x", x), ("y", y)):
+            if a.numel() < 3 * self.n_rows or a.dtype != torch.float64:
+                raise ValueError(f"{name}: expected at least {3 * self.n_rows} float64 values")
+        gm = torch.empty(self.n_tets, dtype=torch.float64, device=mu.device)
+        gl = torch.empty_like(gm)
+        check(self.L.masx_fem_param_grad(self.h, _dev(dminv), _dev(vol), _dev(F), _dev(mu), _dev(lam), _dev(x),
+                                         _dev(y), _dev(gm), _dev(gl), stream))
+        return gm, gl
 
 
 class DeviceHierarchy:

@@ -186,6 +186,30 @@ int masx_hierarchy_host(int n_fem, int n_slots, int bank, int levels, const uint
                         int* out_total, int* level_count, int* level_begin, uint32_t* fine_mask, int* prefix,
                         int* going_next, long long going_next_cap, int* coarse_table);
 
+/* ---- Stable Neo-Hookean element Hessians and their parameter gradient ---
+ * The FEM part of the Newton matrix (femEnergy.cu: first Piola stress
+ * :441-459 with rates u = 4 mu/3, r = lambda + 5 mu/6 (gl_main.cu:586-590),
+ * PSD projection by the analytic eigensystem :1641-1717, element matrix
+ * vol * PFPX^T H PFPX :1719-1855, here without the dt^2 factor: the system is
+ * M/dt^2 + K) assembled on the device, and the backward kernel of an adjoint
+ * solve: for H(theta) x = b and H y = dL/dx, dL/dtheta_e = -y_e^T (dK_e /
+ * dtheta_e) x_e, with the derivative of the PSD projection in closed form.
+ * create: host tets (4 block-row ids each) and the host upper block structure
+ * (rows[k] <= cols[k], the order of the values); every tet edge and vertex
+ * needs its block.  Element inputs (device, per tet): dminv (9, row-major
+ * Dm^-1), vol, F (9, row-major), mu, lambda.  assemble: vals (nb x 9,
+ * col-major blocks) = base (may be NULL) + sum of the element matrices, in a
+ * fixed order (deterministic).  param_grad: x, y are FP64 block vectors (3
+ * per block row); writes one gradient per tet. */
+typedef struct masx_fem masx_fem;
+int masx_fem_create(masx_fem** out, int n_tets, const int* tets, long long nb, const int* rows, const int* cols);
+int masx_fem_destroy(masx_fem* f);
+int masx_fem_assemble(masx_fem* f, const double* d_dminv, const double* d_vol, const double* d_F, const double* d_mu,
+                      const double* d_lambda, const double* d_base, double* d_vals, void* stream);
+int masx_fem_param_grad(masx_fem* f, const double* d_dminv, const double* d_vol, const double* d_F,
+                        const double* d_mu, const double* d_lambda, const double* d_x, const double* d_y,
+                        double* d_grad_mu, double* d_grad_lambda, void* stream);
+
 /* ---- MAS hierarchy on the device ---------------------------------------
  * TraditionalMAS32Preconditioner::ReorderRealtime
  * (TraditionalMAS32Preconditioner.cu:2189-2230: BuildConnectMaskL0 :1963,

@@ -264,3 +264,130 @@ def convert_triplets(rows, cols, vals, drop_lower=False):
             acc = acc + vals[t]
         out[u] = acc
     return rows[order[starts]].astype(np.int32), cols[order[starts]].astype(np.int32), out
+
+
+# ------------------------------------------------- Stable Neo-Hookean (FEM) --
+# Element Hessian of the reference's Stable Neo-Hookean tets
+# (femEnergy.cu:432-459 first Piola stress P = u F + r (J - 1 - u/r) cof F with
+# u = 4 mu / 3, r = lambda + 5 mu / 6 from gl_main.cu:586-590; Hessian made PSD
+# by clamping its eigenvalues, femEnergy.cu:1641-1717; element matrix
+# vol * PFPX^T H PFPX, femEnergy.cu:1719-1855, without the dt^2 factor: the
+# system is M/dt^2 + K).  Restated directly in vec(F) coordinates (column-
+# major, index i + 3 j) with a dense 9x9 symmetric eigen-decomposition; the
+# library computes the same matrix through the analytic eigensystem instead.
+
+_EPS3 = np.zeros((3, 3, 3))
+for _i, _j, _k in ((0, 1, 2), (1, 2, 0), (2, 0, 1)):
+    _EPS3[_i, _j, _k] = 1.0
+    _EPS3[_i, _k, _j] = -1.0
+
+
+def snh_rates(mu, lam):
+    """(u, r) and their derivatives d(u, r)/d(mu, lambda)."""
+    return 4.0 * mu / 3.0, lam + 5.0 * mu / 6.0
+
+
+def snh_stress(F, mu, lam):
+    """First Piola stress (femEnergy.cu:441-459)."""
+    u, r = snh_rates(mu, lam)
+    J = np.linalg.det(F)
+    cof = J * np.linalg.inv(F).T
+    return u * F + r * (J - 1.0 - u / r) * cof
+
+
+def snh_hessian_parts(F):
+    """(I9, g g^T, d2J, J) in vec(F) coordinates: H = u I + r g g^T + s d2J,
+    s = r (J - 1) - u."""
+    J = np.linalg.det(F)
+    cof = J * np.linalg.inv(F).T
+    g = cof.reshape(-1, order="F")
+    d2 = np.einsum("ikm,jln,mn->ijkl", _EPS3, _EPS3, F)  # d2J / dF_ij dF_kl
+    D2 = np.zeros((9, 9))
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                for l in range(3):
+                    D2[i + 3 * j, k + 3 * l] = d2[i, j, k, l]
+    return np.eye(9), np.outer(g, g), D2, J
+
+
+def snh_hessian(F, mu, lam):
+    """Unprojected 9x9 Hessian d vec(P) / d vec(F)."""
+    I9, gg, D2, J = snh_hessian_parts(F)
+    u, r = snh_rates(mu, lam)
+    return u * I9 + r * gg + (r * (J - 1.0) - u) * D2
+
+
+def snh_hessian_dtheta(F):
+    """dH/dmu and dH/dlambda (H is linear in (mu, lambda) before projection)."""
+    I9, gg, D2, J = snh_hessian_parts(F)
+    # u = 4mu/3, r = lambda + 5mu/6, s = r (J-1) - u
+    dmu = (4.0 / 3.0) * I9 + (5.0 / 6.0) * gg + ((5.0 / 6.0) * (J - 1.0) - 4.0 / 3.0) * D2
+    dlam = gg + (J - 1.0) * D2
+    return dmu, dlam
+
+
+def project_psd(H):
+    w, Q = np.linalg.eigh(0.5 * (H + H.T))
+    return (Q * np.maximum(w, 0.0)) @ Q.T
+
+
+def project_psd_derivative(H, dH):
+    """Directional derivative of the PSD projection Q max(w, 0) Q^T along dH
+    (Daleckii-Krein: Q (Gamma o Q^T dH Q) Q^T with divided differences of
+    max(., 0))."""
+    w, Q = np.linalg.eigh(0.5 * (H + H.T))
+    f = np.maximum(w, 0.0)
+    n = len(w)
+    G = np.zeros((n, n))
+    for a in range(n):
+        for b in range(n):
+            if abs(w[a] - w[b]) > 1e-12 * max(1.0, abs(w[a]), abs(w[b])):
+                G[a, b] = (f[a] - f[b]) / (w[a] - w[b])
+            else:
+                G[a, b] = 1.0 if w[a] > 0 else 0.0
+    return Q @ (G * (Q.T @ dH @ Q)) @ Q.T
+
+
+def shape_gradient(dminv):
+    """D (9x12): d vec(F) / d x for a tet, dF_ij / dx_{a,i} = G[a][j] with
+    G[0] = -sum of the rows of Dm^-1, G[a] = row a-1 of Dm^-1
+    (femEnergy.cu __computePFPX3D)."""
+    Dm = np.asarray(dminv, np.float64).reshape(3, 3)
+    G = np.vstack([-Dm.sum(axis=0), Dm])
+    D = np.zeros((9, 12))
+    for a in range(4):
+        for i in range(3):
+            for j in range(3):
+                D[i + 3 * j, 3 * a + i] = G[a, j]
+    return D
+
+
+def element_stiffness(dminv, vol, F, mu, lam):
+    """12x12 PSD element matrix vol * D^T proj(H) D (rows: vertex a, axis i)."""
+    D = shape_gradient(dminv)
+    return vol * D.T @ project_psd(snh_hessian(np.asarray(F).reshape(3, 3), mu, lam)) @ D
+
+
+def element_param_grad(dminv, vol, F, mu, lam, x12, y12):
+    """(-y^T dK/dmu x, -y^T dK/dlambda x) for one tet: the element term of
+    dL/dtheta = -lambda^T (dH/dtheta) x of an adjoint solve."""
+    F = np.asarray(F).reshape(3, 3)
+    D = shape_gradient(dminv)
+    H = snh_hessian(F, mu, lam)
+    dmu, dlam = snh_hessian_dtheta(F)
+    a, b = D @ np.asarray(x12), D @ np.asarray(y12)
+    return (-vol * b @ project_psd_derivative(H, dmu) @ a,
+            -vol * b @ project_psd_derivative(H, dlam) @ a)
+
+
+def fem_assemble_dense(n, tets, dminv, vol, F, mu, lam, diag=None):
+    """Dense (3n x 3n) sum of element matrices (+ diag on the diagonal)."""
+    A = np.zeros((3 * n, 3 * n))
+    for t, tv in enumerate(np.asarray(tets).reshape(-1, 4)):
+        K = element_stiffness(dminv[t], vol[t], F[t], mu[t], lam[t])
+        idx = np.concatenate([np.arange(3 * v, 3 * v + 3) for v in tv])
+        A[np.ix_(idx, idx)] += K
+    if diag is not None:
+        A[np.diag_indices(3 * n)] += np.asarray(diag, np.float64)
+    return A

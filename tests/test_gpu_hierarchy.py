@@ -153,6 +153,18 @@ def test_device_hierarchy_rejects_bad_input():
     r2p_bad[[0, 1]] = r2p_bad[[1, 0]]      # partition maps disagree
     with pytest.raises(RuntimeError, match="disagree"):
         device_build(hier, valid, padded, start, num, lst, p2r, r2p_bad)
+    p2r_dup = np.array(p2r, np.int32)
+    p2r_dup[valid] = p2r_dup[0]            # one vertex in two slots
+    with pytest.raises(RuntimeError, match="disagree"):
+        device_build(hier, valid, padded, start, num, lst, p2r_dup, r2p)
+    # an edge listed at one end only
+    asym = np.array(lst, np.uint32)
+    v0 = 0
+    nbrs0 = set(int(u) for u in asym[start[v0]:start[v0] + num[v0]])
+    w = next(u for u in range(1, valid) if u not in nbrs0)
+    asym[start[v0]] = w
+    with pytest.raises(RuntimeError, match="not symmetric"):
+        device_build(hier, valid, padded, start, num, asym, p2r, r2p)
     # the object is still usable afterwards
     d = device_build(hier, valid, padded, start, num, lst, p2r, r2p)
     assert list(d["prefix"]) == [1, 1]

@@ -90,6 +90,14 @@ def test_library_morton_partition_equals_oracle(oracle, n, bank):
     assert np.array_equal(p2r, q2r) and np.array_equal(r2p, s2p)
 
 
+def test_mesh_structure_equals_the_generator_structure():
+    from agipc_learning_amd import adjoint
+
+    w = W.fem_grid(5, seed=1)
+    rows, cols = adjoint.mesh_structure(w["n_vertices"], w["tets"])
+    assert np.array_equal(rows, w["rows"]) and np.array_equal(cols, w["cols"])
+
+
 def test_library_rejects_bad_input_with_an_error():
     with pytest.raises(RuntimeError):
         native.hierarchy_host(10, 5, np.zeros(10), np.zeros(10), np.zeros(1), np.arange(5), np.arange(10))

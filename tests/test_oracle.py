@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import hier_ref
+from agipc_learning_amd import workloads as W
 from chain_fixture import REFERENCE_ITERATIONS, REFERENCE_TRUE_REL_RES, padded_chain
 
 
@@ -488,3 +489,90 @@ def test_converter_oracle_invariants(oracle, drop_lower):
     for i, a, b in once[:20]:
         j = np.nonzero((ur == a) & (uc == b))[0][0]
         assert np.array_equal(uv[j], v[i])
+
+
+# ------------------------------------------------ Stable Neo-Hookean oracle --
+
+def _rand_tet(rng, far=False):
+    F = rng.standard_normal((3, 3)) if far else np.eye(3) + 0.1 * rng.standard_normal((3, 3))
+    dm = np.linalg.inv(0.0125 * np.eye(3) + 0.002 * rng.standard_normal((3, 3))).reshape(-1)
+    return F, dm, 3e-7, 3.5e4 * np.exp(0.5 * rng.standard_normal()), 1.4e5 * np.exp(0.5 * rng.standard_normal())
+
+
+def test_snh_hessian_is_the_stress_derivative(oracle):
+    # H = d vec(P) / d vec(F) for the reference's first Piola stress, by
+    # central differences (a dropped term or a transposed cofactor fails)
+    rng = np.random.default_rng(11)
+    for t in range(20):
+        F, _, _, mu, lam = _rand_tet(rng, far=t % 4 == 0)
+        H = oracle.snh_hessian(F, mu, lam)
+        h = 1e-6
+        Hfd = np.zeros((9, 9))
+        for k in range(9):
+            dF = np.zeros(9)
+            dF[k] = h
+            dF = dF.reshape(3, 3, order="F")
+            Hfd[:, k] = ((oracle.snh_stress(F + dF, mu, lam) - oracle.snh_stress(F - dF, mu, lam)) / (2 * h)).reshape(
+                -1, order="F")
+        assert np.abs(H - Hfd).max() <= 1e-7 * np.abs(H).max()
+
+
+def test_snh_element_matrix_properties(oracle):
+    rng = np.random.default_rng(12)
+    for t in range(30):
+        F, dm, vol, mu, lam = _rand_tet(rng, far=t % 5 == 0)
+        K = oracle.element_stiffness(dm, vol, F, mu, lam)
+        assert np.allclose(K, K.T, rtol=0, atol=1e-12 * np.abs(K).max())
+        assert np.linalg.eigvalsh(K).min() >= -1e-9 * np.abs(K).max()       # PSD after the projection
+        for axis in range(3):                                                # rigid translations cost nothing
+            tr = np.zeros(12)
+            tr[axis::3] = 1.0
+            assert np.abs(K @ tr).max() <= 1e-9 * np.abs(K).max()
+        # the projection is the identity on an already PSD Hessian
+        H = oracle.snh_hessian(F, mu, lam)
+        if np.linalg.eigvalsh(H).min() > 0:
+            assert np.allclose(oracle.project_psd(H), H, rtol=0, atol=1e-9 * np.abs(H).max())
+
+
+def test_snh_element_matrix_equals_the_workload_generator(oracle):
+    # the generator (Cholesky test + Jacobi eigen clamp) and the oracle
+    # (dense eigh clamp) are independent routes to the same matrix
+    rng = np.random.default_rng(13)
+    for t in range(30):
+        F, dm, vol, mu, lam = _rand_tet(rng, far=t % 5 == 0)
+        K = oracle.element_stiffness(dm, vol, F, mu, lam)
+        Kg = W.element_hessian(dm, vol, F.reshape(-1), mu, lam)
+        assert np.abs(K - Kg).max() <= 1e-11 * np.abs(K).max()
+
+
+def test_snh_param_grad_is_the_derivative_of_the_projected_matrix(oracle):
+    # -y^T dK/dtheta x against central differences of the projected element
+    # matrix (62% of these tets have a clamped eigenvalue)
+    rng = np.random.default_rng(14)
+    for t in range(40):
+        F, dm, vol, mu, lam = _rand_tet(rng, far=t % 8 == 0)
+        x, y = rng.standard_normal(12), rng.standard_normal(12)
+        g = oracle.element_param_grad(dm, vol, F, mu, lam, x, y)
+        h = 1e-6
+        fd = [-(y @ (oracle.element_stiffness(dm, vol, F, mu * (1 + h), lam)
+                     - oracle.element_stiffness(dm, vol, F, mu * (1 - h), lam)) @ x) / (2 * h * mu),
+              -(y @ (oracle.element_stiffness(dm, vol, F, mu, lam * (1 + h))
+                     - oracle.element_stiffness(dm, vol, F, mu, lam * (1 - h))) @ x) / (2 * h * lam)]
+        scale = max(abs(fd[0]), abs(fd[1]))
+        assert abs(g[0] - fd[0]) <= 1e-5 * scale and abs(g[1] - fd[1]) <= 1e-5 * scale
+
+
+def test_adjoint_gradient_equals_finite_differences_of_the_loss(oracle):
+    # L = 1/2 ||x - x*||^2, H(mu, lambda) x = b on a 3^3 mesh in FP64: the
+    # adjoint gradient against central differences (h = 1e-4) of L itself
+    import fem_ref
+
+    w = W.fem_grid(3, young_sigma=0.5, seed=4)
+    rng = np.random.default_rng(0)
+    mu, lam = w["mu"].copy(), w["lam"].copy()
+    b, xstar = w["rhs"], rng.standard_normal(len(w["rhs"]))
+    x, y, gm, gl, cf = fem_ref.adjoint_gradient(w, mu, lam, b, xstar)
+    sel = rng.choice(len(mu), 30, replace=False)
+    fd = fem_ref.fd_gradient(w, mu, lam, b, xstar, sel, h=1e-4, cf=cf)
+    ad = np.stack([gm[sel], gl[sel]], 1)
+    assert np.abs(ad - fd).max() <= 1e-6 * np.abs(fd).max()
