@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 64-system measurement")
     ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-differentiated solve")
+    ap.add_argument("--adjoint-n", type=int, default=60, help="hex cells per edge of the adjoint workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -179,18 +181,42 @@ def batch64(args):
     return out
 
 
-def hybrid(args):
+def hybrid_oracle(w, tol):
+    """The FP64 oracle's solution of the hybrid system (ABD block-Jacobi + MAS,
+    same algorithm), the reference the hybrid leg's error is measured against."""
+    import numpy as np
+    from agipc_learning_amd import session
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+
+    O.build()
+    nv = w["n_vertices"]
+    p2r, r2p = O.morton_partition(w["positions"])
+    st_, nu_, li_ = O.mesh_neighbors(nv, w["tets"])
+    mas = O.Mas(nv, len(p2r), st_, nu_, li_, p2r, r2p)
+    mas.reorder(w["pairs"])
+    fem = (w["rows"] >= w["fem_offset"]) & (w["cols"] >= w["fem_offset"])
+    mas.assemble(w["vals"], w["rows"], w["cols"], indices=np.nonzero(fem)[0].astype(np.uint32),
+                 offset=w["fem_offset"])
+    mas.invert(1)
+    abd = O.abd_block_inverse(w["n_bodies"], w["vals"], w["rows"], w["cols"])
+    xo, so = O.pcg(w["block_rows"], w["vals"], w["rows"], w["cols"], session.rhs_of(w), n_abd=w["n_bodies"],
+                   abd_inv=abd, mas=mas, fem_offset=w["fem_offset"], tol=tol, rule=1)
+    return xo, so["iterations"]
+
+
+def hybrid(args, w, oracle=None):
     """ABD + FEM system: 2,000 affine bodies (dense SPD 12x12 diagonal blocks),
     30,000 body-pair couplings, a 20^3 FEM block (9,261 vertices) coupled to
     200 bodies; ABD 12x12 block-Jacobi + MAS on the FEM part.  Plain FP32 vs
-    FP16 inverse storage; the solution is compared with the FP64 oracle's."""
+    FP16 inverse storage; `oracle` = (FP64 oracle solution, its iterations)
+    from the CPU leg gives the solution error."""
     import numpy as np
     import torch
     from agipc_learning_amd import session
-    from agipc_learning_amd import workloads as W
 
-    log("step: hybrid generate + setup")
-    w = W.hybrid(seed=0)
+    log("step: hybrid setup")
     s = session.solver_for(w)
     vals = torch.from_numpy(np.ascontiguousarray(w["vals"])).cuda()
     b = torch.from_numpy(session.rhs_of(w)).cuda()
@@ -215,28 +241,113 @@ def hybrid(args):
                      "failure": st["failure"]}
         log(f"  hybrid {prec}: {ms:.3f} ms/step, iterations {st['iterations']}, true rel res {st['rel_res_true']:.2e}")
     out["fp16_speedup_vs_fp32"] = round(out["fp32"]["ms"] / out["fp16"]["ms"], 3)
-    if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle as O
-
-        O.build()
-        nv = w["n_vertices"]
-        p2r, r2p = O.morton_partition(w["positions"])
-        st_, nu_, li_ = O.mesh_neighbors(nv, w["tets"])
-        mas = O.Mas(nv, len(p2r), st_, nu_, li_, p2r, r2p)
-        mas.reorder(w["pairs"])
-        fem = (w["rows"] >= w["fem_offset"]) & (w["cols"] >= w["fem_offset"])
-        mas.assemble(w["vals"], w["rows"], w["cols"], indices=np.nonzero(fem)[0].astype(np.uint32),
-                     offset=w["fem_offset"])
-        mas.invert(1)
-        abd = O.abd_block_inverse(w["n_bodies"], w["vals"], w["rows"], w["cols"])
-        xo, so = O.pcg(w["block_rows"], w["vals"], w["rows"], w["cols"], session.rhs_of(w), n_abd=w["n_bodies"],
-                       abd_inv=abd, mas=mas, fem_offset=w["fem_offset"], tol=args.tol, rule=1)
+    if oracle is not None:
+        xo, it = oracle
         for prec in xs:
             out[prec]["solution_rel_err_vs_oracle"] = float(np.linalg.norm(xs[prec] - xo) / np.linalg.norm(xo))
-        out["oracle_iterations"] = so["iterations"]
-        log(f"  hybrid oracle: {so['iterations']} iterations, errors "
-            f"{[out[p]['solution_rel_err_vs_oracle'] for p in xs]}")
+        out["oracle_iterations"] = it
+        log(f"  hybrid oracle: {it} iterations, errors {[out[p]['solution_rel_err_vs_oracle'] for p in xs]}")
+    return out
+
+
+def adjoint_leg(args):
+    """Adjoint-differentiated solve: 60^3 grid (226,981 vertices, 1,296,000
+    tets), per-tet Lame parameters lognormal (sigma 0.5) around E = 1e5 Pa,
+    nu = 0.4; L = 1/2 ||x - x*||^2, x* ~ N(0, 1).  One step = forward
+    (Stable Neo-Hookean assembly on the GPU, MAS build, MAS-PCG) + backward
+    (adjoint MAS-PCG with the same preconditioner, gradient kernel) through
+    the torch.autograd.Function.  The production gradient (both solves at the
+    tolerance) is compared with the FP64-refined one (both solves to 1e-9)."""
+    import numpy as np
+    import torch
+    from agipc_learning_amd import adjoint
+    from agipc_learning_amd import workloads as W
+
+    n = args.adjoint_n
+    log(f"step: adjoint generate {n}^3 grid")
+    w = W.fem_grid(n, young_sigma=0.5, seed=0)
+    log(f"step: adjoint setup ({w['n_vertices']} vertices, {len(w['tets'])} tets)")
+    t0 = time.perf_counter()
+    system = adjoint.ElasticSystem.from_workload(w, tol=args.tol)
+    setup_ms = (time.perf_counter() - t0) * 1e3
+    rng = np.random.default_rng(0)
+    xstar = torch.from_numpy(rng.standard_normal(3 * w["n_vertices"])).cuda()
+    b = torch.from_numpy(np.ascontiguousarray(w["rhs"])).cuda()
+    mu0 = torch.from_numpy(np.ascontiguousarray(w["mu"])).cuda()
+    lam0 = torch.from_numpy(np.ascontiguousarray(w["lam"])).cuda()
+
+    def step():
+        mu = mu0.clone().requires_grad_(True)
+        lam = lam0.clone().requires_grad_(True)
+        x = adjoint.solve(mu, lam, system, b)
+        loss = 0.5 * ((x - xstar) ** 2).sum()
+        loss.backward()
+        return mu.grad, lam.grad, float(loss)
+
+    step()
+    torch.cuda.synchronize()
+    reps = max(2, args.batch_steps)
+    t = time.perf_counter()
+    for _ in range(reps):
+        gm, gl, loss = step()
+    torch.cuda.synchronize()
+    step_ms = (time.perf_counter() - t) * 1e3 / reps
+    fwd, adj = dict(system.last_stats), dict(system.adjoint_stats)
+    # parts: assembly, build, each solve, gradient kernel (device events on
+    # the stream the library calls here run on)
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+
+    def timed(fn, k=3):
+        a, z = ev(), ev()
+        fn()
+        a.record()
+        for _ in range(k):
+            fn()
+        z.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(z) / k
+
+    mu_d, lam_d = mu0.detach(), lam0.detach()
+    assemble_ms = timed(lambda: system.fem.assemble(system.dminv, system.vol, system.F, mu_d, lam_d,
+                                                    base=system.base, out=system.vals))
+    t = time.perf_counter()
+    for _ in range(3):
+        system.build(mu_d, lam_d)
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t) * 1e3 / 3
+    t = time.perf_counter()
+    for _ in range(3):
+        x = system.solve(b)
+    torch.cuda.synchronize()
+    solve_ms = (time.perf_counter() - t) * 1e3 / 3
+    y = system.solve(x - xstar)
+    grad_ms = timed(lambda: system.param_grad(mu_d, lam_d, x, y))
+    # refined reference gradient
+    system.tol = 1e-9
+    t = time.perf_counter()
+    gm_r, gl_r, _ = step()
+    torch.cuda.synchronize()
+    refined_ms = (time.perf_counter() - t) * 1e3
+    fwd_r, adj_r = dict(system.last_stats), dict(system.adjoint_stats)
+    g = torch.cat([gm, gl])
+    g_r = torch.cat([gm_r, gl_r])
+    err = float(torch.linalg.vector_norm(g - g_r) / torch.linalg.vector_norm(g_r))
+    err_mu = float(torch.linalg.vector_norm(gm - gm_r) / torch.linalg.vector_norm(gm_r))
+    out = {"workload": f"fem{n}: {n}^3 hex grid, {w['n_vertices']} vertices, {len(w['tets'])} tets, lognormal "
+                       "Lame parameters, L = 1/2 ||x - x*||^2",
+           "dof": 3 * w["n_vertices"], "upper_blocks": int(system.nb), "setup_ms": round(setup_ms, 1),
+           "step_ms": round(step_ms, 3), "tol": args.tol,
+           "assemble_ms": round(assemble_ms, 3), "build_ms": round(build_ms, 3), "solve_ms": round(solve_ms, 3),
+           "grad_kernel_ms": round(grad_ms, 3),
+           "forward": {"iterations": fwd["iterations"], "rel_res_true": fwd["rel_res_true"]},
+           "adjoint": {"iterations": adj["iterations"], "rel_res_true": adj["rel_res_true"]},
+           "refined": {"tol": 1e-9, "step_ms": round(refined_ms, 3),
+                       "forward_iterations": fwd_r["iterations"], "forward_refinements": fwd_r["refinements"],
+                       "adjoint_iterations": adj_r["iterations"], "adjoint_refinements": adj_r["refinements"]},
+           "grad_rel_err_vs_refined": err, "grad_mu_rel_err_vs_refined": err_mu}
+    log(f"  adjoint: step {step_ms:.2f} ms (assemble {assemble_ms:.3f}, build {build_ms:.3f}, solve {solve_ms:.3f}, "
+        f"grad {grad_ms:.3f}), iterations {fwd['iterations']}/{adj['iterations']}, refined step {refined_ms:.1f} ms, "
+        f"grad rel err vs refined {err:.2e}")
     return out
 
 
@@ -387,13 +498,21 @@ def main():
     log(f"  structure change: {restructure}")
     del hier, d_in, d_pairs
 
-    cpu = None
+    from agipc_learning_amd import workloads as W
+
+    hw = None if args.no_batch else W.hybrid(seed=0)
+    cpu, hyb_oracle = None, None
     if not args.no_cpu:
+        # CPU leg: the oracle timed on one step of this workload, and the
+        # oracle's solution of the hybrid system (the hybrid leg's checker)
         log("step: cpu baseline (oracle, one thread)")
         cpu = cpu_baseline(w, args.levels, args.tol)
         log(f"  cpu: {cpu['value']:.1f} ms")
+        if hw is not None:
+            hyb_oracle = hybrid_oracle(hw, args.tol)
 
-    hyb = None if args.no_batch else hybrid(args)
+    hyb = None if hw is None else hybrid(args, hw, hyb_oracle)
+    adj = None if args.no_adjoint else adjoint_leg(args)
     batch = None
     if not args.no_batch:
         del s, vals, b, x
@@ -435,6 +554,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "hybrid": hyb,
+        "adjoint": adj,
         "batch64": batch,
     }
     print(json.dumps(out), flush=True)
