@@ -977,6 +977,50 @@ __global__ void __launch_bounds__(256) k_precond_finish(DevData d, int force)
 // gathers of a chunk are contiguous and all in flight before the r.z
 // partials are reduced.  Each entry's 3-vector product is staged in LDS and
 // thread t then sums row r0 + t's products in entry order.
+// FP64 true residual r_true = b - A x of rows [r_begin, r_end) (one thread
+// per row, rows strided by the workgroup), from the FP64 input blocks; writes
+// r_true and returns this thread's share of |r_true|^2.
+__device__ double true_residual_rows(const DevData& d, int r_begin, int r_end)
+{
+    double part = 0.0;
+    for(int row = r_begin + (int)threadIdx.x; row < r_end; row += blockDim.x)
+    {
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for(int e = d.row_ptr[row]; e < d.row_ptr[row + 1]; ++e)
+        {
+            const int code = d.entry_src[e];
+            const double* B = d.vals64 + 9 * (long long)(code & 0x7fffffff);
+            const long long c = 3 * (long long)d.col[e];
+            const double x0 = d.x[c], x1 = d.x[c + 1], x2 = d.x[c + 2];
+            if(code >= 0)
+            {
+                a0 += B[0] * x0 + B[3] * x1 + B[6] * x2;
+                a1 += B[1] * x0 + B[4] * x1 + B[7] * x2;
+                a2 += B[2] * x0 + B[5] * x1 + B[8] * x2;
+            }
+            else
+            {
+                a0 += B[0] * x0 + B[1] * x1 + B[2] * x2;
+                a1 += B[3] * x0 + B[4] * x1 + B[5] * x2;
+                a2 += B[6] * x0 + B[7] * x1 + B[8] * x2;
+            }
+        }
+        const long long o = 3 * (long long)row;
+        const double t0 = d.b64[o] - a0, t1 = d.b64[o + 1] - a1, t2 = d.b64[o + 2] - a2;
+        d.rt[o] = (float)t0; d.rt[o + 1] = (float)t1; d.rt[o + 2] = (float)t2;
+        part += t0 * t0 + t1 * t1 + t2 * t2;
+    }
+    return part;
+}
+
+// The final report can reuse the last check when it was taken at the final x
+// (a TRUE-rule system confirmed done by a true-residual check).
+__device__ __forceinline__ bool report_is_current(const SysState& st)
+{
+    return st.rule == kRuleTrueRel && st.status == kDone && st.n_true_checks > 0 && st.k_check == st.k &&
+           st.failure == kOk;
+}
+
 template <int U>
 struct SpmvChunk
 {
@@ -1053,7 +1097,18 @@ __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __re
     const int4 w  = d.wg_spmv[wg];
     const int  s  = w.x;
     SysState&  st = d.state[s];
-    if(st.status != kRunning) return;
+    if(st.status != kRunning)
+    {
+        // a convergence candidate from this iteration's preconditioner step:
+        // the tile's rows of the FP64 true residual, decided by k_update
+        if(st.status == kCandidate && st.need_check == 1)
+        {
+            double part = true_residual_rows(d, w.z, w.w);
+            part = block_sum_last(part, smem);
+            if(threadIdx.x == 0) d.part_tr[wg] = part;
+        }
+        return;
+    }
     // state read before any store of this kernel (scalar loads)
     const int    st_rule = st.rule, st_frozen = st.frozen, st_beta_zero = st.beta_zero, st_k = st.k,
                  st_restarts = st.restarts;
@@ -1130,7 +1185,30 @@ __global__ void __launch_bounds__(256) k_update(DevData d, const float* __restri
     bool init = false;
     if(mode == 0)
     {
-        if(st.status != kRunning) return;
+        if(st.status != kRunning)
+        {
+            // confirm a candidate whose true residual the SpMV just formed:
+            // done if it meets the tolerance, else left to the chunk's check
+            // (need_check 2: no second in-iteration attempt)
+            if(st.status == kCandidate && st.need_check == 1 && (int)blockIdx.x == d.wg_upd_range[2 * s])
+            {
+                const double tr = sys_sum(d.part_tr, d.wg_spmv_range, s, smem);
+                if(threadIdx.x == 0)
+                {
+                    if(tr <= st.tol * st.tol * st.bb)
+                    {
+                        st.true_rr = tr;
+                        st.n_true_checks += 1;
+                        st.k_check = st.k;
+                        st.need_check = 0;
+                        st.status = kDone;
+                    }
+                    else
+                        st.need_check = 2;
+                }
+            }
+            return;
+        }
     }
     else
     {
@@ -1307,34 +1385,8 @@ __global__ void __launch_bounds__(256) k_true_residual(DevData d, int report_onl
     const int  s = w.x;
     const SysState& st = d.state[s];
     if(!report_only && !st.need_check) return;
-    double part = 0.0;
-    for(int row = w.z + (int)threadIdx.x; row < w.w; row += blockDim.x)
-    {
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-        for(int e = d.row_ptr[row]; e < d.row_ptr[row + 1]; ++e)
-        {
-            const int code = d.entry_src[e];
-            const double* B = d.vals64 + 9 * (long long)(code & 0x7fffffff);
-            const long long c = 3 * (long long)d.col[e];
-            const double x0 = d.x[c], x1 = d.x[c + 1], x2 = d.x[c + 2];
-            if(code >= 0)
-            {
-                a0 += B[0] * x0 + B[3] * x1 + B[6] * x2;
-                a1 += B[1] * x0 + B[4] * x1 + B[7] * x2;
-                a2 += B[2] * x0 + B[5] * x1 + B[8] * x2;
-            }
-            else
-            {
-                a0 += B[0] * x0 + B[1] * x1 + B[2] * x2;
-                a1 += B[3] * x0 + B[4] * x1 + B[5] * x2;
-                a2 += B[6] * x0 + B[7] * x1 + B[8] * x2;
-            }
-        }
-        const long long o = 3 * (long long)row;
-        const double t0 = d.b64[o] - a0, t1 = d.b64[o + 1] - a1, t2 = d.b64[o + 2] - a2;
-        d.rt[o] = (float)t0; d.rt[o + 1] = (float)t1; d.rt[o + 2] = (float)t2;
-        part += t0 * t0 + t1 * t1 + t2 * t2;
-    }
+    if(report_only && report_is_current(st)) return;
+    double part = true_residual_rows(d, w.z, w.w);
     part = block_sum_last(part, smem);
     if(threadIdx.x == 0) d.part_tr[blockIdx.x] = part;
 }
@@ -1346,6 +1398,7 @@ __global__ void __launch_bounds__(256) k_check_finish(DevData d, int report_only
     const int s = blockIdx.x;
     SysState& st = d.state[s];
     if(!report_only && !st.need_check) return;
+    if(report_only && report_is_current(st)) return;
     const double tr = sys_sum(d.part_tr, d.wg_spmv_range, s, smem);
     if(threadIdx.x != 0) return;
     st.true_rr = tr;
