@@ -246,31 +246,25 @@ __global__ void __launch_bounds__(256) k_gal_gather(const double* fine, double* 
 // storage precision T.  One workgroup per bank.  The symmetric matrix is
 // assembled in LDS as a packed lower triangle, then inverted in place by
 // Gauss-Jordan elimination without pivoting on register tiles: the DIM x DIM
-// matrix is spread over a 16 x 16 thread grid, thread (ty, tx) holding rows
-// ty + 16 r and columns tx + 16 c.  Step k eliminates with pivot k; its
-// pivot row and column pass through LDS (double-buffered, one barrier per
-// step).  A pivot -- the Schur-complement diagonal -- below
+// matrix is spread over a TY x 16 thread grid, thread (ty, tx) holding rows
+// ty + TY r and columns tx + 16 c (TY = 16).  Step k eliminates with pivot
+// k; its pivot row and column pass through LDS (double-buffered, one barrier
+// per step), the pivot row already divided by the pivot and with the
+// reciprocal beside it: the thread holding the pivot computes it once.  A pivot -- the Schur-complement diagonal -- below
 // max(1e-12, 1e-12 max|diag|) is raised to that floor, which inverts the same
 // modified matrix as the pivot-floored Cholesky of
 // pro/.../TraditionalMAS32Preconditioner.cu:1382-1439.  Half-precision storage
 // is scaled by the largest diagonal entry of the inverse (for an SPD inverse,
 // the largest entry in magnitude).
-template <int BANK, typename T>
-__global__ void __launch_bounds__(256) k_mas_factor(DevData d, const double* E, const int* ent_ptr, const int* ent_pos,
-                                                    const int* ent_src, void* out, float* scale_out, int* status)
+// Assembles bank `bank` as a packed lower triangle L in LDS (zero diagonal
+// -> 1) and leaves the pivot floor max(1e-12, 1e-12 max|diag|) in red[0].
+template <int BANK>
+__device__ void mas_assemble(const DevData& d, const double* E, const int* ent_ptr, const int* ent_pos,
+                             const int* ent_src, double* L, double* red, int bank)
 {
-    constexpr int DIM  = 3 * BANK;
-    constexpr int R    = DIM / 16;  // rows (and columns) per thread
-    constexpr int NL   = DIM * (DIM + 1) / 2;
-    constexpr int NBLK = BANK * (BANK + 1) / 2;
-    constexpr bool kHalf = !std::is_same<T, float>::value;
-    static_assert(DIM % 16 == 0, "bank size must make 3*bank a multiple of 16");
-    __shared__ double L[NL];
-    __shared__ double rowk[2][DIM], colk[2][DIM];
-    __shared__ double red[8];
-    const int bank = blockIdx.x;
-    const int tid  = threadIdx.x;
-    const int tx = tid & 15, ty = tid >> 4;
+    constexpr int DIM = 3 * BANK;
+    constexpr int NL  = DIM * (DIM + 1) / 2;
+    const int tid = threadIdx.x;
     auto at = [](int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; };
     for(int i = tid; i < NL; i += blockDim.x) L[i] = 0.0;
     __syncthreads();
@@ -307,74 +301,18 @@ __global__ void __launch_bounds__(256) k_mas_factor(DevData d, const double* E, 
         for(int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
         if(tid == 0) red[0] = fmax(1.0e-12, 1.0e-12 * m);
     }
-    double a[R][R];
-#pragma unroll
-    for(int r = 0; r < R; ++r)
-#pragma unroll
-        for(int c = 0; c < R; ++c) a[r][c] = L[at(ty + 16 * r, tx + 16 * c)];
-    __syncthreads();
-    const double floor_value = red[0];
-    int bad = 0;
-#pragma unroll
-    for(int kr = 0; kr < R; ++kr)
-    {
-#pragma unroll 1
-        for(int kt = 0; kt < 16; ++kt)
-        {
-            const int k = kt + 16 * kr, buf = k & 1;
-            if(tx == kt)
-            {
-#pragma unroll
-                for(int r = 0; r < R; ++r) colk[buf][ty + 16 * r] = a[r][kr];
-            }
-            if(ty == kt)
-            {
-#pragma unroll
-                for(int c = 0; c < R; ++c) rowk[buf][tx + 16 * c] = a[kr][c];
-            }
-            __syncthreads();
-            double piv = rowk[buf][k];
-            if(!(piv >= floor_value) || isinf(piv))  // also catches NaN
-            {
-                piv = floor_value;
-                ++bad;
-            }
-            const double ip = 1.0 / piv;
-            double rk[R], f[R];
-#pragma unroll
-            for(int c = 0; c < R; ++c) rk[c] = rowk[buf][tx + 16 * c] * ip;
-#pragma unroll
-            for(int r = 0; r < R; ++r) f[r] = colk[buf][ty + 16 * r];
-            // rank-1 update of every element, then the pivot column
-            // (a_ik := -a_ik / p), the pivot row (a_kj := a_kj / p) and the
-            // pivot itself (1 / p) are overwritten by the threads holding them
-#pragma unroll
-            for(int r = 0; r < R; ++r)
-#pragma unroll
-                for(int c = 0; c < R; ++c) a[r][c] -= f[r] * rk[c];
-            if(tx == kt)
-            {
-#pragma unroll
-                for(int r = 0; r < R; ++r) a[r][kr] = -(f[r] * ip);
-            }
-            if(ty == kt)
-            {
-#pragma unroll
-                for(int c = 0; c < R; ++c) a[kr][c] = rk[c];
-                if(tx == kt) a[kr][kr] = ip;
-            }
-        }
-    }
-    // the inverse is symmetric: keep its lower triangle in LDS
-#pragma unroll
-    for(int r = 0; r < R; ++r)
-#pragma unroll
-        for(int c = 0; c < R; ++c)
-        {
-            const int i = ty + 16 * r, j = tx + 16 * c;
-            if(i >= j) L[i * (i + 1) / 2 + j] = a[r][c];
-        }
-    __syncthreads();
+}
+
+// Writes the inverse (packed lower triangle L) of bank `bank` diagonal-major
+// in T, with the per-bank scale of the half-precision formats.
+template <int BANK, typename T>
+__device__ void mas_store_inverse(const double* L, void* out, float* scale_out, double* red, int bank)
+{
+    constexpr int NBLK = BANK * (BANK + 1) / 2;
+    constexpr int DIM  = 3 * BANK;
+    constexpr bool kHalf = !std::is_same<T, float>::value;
+    const int tid = threadIdx.x;
+    auto at = [](int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; };
     double scale = 1.0;
     if(kHalf)
     {
@@ -418,6 +356,106 @@ __global__ void __launch_bounds__(256) k_mas_factor(DevData d, const double* E, 
             dst[o] = lo | (hi << 16);
         }
     }
+}
+
+template <int BANK, typename T, int TY>
+__global__ void __launch_bounds__(TY * 16) k_mas_factor(DevData d, const double* E, const int* ent_ptr,
+                                                        const int* ent_pos, const int* ent_src, void* out,
+                                                        float* scale_out, int* status)
+{
+    constexpr int DIM  = 3 * BANK;
+    constexpr int RR   = DIM / TY;  // rows per thread
+    constexpr int RC   = DIM / 16;  // columns per thread
+    constexpr int NL   = DIM * (DIM + 1) / 2;
+    static_assert(DIM % 16 == 0 && DIM % TY == 0 && TY % 16 == 0, "thread grid must tile the bank matrix");
+    __shared__ double L[NL];
+    __shared__ double rowk[2][DIM], colk[2][DIM];
+    __shared__ double pinv[2];
+    __shared__ int    pflag[2];
+    __shared__ double red[1 + TY * 16 / 64];
+    const int bank = blockIdx.x;
+    const int tid  = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    auto at = [](int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; };
+    mas_assemble<BANK>(d, E, ent_ptr, ent_pos, ent_src, L, red, bank);
+    double a[RR][RC];
+#pragma unroll
+    for(int r = 0; r < RR; ++r)
+#pragma unroll
+        for(int c = 0; c < RC; ++c) a[r][c] = L[at(ty + TY * r, tx + 16 * c)];
+    __syncthreads();
+    const double floor_value = red[0];
+    int bad = 0;
+    // pivot k = kt + 16 cc: column cc of the threads with tx == kt, row
+    // rr = 16 cc / TY of the threads with ty == k - TY rr (both static in cc)
+#pragma unroll
+    for(int cc = 0; cc < RC; ++cc)
+    {
+        const int rr = 16 * cc / TY;
+#pragma unroll 1
+        for(int kt = 0; kt < 16; ++kt)
+        {
+            const int k = kt + 16 * cc, buf = k & 1;
+            const int ty_k = k - TY * rr;
+            if(tx == kt)
+            {
+#pragma unroll
+                for(int r = 0; r < RR; ++r) colk[buf][ty + TY * r] = a[r][cc];
+            }
+            if(ty == ty_k)
+            {
+                // the pivot sits in this row at column tx == kt; every lane
+                // of the row reads it from its own registers after a shuffle
+                const double pv = __shfl(a[rr][cc], (int)(threadIdx.x & ~15u & 63u) + kt, 64);
+                const bool floored = !(pv >= floor_value) || isinf(pv);  // also catches NaN
+                const double ipv = 1.0 / (floored ? floor_value : pv);
+#pragma unroll
+                for(int c = 0; c < RC; ++c) rowk[buf][tx + 16 * c] = a[rr][c] * ipv;
+                if(tx == kt)
+                {
+                    pinv[buf] = ipv;
+                    pflag[buf] = floored ? 1 : 0;
+                }
+            }
+            __syncthreads();
+            const double ip = pinv[buf];
+            bad += pflag[buf];
+            double rk[RC], f[RR];
+#pragma unroll
+            for(int c = 0; c < RC; ++c) rk[c] = rowk[buf][tx + 16 * c];
+#pragma unroll
+            for(int r = 0; r < RR; ++r) f[r] = colk[buf][ty + TY * r];
+            // rank-1 update of every element, then the pivot column
+            // (a_ik := -a_ik / p), the pivot row (a_kj := a_kj / p) and the
+            // pivot itself (1 / p) are overwritten by the threads holding them
+#pragma unroll
+            for(int r = 0; r < RR; ++r)
+#pragma unroll
+                for(int c = 0; c < RC; ++c) a[r][c] -= f[r] * rk[c];
+            if(tx == kt)
+            {
+#pragma unroll
+                for(int r = 0; r < RR; ++r) a[r][cc] = -(f[r] * ip);
+            }
+            if(ty == ty_k)
+            {
+#pragma unroll
+                for(int c = 0; c < RC; ++c) a[rr][c] = rk[c];
+                if(tx == kt) a[rr][cc] = ip;
+            }
+        }
+    }
+    // the inverse is symmetric: keep its lower triangle in LDS
+#pragma unroll
+    for(int r = 0; r < RR; ++r)
+#pragma unroll
+        for(int c = 0; c < RC; ++c)
+        {
+            const int i = ty + TY * r, j = tx + 16 * c;
+            if(i >= j) L[i * (i + 1) / 2 + j] = a[r][c];
+        }
+    __syncthreads();
+    mas_store_inverse<BANK, T>(L, out, scale_out, red, bank);
     if(tid == 0 && bad && status) atomicAdd(status, bad);
 }
 
@@ -1547,9 +1585,10 @@ hipError_t launch_mas_factor(const DevData& d, const double* E, const int* ent_p
                              const int* ent_src, void* out, float* scale, int precision, int* status, hipStream_t st)
 {
     if(d.n_banks <= 0) return hipSuccess;
-    const dim3 g(d.n_banks), b(256);
-#define MASX_FACTOR(BK, T) \
-    hipLaunchKernelGGL((k_mas_factor<BK, T>), g, b, 0, st, d, E, ent_ptr, ent_pos, ent_src, out, scale, status)
+    const dim3 g(d.n_banks);
+#define MASX_FACTOR(BK, T)                                                                                     \
+    hipLaunchKernelGGL((k_mas_factor<BK, T, 16>), g, dim3(256), 0, st, d, E, ent_ptr, ent_pos, ent_src, out, \
+                       scale, status)
     if(d.bank == 32)
     {
         if(precision == kPrecFp16) MASX_FACTOR(32, __half);
