@@ -160,6 +160,7 @@ int masx_set_topology(masx_solver* s, int sys, const uint32_t* nbr_start, const 
             const int slot = real_to_part[v];
             require(slot >= 0 && slot < part_map_size && part_to_real[slot] == v, "inconsistent partition");
         }
+        s->impl.structure_changed(sys);
     });
 }
 
@@ -199,6 +200,7 @@ int masx_set_contacts(masx_solver* s, int sys, const int* pairs, int n_pairs, in
         auto& d = s->impl.system(sys);
         d.pairs.assign(pairs, pairs + 4 * (size_t)n_pairs);
         d.node_offset = node_offset;
+        s->impl.structure_changed(sys);
     });
 }
 

@@ -449,9 +449,10 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
         fine_bank_base[s] = n_fine_banks;
         P.coarse_base[s]  = n_coarse;
         if(!d.use_mas) continue;
-        build_hierarchy(d.hier, d.n_fem, (int)d.part_to_real.size(), bank, d.levels, d.nbr_start.data(),
-                        d.nbr_num.data(), d.nbr_list.data(), d.part_to_real.data(), d.real_to_part.data(),
-                        d.pairs.data(), (int)d.pairs.size() / 4, d.node_offset);
+        if(!d.hier_ready)
+            build_hierarchy(d.hier, d.n_fem, (int)d.part_to_real.size(), bank, d.levels, d.nbr_start.data(),
+                            d.nbr_num.data(), d.nbr_list.data(), d.part_to_real.data(), d.real_to_part.data(),
+                            d.pairs.data(), (int)d.pairs.size() / 4, d.node_offset);
         n_fine_banks += d.hier.n_slots / bank;
         n_coarse += d.hier.total - d.hier.n_slots;
     }

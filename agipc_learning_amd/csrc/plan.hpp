@@ -69,6 +69,7 @@ struct SystemDesc
     std::vector<int>      pairs;        // int4 per pair
     int                   node_offset = 0;
     Hierarchy             hier;
+    bool                  hier_ready = false;  // hier already built (on the device) for this structure
 };
 
 // Global packed layout over a batch of systems (everything the kernels index).

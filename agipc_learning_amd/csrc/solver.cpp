@@ -65,6 +65,64 @@ void Solver::analyze()
     analyzed_ = true;
 }
 
+void Solver::structure_changed(int s)
+{
+    systems_.at(s).hier_ready = false;
+    analyzed_   = false;
+    setup_done_ = false;
+    built_      = false;
+}
+
+// The hierarchy of every MAS system from the device builder (the same arrays
+// as the host build); a system the builder rejects (neighbour lists that are
+// not symmetric, which the host build reads as undirected) keeps the host path.
+void Solver::build_hierarchies_on_device()
+{
+    hipStream_t st = work_;
+    for(auto& d : systems_)
+    {
+        if(!d.use_mas || d.n_fem <= 0 || d.hier_ready) continue;
+        if(!hier_dev_) hier_dev_ = std::make_unique<DeviceHierarchy>();
+        d_h_start_.upload(d.nbr_start, st);
+        d_h_num_.upload(d.nbr_num, st);
+        d_h_list_.upload(d.nbr_list, st);
+        d_h_p2r_.upload(d.part_to_real, st);
+        d_h_r2p_.upload(d.real_to_part, st);
+        d_h_pairs_.upload(d.pairs, st);
+        const int n_slots = (int)d.part_to_real.size(), n_pairs = (int)d.pairs.size() / 4;
+        try
+        {
+            hier_dev_->build(d.n_fem, n_slots, d.bank, d.levels, d_h_start_.ptr, d_h_num_.ptr, d_h_list_.ptr,
+                             (long long)d.nbr_list.size(), d_h_p2r_.ptr, d_h_r2p_.ptr, n_pairs > 0 ? d_h_pairs_.ptr : nullptr,
+                             n_pairs, d.node_offset, st);
+        }
+        catch(const std::invalid_argument&)
+        {
+            continue;
+        }
+        Hierarchy& h = d.hier;
+        const DeviceHierarchy& g = *hier_dev_;
+        h.bank    = d.bank;
+        h.levels  = g.levels();
+        h.n_fem   = d.n_fem;
+        h.n_slots = n_slots;
+        h.total   = g.total();
+        h.level_count.assign(h.levels + 1, 0);
+        h.level_begin.assign(h.levels + 1, 0);
+        h.fine_mask.assign(d.n_fem, 0u);
+        h.prefix.assign(n_slots / d.bank, 0);
+        h.going_next.assign(std::max(h.total, d.n_fem), -1);
+        h.coarse_table.assign(6 * (size_t)d.n_fem, 0);
+        h.comp_ptr.assign(h.total / d.bank + 1, 0);
+        h.comp_mask.assign(g.n_components(), 0u);
+        h.comp_parent.assign(g.n_components(), 0);
+        g.download(h.level_count.data(), h.level_begin.data(), h.fine_mask.data(), h.prefix.data(),
+                   h.going_next.data(), h.coarse_table.data(), h.comp_ptr.data(), h.comp_mask.data(),
+                   h.comp_parent.data(), st);
+        d.hier_ready = true;
+    }
+}
+
 Solver::~Solver()
 {
     if(graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
@@ -135,8 +193,12 @@ Solver::Tiling Solver::autotune_spmv(int reps)
 
 void Solver::setup()
 {
-    if(!analyzed_) analyze();
     ensure_stream();
+    if(!analyzed_)
+    {
+        build_hierarchies_on_device();
+        analyze();
+    }
     const Plan& P = plan_;
     hipStream_t st = work_;
     d_row_ptr_.upload(P.row_ptr, st);

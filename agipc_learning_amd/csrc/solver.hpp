@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "device.hpp"
+#include "hierarchy.hpp"
 #include "plan.hpp"
 
 namespace masx
@@ -64,7 +65,11 @@ class Solver
     SystemDesc& system(int s) { return systems_.at(s); }
     int  n_systems() const { return (int)systems_.size(); }
     void analyze();                    // host-only structure analysis (no device needed)
-    void setup();                      // analyze (if needed) + allocation and upload
+    // analyze (if needed; the MAS hierarchies are then built on the device)
+    // + allocation and upload
+    void setup();
+    // topology or contacts of system s changed: the next setup() re-analyzes
+    void structure_changed(int s);
     bool is_setup() const { return setup_done_; }
     bool is_analyzed() const { return analyzed_; }
     void set_values(int s, const double* d_values, hipStream_t st);      // device FP64 col-major
@@ -131,6 +136,11 @@ class Solver
     DeviceArray<double>    d_part_pre_, d_part_spmv_, d_part_upd_, d_part_tr_;
     DeviceArray<SysState>  d_state_;
     DeviceArray<int>       d_max_iter_, d_factor_status_;
+    // device MAS hierarchy build of setup(): inputs and the builder's scratch
+    void build_hierarchies_on_device();
+    DeviceArray<uint32_t>  d_h_start_, d_h_num_, d_h_list_;
+    DeviceArray<int>       d_h_p2r_, d_h_r2p_, d_h_pairs_;
+    std::unique_ptr<DeviceHierarchy> hier_dev_;
     int n_wg_pre_ = 0, n_wg_spmv_ = 0, n_wg_upd_ = 0;
     DevData dd_{};
 

@@ -453,22 +453,36 @@ def main():
     nbr = W.neighbors(w2["n_vertices"], w2["tets"])
     vals2 = torch.from_numpy(np.ascontiguousarray(w2["vals"])).cuda()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    s2 = native.Solver()
-    s2.add_system(block_rows=w2["block_rows"], rows=w2["rows"], cols=w2["cols"], use_mas=True, levels=args.levels)
-    s2.set_topology_morton(0, *nbr, w2["positions"])
-    s2.set_contacts(0, w2["pairs"])
-    t1 = time.perf_counter()
-    s2.analyze()
-    t2 = time.perf_counter()
-    s2.setup()
-    s2.set_values(0, vals2)
-    s2.build("fp32")
-    torch.cuda.synchronize()
-    t3 = time.perf_counter()
-    restructure = {"ms": round((t3 - t0) * 1e3, 2), "host_analysis_ms": round((t2 - t1) * 1e3, 2),
-                   "device_setup_and_build_ms": round((t3 - t2) * 1e3, 2)}
-    del s2, vals2
+    def restructure_once():
+        t0 = time.perf_counter()
+        s2 = native.Solver()
+        s2.add_system(block_rows=w2["block_rows"], rows=w2["rows"], cols=w2["cols"], use_mas=True,
+                      levels=args.levels)
+        s2.set_topology_morton(0, *nbr, w2["positions"])
+        s2.set_contacts(0, w2["pairs"])
+        t1 = time.perf_counter()
+        s2.setup()  # MAS hierarchy on the device, host plan, device upload
+        t2 = time.perf_counter()
+        s2.set_values(0, vals2)
+        s2.build("fp32")
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        return (t3 - t0) * 1e3, (t2 - t1) * 1e3, (t3 - t2) * 1e3
+
+    restructure_once()
+    times = sorted((restructure_once() for _ in range(3)), key=lambda r: r[0])
+    tot, setup_ms, build_ms = times[1]
+    # the host-only analysis (hierarchy built on the host), for comparison
+    s3 = native.Solver()
+    s3.add_system(block_rows=w2["block_rows"], rows=w2["rows"], cols=w2["cols"], use_mas=True, levels=args.levels)
+    s3.set_topology_morton(0, *nbr, w2["positions"])
+    s3.set_contacts(0, w2["pairs"])
+    t = time.perf_counter()
+    s3.analyze()
+    host_analysis_ms = (time.perf_counter() - t) * 1e3
+    restructure = {"ms": round(tot, 2), "setup_ms": round(setup_ms, 2), "values_and_build_ms": round(build_ms, 2),
+                   "host_only_analysis_ms": round(host_analysis_ms, 2)}
+    del s3, vals2
     # the MAS hierarchy alone (the reference's per-step ReorderRealtime): host
     # build vs the device build, inputs resident where each one runs
     p2r, r2p = native.morton_partition(w2["positions"])
