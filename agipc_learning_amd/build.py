@@ -39,8 +39,8 @@ def build_masx(jobs: int = 4):
         if src.endswith(".hip"):
             cmd = [HIPCC, f"--offload-arch={ARCH}", *common, "-Xclang", "-target-feature", "-Xclang",
                    "-packed-fp32-ops", "-c", os.path.join(CSRC, src), "-o", obj]
-        else:
-            cmd = [HIPCC, f"--offload-arch={ARCH}", *common, "-c", os.path.join(CSRC, src), "-o", obj]
+        else:  # host code; the structure analysis uses OpenMP
+            cmd = [HIPCC, f"--offload-arch={ARCH}", *common, "-fopenmp", "-c", os.path.join(CSRC, src), "-o", obj]
         print("+", " ".join(cmd), flush=True)
         procs.append(subprocess.Popen(cmd))
         if len(procs) >= jobs:
@@ -52,7 +52,7 @@ def build_masx(jobs: int = 4):
         if p.wait() != 0:
             raise RuntimeError("compilation failed")
     out = os.path.join(LIB, "libmasx.so")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out + ".tmp", *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-fopenmp", "-o", out + ".tmp", *objs])
     os.replace(out + ".tmp", out)
     for o in objs:
         os.remove(o)

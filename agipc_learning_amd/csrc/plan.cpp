@@ -2,6 +2,8 @@
 #include "plan.hpp"
 #include "pair_code.hpp"
 
+#include <omp.h>
+
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -16,6 +18,44 @@ inline int pad_to(int v, int b) { return (v + b - 1) / b * b; }
 inline int packed_index(int bank, int r, int c) { return bank * r - r * (r + 1) / 2 + c; }
 inline uint32_t lane_bit(int l) { return 1u << l; }
 }  // namespace
+
+// Stable parallel counting sort: item k (k < n) goes to up to two buckets
+// (emit(k, sink) calls sink(bucket, slot) for each); buckets keep items in k
+// order.  Returns the bucket pointers (nbuckets + 1); place(k, slot, pos)
+// receives every item's final position.
+template <class Emit, class Place>
+std::vector<long long> bucket_scatter(long long n, int nbuckets, Emit emit, Place place)
+{
+    const int T = std::max(1, std::min(omp_get_max_threads(), (int)((n + 65535) / 65536)));
+    std::vector<long long> cnt((size_t)T * (nbuckets + 1), 0);
+    const long long chunk = (n + T - 1) / T;
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+    for(int t = 0; t < T; ++t)
+    {
+        long long* c = cnt.data() + (size_t)t * (nbuckets + 1);
+        for(long long k = t * chunk; k < std::min(n, (t + 1) * chunk); ++k) emit(k, [&](int b, int) { ++c[b]; });
+    }
+    std::vector<long long> ptr(nbuckets + 1, 0);
+    for(int b = 0; b < nbuckets; ++b)
+    {
+        long long acc = ptr[b];
+        for(int t = 0; t < T; ++t)
+        {
+            const long long v = cnt[(size_t)t * (nbuckets + 1) + b];
+            cnt[(size_t)t * (nbuckets + 1) + b] = acc;
+            acc += v;
+        }
+        ptr[b + 1] = acc;
+    }
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+    for(int t = 0; t < T; ++t)
+    {
+        long long* c = cnt.data() + (size_t)t * (nbuckets + 1);
+        for(long long k = t * chunk; k < std::min(n, (t + 1) * chunk); ++k)
+            emit(k, [&](int b, int slot) { place(k, slot, c[b]++); });
+    }
+    return ptr;
+}
 
 int reference_level_count(int vert_num, int part_map_size, int bank)
 {
@@ -261,23 +301,6 @@ void build_hierarchy(Hierarchy& h, int n_fem, int n_slots, int bank, int levels,
 
 // ----------------------------------------------------------------- plan ----
 
-// Stable LSD radix sort of records with a non-negative 64-bit `key` below
-// key_bound (12-bit digits, as many passes as the bound needs).
-template <typename T>
-static void radix_sort_stable(std::vector<T>& a, unsigned long long key_bound)
-{
-    constexpr int kBits = 12, kBuckets = 1 << kBits;
-    std::vector<T> tmp(a.size());
-    std::vector<size_t> cnt(kBuckets + 1);
-    for(int shift = 0; shift < 64 && (key_bound >> shift) != 0; shift += kBits)
-    {
-        std::fill(cnt.begin(), cnt.end(), 0);
-        for(const T& x : a) ++cnt[((unsigned long long)x.key >> shift & (kBuckets - 1)) + 1];
-        for(int k = 0; k < kBuckets; ++k) cnt[k + 1] += cnt[k];
-        for(const T& x : a) tmp[cnt[(unsigned long long)x.key >> shift & (kBuckets - 1)]++] = x;
-        a.swap(tmp);
-    }
-}
 
 void build_plan(Plan& P, std::vector<SystemDesc>& sys)
 {
@@ -329,52 +352,59 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
         for(int r = P.row_off[s]; r < P.row_off[s + 1]; ++r) P.row_sys[r] = s;
     }
     // ---- full block CSR (both triangles)
-    std::vector<int> cnt(P.n_rows + 1, 0);
     for(int s = 0; s < S; ++s)
     {
         auto& d = sys[s];
         if((long long)d.rows.size() != d.nb || (long long)d.cols.size() != d.nb)
             throw std::invalid_argument("row/col arrays do not match the block count");
+        int bad = 0;
+#pragma omp parallel for reduction(| : bad) schedule(static)
         for(long long k = 0; k < d.nb; ++k)
-        {
-            if(d.rows[k] < 0 || d.rows[k] >= d.block_rows || d.cols[k] < 0 || d.cols[k] >= d.block_rows)
-                throw std::invalid_argument("block index out of range");
-            const int R = P.ext_to_int[P.ext_off[s] + d.rows[k]];
-            const int C = P.ext_to_int[P.ext_off[s] + d.cols[k]];
-            cnt[R + 1]++;
-            if(d.rows[k] != d.cols[k]) cnt[C + 1]++;
-        }
+            if(d.rows[k] < 0 || d.rows[k] >= d.block_rows || d.cols[k] < 0 || d.cols[k] >= d.block_rows) bad |= 1;
+        if(bad) throw std::invalid_argument("block index out of range");
     }
-    P.row_ptr.assign(P.n_rows + 1, 0);
-    for(int r = 0; r < P.n_rows; ++r) P.row_ptr[r + 1] = P.row_ptr[r] + cnt[r + 1];
-    P.nnz = P.row_ptr[P.n_rows];
-    P.col.assign(P.nnz, 0);
-    P.entry_src.assign(P.nnz, 0);
     {
-        std::vector<int> fill(P.row_ptr.begin(), P.row_ptr.end() - 1);
-        for(int s = 0; s < S; ++s)
-        {
-            auto& d = sys[s];
-            for(long long k = 0; k < d.nb; ++k)
-            {
-                const int g = (int)(P.src_off[s] + k);
-                const int R = P.ext_to_int[P.ext_off[s] + d.rows[k]];
-                const int C = P.ext_to_int[P.ext_off[s] + d.cols[k]];
-                int e = fill[R]++;
-                P.col[e] = C;
-                P.entry_src[e] = g;
-                if(d.rows[k] != d.cols[k])
-                {
-                    e = fill[C]++;
-                    P.col[e] = R;
-                    P.entry_src[e] = g | (int)0x80000000u;
-                }
-            }
-        }
+        // items: the upper blocks of all systems; block k of system s lands in
+        // row R as (C, g) and, off the diagonal, in row C as (R, g | T)
+        std::vector<long long> item_base(S + 1, 0);
+        for(int s = 0; s < S; ++s) item_base[s + 1] = item_base[s] + sys[s].nb;
+        auto locate = [&](long long k, int& s_, long long& kk) {
+            s_ = (int)(std::upper_bound(item_base.begin(), item_base.end(), k) - item_base.begin()) - 1;
+            kk = k - item_base[s_];
+        };
+        auto emit = [&](long long k, auto&& sink) {
+            int s_;
+            long long kk;
+            locate(k, s_, kk);
+            const auto& d = sys[s_];
+            const int R = P.ext_to_int[P.ext_off[s_] + d.rows[kk]];
+            const int C = P.ext_to_int[P.ext_off[s_] + d.cols[kk]];
+            sink(R, 0);
+            if(d.rows[kk] != d.cols[kk]) sink(C, 1);
+        };
+        P.col.assign(2 * item_base[S], 0);
+        P.entry_src.assign(2 * item_base[S], 0);
+        auto ptr = bucket_scatter(item_base[S], P.n_rows, emit, [&](long long k, int slot, long long pos) {
+            int s_;
+            long long kk;
+            locate(k, s_, kk);
+            const auto& d = sys[s_];
+            const int g = (int)(P.src_off[s_] + kk);
+            const int R = P.ext_to_int[P.ext_off[s_] + d.rows[kk]];
+            const int C = P.ext_to_int[P.ext_off[s_] + d.cols[kk]];
+            P.col[pos] = slot == 0 ? C : R;
+            P.entry_src[pos] = slot == 0 ? g : (g | (int)0x80000000u);
+        });
+        P.row_ptr.assign(P.n_rows + 1, 0);
+        for(int r = 0; r <= P.n_rows; ++r) P.row_ptr[r] = (int)ptr[r];
+        P.nnz = ptr[P.n_rows];
+        P.col.resize(P.nnz);
+        P.entry_src.resize(P.nnz);
         // sort each row by column (stable) for gather locality
-        std::vector<std::pair<int, int>> tmp;
+#pragma omp parallel for schedule(dynamic, 512)
         for(int r = 0; r < P.n_rows; ++r)
         {
+            thread_local std::vector<std::pair<int, int>> tmp;
             const int b = P.row_ptr[r], e = P.row_ptr[r + 1];
             tmp.resize(e - b);
             for(int i = b; i < e; ++i) tmp[i - b] = {P.col[i], P.entry_src[i]};
@@ -674,47 +704,101 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
             auto& d = sys[s];
             if(!d.use_mas || lvl >= d.hier.levels) continue;
             const auto& h = d.hier;
-            std::vector<Ent> ents;
-            auto push = [&](int I, int J, int src, bool src_diag) {
-                // canonical (min, max): as is, transposed, or B + B^T when both
-                // ends merge into one node
-                if(I < J) ents.push_back({(long long)I * h.total + J, src, 0});
-                else if(I > J) ents.push_back({(long long)J * h.total + I, src, 1});
-                else ents.push_back({(long long)I * h.total + I, src, src_diag ? 0 : 2});
-            };
-            if(lvl == 1)
-            {
-                for(long long k = 0; k < d.nb; ++k)
+            // entries (I, J, source, mode) in canonical (min, max) order: as
+            // is, transposed, or B + B^T when both ends merge into one node;
+            // bucketed by I (stable), then each bucket sorted by J (stable),
+            // which is the (I, J) order with sources in input order
+            const int lb = h.level_begin[lvl], lc = h.level_count[lvl];
+            const long long n_src = lvl == 1 ? d.nb : (long long)prev_ij[s].size();
+            auto canon = [&](long long k, int& I, int& J, int& src, int& mode) -> bool {
+                int a, b;
+                bool diag;
+                if(lvl == 1)
                 {
                     const int r = d.rows[k] - d.fem_offset, c = d.cols[k] - d.fem_offset;
-                    if(r < 0 || c < 0) continue;
-                    push(h.going_next[r], h.going_next[c], (int)(P.src_off[s] + k), r == c);
+                    if(r < 0 || c < 0) return false;
+                    a = h.going_next[r];
+                    b = h.going_next[c];
+                    diag = r == c;
+                    src = (int)(P.src_off[s] + k);
                 }
-            }
-            else
-            {
-                for(size_t e = 0; e < prev_ij[s].size(); ++e)
+                else
                 {
-                    const int I = prev_ij[s][e].first, J = prev_ij[s][e].second;
-                    push(h.going_next[I], h.going_next[J], prev_idx[s][e], I == J);
+                    a = h.going_next[prev_ij[s][k].first];
+                    b = h.going_next[prev_ij[s][k].second];
+                    diag = prev_ij[s][k].first == prev_ij[s][k].second;
+                    src = prev_idx[s][k];
                 }
-            }
-            radix_sort_stable(ents, (unsigned long long)h.total * (unsigned long long)h.total);
-            prev_ij[s].clear();
-            prev_idx[s].clear();
-            for(size_t i = 0; i < ents.size();)
+                if(a < b) { I = a; J = b; mode = 0; }
+                else if(a > b) { I = b; J = a; mode = 1; }
+                else { I = a; J = a; mode = diag ? 0 : 2; }
+                return true;
+            };
+            std::vector<Ent> ents(n_src);
+            const auto bp = bucket_scatter(
+                n_src, lc,
+                [&](long long k, auto&& sink) {
+                    int I, J, src, mode;
+                    if(canon(k, I, J, src, mode)) sink(I - lb, 0);
+                },
+                [&](long long k, int, long long pos) {
+                    int I, J, src, mode;
+                    canon(k, I, J, src, mode);
+                    ents[pos] = {(long long)J, src, mode};
+                });
+            ents.resize(bp[lc]);
+            std::vector<int> bptr(lc + 1);
+            for(int i = 0; i <= lc; ++i) bptr[i] = (int)bp[i];
+            std::vector<int> n_uniq(lc + 1, 0);
+#pragma omp parallel for schedule(dynamic, 64)
+            for(int i = 0; i < lc; ++i)
             {
-                size_t j = i;
-                while(j < ents.size() && ents[j].key == ents[i].key) ++j;
-                const int I = (int)(ents[i].key / h.total), J = (int)(ents[i].key % h.total);
-                const int eidx = (int)E_ij.size();
-                E_ij.push_back({I, J});
-                E_sys.push_back(s);
-                prev_ij[s].push_back({I, J});
-                prev_idx[s].push_back(eidx);
-                for(size_t t = i; t < j; ++t) P.gal_src.push_back((ents[t].src << 2) | ents[t].mode);
-                P.gal_ptr.push_back((int)P.gal_src.size());
-                i = j;
+                Ent* b = ents.data() + bptr[i];
+                const int n = bptr[i + 1] - bptr[i];
+                if(n <= 64)
+                    for(int x = 1; x < n; ++x)
+                    {
+                        const Ent v = b[x];
+                        int y = x;
+                        for(; y > 0 && b[y - 1].key > v.key; --y) b[y] = b[y - 1];
+                        b[y] = v;
+                    }
+                else
+                    std::stable_sort(b, b + n, [](const Ent& u, const Ent& v) { return u.key < v.key; });
+                int u = 0;
+                for(int x = 0; x < n; ++x)
+                    if(x == 0 || b[x].key != b[x - 1].key) ++u;
+                n_uniq[i + 1] = u;
+            }
+            for(int i = 0; i < lc; ++i) n_uniq[i + 1] += n_uniq[i];
+            const int e_base = (int)E_ij.size(), g_base = (int)P.gal_src.size();
+            const int n_new = n_uniq[lc];
+            E_ij.resize(e_base + n_new);
+            E_sys.resize(e_base + n_new, s);
+            prev_ij[s].assign(n_new, {0, 0});
+            prev_idx[s].assign(n_new, 0);
+            P.gal_src.resize(g_base + ents.size());
+            const size_t gp_base = P.gal_ptr.size();
+            P.gal_ptr.resize(gp_base + n_new);
+#pragma omp parallel for schedule(dynamic, 64)
+            for(int i = 0; i < lc; ++i)
+            {
+                const Ent* b = ents.data() + bptr[i];
+                const int n = bptr[i + 1] - bptr[i];
+                int q = n_uniq[i];
+                for(int x = 0; x < n; ++x)
+                {
+                    P.gal_src[g_base + bptr[i] + x] = (b[x].src << 2) | b[x].mode;
+                    if(x + 1 == n || b[x + 1].key != b[x].key)
+                    {
+                        const int I = lb + i, J = (int)b[x].key;
+                        E_ij[e_base + q] = {I, J};
+                        prev_ij[s][q] = {I, J};
+                        prev_idx[s][q] = e_base + q;
+                        P.gal_ptr[gp_base + q] = g_base + bptr[i] + x + 1;
+                        ++q;
+                    }
+                }
             }
         }
         P.gal_level_ptr.push_back((int)E_ij.size());
@@ -723,44 +807,65 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
     P.n_gal = (long long)E_ij.size();
     if((long long)P.gal_src.size() >= (1ll << 29)) throw std::invalid_argument("Galerkin map too large");
 
-    // per-bank entries for the dense local assembly
-    std::vector<std::vector<std::pair<int, int>>> be(P.n_banks);  // (pos|T<<16, src)
+    // per-bank entries for the dense local assembly: a fine bank's upper
+    // blocks (from its rows of the full CSR, as-is entries with both ends in
+    // the bank), then the coarse entries with both ends in one coarse bank.
+    // Every entry has its own packed position, so the order within a bank
+    // does not change the assembled matrix.
+    P.bank_ent_ptr.assign(P.n_banks + 1, 0);
+    std::vector<int> fine_row0_of_bank(n_fine_banks, 0);
     for(int s = 0; s < S; ++s)
     {
-        auto& d = sys[s];
-        if(!d.use_mas) continue;
-        const auto& h = d.hier;
-        for(long long k = 0; k < d.nb; ++k)
-        {
-            const int r = d.rows[k] - d.fem_offset, c = d.cols[k] - d.fem_offset;
-            if(r < 0 || c < 0) continue;
-            const int sr = d.real_to_part[r], sc = d.real_to_part[c];
-            if(sr / bank != sc / bank) continue;
-            const int gb = gbank_of_slot(s, sr);
-            const int lr = sr % bank, lc = sc % bank;
-            const int pos = lr <= lc ? packed_index(bank, lr, lc) : packed_index(bank, lc, lr);
-            be[gb].push_back({pos | ((lr > lc ? 1 : 0) << 16), (int)(P.src_off[s] + k)});
-        }
-        (void)h;
+        if(!sys[s].use_mas) continue;
+        for(int b = 0; b < sys[s].hier.n_slots / bank; ++b) fine_row0_of_bank[fine_bank_base[s] + b] = P.fem_row0[s] + b * bank;
+    }
+    auto fine_bank_entries = [&](int gb, auto&& out) {
+        const int r0 = fine_row0_of_bank[gb];
+        for(int lr = 0; lr < bank; ++lr)
+            for(int e = P.row_ptr[r0 + lr]; e < P.row_ptr[r0 + lr + 1]; ++e)
+            {
+                const int lc_ = P.col[e] - r0;
+                if(P.entry_src[e] < 0 || lc_ < 0 || lc_ >= bank) continue;  // transposed copy, or another bank
+                const int pos = lr <= lc_ ? packed_index(bank, lr, lc_) : packed_index(bank, lc_, lr);
+                out(pos | ((lr > lc_ ? 1 : 0) << 16), P.entry_src[e]);
+            }
+    };
+#pragma omp parallel for schedule(dynamic, 64)
+    for(int gb = 0; gb < n_fine_banks; ++gb)
+    {
+        int c = 0;
+        fine_bank_entries(gb, [&](int, int) { ++c; });
+        P.bank_ent_ptr[gb + 1] = c;
     }
     for(size_t e = 0; e < E_ij.size(); ++e)
     {
-        const int s = E_sys[e];
         const int I = E_ij[e].first, J = E_ij[e].second;
-        if(I / bank != J / bank) continue;
-        const int gb = gbank_of_slot(s, I);
-        be[gb].push_back({packed_index(bank, I % bank, J % bank), (int)e});
+        if(I / bank == J / bank) ++P.bank_ent_ptr[gbank_of_slot(E_sys[e], I) + 1];
     }
-    P.bank_ent_ptr.assign(P.n_banks + 1, 0);
-    for(int b = 0; b < P.n_banks; ++b) P.bank_ent_ptr[b + 1] = P.bank_ent_ptr[b] + (int)be[b].size();
+    for(int b = 0; b < P.n_banks; ++b) P.bank_ent_ptr[b + 1] += P.bank_ent_ptr[b];
     P.bank_ent_pos.resize(P.bank_ent_ptr[P.n_banks]);
     P.bank_ent_src.resize(P.bank_ent_ptr[P.n_banks]);
-    for(int b = 0; b < P.n_banks; ++b)
-        for(size_t k = 0; k < be[b].size(); ++k)
+#pragma omp parallel for schedule(dynamic, 64)
+    for(int gb = 0; gb < n_fine_banks; ++gb)
+    {
+        int at = P.bank_ent_ptr[gb];
+        fine_bank_entries(gb, [&](int code, int src) {
+            P.bank_ent_pos[at] = code;
+            P.bank_ent_src[at] = src;
+            ++at;
+        });
+    }
+    {
+        std::vector<int> cur(P.bank_ent_ptr.begin() + n_fine_banks, P.bank_ent_ptr.end() - 1);
+        for(size_t e = 0; e < E_ij.size(); ++e)
         {
-            P.bank_ent_pos[P.bank_ent_ptr[b] + k] = be[b][k].first;
-            P.bank_ent_src[P.bank_ent_ptr[b] + k] = be[b][k].second;
+            const int I = E_ij[e].first, J = E_ij[e].second;
+            if(I / bank != J / bank) continue;
+            const int at = cur[gbank_of_slot(E_sys[e], I) - n_fine_banks]++;
+            P.bank_ent_pos[at] = packed_index(bank, I % bank, J % bank);
+            P.bank_ent_src[at] = (int)e;
         }
+    }
 }
 
 
@@ -772,11 +877,15 @@ void validate_plan(const Plan& P)
     if((int)P.row_ptr.size() != P.n_rows + 1 || P.row_ptr[0] != 0 || P.row_ptr.back() != P.nnz) fail("row_ptr");
     for(int r = 0; r < P.n_rows; ++r)
         if(P.row_ptr[r + 1] < P.row_ptr[r]) fail("row_ptr order");
+    int bad_e = 0;
+#pragma omp parallel for reduction(| : bad_e) schedule(static)
     for(long long e = 0; e < P.nnz; ++e)
     {
-        if(P.col[e] < 0 || P.col[e] >= P.n_rows) fail("col");
-        if((P.entry_src[e] & 0x7fffffff) >= nb) fail("entry_src");
+        if(P.col[e] < 0 || P.col[e] >= P.n_rows) bad_e |= 1;
+        if((P.entry_src[e] & 0x7fffffff) >= nb) bad_e |= 2;
     }
+    if(bad_e & 1) fail("col");
+    if(bad_e & 2) fail("entry_src");
     for(int v : P.ext_to_int)
         if(v < 0 || v >= P.n_rows) fail("ext_to_int");
     if((int)P.bank_vec.size() != P.n_banks || (int)P.bank_kind.size() != P.n_banks) fail("bank arrays");
