@@ -35,7 +35,18 @@ std::vector<long long> bucket_scatter(long long n, int nbuckets, Emit emit, Plac
         long long* c = cnt.data() + (size_t)t * (nbuckets + 1);
         for(long long k = t * chunk; k < std::min(n, (t + 1) * chunk); ++k) emit(k, [&](int b, int) { ++c[b]; });
     }
+    // bucket totals (parallel over buckets), their scan, then each thread's
+    // start inside every bucket
     std::vector<long long> ptr(nbuckets + 1, 0);
+#pragma omp parallel for schedule(static)
+    for(int b = 0; b < nbuckets; ++b)
+    {
+        long long acc = 0;
+        for(int t = 0; t < T; ++t) acc += cnt[(size_t)t * (nbuckets + 1) + b];
+        ptr[b + 1] = acc;
+    }
+    for(int b = 0; b < nbuckets; ++b) ptr[b + 1] += ptr[b];
+#pragma omp parallel for schedule(static)
     for(int b = 0; b < nbuckets; ++b)
     {
         long long acc = ptr[b];
@@ -45,7 +56,6 @@ std::vector<long long> bucket_scatter(long long n, int nbuckets, Emit emit, Plac
             cnt[(size_t)t * (nbuckets + 1) + b] = acc;
             acc += v;
         }
-        ptr[b + 1] = acc;
     }
 #pragma omp parallel for num_threads(T) schedule(static, 1)
     for(int t = 0; t < T; ++t)
