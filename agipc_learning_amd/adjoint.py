@@ -146,7 +146,7 @@ def _function():
             x = system.solve(b)
             ctx.system = system
             ctx.save_for_backward(mu, lam, x)
-            ctx.forward_stats = system.last_stats
+            system.forward_stats = system.last_stats
             return x
 
         @staticmethod

@@ -292,7 +292,7 @@ def adjoint_leg(args):
         gm, gl, loss = step()
     torch.cuda.synchronize()
     step_ms = (time.perf_counter() - t) * 1e3 / reps
-    fwd, adj = dict(system.last_stats), dict(system.adjoint_stats)
+    fwd, adj = dict(system.forward_stats), dict(system.adjoint_stats)
     # parts: assembly, build, each solve, gradient kernel (device events on
     # the stream the library calls here run on)
     ev = lambda: torch.cuda.Event(enable_timing=True)
@@ -328,7 +328,7 @@ def adjoint_leg(args):
     gm_r, gl_r, _ = step()
     torch.cuda.synchronize()
     refined_ms = (time.perf_counter() - t) * 1e3
-    fwd_r, adj_r = dict(system.last_stats), dict(system.adjoint_stats)
+    fwd_r, adj_r = dict(system.forward_stats), dict(system.adjoint_stats)
     g = torch.cat([gm, gl])
     g_r = torch.cat([gm_r, gl_r])
     err = float(torch.linalg.vector_norm(g - g_r) / torch.linalg.vector_norm(g_r))

@@ -83,6 +83,32 @@ def test_device_solver_hierarchy_equals_oracle(oracle):
     assert np.array_equal(h["fine_mask"], o["fine_mask"])
 
 
+def test_new_contact_set_on_the_same_solver_is_reanalyzed(oracle):
+    # contacts set after set-up mark the structure for re-analysis: the next
+    # setup() rebuilds the hierarchy (on the device) with the new pairs, and
+    # the solve on the rebuilt preconditioner converges
+    w = W.fem_grid(10, n_contacts=0, seed=4, young_sigma=0.5)
+    s = session.solver_for(w)
+    rng = np.random.default_rng(3)
+    pairs = np.sort(rng.choice(w["n_vertices"], size=(200, 4), replace=True), axis=1).astype(np.int32)
+    pairs = pairs[(pairs[:, 0] != pairs[:, 1]) & (pairs[:, 1] != pairs[:, 2]) & (pairs[:, 2] != pairs[:, 3])]
+    for contact_set in (pairs, pairs[:50]):
+        s.set_contacts(0, contact_set)
+        s.setup()
+        s.set_values(0, dev(w["vals"]))
+        s.build("fp32")
+        wo = dict(w, pairs=contact_set)
+        mas = oracle_mas(oracle, wo)
+        h, o = s.hierarchy(0), mas.hierarchy()
+        assert h["total"] == mas.total_clusters
+        assert np.array_equal(h["coarse_table"], o["coarse_table"])
+        assert np.array_equal(h["fine_mask"], o["fine_mask"])
+        b = session.rhs_of(w)
+        x = torch.zeros(len(b), dtype=torch.float64, device="cuda")
+        st = s.solve(dev(b), x, rule=1, tol=1e-4)[0]
+        assert st["failure"] == 0 and st["rel_res_true"] <= 1e-4 * (1 + 1e-6)
+
+
 # ------------------------------------------------------------------- MAS ----
 
 @pytest.mark.parametrize("n,contacts", [(8, 0), (14, 400)])
