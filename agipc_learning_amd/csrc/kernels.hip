@@ -17,6 +17,7 @@
 #include "kernels.hpp"
 
 #include <hip/hip_bf16.h>
+#include <hip/hip_ext.h>
 #include <hip/hip_fp16.h>
 
 #include <type_traits>
@@ -1670,12 +1671,20 @@ hipError_t launch_phase_finish(const DevData& d, hipStream_t st)
     return hipSuccess;
 }
 
-hipError_t launch_mas_local(const DevData& d, int precision, int force, hipStream_t st)
+// plain launch, or one whose events are stamped by the dispatch itself
+#define MASX_DISPATCH(K, G, B, ...)                                                        \
+    do                                                                                      \
+    {                                                                                       \
+        if(ev) hipExtLaunchKernelGGL(K, G, B, 0, st, ev->start, ev->stop, 0, __VA_ARGS__); \
+        else hipLaunchKernelGGL(K, G, B, 0, st, __VA_ARGS__);                               \
+    } while(0)
+
+hipError_t launch_mas_local(const DevData& d, int precision, int force, hipStream_t st, const LaunchEvents* ev)
 {
     if(d.n_banks <= 0) return hipSuccess;
     const int bpw = 64 / d.bank;
     const dim3 g(blocks_for(d.n_banks, bpw)), b(256);
-#define MASX_LOCAL(BK, T) hipLaunchKernelGGL((k_mas_local<BK, T>), g, b, 0, st, d, force)
+#define MASX_LOCAL(BK, T) MASX_DISPATCH((k_mas_local<BK, T>), g, b, d, force)
     if(d.bank == 32)
     {
         if(precision == kPrecFp16) MASX_LOCAL(32, __half);
@@ -1695,10 +1704,10 @@ hipError_t launch_mas_local(const DevData& d, int precision, int force, hipStrea
     return hipSuccess;
 }
 
-hipError_t launch_precond_finish(const DevData& d, int n_wg, int force, hipStream_t st)
+hipError_t launch_precond_finish(const DevData& d, int n_wg, int force, hipStream_t st, const LaunchEvents* ev)
 {
     if(n_wg <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_precond_finish, dim3(n_wg), dim3(256), 0, st, d, force);
+    MASX_DISPATCH(k_precond_finish, dim3(n_wg), dim3(256), d, force);
     MASX_CHECK_LAUNCH();
     return hipSuccess;
 }
@@ -1709,27 +1718,28 @@ bool spmv_tiling_supported(int entries_per_thread)
 }
 
 hipError_t launch_spmv_fused(const DevData& d, int n_wg, int entries_per_thread, const float* p_old, float* p_new,
-                             hipStream_t st)
+                             hipStream_t st, const LaunchEvents* ev)
 {
     if(n_wg <= 0) return hipSuccess;
     switch(entries_per_thread)
     {
-        case 2: hipLaunchKernelGGL(k_spmv_fused<2>, dim3(n_wg), dim3(256), 0, st, d, p_old, p_new); break;
-        case 4: hipLaunchKernelGGL(k_spmv_fused<4>, dim3(n_wg), dim3(256), 0, st, d, p_old, p_new); break;
-        case 8: hipLaunchKernelGGL(k_spmv_fused<8>, dim3(n_wg), dim3(256), 0, st, d, p_old, p_new); break;
+        case 2: MASX_DISPATCH(k_spmv_fused<2>, dim3(n_wg), dim3(256), d, p_old, p_new); break;
+        case 4: MASX_DISPATCH(k_spmv_fused<4>, dim3(n_wg), dim3(256), d, p_old, p_new); break;
+        case 8: MASX_DISPATCH(k_spmv_fused<8>, dim3(n_wg), dim3(256), d, p_old, p_new); break;
         default: return hipErrorInvalidValue;
     }
     MASX_CHECK_LAUNCH();
     return hipSuccess;
 }
 
-hipError_t launch_update(const DevData& d, int n_wg, const float* p, int mode, hipStream_t st)
+hipError_t launch_update(const DevData& d, int n_wg, const float* p, int mode, hipStream_t st, const LaunchEvents* ev)
 {
     if(n_wg <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_update, dim3(n_wg), dim3(256), 0, st, d, p, mode);
+    MASX_DISPATCH(k_update, dim3(n_wg), dim3(256), d, p, mode);
     MASX_CHECK_LAUNCH();
     return hipSuccess;
 }
+#undef MASX_DISPATCH
 
 hipError_t launch_true_residual(const DevData& d, int n_wg, int report_only, hipStream_t st)
 {

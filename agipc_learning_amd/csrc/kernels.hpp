@@ -24,12 +24,22 @@ hipError_t launch_solve_prepare(const DevData& d, const int* int_to_ext, const d
 // host[0] = systems not done, then host[1] = ++*seq (device counter)
 hipError_t launch_export_status(const DevData& d, int* host, int* seq, hipStream_t st);
 hipError_t launch_phase_finish(const DevData& d, hipStream_t st);
-hipError_t launch_mas_local(const DevData& d, int precision, int force, hipStream_t st);
-hipError_t launch_precond_finish(const DevData& d, int n_wg, int force, hipStream_t st);
+// Events stamped by one kernel's own dispatch (start and end of its
+// execution, via hipExtLaunchKernelGGL): their elapsed time is the kernel's
+// duration alone, without the gaps between launches.  Null: a plain launch.
+struct LaunchEvents
+{
+    hipEvent_t start, stop;
+};
+hipError_t launch_mas_local(const DevData& d, int precision, int force, hipStream_t st,
+                            const LaunchEvents* ev = nullptr);
+hipError_t launch_precond_finish(const DevData& d, int n_wg, int force, hipStream_t st,
+                                 const LaunchEvents* ev = nullptr);
 // row-aligned tiles of at most 256 rows and 256 * entries_per_thread entries (2, 4 or 8)
 hipError_t launch_spmv_fused(const DevData& d, int n_wg, int entries_per_thread, const float* p_old, float* p_new,
-                             hipStream_t st);
-hipError_t launch_update(const DevData& d, int n_wg, const float* p, int mode, hipStream_t st);
+                             hipStream_t st, const LaunchEvents* ev = nullptr);
+hipError_t launch_update(const DevData& d, int n_wg, const float* p, int mode, hipStream_t st,
+                         const LaunchEvents* ev = nullptr);
 // true residual of the checked (or, report_only, all) systems and its decision
 hipError_t launch_true_residual(const DevData& d, int n_wg, int report_only, hipStream_t st);
 hipError_t launch_spmv64(const DevData& d, const double* x, double* y, hipStream_t st);

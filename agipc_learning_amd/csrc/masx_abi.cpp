@@ -316,7 +316,7 @@ int masx_autotune(masx_solver* s, int reps, int* entries_per_thread, double* ms)
 int masx_time_kernel(masx_solver* s, int which, int reps, double* ms)
 {
     return guard([&] {
-        require(s && ms && reps > 0, "bad argument");
+        require(s && ms && reps > 0 && reps <= 65536, "bad argument (reps in 1..65536)");
         *ms = s->impl.time_kernel(which, reps);
     });
 }

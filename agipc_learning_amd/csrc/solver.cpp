@@ -130,6 +130,7 @@ Solver::~Solver()
     if(h_export_) (void)hipHostFree(h_export_);
     if(ev_a_) (void)hipEventDestroy(ev_a_);
     if(ev_b_) (void)hipEventDestroy(ev_b_);
+    for(auto e : timing_ev_) (void)hipEventDestroy(e);
     for(auto e : chunk_ev_)
         if(e) (void)hipEventDestroy(e);
     if(work_) (void)hipStreamDestroy(work_);
@@ -751,28 +752,43 @@ double Solver::time_kernel(int which, int reps)
         s.frozen = 1;
     }
     MASX_HIP(hipMemcpy(d_state_.ptr, h.data(), sizeof(SysState) * plan_.n_sys, hipMemcpyHostToDevice));
-    auto launch = [&]() {
+    auto launch = [&](const LaunchEvents* ev) {
         switch(which)
         {
-            case 0: MASX_HIP(launch_mas_local(dd_, precision_, 0, st)); break;
-            case 1: MASX_HIP(launch_precond_finish(dd_, n_wg_pre_, 0, st)); break;
+            case 0: MASX_HIP(launch_mas_local(dd_, precision_, 0, st, ev)); break;
+            case 1: MASX_HIP(launch_precond_finish(dd_, n_wg_pre_, 0, st, ev)); break;
             case 2:
-                MASX_HIP(launch_spmv_fused(dd_, n_wg_spmv_, entries_, d_p0_.ptr, d_p1_.ptr, st));
+                MASX_HIP(launch_spmv_fused(dd_, n_wg_spmv_, entries_, d_p0_.ptr, d_p1_.ptr, st, ev));
                 break;
-            case 3: MASX_HIP(launch_update(dd_, n_wg_upd_, d_p1_.ptr, 0, st)); break;
+            case 3: MASX_HIP(launch_update(dd_, n_wg_upd_, d_p1_.ptr, 0, st, ev)); break;
             default: throw std::invalid_argument("unknown kernel id");
         }
     };
-    launch();
+    launch(nullptr);
     MASX_HIP(hipStreamSynchronize(st));
-    // the update kernel may end a system; keep them running between launches
-    MASX_HIP(hipEventRecord(ev_a_, st));
-    for(int i = 0; i < reps; ++i) launch();
-    MASX_HIP(hipEventRecord(ev_b_, st));
-    MASX_HIP(hipEventSynchronize(ev_b_));
-    float ms = 0.f;
-    MASX_HIP(hipEventElapsedTime(&ms, ev_a_, ev_b_));
-    return ms / reps;
+    // each launch carries events stamped by its own dispatch, so the average
+    // is of the kernel's execution time (what a kernel trace reports), not of
+    // the launch rate; the frozen state keeps every system running
+    while(timing_ev_.size() < 2 * (size_t)reps)
+    {
+        hipEvent_t e = nullptr;
+        MASX_HIP(hipEventCreate(&e));
+        timing_ev_.push_back(e);
+    }
+    for(int i = 0; i < reps; ++i)
+    {
+        const LaunchEvents ev{timing_ev_[2 * i], timing_ev_[2 * i + 1]};
+        launch(&ev);
+    }
+    MASX_HIP(hipStreamSynchronize(st));
+    double total = 0.0;
+    for(int i = 0; i < reps; ++i)
+    {
+        float ms = 0.f;
+        MASX_HIP(hipEventElapsedTime(&ms, timing_ev_[2 * i], timing_ev_[2 * i + 1]));
+        total += ms;
+    }
+    return total / reps;
 }
 
 }  // namespace masx

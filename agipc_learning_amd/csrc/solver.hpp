@@ -146,6 +146,7 @@ class Solver
 
     hipStream_t work_ = nullptr;
     hipEvent_t  ev_a_ = nullptr, ev_b_ = nullptr;
+    std::vector<hipEvent_t> timing_ev_;    // time_kernel: one start/stop pair per launch
     static constexpr int kDepth = 2;       // graph chunks in flight during a solve
     hipEvent_t  chunk_ev_[kDepth] = {};
     SysState*   h_state_ = nullptr;        // pinned: kDepth + 1 slots of n_sys states

@@ -1,10 +1,15 @@
 """Summaries of rocprofv3 CSV output for the solver's kernels.
 
   python -m agipc_learning_amd.prof_summary stats  <dir>/<name>_kernel_stats.csv  [--cmd "..."]
+  python -m agipc_learning_amd.prof_summary busy   <dir>/<name>_kernel_trace.csv  [--kernels "a;b;..."]
   python -m agipc_learning_amd.prof_summary traffic <fetch.csv> <write.csv> <copy_fetch.csv> <copy_write.csv>
          [--cmd "..."] [--json out.json]
 
-`stats` prints a markdown table of the kernel-trace statistics.  `traffic`
+`stats` prints a markdown table of the kernel-trace statistics.  `busy`
+reports each kernel's dispatch durations over the launches that did the work
+(at least half the 95th-percentile duration; launches for systems already
+converged exit at once and would pull a plain mean down), mean and median.
+`traffic`
 turns FETCH_SIZE / WRITE_SIZE (KiB per dispatch) into HBM bytes per launch,
 each scaled by what the same counter reports for a plain device copy of known
 size (the copy's true read and write bytes over its counter values), and
@@ -22,6 +27,7 @@ COPY_BYTES = 256 << 20  # bench.py --copy-calibration copies 256 MiB per launch
 
 
 def _short(name):
+    name = name.replace("(anonymous namespace)::", "")
     return name.split("(")[0].replace("void ", "").replace("masx::", "")
 
 
@@ -37,6 +43,24 @@ def stats(path, cmd=None, top=25):
         out.append(f"| {_short(r['Name'])} | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
                    f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['MinNs']) / 1e3:.2f} | "
                    f"{float(r['MaxNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
+    return "\n".join(out)
+
+
+def busy(path, kernels=None):
+    d = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        d[_short(r["Kernel_Name"])].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    names = kernels or sorted(d, key=lambda k: -sum(d[k]))
+    out = ["| kernel | dispatches (busy/all) | busy avg us | busy median us | busy min us | busy max us |",
+           "|---|---|---|---|---|---|"]
+    for k in names:
+        v = sorted(d.get(k, []))
+        if not v:
+            continue
+        cut = 0.5 * v[min(len(v) - 1, int(0.95 * len(v)))]
+        b = [x for x in v if x >= cut]
+        out.append(f"| {k} | {len(b)}/{len(v)} | {sum(b) / len(b):.2f} | {b[len(b) // 2]:.2f} | {b[0]:.2f} | "
+                   f"{b[-1]:.2f} |")
     return "\n".join(out)
 
 
@@ -58,11 +82,15 @@ def _copy_factor(path, counter):
 
 
 def _busy(vals):
-    """Mean over the dispatches that did the work (the early-exit launches of
-    systems already converged read almost nothing)."""
-    m = max(vals)
-    busy = [v for v in vals if v >= 0.5 * m]
-    return sum(busy) / len(busy), len(busy), len(vals)
+    """Median over the dispatches that did the work: those of at least half
+    the 95th-percentile count (the early-exit launches of systems already
+    converged read almost nothing; the median keeps the few heavier launches
+    of a kernel, such as the SpMV that also forms the FP64 true residual once
+    per solve, from shifting the figure of the regular launch)."""
+    v = sorted(vals)
+    cut = 0.5 * v[min(len(v) - 1, int(0.95 * len(v)))]
+    busy = [x for x in v if x >= cut]
+    return busy[len(busy) // 2], len(busy), len(vals)
 
 
 def traffic(fetch, write, copy_fetch, copy_write, cmd=None, json_out=None):
@@ -96,6 +124,9 @@ def main(argv=None):
     a = sub.add_parser("stats")
     a.add_argument("csv")
     a.add_argument("--cmd")
+    c = sub.add_parser("busy")
+    c.add_argument("csv")
+    c.add_argument("--kernels")
     b = sub.add_parser("traffic")
     b.add_argument("fetch")
     b.add_argument("write")
@@ -106,6 +137,8 @@ def main(argv=None):
     args = ap.parse_args(argv)
     if args.what == "stats":
         print(stats(args.csv, args.cmd))
+    elif args.what == "busy":
+        print(busy(args.csv, args.kernels.split(";") if args.kernels else None))
     else:
         print(traffic(args.fetch, args.write, args.copy_fetch, args.copy_write, args.cmd, args.json))
 

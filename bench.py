@@ -50,6 +50,10 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-differentiated solve")
     ap.add_argument("--adjoint-n", type=int, default=60, help="hex cells per edge of the adjoint workload")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only run the adjoint-differentiated solve and print its figures (profiling aid)")
+    ap.add_argument("--spmv-tiling", type=int, default=0, choices=(0, 2, 4, 8),
+                    help="block entries per SpMV thread; 0 times every tiling and keeps the fastest")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -282,7 +286,7 @@ def adjoint_leg(args):
         x = adjoint.solve(mu, lam, system, b)
         loss = 0.5 * ((x - xstar) ** 2).sum()
         loss.backward()
-        return mu.grad, lam.grad, float(loss)
+        return mu.grad, lam.grad, float(loss.detach())
 
     step()
     torch.cuda.synchronize()
@@ -360,6 +364,9 @@ def main():
     if args.copy_calibration:
         copy_calibration()
         return
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_leg(args)}), flush=True)
+        return
 
     import numpy as np
     import torch
@@ -399,9 +406,13 @@ def main():
         ms = (time.perf_counter() - t) * 1e3 / steps
         return ms, st
 
-    log("step: autotune SpMV tiling")
     s.build("fp32")
-    tiling = s.autotune(50)
+    if args.spmv_tiling:
+        s.set_spmv_tiling(args.spmv_tiling)
+        tiling = {"entries_per_thread": args.spmv_tiling, "ms": s.time_kernel(2, 50)}
+    else:
+        log("step: autotune SpMV tiling")
+        tiling = s.autotune(50)
     log(f"  tiling {tiling}")
     precs = ["fp32"] + [p for p in args.precisions.split(",") if p and p != "fp32"]
     results = {}

@@ -147,8 +147,10 @@ int masx_set_spmv_tiling(masx_solver* s, int entries_per_thread);
  * each) and keeps the fastest; reports it and its time per launch (ms).
  * Leaves the iteration state frozen: call masx_solve afterwards as usual. */
 int masx_autotune(masx_solver* s, int reps, int* entries_per_thread, double* ms);
-/* Average device time of one kernel (0 MAS local, 1 preconditioner finish,
- * 2 fused SpMV, 3 update) over `reps` back-to-back launches, milliseconds. */
+/* Average execution time of one kernel (0 MAS local, 1 preconditioner finish,
+ * 2 fused SpMV, 3 update) over `reps` back-to-back launches, milliseconds:
+ * each launch is timed by events stamped by its own dispatch, so the figure
+ * excludes the gaps between launches. */
 int masx_time_kernel(masx_solver* s, int which, int reps, double* ms);
 
 /* Introspection (tests, benchmark). */
