@@ -55,14 +55,20 @@ struct DevData
 {
     // sizes
     int n_rows, n_sys, bank, n_banks, n_fine_banks;
-    long long nnz, plane;  // plane = padded entry stride of the SoA value planes
-    // structure
+    long long nnz;          // entries of the full block CSR
+    long long nnz_s, plane; // entries of the FP32 SpMV layout; padded entry stride of its value planes
+    // structure: full block CSR (FP64 paths)
     const int* row_ptr;
     const int* col;
     const int* entry_src;   // full entry -> upper block | transposed<<31
     const int* row_sys;
+    // FP32 SpMV layout (tile-local symmetric pairs stored once; plan.hpp)
+    const int* sy_row_ptr;  // n_rows+1
+    const int* sy_col;      // column word: >= 0 outside the tile, else tile-local code
+    const int* sy_src;      // entry -> upper block | transposed<<31
+    const int* sy_tptr;     // n_rows+1: cumulative transposed products
     // values
-    const float*  vals32;   // 9 planes of `plane` floats, entry-major, block row-major (r*3+c)
+    const float*  vals32;   // 9 planes of `plane` floats over the SpMV layout's entries, block row-major (r*3+c)
     const double* vals64;   // upper blocks, col-major (r + 3c), global upper order
     // vectors (float3 per internal row)
     float*  x;

@@ -15,6 +15,7 @@
 //  * kernels of a finished system return at once, so a captured chunk of
 //    iterations can run past convergence at negligible cost.
 #include "kernels.hpp"
+#include "spmv_layout.hpp"
 
 #include <hip/hip_bf16.h>
 #include <hip/hip_ext.h>
@@ -164,16 +165,22 @@ __device__ __forceinline__ bool finite_d(double v) { return isfinite(v); }
 
 // ===================================================================== build
 
-// Upper FP64 blocks (col-major) -> FP32 SoA planes of the full block CSR, one
-// thread per full entry so the plane writes are contiguous; an entry marked
-// transposed takes its upper block's transpose.
+// Upper FP64 blocks (col-major) -> FP32 SoA planes of the SpMV layout, one
+// thread per entry so the plane writes are contiguous; an entry marked
+// transposed takes its upper block's transpose, a padding entry zeros.
 __global__ void __launch_bounds__(256) k_pack_values(DevData d)
 {
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if(e >= d.nnz) return;
-    const int code = d.entry_src[e];
-    const double* B = d.vals64 + 9 * (long long)(code & 0x7fffffff);
+    if(e >= d.nnz_s) return;
+    const int code = d.sy_src[e];
     float* planes = const_cast<float*>(d.vals32);
+    if(code == kSpmvPadSrc)
+    {
+#pragma unroll
+        for(int q = 0; q < 9; ++q) planes[q * d.plane + e] = 0.f;
+        return;
+    }
+    const double* B = d.vals64 + 9 * (long long)(code & 0x7fffffff);
     if(code >= 0)
     {
 #pragma unroll
@@ -1063,74 +1070,163 @@ __device__ __forceinline__ bool report_is_current(const SysState& st)
 template <int U>
 struct SpmvChunk
 {
-    int   cix[U];
+    int   cix[U];    // column words of the thread's run of U consecutive entries
     float v[U][9];
-    float g[U][6];
+    float g[U][6];   // z and old p of a column outside the tile
 };
 
+// U consecutive floats (ints) from a run start aligned to min(U, 4) entries
+template <int U, typename T>
+__device__ __forceinline__ void load_run(const T* __restrict__ p, T (&o)[U])
+{
+    if constexpr(U == 1)
+        o[0] = p[0];
+    else if constexpr(U == 2)
+    {
+        using V2 = std::conditional_t<std::is_same_v<T, int>, int2, float2>;
+        const V2 t = *reinterpret_cast<const V2*>(p);
+        o[0] = t.x; o[1] = t.y;
+    }
+    else
+    {
+        using V4 = std::conditional_t<std::is_same_v<T, int>, int4, float4>;
+#pragma unroll
+        for(int h = 0; h < U / 4; ++h)
+        {
+            const V4 t = reinterpret_cast<const V4*>(p)[h];
+            o[4 * h] = t.x; o[4 * h + 1] = t.y; o[4 * h + 2] = t.z; o[4 * h + 3] = t.w;
+        }
+    }
+}
+
+// This thread's run: entries [base + U t, base + U t + U) of the chunk (those
+// at or past n are read -- the layout's spare entries keep that in bounds --
+// and ignored)
 template <int U>
 __device__ __forceinline__ void spmv_load_chunk(const DevData& d, const float* __restrict__ p_old, int base, int n,
                                                 SpmvChunk<U>& c)
 {
-#pragma unroll
-    for(int j = 0; j < U; ++j)
+    const int k0 = U * (int)threadIdx.x;
+    if(k0 >= n)
     {
-        const int  k  = (int)threadIdx.x + 256 * j;
-        const bool ok = k < n;
-        const int  e  = base + (ok ? k : 0);
-        c.cix[j] = ok ? d.col[e] : -1;
 #pragma unroll
-        for(int q = 0; q < 9; ++q) c.v[j][q] = d.vals32[q * d.plane + e];
+        for(int j = 0; j < U; ++j) c.cix[j] = 0;
+        return;
+    }
+    const int e = base + k0;
+    load_run<U>(d.sy_col + e, c.cix);
+#pragma unroll
+    for(int q = 0; q < 9; ++q)
+    {
+        float t[U];
+        load_run<U>(d.vals32 + q * d.plane + e, t);
+#pragma unroll
+        for(int j = 0; j < U; ++j) c.v[j][q] = t[j];
     }
 #pragma unroll
     for(int j = 0; j < U; ++j)
     {
-        const long long x = 3 * (long long)(c.cix[j] >= 0 ? c.cix[j] : 0);
-        c.g[j][0] = d.z[x]; c.g[j][1] = d.z[x + 1]; c.g[j][2] = d.z[x + 2];
-        c.g[j][3] = p_old[x]; c.g[j][4] = p_old[x + 1]; c.g[j][5] = p_old[x + 2];
+        // columns inside the tile come from its x in LDS, not from memory
+        const unsigned w = (unsigned)c.cix[j];
+        if(!(w & kSpmvInTile) && k0 + j < n)
+        {
+            const long long x = 3 * (long long)(w & kSpmvColMask);
+            c.g[j][0] = d.z[x]; c.g[j][1] = d.z[x + 1]; c.g[j][2] = d.z[x + 2];
+            c.g[j][3] = p_old[x]; c.g[j][4] = p_old[x + 1]; c.g[j][5] = p_old[x + 2];
+        }
     }
 }
 
-// products of a loaded chunk -> LDS, then the owning rows add them in order
+// Products of the thread's run, summed per row in registers (a row-start
+// flag ends the running sum); each sum goes to LDS at the last entry of its
+// row within the run.  An entry inside the tile takes x_c from the tile's x
+// (sX) and, off the diagonal, also gives A^T x_r for its column (slot tpos).
+// Then each owning row adds the run ends of its entries in order, then
+// (tb < te) its transposed products.
 template <int U>
 __device__ __forceinline__ void spmv_reduce_chunk(const SpmvChunk<U>& c, float beta, int base, int n, bool has_row,
-                                                  int rb, int re, float* sP, float acc[3])
+                                                  int rb, int re, int tb, int te, const float* sX, float* sP,
+                                                  float* sQ, float acc[3])
 {
+    const int k0 = U * (int)threadIdx.x;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
 #pragma unroll
     for(int j = 0; j < U; ++j)
     {
-        const int k = (int)threadIdx.x + 256 * j;
+        const int k = k0 + j;
         if(k < n)
         {
-            const float x0 = c.g[j][0] + beta * c.g[j][3];
-            const float x1 = c.g[j][1] + beta * c.g[j][4];
-            const float x2 = c.g[j][2] + beta * c.g[j][5];
             const float* v = c.v[j];
-            sP[3 * k]     = v[0] * x0 + v[1] * x1 + v[2] * x2;
-            sP[3 * k + 1] = v[3] * x0 + v[4] * x1 + v[5] * x2;
-            sP[3 * k + 2] = v[6] * x0 + v[7] * x1 + v[8] * x2;
+            const unsigned w = (unsigned)c.cix[j];
+            if(j > 0 && (w & kSpmvRowStart))
+            {
+                sP[3 * (k - 1)] = s0; sP[3 * (k - 1) + 1] = s1; sP[3 * (k - 1) + 2] = s2;
+                s0 = s1 = s2 = 0.f;
+            }
+            float x0, x1, x2;
+            if(!(w & kSpmvInTile))
+            {
+                x0 = c.g[j][0] + beta * c.g[j][3];
+                x1 = c.g[j][1] + beta * c.g[j][4];
+                x2 = c.g[j][2] + beta * c.g[j][5];
+            }
+            else
+            {
+                const int cl = w & 0xff, rl = (w >> 8) & 0xff, tpos = (w >> 16) & 0x3fff;
+                x0 = sX[3 * cl]; x1 = sX[3 * cl + 1]; x2 = sX[3 * cl + 2];
+                if(rl != cl)
+                {
+                    const float y0 = sX[3 * rl], y1 = sX[3 * rl + 1], y2 = sX[3 * rl + 2];
+                    sQ[3 * tpos]     = v[0] * y0 + v[3] * y1 + v[6] * y2;
+                    sQ[3 * tpos + 1] = v[1] * y0 + v[4] * y1 + v[7] * y2;
+                    sQ[3 * tpos + 2] = v[2] * y0 + v[5] * y1 + v[8] * y2;
+                }
+            }
+            s0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
+            s1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
+            s2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
         }
+    }
+    if(k0 < n)
+    {
+        const int last = min(n, k0 + U) - 1;
+        sP[3 * last] = s0; sP[3 * last + 1] = s1; sP[3 * last + 2] = s2;
     }
     __syncthreads();
     if(has_row)
     {
-        const int q0 = max(rb, base), q1 = min(re, base + n);
-        for(int q = q0; q < q1; ++q)
+        int q        = max(rb, base) - base;
+        const int qb = min(re, base + n) - base;
+        while(q < qb)
         {
-            acc[0] += sP[3 * (q - base)];
-            acc[1] += sP[3 * (q - base) + 1];
-            acc[2] += sP[3 * (q - base) + 2];
+            const int end = min(qb, (q / U + 1) * U);
+            acc[0] += sP[3 * (end - 1)];
+            acc[1] += sP[3 * (end - 1) + 1];
+            acc[2] += sP[3 * (end - 1) + 2];
+            q = end;
+        }
+        for(int t = tb; t < te; ++t)
+        {
+            acc[0] += sQ[3 * t];
+            acc[1] += sQ[3 * t + 1];
+            acc[2] += sQ[3 * t + 2];
         }
     }
     __syncthreads();
 }
 
+// Ap = A p with p = z + beta p_old formed on the fly (and stored), over one
+// tile of the SpMV layout: at most 256 rows, one thread per row; a tile of
+// several rows fits one chunk of 256 * U entries, a longer single row takes
+// several chunks (it has no transposed products).  Also the p.Ap partial.
 template <int U>
 __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __restrict__ p_old,
                                                     float* __restrict__ p_new)
 {
     constexpr int CH = 256 * U;
     __shared__ float  sP[CH * 3];
+    __shared__ float  sQ[CH * 3];
+    __shared__ float  sX[kSpmvTileRows * 3];
     __shared__ double smem[8];
     const int  wg = xcd_tile(blockIdx.x, gridDim.x);
     const int4 w  = d.wg_spmv[wg];
@@ -1158,12 +1254,15 @@ __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __re
     const int e0 = er.x, e1 = er.y;
     const int row = r0 + (int)threadIdx.x;
     const bool has_row = row < r1;
-    int rb = 0, re = 0;
+    int rb = 0, re = 0, tb = 0, te = 0;
     float own[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if(has_row)
     {
-        rb = d.row_ptr[row];
-        re = d.row_ptr[row + 1];
+        rb = d.sy_row_ptr[row];
+        re = d.sy_row_ptr[row + 1];
+        const int t0 = d.sy_tptr[r0];
+        tb = d.sy_tptr[row] - t0;
+        te = d.sy_tptr[row + 1] - t0;
         const long long o = 3 * (long long)row;
         own[0] = d.z[o]; own[1] = d.z[o + 1]; own[2] = d.z[o + 2];
         own[3] = p_old[o]; own[4] = p_old[o + 1]; own[5] = p_old[o + 2];
@@ -1186,23 +1285,29 @@ __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __re
     }
     if(bad && !st_frozen) return;
     const float beta = st_beta_zero ? 0.f : (float)(rz / st_rz_prev);
+    // this row's p, for the store below and for the tile's own columns
+    const float y0 = own[0] + beta * own[3];
+    const float y1 = own[1] + beta * own[4];
+    const float y2 = own[2] + beta * own[5];
+    if(has_row)
+    {
+        sX[3 * threadIdx.x] = y0; sX[3 * threadIdx.x + 1] = y1; sX[3 * threadIdx.x + 2] = y2;
+    }
+    __syncthreads();
     float acc[3] = {0.f, 0.f, 0.f};
-    if(n0 > 0) spmv_reduce_chunk<U>(c, beta, e0, n0, has_row, rb, re, sP, acc);
-    // tiles holding a row longer than one chunk
+    if(n0 > 0) spmv_reduce_chunk<U>(c, beta, e0, n0, has_row, rb, re, tb, te, sX, sP, sQ, acc);
+    // a single row longer than one chunk
 #pragma unroll 1
     for(int base = e0 + CH; base < e1; base += CH)
     {
         const int n = min(CH, e1 - base);
         spmv_load_chunk<U>(d, p_old, base, n, c);
-        spmv_reduce_chunk<U>(c, beta, base, n, has_row, rb, re, sP, acc);
+        spmv_reduce_chunk<U>(c, beta, base, n, has_row, rb, re, 0, 0, sX, sP, sQ, acc);
     }
     double part = 0.0;
     if(has_row)
     {
         const long long o = 3 * (long long)row;
-        const float y0 = own[0] + beta * own[3];
-        const float y1 = own[1] + beta * own[4];
-        const float y2 = own[2] + beta * own[5];
         p_new[o] = y0; p_new[o + 1] = y1; p_new[o + 2] = y2;
         d.Ap[o] = acc[0]; d.Ap[o + 1] = acc[1]; d.Ap[o + 2] = acc[2];
         part = (double)y0 * acc[0] + (double)y1 * acc[1] + (double)y2 * acc[2];
@@ -1554,8 +1659,8 @@ static inline unsigned blocks_for(long long n, int b) { return (unsigned)((n + b
 
 hipError_t launch_pack_values(const DevData& d, hipStream_t st)
 {
-    if(d.nnz <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_pack_values, dim3(blocks_for(d.nnz, 256)), dim3(256), 0, st, d);
+    if(d.nnz_s <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_values, dim3(blocks_for(d.nnz_s, 256)), dim3(256), 0, st, d);
     MASX_CHECK_LAUNCH();
     return hipSuccess;
 }

@@ -3,6 +3,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <array>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -428,6 +431,67 @@ int masx_hierarchy_host(int n_fem, int n_slots, int bank, int levels, const uint
         if(prefix) std::memcpy(prefix, h.prefix.data(), sizeof(int) * h.prefix.size());
         std::memcpy(going_next, h.going_next.data(), sizeof(int) * std::max(h.total, n_fem));
         if(coarse_table) std::memcpy(coarse_table, h.coarse_table.data(), sizeof(int) * 6 * (size_t)n_fem);
+    });
+}
+
+int masx_spmv_layout_check(int n_rows, long long nb, const int* rows, const int* cols, int entries_per_thread,
+                           unsigned seed, double* max_rel_err, long long* n_entries, long long* n_in_tile)
+{
+    return guard([&] {
+        require(n_rows >= 1 && nb >= 1 && rows && cols && max_rel_err, "bad argument");
+        require(entries_per_thread >= 1 && 256LL * entries_per_thread <= masx::kSpmvMaxCap, "bad entries per thread");
+        // full CSR: upper (r, c) -> row r as is, row c transposed
+        std::vector<std::array<int, 3>> ent;
+        ent.reserve(2 * (size_t)nb);
+        for(long long k = 0; k < nb; ++k)
+        {
+            const int r = rows[k], c = cols[k];
+            require(r >= 0 && c < n_rows && r <= c, "upper structure expected (0 <= rows <= cols < n_rows)");
+            ent.push_back({r, c, (int)k});
+            if(r != c) ent.push_back({c, r, (int)(k | 0x80000000LL)});
+        }
+        std::sort(ent.begin(), ent.end(),
+                  [](const auto& a, const auto& b) { return a[0] != b[0] ? a[0] < b[0] : a[1] < b[1]; });
+        for(size_t i = 1; i < ent.size(); ++i)
+            require(ent[i][0] != ent[i - 1][0] || ent[i][1] != ent[i - 1][1], "duplicate block");
+        std::vector<int> row_ptr(n_rows + 1, 0), col(ent.size()), src(ent.size());
+        for(size_t i = 0; i < ent.size(); ++i)
+        {
+            ++row_ptr[ent[i][0] + 1];
+            col[i] = ent[i][1];
+            src[i] = ent[i][2];
+        }
+        for(int r = 0; r < n_rows; ++r) row_ptr[r + 1] += row_ptr[r];
+        masx::SpmvLayout L;
+        masx::build_spmv_layout(L, 1, {0, n_rows}, row_ptr, col, src, 256 * entries_per_thread);
+        // seeded values (64-bit LCG, uniform in [-1, 1))
+        uint64_t z = 0x9e3779b97f4a7c15ULL ^ seed;
+        auto uni = [&z]() {
+            z = z * 6364136223846793005ULL + 1442695040888963407ULL;
+            return (double)(z >> 11) * (1.0 / 4503599627370496.0) - 1.0;
+        };
+        std::vector<double> upper(9 * (size_t)nb), x(3 * (size_t)n_rows), y(3 * (size_t)n_rows),
+            yf(3 * (size_t)n_rows, 0.0);
+        for(auto& v : upper) v = uni();
+        for(auto& v : x) v = uni();
+        masx::spmv_layout_apply(L, upper.data(), x.data(), y.data());
+        double scale = 0.0, err = 0.0;
+        for(int r = 0; r < n_rows; ++r)
+            for(int e = row_ptr[r]; e < row_ptr[r + 1]; ++e)
+            {
+                const double* U = upper.data() + 9 * (long long)(src[e] & 0x7fffffff);
+                const double* xc = x.data() + 3 * (long long)col[e];
+                for(int i = 0; i < 3; ++i)
+                    for(int j = 0; j < 3; ++j) yf[3 * r + i] += (src[e] >= 0 ? U[i + 3 * j] : U[j + 3 * i]) * xc[j];
+            }
+        for(size_t i = 0; i < y.size(); ++i)
+        {
+            scale = std::max(scale, std::fabs(yf[i]));
+            err   = std::max(err, std::fabs(y[i] - yf[i]));
+        }
+        *max_rel_err = scale > 0.0 ? err / scale : err;
+        if(n_entries) *n_entries = L.n_entries;
+        if(n_in_tile) *n_in_tile = L.n_in_tile;
     });
 }
 

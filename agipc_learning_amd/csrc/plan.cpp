@@ -959,4 +959,197 @@ void validate_plan(const Plan& P)
         if(v < -1 || v >= nb) fail("diag_src");
 }
 
+// ------------------------------------------------------------ SpMV layout --
+
+void build_spmv_layout(SpmvLayout& L, int n_sys, const std::vector<int>& row_off, const std::vector<int>& row_ptr,
+                       const std::vector<int>& col, const std::vector<int>& entry_src, int cap)
+{
+    if(cap < 1 || cap > kSpmvMaxCap) throw std::invalid_argument("SpMV tile capacity out of range");
+    const int n = (int)row_ptr.size() - 1;
+    if((unsigned)n > kSpmvColMask) throw std::length_error("too many rows for the SpMV layout");
+    // first entry of row r with column >= c (rows are sorted by column)
+    auto first_ge = [&](int r, int c) {
+        return (int)(std::lower_bound(col.begin() + row_ptr[r], col.begin() + row_ptr[r + 1], c) - col.begin());
+    };
+    // entries row r keeps in a tile starting at r0: columns < r0 or >= r
+    auto kept = [&](int r, int r0) { return (row_ptr[r + 1] - first_ge(r, r)) + (first_ge(r, r0) - row_ptr[r]); };
+    L = SpmvLayout{};
+    L.cap = cap;
+    L.sys_range.resize(2 * (size_t)n_sys);
+    std::vector<long long> t_kept;
+    for(int s = 0; s < n_sys; ++s)
+    {
+        L.sys_range[2 * s] = (int)(L.tiles.size() / 4);
+        int r0 = row_off[s];
+        while(r0 < row_off[s + 1])
+        {
+            long long k = row_ptr[r0 + 1] - row_ptr[r0];
+            int r1 = r0 + 1;
+            while(r1 < row_off[s + 1] && r1 - r0 < kSpmvTileRows)
+            {
+                const int q = kept(r1, r0);
+                if(k + q > cap) break;
+                k += q;
+                ++r1;
+            }
+            L.tiles.insert(L.tiles.end(), {s, 0, r0, r1});
+            t_kept.push_back(k);
+            r0 = r1;
+        }
+        L.sys_range[2 * s + 1] = (int)(L.tiles.size() / 4);
+    }
+    const int T = (int)t_kept.size();
+    // transposed products per tile: pairs r < c inside it
+    std::vector<long long> t_trans(T);
+#pragma omp parallel for schedule(dynamic, 64)
+    for(int t = 0; t < T; ++t)
+    {
+        const int r0 = L.tiles[4 * t + 2], r1 = L.tiles[4 * t + 3];
+        long long q = 0;
+        for(int r = r0; r < r1; ++r) q += first_ge(r, r1) - first_ge(r, r + 1);
+        t_trans[t] = q;
+    }
+    // tile starts aligned to kSpmvAlign entries; the gaps are padding
+    std::vector<long long> e_base(T + 1, 0), e_end(T, 0), q_base(T + 1, 0);
+    long long stored = 0;
+    for(int t = 0; t < T; ++t)
+    {
+        e_end[t]      = e_base[t] + t_kept[t];
+        e_base[t + 1] = (e_end[t] + kSpmvAlign - 1) / kSpmvAlign * kSpmvAlign;
+        q_base[t + 1] = q_base[t] + t_trans[t];
+        stored += t_kept[t];
+    }
+    const long long total = e_base[T] + kSpmvSlack;
+    if(total > 0x7fffffffLL || q_base[T] > 0x7fffffffLL) throw std::length_error("SpMV layout too large");
+    L.n_entries = stored;
+    L.n_in_tile = q_base[T];
+    L.row_ptr.assign(n + 1, 0);
+    L.tptr.assign(n + 1, 0);
+    L.col.assign(total, 0);
+    L.src.assign(total, kSpmvPadSrc);
+    L.tile_ent.resize(2 * (size_t)T);
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(| : bad)
+    for(int t = 0; t < T; ++t)
+    {
+        const int r0 = L.tiles[4 * t + 2], r1 = L.tiles[4 * t + 3], nr = r1 - r0;
+        int tstart[kSpmvTileRows + 1];
+        std::fill(tstart, tstart + nr + 1, 0);
+        for(int r = r0; r < r1; ++r)
+            for(int e = first_ge(r, r + 1), e_end = first_ge(r, r1); e < e_end; ++e) ++tstart[col[e] - r0 + 1];
+        for(int j = 0; j < nr; ++j) tstart[j + 1] += tstart[j];
+        if(t_trans[t] != tstart[nr]) bad |= 1;
+        if(nr > 1 && t_kept[t] > cap) bad |= 2;
+        if(tstart[nr] > kSpmvMaxCap) bad |= 4;
+        int tcur[kSpmvTileRows];
+        std::copy(tstart, tstart + nr, tcur);
+        long long o = e_base[t];
+        for(int r = r0; r < r1; ++r)
+        {
+            L.row_ptr[r] = (int)o;
+            L.tptr[r]    = (int)(q_base[t] + tstart[r - r0]);
+            unsigned first = kSpmvRowStart;
+            for(int e = row_ptr[r]; e < row_ptr[r + 1]; ++e)
+            {
+                const int c = col[e];
+                unsigned word = (unsigned)c;
+                if(c >= r0 && c < r1)
+                {
+                    if(c < r) continue;  // the tile's lower copy: its upper twin covers it
+                    const int tpos = c == r ? 0 : tcur[c - r0]++;
+                    word = kSpmvInTile | (unsigned)tpos << 16 | (unsigned)(r - r0) << 8 | (unsigned)(c - r0);
+                }
+                L.col[o] = (int)(word | first);
+                L.src[o] = entry_src[e];
+                first = 0;
+                ++o;
+            }
+        }
+        if(o != e_end[t]) bad |= 8;
+        L.tile_ent[2 * t]     = (int)e_base[t];
+        L.tile_ent[2 * t + 1] = (int)e_end[t];
+    }
+    if(bad) throw std::logic_error("SpMV layout: tile codes inconsistent (check mask " + std::to_string(bad) + ")");
+    L.row_ptr[n] = (int)(T ? e_end[T - 1] : 0);
+    L.tptr[n]    = (int)q_base[T];
+}
+
+void spmv_layout_apply(const SpmvLayout& L, const double* upper, const double* x, double* y)
+{
+    const int T = (int)L.tiles.size() / 4;
+    const int U = std::max(1, L.cap / 256);
+    // block of entry e as row-major 3x3 (an entry flagged transposed takes its
+    // upper block's transpose; upper blocks are col-major)
+    auto block = [&](long long e, double B[9]) {
+        const int code = L.src[e];
+        const double* Ub = upper + 9 * (long long)(code & 0x7fffffff);
+        for(int i = 0; i < 3; ++i)
+            for(int j = 0; j < 3; ++j) B[3 * i + j] = code >= 0 ? Ub[i + 3 * j] : Ub[j + 3 * i];
+    };
+#pragma omp parallel for schedule(dynamic, 16)
+    for(int t = 0; t < T; ++t)
+    {
+        const int r0 = L.tiles[4 * t + 2], r1 = L.tiles[4 * t + 3];
+        const int e0 = L.tile_ent[2 * t], e1 = L.tile_ent[2 * t + 1];
+        std::vector<double> acc(3 * (size_t)(r1 - r0), 0.0), Q(3 * (size_t)(L.tptr[r1] - L.tptr[r0]));
+        const int CH = 256 * U;
+        for(int base = e0; base < e1; base += CH)
+        {
+            const int n_ch = std::min(CH, e1 - base);
+            // run ends: a thread's entries [U*th, U*th+U) summed per row run
+            std::vector<double> P(3 * (size_t)n_ch, 0.0);
+            for(int th = 0; th * U < n_ch; ++th)
+            {
+                double run[3] = {0.0, 0.0, 0.0};
+                const int k_end = std::min(n_ch, th * U + U);
+                for(int k = th * U; k < k_end; ++k)
+                {
+                    const long long e = base + k;
+                    const unsigned w  = (unsigned)L.col[e];
+                    if((w & kSpmvRowStart) && k > th * U)
+                    {
+                        for(int i = 0; i < 3; ++i) P[3 * (k - 1) + i] = run[i];
+                        run[0] = run[1] = run[2] = 0.0;
+                    }
+                    double B[9];
+                    block(e, B);
+                    const double* xc;
+                    if(w & kSpmvInTile)
+                    {
+                        const int cl = w & 0xff, rl = (w >> 8) & 0xff, tpos = (w >> 16) & 0x3fff;
+                        xc = x + 3 * (long long)(r0 + cl);
+                        if(cl != rl)
+                        {
+                            const double* xr = x + 3 * (long long)(r0 + rl);
+                            for(int i = 0; i < 3; ++i)
+                                Q[3 * tpos + i] = B[i] * xr[0] + B[3 + i] * xr[1] + B[6 + i] * xr[2];
+                        }
+                    }
+                    else
+                        xc = x + 3 * (long long)(w & kSpmvColMask);
+                    for(int i = 0; i < 3; ++i) run[i] += B[3 * i] * xc[0] + B[3 * i + 1] * xc[1] + B[3 * i + 2] * xc[2];
+                }
+                for(int i = 0; i < 3; ++i) P[3 * (k_end - 1) + i] = run[i];
+            }
+            for(int r = r0; r < r1; ++r)
+            {
+                int q        = std::max(L.row_ptr[r], base) - base;
+                const int qb = std::min(L.row_ptr[r + 1], base + n_ch) - base;
+                while(q < qb)
+                {
+                    const int end = std::min(qb, (q / U + 1) * U);
+                    for(int i = 0; i < 3; ++i) acc[3 * (r - r0) + i] += P[3 * (end - 1) + i];
+                    q = end;
+                }
+            }
+        }
+        for(int r = r0; r < r1; ++r)
+        {
+            for(int q = L.tptr[r] - L.tptr[r0]; q < L.tptr[r + 1] - L.tptr[r0]; ++q)
+                for(int i = 0; i < 3; ++i) acc[3 * (r - r0) + i] += Q[3 * q + i];
+            for(int i = 0; i < 3; ++i) y[3 * (long long)r + i] = acc[3 * (r - r0) + i];
+        }
+    }
+}
+
 }  // namespace masx

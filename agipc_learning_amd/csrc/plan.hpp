@@ -10,6 +10,8 @@
 #include <string>
 #include <vector>
 
+#include "spmv_layout.hpp"
+
 namespace masx
 {
 
@@ -143,5 +145,43 @@ struct Plan
 void build_plan(Plan& plan, std::vector<SystemDesc>& systems);
 // Bounds of every index array the kernels dereference; throws on violation.
 void validate_plan(const Plan& plan);
+
+// ------------------------------------------------------------ SpMV layout --
+// The FP32 SpMV's matrix layout.  Rows are cut into tiles of consecutive rows
+// of one system (one thread per row, at most kSpmvTileRows); a tile of several
+// rows holds at most `cap` entries.  A symmetric pair whose two rows share a
+// tile is stored once, as its upper block: it yields A x_c for row r and
+// A^T x_r for row c, both from the tile's own x.  Every other off-diagonal
+// block keeps both copies, one in each row, so each row is still summed by
+// the tile that owns it.  Row r of a tile starting at r0 keeps its entries
+// with column < r0 or >= r, in column order.
+//
+// Entry column words and tile limits: spmv_layout.hpp.
+
+struct SpmvLayout
+{
+    int cap = 0;
+    std::vector<int> tiles;       // 4 per tile: system, 0, first row, end row
+    std::vector<int> tile_ent;    // 2 per tile: entry range [e0, e1)
+    std::vector<int> sys_range;   // 2 per system: tile range
+    std::vector<int> row_ptr;     // n_rows+1
+    std::vector<int> col;         // column word per entry (padding and slack included)
+    std::vector<int> src;         // per entry: upper block | transposed flag (bit 31), as entry_src;
+                                  // kSpmvPadSrc for padding and slack
+    std::vector<int> tptr;        // n_rows+1: cumulative transposed products
+    long long n_entries = 0;      // stored matrix entries (no padding)
+    long long n_in_tile = 0;      // off-diagonal pairs stored once
+};
+
+// From the full block CSR (rows sorted by column) and its entry sources.
+// Throws if a tile's codes would not fit their fields.
+void build_spmv_layout(SpmvLayout& L, int n_sys, const std::vector<int>& row_off, const std::vector<int>& row_ptr,
+                       const std::vector<int>& col, const std::vector<int>& entry_src, int cap);
+// y = A x with the layout (FP64, host), from FP64 upper blocks (col-major),
+// read the way the kernel reads it: each thread's run of cap/256 consecutive
+// entries summed per row (restarting at row-start flags), the runs' ends
+// summed by the owning row, then its transposed products.  For tests of the
+// layout.
+void spmv_layout_apply(const SpmvLayout& L, const double* upper, const double* x, double* y);
 
 }  // namespace masx

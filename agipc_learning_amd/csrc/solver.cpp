@@ -175,6 +175,12 @@ void Solver::set_spmv_tiling(int entries_per_thread)
         make_devdata();
         if(graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
         graph_exec_ = nullptr;
+        // the value planes follow the layout: pack them again from the FP64 blocks
+        if(built_)
+        {
+            MASX_HIP(launch_pack_values(dd_, work_));
+            MASX_HIP(hipStreamSynchronize(work_));
+        }
     }
 }
 
@@ -207,8 +213,6 @@ void Solver::setup()
     d_entry_src_.upload(P.entry_src, st);
     d_row_sys_.upload(P.row_sys, st);
     d_e2i_.upload(P.ext_to_int, st);
-    const long long plane = std::max<long long>(64, (P.nnz + 63) / 64 * 64);
-    d_vals32_.alloc(9 * (size_t)plane);
     d_vals64_.alloc(9 * (size_t)P.nb_total);
     d_E_.alloc(9 * (size_t)std::max<long long>(P.n_gal, 1));
     const size_t nv = 3 * (size_t)P.n_rows;
@@ -269,7 +273,6 @@ void Solver::setup()
     MASX_HIP(hipHostMalloc((void**)&h_state_, sizeof(SysState) * P.n_sys * (kDepth + 1), hipHostMallocDefault));
     build_work_tables();
     make_devdata();
-    dd_.plane = plane;
     MASX_HIP(hipStreamSynchronize(st));
     if(graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     graph_exec_ = nullptr;
@@ -318,24 +321,17 @@ void Solver::build_work_tables()
         rows_chunks(pre, s, P.fem_row0[s], P.row_off[s + 1], fem_type, kBlock, fine_bank_base[s]);
         pre_r[2 * s + 1] = (int)pre.size();
         body_base += d.n_abd;
-        // SpMV: row-aligned tiles of at most 256 rows and 256 * entries_ entries
-        // (a row longer than that gets a tile of its own)
-        spmv_r[2 * s] = (int)spmv.size();
-        {
-            const int cap = 256 * entries_;
-            int r0 = P.row_off[s];
-            while(r0 < P.row_off[s + 1])
-            {
-                int r1 = r0 + 1;
-                while(r1 < P.row_off[s + 1] && r1 - r0 < 256 && P.row_ptr[r1 + 1] - P.row_ptr[r0] <= cap) ++r1;
-                spmv.push_back(make_int4(s, kWgPlain << 28, r0, r1));
-                r0 = r1;
-            }
-        }
-        spmv_r[2 * s + 1] = (int)spmv.size();
-        if(upd_r[2 * s] == upd_r[2 * s + 1] || pre_r[2 * s] == pre_r[2 * s + 1] || spmv_r[2 * s] == spmv_r[2 * s + 1])
-            throw std::runtime_error("empty system");
+        if(upd_r[2 * s] == upd_r[2 * s + 1] || pre_r[2 * s] == pre_r[2 * s + 1]) throw std::runtime_error("empty system");
     }
+    // SpMV: the layout's tiles (at most 256 rows; several rows only within
+    // 256 * entries_ entries, a longer row gets a tile of its own)
+    SpmvLayout& L = spmv_layout_;
+    build_spmv_layout(L, S, P.row_off, P.row_ptr, P.col, P.entry_src, 256 * entries_);
+    for(size_t t = 0; t < L.tiles.size() / 4; ++t)
+        spmv.push_back(make_int4(L.tiles[4 * t], kWgPlain << 28, L.tiles[4 * t + 2], L.tiles[4 * t + 3]));
+    spmv_r = L.sys_range;
+    for(int s = 0; s < S; ++s)
+        if(spmv_r[2 * s] == spmv_r[2 * s + 1]) throw std::runtime_error("empty system");
     // the update workgroups of MAS rows must be exactly the plan's update groups
     std::vector<int4> upd_grp(upd.size(), make_int4(0, 0, 0, 0));
     for(size_t k = 0; k < upd.size(); ++k)
@@ -349,8 +345,13 @@ void Solver::build_work_tables()
         upd_grp[k] = make_int4(P.ugroup[4 * g], P.ugroup[4 * g + 1], P.ugroup[4 * g + 2], P.ugroup[4 * g + 3]);
     }
     std::vector<int2> spmv_ent(spmv.size());
-    for(size_t k = 0; k < spmv.size(); ++k) spmv_ent[k] = make_int2(P.row_ptr[spmv[k].z], P.row_ptr[spmv[k].w]);
+    for(size_t k = 0; k < spmv.size(); ++k) spmv_ent[k] = make_int2(L.tile_ent[2 * k], L.tile_ent[2 * k + 1]);
     hipStream_t st = work_;
+    d_sy_row_ptr_.upload(L.row_ptr, st);
+    d_sy_col_.upload(L.col, st);
+    d_sy_src_.upload(L.src, st);
+    d_sy_tptr_.upload(L.tptr, st);
+    d_vals32_.alloc(9 * (size_t)std::max<long long>(64, ((long long)L.col.size() + 63) / 64 * 64));
     d_wg_pre_.upload(pre, st);
     d_wg_spmv_.upload(spmv, st);
     d_wg_upd_.upload(upd, st);
@@ -379,10 +380,15 @@ void Solver::make_devdata()
     d.n_banks = P.n_banks;
     d.n_fine_banks = P.n_fine_banks;
     d.nnz = P.nnz;
-    d.plane = std::max<long long>(64, (P.nnz + 63) / 64 * 64);
+    d.nnz_s = (long long)spmv_layout_.col.size();
+    d.plane = std::max<long long>(64, (d.nnz_s + 63) / 64 * 64);
     d.row_ptr = d_row_ptr_.ptr;
     d.col = d_col_.ptr;
     d.entry_src = d_entry_src_.ptr;
+    d.sy_row_ptr = d_sy_row_ptr_.ptr;
+    d.sy_col = d_sy_col_.ptr;
+    d.sy_src = d_sy_src_.ptr;
+    d.sy_tptr = d_sy_tptr_.ptr;
     d.row_sys = d_row_sys_.ptr;
     d.vals32 = d_vals32_.ptr;
     d.vals64 = d_vals64_.ptr;

@@ -88,6 +88,8 @@ def load():
         "masx_set_topology_morton": ([vp, C.c_int, up, up, up, C.c_int, dp], C.c_int),
         "masx_set_contacts": ([vp, C.c_int, ip, C.c_int, C.c_int], C.c_int),
         "masx_morton_partition": ([C.c_int, dp, C.c_int, ip, C.c_int, ip], C.c_int),
+        "masx_spmv_layout_check": ([C.c_int, C.c_longlong, ip, ip, C.c_int, C.c_uint, dp, C.POINTER(C.c_longlong),
+                                    C.POINTER(C.c_longlong)], C.c_int),
         "masx_analyze": ([vp], C.c_int),
         "masx_setup": ([vp], C.c_int),
         "masx_set_values": ([vp, C.c_int, vp, vp], C.c_int),
@@ -183,6 +185,18 @@ def morton_partition(positions, bank=32):
     r2p = np.zeros(n, np.int32)
     check(L.masx_morton_partition(n, _ptr(pos, C.c_double), bank, _ptr(p2r, C.c_int), len(p2r), _ptr(r2p, C.c_int)))
     return p2r, r2p
+
+
+def spmv_layout_check(n_rows, rows, cols, entries_per_thread, seed=0):
+    """Host check of the FP32 SpMV's tile layout for one system's upper block
+    structure: (max |y_layout - y_full| / max |y_full| in FP64 on seeded
+    random blocks and x, layout entries, pairs stored once)."""
+    L = load()
+    r, c = _i32(rows), _i32(cols)
+    err, ne, nt = C.c_double(), C.c_longlong(), C.c_longlong()
+    check(L.masx_spmv_layout_check(n_rows, len(r), _ptr(r, C.c_int), _ptr(c, C.c_int), entries_per_thread, seed,
+                                   C.byref(err), C.byref(ne), C.byref(nt)))
+    return err.value, ne.value, nt.value
 
 
 def hierarchy_host(n_fem, n_slots, nbr_start, nbr_num, nbr_list, part_to_real, real_to_part, *, pairs=None,

@@ -188,6 +188,18 @@ int masx_hierarchy_host(int n_fem, int n_slots, int bank, int levels, const uint
                         int* out_total, int* level_count, int* level_begin, uint32_t* fine_mask, int* prefix,
                         int* going_next, long long going_next_cap, int* coarse_table);
 
+/* Check of the FP32 SpMV's tile layout on the host (no device): from the
+ * upper block structure (rows[k] <= cols[k], unique) of one system of n_rows
+ * block rows, build the full block CSR, then the layout for tiles of
+ * 256 * entries_per_thread entries (pairs inside a tile stored once), and
+ * apply both to seeded random upper blocks and x in FP64.  Outputs: the
+ * largest |y_layout - y_full| over max |y_full|, the layout's entry count and
+ * its pairs stored once.  Not part of the solve; the layout itself has no
+ * counterpart in the reference (MASPreconditioner.cu and the BCOO SpMV of
+ * pcg_solver.cu store and multiply the full matrix). */
+int masx_spmv_layout_check(int n_rows, long long nb, const int* rows, const int* cols, int entries_per_thread,
+                           unsigned seed, double* max_rel_err, long long* n_entries, long long* n_in_tile);
+
 /* ---- Stable Neo-Hookean element Hessians and their parameter gradient ---
  * The FEM part of the Newton matrix (femEnergy.cu: first Piola stress
  * :441-459 with rates u = 4 mu/3, r = lambda + 5 mu/6 (gl_main.cu:586-590),

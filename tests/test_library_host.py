@@ -101,3 +101,60 @@ def test_mesh_structure_equals_the_generator_structure():
 def test_library_rejects_bad_input_with_an_error():
     with pytest.raises(RuntimeError):
         native.hierarchy_host(10, 5, np.zeros(10), np.zeros(10), np.zeros(1), np.arange(5), np.arange(10))
+
+
+def _upper_of(rows, cols, perm=None):
+    """Upper structure (r <= c, sorted, unique) after renumbering rows by perm."""
+    r, c = np.asarray(rows, np.int64), np.asarray(cols, np.int64)
+    if perm is not None:
+        r, c = perm[r], perm[c]
+    lo, hi = np.minimum(r, c), np.maximum(r, c)
+    key = np.unique(lo * (1 << 32) + hi)
+    return (key >> 32).astype(np.int32), (key & 0xffffffff).astype(np.int32)
+
+
+@pytest.mark.parametrize("order", ["grid", "morton"])
+@pytest.mark.parametrize("entries", [1, 2, 4, 8])
+def test_spmv_layout_equals_the_full_csr_product(order, entries):
+    # the 40^3 + 20,000-contact structure: the layout's product (pairs inside a
+    # tile stored once, their transposed products summed by the column row)
+    # equals the full both-triangle CSR product; every pair stored once saves
+    # exactly one entry
+    w = W.fem_grid(40, n_contacts=20000, seed=0)
+    n = w["n_vertices"]
+    perm = None
+    if order == "morton":
+        _, r2p = native.morton_partition(w["positions"], 32)
+        perm = r2p.astype(np.int64)
+    rows, cols = _upper_of(w["rows"], w["cols"], perm)
+    err, n_ent, n_once = native.spmv_layout_check(n, rows, cols, entries, seed=3)
+    off = int((rows != cols).sum())
+    assert err <= 1e-13
+    assert n_ent == n + 2 * off - n_once
+    assert 0 < n_once < off
+    if order == "morton" and entries == 8:
+        assert n_once >= 0.55 * off  # Morton tiles keep most pairs local
+
+
+def test_spmv_layout_long_rows_and_empty_rows():
+    # a hub row coupled to every other row (longer than one chunk: a tile of
+    # its own, several chunks), rows with only their diagonal, a tridiagonal
+    # band, and the hub as the last row too
+    n = 3000
+    rows = [np.arange(n), np.arange(0, n - 1, 2), np.zeros(n - 1, np.int64), np.arange(n - 1)]
+    cols = [np.arange(n), np.arange(1, n, 2), np.arange(1, n), np.full(n - 1, n - 1)]
+    r, c = _upper_of(np.concatenate(rows), np.concatenate(cols))
+    for entries in (1, 2, 8):
+        err, n_ent, n_once = native.spmv_layout_check(n, r, c, entries, seed=entries)
+        off = int((r != c).sum())
+        assert err <= 1e-13
+        assert n_ent == n + 2 * off - n_once
+
+
+def test_spmv_layout_rejects_bad_structure():
+    with pytest.raises(RuntimeError, match="upper structure"):
+        native.spmv_layout_check(4, np.array([1, 0]), np.array([0, 0]), 2)
+    with pytest.raises(RuntimeError, match="duplicate"):
+        native.spmv_layout_check(4, np.array([0, 0]), np.array([1, 1]), 2)
+    with pytest.raises(RuntimeError, match="entries per thread"):
+        native.spmv_layout_check(4, np.array([0]), np.array([0]), 512)
