@@ -2,6 +2,7 @@
 
   python -m agipc_learning_amd.prof_summary stats  <dir>/<name>_kernel_stats.csv  [--cmd "..."]
   python -m agipc_learning_amd.prof_summary busy   <dir>/<name>_kernel_trace.csv  [--kernels "a;b;..."]
+  python -m agipc_learning_amd.prof_summary compare <bench.json> <dir>/<name>_kernel_trace.csv
   python -m agipc_learning_amd.prof_summary traffic <fetch.csv> <write.csv> <copy_fetch.csv> <copy_write.csv>
          [--cmd "..."] [--json out.json]
 
@@ -9,7 +10,8 @@
 reports each kernel's dispatch durations over the launches that did the work
 (at least half the 95th-percentile duration; launches for systems already
 converged exit at once and would pull a plain mean down), mean and median.
-`traffic`
+`compare` sets bench.py's per-kernel times (its JSON line's roofline
+object) beside the busy medians of a trace of the same workload.  `traffic`
 turns FETCH_SIZE / WRITE_SIZE (KiB per dispatch) into HBM bytes per launch,
 each scaled by what the same counter reports for a plain device copy of known
 size (the copy's true read and write bytes over its counter values), and
@@ -61,6 +63,32 @@ def busy(path, kernels=None):
         b = [x for x in v if x >= cut]
         out.append(f"| {k} | {len(b)}/{len(v)} | {sum(b) / len(b):.2f} | {b[len(b) // 2]:.2f} | {b[0]:.2f} | "
                    f"{b[-1]:.2f} |")
+    return "\n".join(out)
+
+
+def _busy_median(durations):
+    v = sorted(durations)
+    cut = 0.5 * v[min(len(v) - 1, int(0.95 * len(v)))]
+    b = [x for x in v if x >= cut]
+    return b[len(b) // 2]
+
+
+def compare(bench_json, trace_csv):
+    line = [l for l in open(bench_json).read().splitlines() if l.strip().startswith("{")][-1]
+    bench = json.loads(line)
+    tiling = bench["spmv_tiling"]["entries_per_thread"]
+    names = {"spmv": f"k_spmv_fused<{tiling}>", "mas_local": "k_mas_local<32, float>",
+             "precond_finish": "k_precond_finish", "update": "k_update"}
+    d = collections.defaultdict(list)
+    for r in csv.DictReader(open(trace_csv)):
+        d[_short(r["Kernel_Name"])].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    out = ["| kernel | bench.py events, unprofiled (us) | rocprofv3 busy median (us) | trace / events |",
+           "|---|---|---|---|"]
+    for k, n in names.items():
+        e = bench["roofline"]["per_kernel"][k]["us"]
+        if n in d:
+            m = _busy_median(d[n])
+            out.append(f"| {n} | {e:.2f} | {m:.2f} | {m / e:.3f} |")
     return "\n".join(out)
 
 
@@ -124,6 +152,9 @@ def main(argv=None):
     a = sub.add_parser("stats")
     a.add_argument("csv")
     a.add_argument("--cmd")
+    cp = sub.add_parser("compare")
+    cp.add_argument("bench_json")
+    cp.add_argument("trace_csv")
     c = sub.add_parser("busy")
     c.add_argument("csv")
     c.add_argument("--kernels")
@@ -137,6 +168,8 @@ def main(argv=None):
     args = ap.parse_args(argv)
     if args.what == "stats":
         print(stats(args.csv, args.cmd))
+    elif args.what == "compare":
+        print(compare(args.bench_json, args.trace_csv))
     elif args.what == "busy":
         print(busy(args.csv, args.kernels.split(";") if args.kernels else None))
     else:
