@@ -55,6 +55,8 @@ struct DevData
 {
     // sizes
     int n_rows, n_sys, bank, n_banks, n_fine_banks;
+    int single;                         // 1: one system (system index 0, whole work tables)
+    int n_wg_pre, n_wg_spmv, n_wg_upd;  // workgroups of the iteration kernels' tables
     long long nnz;          // entries of the full block CSR
     long long nnz_s, plane; // entries of the FP32 SpMV layout; padded entry stride of its value planes
     // structure: full block CSR (FP64 paths)

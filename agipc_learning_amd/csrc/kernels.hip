@@ -112,22 +112,31 @@ __device__ double block_sum_last(double v, double* smem)
     return t;
 }
 
-// Sum of the partials of system s's workgroups of the producing kernel
-// (range = per-system [begin, end) pairs).  Identical in every workgroup.
-__device__ double sys_sum(const double* part, const int* range, int s, double* smem)
+// [first, end) of system s's workgroups in a kernel's table (range = the
+// per-system pairs, n_wg = the table's size).  With one system that is the
+// whole table, known without a load, so the loads that depend on it can
+// issue at once.
+__device__ __forceinline__ int2 wg_range(const DevData& d, const int* range, int n_wg, int s)
 {
-    const int b = range[2 * s], e = range[2 * s + 1];
+    return d.single ? make_int2(0, n_wg) : make_int2(range[2 * s], range[2 * s + 1]);
+}
+
+// Sum of the partials of one system's workgroups of the producing kernel
+// (r = their range).  Identical in every workgroup.
+__device__ double sys_sum(const double* part, int2 r, double* smem)
+{
+    const int b = r.x, e = r.y;
     double v = 0.0;
     for(int i = b + (int)threadIdx.x; i < e; i += blockDim.x) v += part[i];
     return block_sum(v, smem);
 }
 
 // The loads of sys_sum without the reduction: this thread's share of the
-// partials of system s (eight loads in flight per pass).  Forming the total
-// with block_sum later lets the caller issue its other loads in between.
-__device__ __forceinline__ double partial_share(const double* part, const int* range, int s)
+// partials (eight loads in flight per pass).  Forming the total with
+// block_sum later lets the caller issue its other loads in between.
+__device__ __forceinline__ double partial_share(const double* part, int2 r)
 {
-    const int b = range[2 * s], e = range[2 * s + 1];
+    const int b = r.x, e = r.y;
     double v = 0.0;
 #pragma unroll 1
     for(int i = b + (int)threadIdx.x; i < e; i += 8 * 256)
@@ -659,7 +668,7 @@ __global__ void __launch_bounds__(256) k_phase_finish(DevData d)
     const int s = blockIdx.x;
     SysState& st = d.state[s];
     if(st.status != kInit && st.status != kRestarting) return;
-    const double v = sys_sum(d.part_upd, d.wg_upd_range, s, smem);
+    const double v = sys_sum(d.part_upd, wg_range(d, d.wg_upd_range, d.n_wg_upd, s), smem);
     if(threadIdx.x == 0)
     {
         if(st.status == kInit)
@@ -715,7 +724,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
         {
             s_kind[k] = d.bank_kind[bb];
             s_vec[k]  = d.bank_vec[bb];
-            s_live[k] = force || d.state[d.bank_sys[bb]].status == kRunning;
+            s_live[k] = force || d.state[d.single ? 0 : d.bank_sys[bb]].status == kRunning;
         }
     }
     {
@@ -886,10 +895,10 @@ __global__ void __launch_bounds__(256) k_precond_finish(DevData d, int force)
 {
     __shared__ double smem[8];
     const int4 w = d.wg_pre[blockIdx.x];
-    const int  s = w.x;
+    const int  s = d.single ? 0 : w.x;
     SysState&  st = d.state[s];
     if(!force && st.status != kRunning) return;
-    const double share = force ? 0.0 : partial_share(d.part_upd, d.wg_upd_range, s);
+    const double share = force ? 0.0 : partial_share(d.part_upd, wg_range(d, d.wg_upd_range, d.n_wg_upd, s));
     const int type = w.y >> 28;
     // row loads, independent of the decision
     const int row = w.z + threadIdx.x;
@@ -969,7 +978,7 @@ __global__ void __launch_bounds__(256) k_precond_finish(DevData d, int force)
                 else if(st.check_every > 0 && st.k - st.k_check >= st.check_every) chk = 1;
             }
         }
-        if(blockIdx.x == d.wg_pre_range[2 * s] && threadIdx.x == 0)
+        if((int)blockIdx.x == wg_range(d, d.wg_pre_range, d.n_wg_pre, s).x && threadIdx.x == 0)
         {
             st.rr = rr;
             if(ns != kRunning) st.status = ns;
@@ -1230,7 +1239,7 @@ __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __re
     __shared__ double smem[8];
     const int  wg = xcd_tile(blockIdx.x, gridDim.x);
     const int4 w  = d.wg_spmv[wg];
-    const int  s  = w.x;
+    const int  s  = d.single ? 0 : w.x;
     SysState&  st = d.state[s];
     if(st.status != kRunning)
     {
@@ -1248,7 +1257,7 @@ __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __re
     const int    st_rule = st.rule, st_frozen = st.frozen, st_beta_zero = st.beta_zero, st_k = st.k,
                  st_restarts = st.restarts;
     const double st_rz_prev = st.rz_prev;
-    const double share = partial_share(d.part_pre, d.wg_pre_range, s);
+    const double share = partial_share(d.part_pre, wg_range(d, d.wg_pre_range, d.n_wg_pre, s));
     const int r0 = w.z, r1 = w.w;
     const int2 er = d.wg_spmv_ent[wg];
     const int e0 = er.x, e1 = er.y;
@@ -1272,7 +1281,7 @@ __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __re
     if(n0 > 0) spmv_load_chunk<U>(d, p_old, e0, n0, c);
     const double rz = block_sum(share, smem);
     const bool bad = !finite_d(rz) || (st_rule == kRuleTrueRel && rz <= 0.0);
-    if(wg == d.wg_spmv_range[2 * s] && threadIdx.x == 0)
+    if(wg == wg_range(d, d.wg_spmv_range, d.n_wg_spmv, s).x && threadIdx.x == 0)
     {
         if(bad && !st_frozen)
         {
@@ -1324,7 +1333,7 @@ __global__ void __launch_bounds__(256) k_update(DevData d, const float* __restri
 {
     __shared__ double smem[8];
     const int4 w = d.wg_upd[blockIdx.x];
-    const int  s = w.x;
+    const int  s = d.single ? 0 : w.x;
     SysState&  st = d.state[s];
     bool init = false;
     if(mode == 0)
@@ -1334,9 +1343,10 @@ __global__ void __launch_bounds__(256) k_update(DevData d, const float* __restri
             // confirm a candidate whose true residual the SpMV just formed:
             // done if it meets the tolerance, else left to the chunk's check
             // (need_check 2: no second in-iteration attempt)
-            if(st.status == kCandidate && st.need_check == 1 && (int)blockIdx.x == d.wg_upd_range[2 * s])
+            if(st.status == kCandidate && st.need_check == 1 &&
+               (int)blockIdx.x == wg_range(d, d.wg_upd_range, d.n_wg_upd, s).x)
             {
-                const double tr = sys_sum(d.part_tr, d.wg_spmv_range, s, smem);
+                const double tr = sys_sum(d.part_tr, wg_range(d, d.wg_spmv_range, d.n_wg_spmv, s), smem);
                 if(threadIdx.x == 0)
                 {
                     if(tr <= st.tol * st.tol * st.bb)
@@ -1361,7 +1371,7 @@ __global__ void __launch_bounds__(256) k_update(DevData d, const float* __restri
     }
     // read before this kernel's first store, so it stays a scalar load
     const int4 grp = d.wg_upd_grp[blockIdx.x];  // uniform over the workgroup
-    const double share = mode == 0 ? partial_share(d.part_spmv, d.wg_spmv_range, s) : 0.0;
+    const double share = mode == 0 ? partial_share(d.part_spmv, wg_range(d, d.wg_spmv_range, d.n_wg_spmv, s)) : 0.0;
     const int type = w.y >> 28;
     const int aux  = w.y & 0x0fffffff;
     const int row  = w.z + threadIdx.x;
@@ -1415,7 +1425,7 @@ __global__ void __launch_bounds__(256) k_update(DevData d, const float* __restri
     {
         const double pAp = block_sum(share, smem);
         const bool bad = !finite_d(pAp) || (st.rule == kRuleTrueRel && pAp <= 0.0);
-        if(blockIdx.x == d.wg_upd_range[2 * s] && threadIdx.x == 0)
+        if((int)blockIdx.x == wg_range(d, d.wg_upd_range, d.n_wg_upd, s).x && threadIdx.x == 0)
         {
             if(bad && !st.frozen)
             {
@@ -1543,7 +1553,7 @@ __global__ void __launch_bounds__(256) k_check_finish(DevData d, int report_only
     SysState& st = d.state[s];
     if(!report_only && !st.need_check) return;
     if(report_only && report_is_current(st)) return;
-    const double tr = sys_sum(d.part_tr, d.wg_spmv_range, s, smem);
+    const double tr = sys_sum(d.part_tr, wg_range(d, d.wg_spmv_range, d.n_wg_spmv, s), smem);
     if(threadIdx.x != 0) return;
     st.true_rr = tr;
     if(report_only) return;

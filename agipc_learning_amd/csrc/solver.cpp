@@ -376,6 +376,10 @@ void Solver::make_devdata()
     DevData& d = dd_;
     d.n_rows = P.n_rows;
     d.n_sys = P.n_sys;
+    d.single = P.n_sys == 1 ? 1 : 0;
+    d.n_wg_pre = n_wg_pre_;
+    d.n_wg_spmv = n_wg_spmv_;
+    d.n_wg_upd = n_wg_upd_;
     d.bank = P.bank;
     d.n_banks = P.n_banks;
     d.n_fine_banks = P.n_fine_banks;
