@@ -76,7 +76,12 @@ def _busy_median(durations):
 def compare(bench_json, trace_csv):
     line = [l for l in open(bench_json).read().splitlines() if l.strip().startswith("{")][-1]
     bench = json.loads(line)
-    tiling = bench["spmv_tiling"]["entries_per_thread"]
+    if "batch64" in bench:  # bench.py --batch-only: FP32 kernels, SpMV tiling 4
+        us = bench["batch64"]["fp32"]["kernels_us"]
+        tiling = 4
+    else:
+        us = {k: v["us"] for k, v in bench["roofline"]["per_kernel"].items()}
+        tiling = bench["spmv_tiling"]["entries_per_thread"]
     names = {"spmv": f"k_spmv_fused<{tiling}>", "mas_local": "k_mas_local<32, float>",
              "precond_finish": "k_precond_finish", "update": "k_update"}
     d = collections.defaultdict(list)
@@ -85,7 +90,7 @@ def compare(bench_json, trace_csv):
     out = ["| kernel | bench.py events, unprofiled (us) | rocprofv3 busy median (us) | trace / events |",
            "|---|---|---|---|"]
     for k, n in names.items():
-        e = bench["roofline"]["per_kernel"][k]["us"]
+        e = us[k]
         if n in d:
             m = _busy_median(d[n])
             out.append(f"| {n} | {e:.2f} | {m:.2f} | {m / e:.3f} |")

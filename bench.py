@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-differentiated solve")
     ap.add_argument("--adjoint-n", type=int, default=60, help="hex cells per edge of the adjoint workload")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only run the batched 64-system measurement and print its figures (profiling aid)")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only run the adjoint-differentiated solve and print its figures (profiling aid)")
     ap.add_argument("--spmv-tiling", type=int, default=0, choices=(0, 2, 4, 8),
@@ -87,10 +89,10 @@ KERNEL_PREFIX = {"spmv": "k_spmv_fused", "mas_local": "k_mas_local<32, float>", 
                  "precond_finish": "k_precond_finish"}
 
 
-def measured_traffic(kernel, workload_is_default, spmv_entries):
+def measured_traffic(kernel, workload_is_default, spmv_entries, summary="fem40c_hbm_traffic.json"):
     """HBM bytes per launch of `kernel` from the committed counter summary of
     this workload (FETCH_SIZE + WRITE_SIZE runs, copy-calibrated), or None."""
-    path = os.path.join(ROOT, "profiles", "fem40c_hbm_traffic.json")
+    path = os.path.join(ROOT, "profiles", summary)
     if not workload_is_default or not os.path.exists(path):
         return None
     data = json.load(open(path))["kernels"]
@@ -152,6 +154,8 @@ def batch64(args):
     nv = sum(w["n_vertices"] for w in ws)
     log(f"step: batch64 analyze + setup ({nv} vertices, {sum(w['nb'] for w in ws)} upper blocks)")
     s = session.solver_for(ws)
+    batch_tiling = 4  # SpMV entries per thread (the library default), pinned for the traffic lookup
+    s.set_spmv_tiling(batch_tiling)
     vals = [torch.from_numpy(np.ascontiguousarray(w["vals"])).cuda() for w in ws]
     b = torch.from_numpy(session.rhs_of(ws)).cuda()
     x = torch.zeros_like(b)
@@ -179,6 +183,15 @@ def batch64(args):
         alg = algorithmic_bytes(s.info(), nv, 4 if prec == "fp32" else 2)
         out[prec]["kernels_us"] = kern
         out[prec]["kernels_GBs"] = {k: round(alg[k] / (kern[k] * 1e-6) / 1e9, 1) for k in kern}
+        if prec == "fp32":
+            # the clean HBM case: this working set is far beyond the 256 MB
+            # Infinity Cache, so its dominant kernel's rate is an HBM rate
+            dom = max(kern, key=lambda k: kern[k])
+            ach = alg[dom] / (kern[dom] * 1e-6) / 1e9
+            out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                               "traffic": measured_traffic(dom, True, batch_tiling, "batch64_hbm_traffic.json"),
+                               "algorithmic_bytes": alg[dom], "avg_us": kern[dom]}
         log(f"  batch64 {prec}: {ms:.2f} ms/step, iterations max {max(its)}, worst true rel res {worst:.2e}, "
             f"kernels us {kern}, GB/s {out[prec]['kernels_GBs']}")
     out["fp16_speedup_vs_fp32"] = round(out["fp32"]["ms"] / out["fp16"]["ms"], 3)
@@ -366,6 +379,9 @@ def main():
         return
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_leg(args)}), flush=True)
+        return
+    if args.batch_only:
+        print(json.dumps({"batch64": batch64(args)}), flush=True)
         return
 
     import numpy as np
