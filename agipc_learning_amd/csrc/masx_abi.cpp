@@ -15,6 +15,7 @@
 #include "converter.hpp"
 #include "fem.hpp"
 #include "hierarchy.hpp"
+#include "morton.hpp"
 #include "solver.hpp"
 
 namespace
@@ -431,6 +432,34 @@ int masx_hierarchy_host(int n_fem, int n_slots, int bank, int levels, const uint
         if(prefix) std::memcpy(prefix, h.prefix.data(), sizeof(int) * h.prefix.size());
         std::memcpy(going_next, h.going_next.data(), sizeof(int) * std::max(h.total, n_fem));
         if(coarse_table) std::memcpy(coarse_table, h.coarse_table.data(), sizeof(int) * 6 * (size_t)n_fem);
+    });
+}
+
+struct masx_morton
+{
+    masx::DeviceMorton impl;
+};
+
+int masx_morton_create(masx_morton** out)
+{
+    return guard([&] {
+        require(out != nullptr, "bad argument");
+        *out = new masx_morton();
+    });
+}
+
+int masx_morton_destroy(masx_morton* m)
+{
+    delete m;
+    return 0;
+}
+
+int masx_morton_partition_device(masx_morton* m, int n, const double* d_positions, int bank, int* d_part_to_real,
+                                 long long part_cap, int* d_real_to_part, void* stream)
+{
+    return guard([&] {
+        require(m != nullptr, "bad argument");
+        m->impl.partition(n, d_positions, bank, d_part_to_real, part_cap, d_real_to_part, (hipStream_t)stream);
     });
 }
 

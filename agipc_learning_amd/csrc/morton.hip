@@ -1,0 +1,306 @@
+// Morton-order vertex partition on the device (gl_main.cu:1176-1243): the
+// bounding box, a 30-bit code per vertex (10 bits per axis, x | y << 1 |
+// z << 2), vertices ordered by (code, index), banks padded with -1.
+//
+// The order is a stable least-significant-digit radix sort of the codes
+// (four passes of 8 bits), so ties keep the index order without a key
+// carrying the index, and every input distribution costs the same (a
+// bucket-and-rank scheme would go quadratic on clustered points).  Per pass:
+// per-tile digit counts, one exclusive scan over (digit, tile) in digit-major
+// order, then a stable scatter in which each wavefront ranks equal digits by
+// eight ballots.  Integer work, latency-bound at mesh sizes.
+#include "morton.hpp"
+
+#include <stdexcept>
+#include <string>
+
+namespace masx
+{
+namespace
+{
+
+constexpr int kThreads = 256;
+constexpr int kItems   = 8;
+constexpr int kTile    = kThreads * kItems;   // items per scatter tile
+constexpr int kBins    = 256;                  // 8-bit digits
+constexpr int kPasses  = 4;                    // 32 key bits (codes use 30)
+constexpr int kScan    = 1024;                 // threads of the single-workgroup scan
+constexpr int kBoxMax  = 1024;                 // partial bounding boxes
+
+#define MT_HIP(x)                                                                                    \
+    do                                                                                               \
+    {                                                                                                \
+        const hipError_t e_ = (x);                                                                   \
+        if(e_ != hipSuccess) throw std::runtime_error(std::string("HIP: ") + hipGetErrorString(e_)); \
+    } while(0)
+
+// per-block partial bounds: lo xyz, hi xyz
+__global__ void __launch_bounds__(kThreads) k_mt_box_partial(int n, const double* __restrict__ pos, double* part)
+{
+    __shared__ double s[6][kThreads];
+    const int t = threadIdx.x;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for(long long v = (long long)blockIdx.x * kThreads + t; v < n; v += (long long)gridDim.x * kThreads)
+#pragma unroll
+        for(int d = 0; d < 3; ++d)
+        {
+            const double p = pos[3 * v + d];
+            lo[d] = fmin(lo[d], p);
+            hi[d] = fmax(hi[d], p);
+        }
+#pragma unroll
+    for(int d = 0; d < 3; ++d)
+    {
+        s[d][t]     = lo[d];
+        s[3 + d][t] = hi[d];
+    }
+    __syncthreads();
+    for(int off = kThreads / 2; off > 0; off >>= 1)
+    {
+        if(t < off)
+#pragma unroll
+            for(int d = 0; d < 3; ++d)
+            {
+                s[d][t]     = fmin(s[d][t], s[d][t + off]);
+                s[3 + d][t] = fmax(s[3 + d][t], s[3 + d][t + off]);
+            }
+        __syncthreads();
+    }
+    if(t < 6) part[6 * blockIdx.x + t] = s[t][0];
+}
+
+// the partials -> the box (written after the partials, at part + 6 * g)
+__global__ void __launch_bounds__(kThreads) k_mt_box_final(int g, double* part)
+{
+    __shared__ double s[6][kThreads];
+    const int t = threadIdx.x;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for(int b = t; b < g; b += kThreads)
+#pragma unroll
+        for(int d = 0; d < 3; ++d)
+        {
+            lo[d] = fmin(lo[d], part[6 * b + d]);
+            hi[d] = fmax(hi[d], part[6 * b + 3 + d]);
+        }
+#pragma unroll
+    for(int d = 0; d < 3; ++d)
+    {
+        s[d][t]     = lo[d];
+        s[3 + d][t] = hi[d];
+    }
+    __syncthreads();
+    for(int off = kThreads / 2; off > 0; off >>= 1)
+    {
+        if(t < off)
+#pragma unroll
+            for(int d = 0; d < 3; ++d)
+            {
+                s[d][t]     = fmin(s[d][t], s[d][t + off]);
+                s[3 + d][t] = fmax(s[3 + d][t], s[3 + d][t + off]);
+            }
+        __syncthreads();
+    }
+    if(t < 6) part[6 * g + t] = s[t][0];
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v)
+{
+    uint32_t out = 0;
+#pragma unroll
+    for(int b = 0; b < 10; ++b) out |= ((v >> b) & 1u) << (3 * b);
+    return out;
+}
+
+// code per vertex: t = clamp((p - lo) / (hi - lo), 0, 1) per axis (0 for a
+// flat axis), q = (uint)(t * 1023 + 0.5) with the multiply and the add
+// rounded separately (as the host forms it), 10 bits interleaved
+__global__ void __launch_bounds__(kThreads) k_mt_codes(int n, const double* __restrict__ pos,
+                                                       const double* __restrict__ box, uint32_t* key, int* idx)
+{
+    const int v = blockIdx.x * kThreads + threadIdx.x;
+    if(v >= n) return;
+    uint32_t code = 0;
+#pragma unroll
+    for(int d = 0; d < 3; ++d)
+    {
+        const double lo = box[d], e = __dsub_rn(box[3 + d], lo);
+        double t = 0.0;
+        if(e > 0.0) t = fmin(fmax(__ddiv_rn(__dsub_rn(pos[3 * (long long)v + d], lo), e), 0.0), 1.0);
+        const double q = __dadd_rn(__dmul_rn(t, 1023.0), 0.5);
+        code |= spread10((uint32_t)q) << d;
+    }
+    key[v] = code;
+    idx[v] = v;
+}
+
+// digit counts of one tile -> hist[digit * n_tiles + tile]
+__global__ void __launch_bounds__(kThreads) k_mt_hist(int n, const uint32_t* __restrict__ key, int shift, int* hist,
+                                                      int n_tiles)
+{
+    __shared__ int cnt[kBins];
+    const int t = threadIdx.x, tile = blockIdx.x;
+    cnt[t] = 0;
+    __syncthreads();
+#pragma unroll
+    for(int j = 0; j < kItems; ++j)
+    {
+        const long long i = (long long)tile * kTile + j * kThreads + t;
+        if(i < n) atomicAdd(&cnt[(key[i] >> shift) & (kBins - 1)], 1);
+    }
+    __syncthreads();
+    hist[(long long)t * n_tiles + tile] = cnt[t];
+}
+
+// exclusive scan of a[0..m) in place, one workgroup: each thread sums a
+// contiguous chunk, the chunk sums are scanned in LDS, each chunk is rewritten
+__global__ void __launch_bounds__(kScan) k_mt_scan(int m, int* a)
+{
+    __shared__ int s[kScan];
+    const int t = threadIdx.x;
+    const int chunk = (m + kScan - 1) / kScan;
+    const int b = min(m, t * chunk), e = min(m, b + chunk);
+    int sum = 0;
+    for(int i = b; i < e; ++i) sum += a[i];
+    s[t] = sum;
+    __syncthreads();
+    for(int off = 1; off < kScan; off <<= 1)
+    {
+        const int v = t >= off ? s[t - off] : 0;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
+    }
+    int run = s[t] - sum;
+    for(int i = b; i < e; ++i)
+    {
+        const int v = a[i];
+        a[i] = run;
+        run += v;
+    }
+}
+
+// stable scatter of one tile: items in rounds of 256 consecutive indices;
+// inside a round, a lane's rank among equal digits comes from eight ballots
+// (lanes of its wavefront) plus the counts of the earlier wavefronts, and
+// `running` carries each digit's count over the rounds
+__global__ void __launch_bounds__(kThreads) k_mt_scatter(int n, const uint32_t* __restrict__ key_in,
+                                                         const int* __restrict__ idx_in, uint32_t* key_out,
+                                                         int* idx_out, int shift, const int* __restrict__ offs,
+                                                         int n_tiles)
+{
+    __shared__ int running[kBins];
+    __shared__ int wcnt[kThreads / 64][kBins];
+    const int t = threadIdx.x, tile = blockIdx.x, lane = t & 63, wave = t >> 6;
+    running[t] = 0;
+    const int base = offs[(long long)t * n_tiles + tile];  // this tile's start for digit t
+    __shared__ int sbase[kBins];
+    sbase[t] = base;
+    for(int j = 0; j < kItems; ++j)
+    {
+#pragma unroll
+        for(int w = 0; w < kThreads / 64; ++w) wcnt[w][t] = 0;
+        __syncthreads();
+        const long long i = (long long)tile * kTile + j * kThreads + t;
+        const bool valid = i < n;
+        const uint32_t k = valid ? key_in[i] : 0u;
+        const int d = (int)((k >> shift) & (kBins - 1));
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for(int b = 0; b < 8; ++b)
+        {
+            const bool bit = (d >> b) & 1;
+            const unsigned long long m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const unsigned long long below = peers & ((1ull << lane) - 1ull);
+        if(valid && below == 0ull) wcnt[wave][d] = __popcll(peers);
+        __syncthreads();
+        if(valid)
+        {
+            int pre = running[d];
+            for(int w = 0; w < wave; ++w) pre += wcnt[w][d];
+            const int p = sbase[d] + pre + __popcll(below);
+            key_out[p] = k;
+            idx_out[p] = idx_in[i];
+        }
+        __syncthreads();
+        int add = 0;
+#pragma unroll
+        for(int w = 0; w < kThreads / 64; ++w) add += wcnt[w][t];
+        running[t] += add;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_mt_maps(int n, int padded, const int* __restrict__ order, int* p2r,
+                                                      int* r2p)
+{
+    const int s = blockIdx.x * kThreads + threadIdx.x;
+    if(s >= padded) return;
+    const int v = s < n ? order[s] : -1;
+    p2r[s] = v;
+    if(s < n) r2p[v] = s;
+}
+
+template <typename T>
+void grow(T*& p, size_t& cap, size_t need)
+{
+    if(need <= cap && p) return;
+    if(p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    MT_HIP(hipMalloc((void**)&p, need * sizeof(T)));
+    cap = need;
+}
+
+}  // namespace
+
+DeviceMorton::~DeviceMorton()
+{
+    for(void* p : {(void*)box_, (void*)key_[0], (void*)key_[1], (void*)idx_[0], (void*)idx_[1], (void*)hist_,
+                   (void*)part_})
+        if(p) (void)hipFree(p);
+}
+
+void DeviceMorton::partition(int n, const double* pos, int bank, int* part_to_real, long long part_cap,
+                             int* real_to_part, hipStream_t st)
+{
+    if(n < 1 || bank < 1 || !pos || !part_to_real || !real_to_part)
+        throw std::invalid_argument("morton partition: n >= 1, bank >= 1 and device arrays required");
+    const long long padded = ((long long)n + bank - 1) / bank * bank;
+    if(part_cap < padded) throw std::invalid_argument("part_to_real buffer smaller than the padded vertex count");
+    if(padded > 0x7fffffffLL) throw std::invalid_argument("too many vertices");
+    const int n_tiles = (int)(((long long)n + kTile - 1) / kTile);
+    const int g = (int)std::min<long long>(kBoxMax, ((long long)n + kThreads - 1) / kThreads);
+    grow(box_, box_cap_, 6 * (size_t)(g + 1));
+    for(int b = 0; b < 2; ++b)
+    {
+        grow(key_[b], key_cap_[b], (size_t)n);
+        grow(idx_[b], idx_cap_[b], (size_t)n);
+    }
+    grow(hist_, hist_cap_, (size_t)kBins * n_tiles);
+    const unsigned blocks = (unsigned)(((long long)n + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(k_mt_box_partial, dim3(g), dim3(kThreads), 0, st, n, pos, box_);
+    MT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_mt_box_final, dim3(1), dim3(kThreads), 0, st, g, box_);
+    MT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_mt_codes, dim3(blocks), dim3(kThreads), 0, st, n, pos, box_ + 6 * g, key_[0], idx_[0]);
+    MT_HIP(hipGetLastError());
+    int cur = 0;
+    for(int p = 0; p < kPasses; ++p)
+    {
+        const int shift = 8 * p;
+        hipLaunchKernelGGL(k_mt_hist, dim3(n_tiles), dim3(kThreads), 0, st, n, key_[cur], shift, hist_, n_tiles);
+        MT_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_mt_scan, dim3(1), dim3(kScan), 0, st, kBins * n_tiles, hist_);
+        MT_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_mt_scatter, dim3(n_tiles), dim3(kThreads), 0, st, n, key_[cur], idx_[cur], key_[1 - cur],
+                           idx_[1 - cur], shift, hist_, n_tiles);
+        MT_HIP(hipGetLastError());
+        cur = 1 - cur;
+    }
+    hipLaunchKernelGGL(k_mt_maps, dim3((unsigned)((padded + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, n,
+                       (int)padded, idx_[cur], part_to_real, real_to_part);
+    MT_HIP(hipGetLastError());
+}
+
+}  // namespace masx

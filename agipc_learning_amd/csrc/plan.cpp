@@ -106,9 +106,13 @@ void morton_partition(int n, const double* pos, int bank, std::vector<int>& p2r,
         uint32_t code = 0;
         for(int d = 0; d < 3; ++d)
         {
+            // separate statements: no fused multiply-add, so the device
+            // partition (morton.hip) forms the same codes bit for bit
             const double e = hi[d] - lo[d];
             const double t = e > 0.0 ? std::clamp((pos[3 * v + d] - lo[d]) / e, 0.0, 1.0) : 0.0;
-            code |= spread((uint32_t)(t * 1023.0 + 0.5)) << d;
+            const double s = t * 1023.0;
+            const double q = s + 0.5;
+            code |= spread((uint32_t)q) << d;
         }
         key[v] = {code, v};
     }

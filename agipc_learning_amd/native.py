@@ -88,6 +88,9 @@ def load():
         "masx_set_topology_morton": ([vp, C.c_int, up, up, up, C.c_int, dp], C.c_int),
         "masx_set_contacts": ([vp, C.c_int, ip, C.c_int, C.c_int], C.c_int),
         "masx_morton_partition": ([C.c_int, dp, C.c_int, ip, C.c_int, ip], C.c_int),
+        "masx_morton_create": ([C.POINTER(vp)], C.c_int),
+        "masx_morton_destroy": ([vp], C.c_int),
+        "masx_morton_partition_device": ([vp, C.c_int, vp, C.c_int, vp, C.c_longlong, vp, vp], C.c_int),
         "masx_spmv_layout_check": ([C.c_int, C.c_longlong, ip, ip, C.c_int, C.c_uint, dp, C.POINTER(C.c_longlong),
                                     C.POINTER(C.c_longlong)], C.c_int),
         "masx_analyze": ([vp], C.c_int),
@@ -185,6 +188,39 @@ def morton_partition(positions, bank=32):
     r2p = np.zeros(n, np.int32)
     check(L.masx_morton_partition(n, _ptr(pos, C.c_double), bank, _ptr(p2r, C.c_int), len(p2r), _ptr(r2p, C.c_int)))
     return p2r, r2p
+
+
+class DeviceMorton:
+    """Morton vertex partition on the device (masx_morton_*): the same arrays
+    as morton_partition, from a CUDA float64 tensor of positions (n, 3)."""
+
+    def __init__(self):
+        self.L = load()
+        h = C.c_void_p()
+        check(self.L.masx_morton_create(C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.L.masx_morton_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def partition(self, positions, bank=32, stream=None):
+        import torch
+
+        pos = positions.contiguous()
+        if pos.dtype != torch.float64 or not pos.is_cuda or pos.dim() != 2 or pos.shape[1] != 3:
+            raise ValueError("positions must be a CUDA float64 tensor of shape (n, 3)")
+        n = pos.shape[0]
+        padded = (n + bank - 1) // bank * bank
+        p2r = torch.empty(padded, dtype=torch.int32, device=pos.device)
+        r2p = torch.empty(n, dtype=torch.int32, device=pos.device)
+        check(self.L.masx_morton_partition_device(self.h, n, _dev(pos), bank, _dev(p2r), padded, _dev(r2p),
+                                                  stream))
+        return p2r, r2p
 
 
 def spmv_layout_check(n_rows, rows, cols, entries_per_thread, seed=0):

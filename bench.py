@@ -536,6 +536,22 @@ def main():
         raise SystemExit("device hierarchy differs from the host build")
     restructure["hierarchy_host_ms"] = round(sorted(host_ms)[1], 3)
     restructure["hierarchy_device_ms"] = round(sorted(dev_ms[1:])[2], 3)
+    # the Morton partition the banks are cut from: host vs device (same arrays)
+    mt_host, mt_dev = [], []
+    dm = native.DeviceMorton()
+    d_pos = torch.from_numpy(np.ascontiguousarray(w["positions"], np.float64)).cuda()
+    for _ in range(5):
+        t = time.perf_counter()
+        native.morton_partition(w["positions"], 32)
+        mt_host.append((time.perf_counter() - t) * 1e3)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        dm.partition(d_pos, 32)
+        torch.cuda.synchronize()
+        mt_dev.append((time.perf_counter() - t) * 1e3)
+    restructure["morton_host_ms"] = round(sorted(mt_host)[2], 3)
+    restructure["morton_device_ms"] = round(sorted(mt_dev[1:])[1], 3)
+    dm.close()
     log(f"  structure change: {restructure}")
     del hier, d_in, d_pairs
 

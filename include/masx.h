@@ -188,6 +188,19 @@ int masx_hierarchy_host(int n_fem, int n_slots, int bank, int levels, const uint
                         int* out_total, int* level_count, int* level_begin, uint32_t* fine_mask, int* prefix,
                         int* going_next, long long going_next_cap, int* coarse_table);
 
+/* ---- Morton vertex partition on the device ------------------------------
+ * gl_main.cu:1176-1243 (the mesh reordering the MAS banks are cut from) on
+ * the GPU: the same result as masx_morton_partition bit for bit, from device
+ * positions (3 doubles per vertex) into device arrays part_to_real
+ * (pad(n, bank) ints, -1 in the padding; part_cap >= that) and real_to_part
+ * (n ints).  Asynchronous on `stream`; scratch is kept in the object and
+ * grows as needed. */
+typedef struct masx_morton masx_morton;
+int masx_morton_create(masx_morton** out);
+int masx_morton_destroy(masx_morton* m);
+int masx_morton_partition_device(masx_morton* m, int n, const double* d_positions, int bank, int* d_part_to_real,
+                                 long long part_cap, int* d_real_to_part, void* stream);
+
 /* Check of the FP32 SpMV's tile layout on the host (no device): from the
  * upper block structure (rows[k] <= cols[k], unique) of one system of n_rows
  * block rows, build the full block CSR, then the layout for tiles of

@@ -1,0 +1,87 @@
+"""Morton vertex partition on the device (masx_morton_*, gl_main.cu:1176-1243
+on the GPU) against the host partition (masx_morton_partition, itself pinned
+to the oracle): identical arrays, bit for bit, on grids, random clouds,
+duplicate points (ties keep the index order), flat axes, a single point, all
+points equal, sizes around the bank and a cloud of a million points (many
+scatter tiles)."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from agipc_learning_amd import native
+from agipc_learning_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def both(pos, bank=32):
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    hp, hr = native.morton_partition(pos, bank)
+    dm = native.DeviceMorton()
+    dp, dr = dm.partition(torch.from_numpy(pos).cuda(), bank)
+    torch.cuda.synchronize()
+    return hp, hr, dp.cpu().numpy(), dr.cpu().numpy()
+
+
+def check(pos, bank=32):
+    hp, hr, dp, dr = both(pos, bank)
+    assert np.array_equal(hp, dp)
+    assert np.array_equal(hr, dr)
+    n = len(hr)
+    assert np.array_equal(np.sort(dp[:n]), np.arange(n)) and np.all(dp[n:] == -1)
+    return dp
+
+
+def test_grid_positions():
+    w = W.fem_grid(20, seed=0)
+    check(w["positions"])
+
+
+@pytest.mark.parametrize("n,seed", [(1000, 0), (100_000, 1), (33, 2), (32, 3), (31, 4)])
+def test_random_clouds(n, seed):
+    rng = np.random.default_rng(seed)
+    check(rng.standard_normal((n, 3)) * [1.0, 3.0, 0.01] + [5.0, -2.0, 1e6])
+
+
+def test_duplicate_points_keep_index_order():
+    rng = np.random.default_rng(5)
+    base = rng.random((500, 3))
+    pos = base[rng.integers(0, 500, 20_000)]
+    p = check(pos)[: len(pos)]
+    same = np.all(pos[p[:-1]] == pos[p[1:]], axis=1)
+    assert same.any() and np.all(np.diff(p)[same] > 0)  # equal points stay in index order
+
+
+def test_flat_axis_single_point_and_all_equal():
+    rng = np.random.default_rng(6)
+    flat = rng.random((5000, 3))
+    flat[:, 2] = 0.25
+    check(flat)
+    check(np.array([[1.0, 2.0, 3.0]]))
+    check(np.ones((7000, 3)))  # every code is 0: the identity order
+
+
+@pytest.mark.parametrize("bank", [16, 32])
+def test_bank_padding(bank):
+    rng = np.random.default_rng(7)
+    check(rng.random((4097, 3)), bank)
+
+
+def test_a_million_points_and_reuse():
+    rng = np.random.default_rng(8)
+    dm = native.DeviceMorton()
+    for n in (1_000_000, 3000):  # the second call reuses (and must not read stale) scratch
+        pos = rng.random((n, 3))
+        hp, hr = native.morton_partition(pos, 32)
+        dp, dr = dm.partition(torch.from_numpy(pos).cuda(), 32)
+        torch.cuda.synchronize()
+        assert np.array_equal(hp, dp.cpu().numpy()) and np.array_equal(hr, dr.cpu().numpy())
+
+
+def test_rejects_bad_arguments():
+    dm = native.DeviceMorton()
+    with pytest.raises(ValueError):
+        dm.partition(torch.zeros((4, 2), dtype=torch.float64, device="cuda"))
+    with pytest.raises(RuntimeError, match="n >= 1"):
+        dm.partition(torch.zeros((0, 3), dtype=torch.float64, device="cuda"))
