@@ -93,7 +93,9 @@ int masx_add_system(masx_solver* s, const masx_system_desc* desc, const int* row
 int masx_set_topology(masx_solver* s, int sys, const uint32_t* nbr_start, const uint32_t* nbr_num,
                       const uint32_t* nbr_list, int list_len, const int* part_to_real, int part_map_size,
                       const int* real_to_part);
-/* Same, with the partition computed from vertex positions by Morton order. */
+/* Same, with the partition computed from vertex positions (host array) by
+ * Morton order; meshes of 4,096 vertices or more are sorted on the device
+ * when one is present (masx_morton_partition_device: identical arrays). */
 int masx_set_topology_morton(masx_solver* s, int sys, const uint32_t* nbr_start, const uint32_t* nbr_num,
                              const uint32_t* nbr_list, int list_len, const double* positions);
 /* The Morton partition masx_set_topology_morton uses (gl_main.cu:1176-1243),

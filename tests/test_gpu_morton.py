@@ -85,3 +85,32 @@ def test_rejects_bad_arguments():
         dm.partition(torch.zeros((4, 2), dtype=torch.float64, device="cuda"))
     with pytest.raises(RuntimeError, match="n >= 1"):
         dm.partition(torch.zeros((0, 3), dtype=torch.float64, device="cuda"))
+
+
+def test_solver_topology_from_positions_uses_the_same_partition():
+    # masx_set_topology_morton sorts meshes of >= 4,096 vertices on the device:
+    # the solve must equal the one set up with the host partition, bit for bit
+    w = W.fem_grid(20, young_sigma=0.5, n_contacts=500, seed=3)
+    start, num, lst = W.neighbors(w["n_vertices"], w["tets"])
+    p2r, r2p = native.morton_partition(w["positions"], 32)
+    b = torch.from_numpy(np.ascontiguousarray(w["rhs"])).cuda()
+    vals = torch.from_numpy(np.ascontiguousarray(w["vals"])).cuda()
+    out = []
+    for by_positions in (True, False):
+        s = native.Solver()
+        s.add_system(block_rows=w["block_rows"], rows=w["rows"], cols=w["cols"], use_mas=True, levels=3)
+        if by_positions:
+            s.set_topology_morton(0, start, num, lst, w["positions"])
+        else:
+            s.set_topology(0, start, num, lst, p2r, r2p)
+        s.set_contacts(0, w["pairs"])
+        s.setup()
+        s.set_values(0, vals)
+        s.build("fp32")
+        x = torch.zeros_like(b)
+        st = s.solve(b, x, rule=1, tol=1e-4)[0]
+        torch.cuda.synchronize()
+        out.append((x.cpu().numpy(), st["iterations"]))
+        s.close()
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][0], out[1][0])
