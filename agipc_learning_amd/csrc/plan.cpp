@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <memory>
 #include <numeric>
 #include <stdexcept>
 #include <unordered_map>
@@ -27,12 +28,14 @@ template <class Emit, class Place>
 std::vector<long long> bucket_scatter(long long n, int nbuckets, Emit emit, Place place)
 {
     const int T = std::max(1, std::min(omp_get_max_threads(), (int)((n + 65535) / 65536)));
-    std::vector<long long> cnt((size_t)T * (nbuckets + 1), 0);
+    // per-thread counters, zeroed (first touched) by the thread that uses them
+    std::unique_ptr<long long[]> cnt(new long long[(size_t)T * (nbuckets + 1)]);
     const long long chunk = (n + T - 1) / T;
 #pragma omp parallel for num_threads(T) schedule(static, 1)
     for(int t = 0; t < T; ++t)
     {
-        long long* c = cnt.data() + (size_t)t * (nbuckets + 1);
+        long long* c = cnt.get() + (size_t)t * (nbuckets + 1);
+        std::fill(c, c + nbuckets + 1, 0LL);
         for(long long k = t * chunk; k < std::min(n, (t + 1) * chunk); ++k) emit(k, [&](int b, int) { ++c[b]; });
     }
     // bucket totals (parallel over buckets), their scan, then each thread's
@@ -60,7 +63,7 @@ std::vector<long long> bucket_scatter(long long n, int nbuckets, Emit emit, Plac
 #pragma omp parallel for num_threads(T) schedule(static, 1)
     for(int t = 0; t < T; ++t)
     {
-        long long* c = cnt.data() + (size_t)t * (nbuckets + 1);
+        long long* c = cnt.get() + (size_t)t * (nbuckets + 1);
         for(long long k = t * chunk; k < std::min(n, (t + 1) * chunk); ++k)
             emit(k, [&](int b, int slot) { place(k, slot, c[b]++); });
     }
@@ -382,31 +385,30 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
         // row R as (C, g) and, off the diagonal, in row C as (R, g | T)
         std::vector<long long> item_base(S + 1, 0);
         for(int s = 0; s < S; ++s) item_base[s + 1] = item_base[s] + sys[s].nb;
-        auto locate = [&](long long k, int& s_, long long& kk) {
-            s_ = (int)(std::upper_bound(item_base.begin(), item_base.end(), k) - item_base.begin()) - 1;
-            kk = k - item_base[s_];
-        };
+        // internal (row, col) of every upper block, once (items are the
+        // global upper block indices: item k is block k of the concatenation)
+        std::unique_ptr<int[]> iR(new int[item_base[S] + 1]), iC(new int[item_base[S] + 1]);
+        for(int s = 0; s < S; ++s)
+        {
+            const auto& d = sys[s];
+            const int* e2i = P.ext_to_int.data() + P.ext_off[s];
+            const long long b = item_base[s];
+#pragma omp parallel for schedule(static)
+            for(long long kk = 0; kk < d.nb; ++kk)
+            {
+                iR[b + kk] = e2i[d.rows[kk]];
+                iC[b + kk] = d.rows[kk] == d.cols[kk] ? -1 : e2i[d.cols[kk]];  // -1: diagonal
+            }
+        }
         auto emit = [&](long long k, auto&& sink) {
-            int s_;
-            long long kk;
-            locate(k, s_, kk);
-            const auto& d = sys[s_];
-            const int R = P.ext_to_int[P.ext_off[s_] + d.rows[kk]];
-            const int C = P.ext_to_int[P.ext_off[s_] + d.cols[kk]];
-            sink(R, 0);
-            if(d.rows[kk] != d.cols[kk]) sink(C, 1);
+            sink(iR[k], 0);
+            if(iC[k] >= 0) sink(iC[k], 1);
         };
         P.col.assign(2 * item_base[S], 0);
         P.entry_src.assign(2 * item_base[S], 0);
         auto ptr = bucket_scatter(item_base[S], P.n_rows, emit, [&](long long k, int slot, long long pos) {
-            int s_;
-            long long kk;
-            locate(k, s_, kk);
-            const auto& d = sys[s_];
-            const int g = (int)(P.src_off[s_] + kk);
-            const int R = P.ext_to_int[P.ext_off[s_] + d.rows[kk]];
-            const int C = P.ext_to_int[P.ext_off[s_] + d.cols[kk]];
-            P.col[pos] = slot == 0 ? C : R;
+            const int g = (int)k;  // == src_off[s] + local index
+            P.col[pos] = slot == 0 ? (iC[k] >= 0 ? iC[k] : iR[k]) : iR[k];
             P.entry_src[pos] = slot == 0 ? g : (g | (int)0x80000000u);
         });
         P.row_ptr.assign(P.n_rows + 1, 0);
@@ -702,7 +704,7 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
     int max_levels = 0;
     for(auto& d : sys)
         if(d.use_mas) max_levels = std::max(max_levels, d.hier.levels);
-    struct Ent { long long key; int src; int mode; };
+    struct Ent { long long key; int srcmode; int pos; };  // srcmode = source << 2 | mode
     // level-(l) entries per system: (I,J) slots, global E index
     std::vector<std::vector<std::pair<int, int>>> prev_ij(S);   // (I,J) of the previous level's entries
     std::vector<std::vector<int>>                 prev_idx(S);  // their global E index
@@ -748,7 +750,7 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
                 else { I = a; J = a; mode = diag ? 0 : 2; }
                 return true;
             };
-            std::vector<Ent> ents(n_src);
+            std::unique_ptr<Ent[]> ents(new Ent[n_src > 0 ? n_src : 1]);  // every slot used is written
             const auto bp = bucket_scatter(
                 n_src, lc,
                 [&](long long k, auto&& sink) {
@@ -758,18 +760,18 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
                 [&](long long k, int, long long pos) {
                     int I, J, src, mode;
                     canon(k, I, J, src, mode);
-                    ents[pos] = {(long long)J, src, mode};
+                    ents[pos] = {(long long)J, (src << 2) | mode, 0};
                 });
-            ents.resize(bp[lc]);
+            const long long n_ents = bp[lc];
             std::vector<int> bptr(lc + 1);
             for(int i = 0; i <= lc; ++i) bptr[i] = (int)bp[i];
             std::vector<int> n_uniq(lc + 1, 0);
 #pragma omp parallel for schedule(dynamic, 64)
             for(int i = 0; i < lc; ++i)
             {
-                Ent* b = ents.data() + bptr[i];
+                Ent* b = ents.get() + bptr[i];
                 const int n = bptr[i + 1] - bptr[i];
-                if(n <= 64)
+                if(n <= 32)
                     for(int x = 1; x < n; ++x)
                     {
                         const Ent v = b[x];
@@ -778,7 +780,14 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
                         b[y] = v;
                     }
                 else
-                    std::stable_sort(b, b + n, [](const Ent& u, const Ent& v) { return u.key < v.key; });
+                {
+                    // (key, input position): the stable order, without the
+                    // temporary buffer a stable sort allocates per bucket
+                    for(int x = 0; x < n; ++x) b[x].pos = x;
+                    std::sort(b, b + n, [](const Ent& u, const Ent& v) {
+                        return u.key != v.key ? u.key < v.key : u.pos < v.pos;
+                    });
+                }
                 int u = 0;
                 for(int x = 0; x < n; ++x)
                     if(x == 0 || b[x].key != b[x - 1].key) ++u;
@@ -791,18 +800,18 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
             E_sys.resize(e_base + n_new, s);
             prev_ij[s].assign(n_new, {0, 0});
             prev_idx[s].assign(n_new, 0);
-            P.gal_src.resize(g_base + ents.size());
+            P.gal_src.resize(g_base + n_ents);
             const size_t gp_base = P.gal_ptr.size();
             P.gal_ptr.resize(gp_base + n_new);
 #pragma omp parallel for schedule(dynamic, 64)
             for(int i = 0; i < lc; ++i)
             {
-                const Ent* b = ents.data() + bptr[i];
+                const Ent* b = ents.get() + bptr[i];
                 const int n = bptr[i + 1] - bptr[i];
                 int q = n_uniq[i];
                 for(int x = 0; x < n; ++x)
                 {
-                    P.gal_src[g_base + bptr[i] + x] = (b[x].src << 2) | b[x].mode;
+                    P.gal_src[g_base + bptr[i] + x] = b[x].srcmode;
                     if(x + 1 == n || b[x + 1].key != b[x].key)
                     {
                         const int I = lb + i, J = (int)b[x].key;
