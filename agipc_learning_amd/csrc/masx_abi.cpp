@@ -164,6 +164,19 @@ int masx_add_system(masx_solver* s, const masx_system_desc* desc, const int* row
     });
 }
 
+int masx_set_structure(masx_solver* s, int sys, long long nb, const int* rows, const int* cols)
+{
+    return guard([&] {
+        require(s && sys >= 0 && sys < s->impl.n_systems(), "bad system index");
+        require(nb >= 0 && (nb == 0 || (rows && cols)), "null rows/cols");
+        auto& d = s->impl.system(sys);
+        d.nb = nb;
+        d.rows.assign(rows, rows + nb);
+        d.cols.assign(cols, cols + nb);
+        s->impl.structure_changed(sys);
+    });
+}
+
 int masx_set_topology(masx_solver* s, int sys, const uint32_t* nbr_start, const uint32_t* nbr_num,
                       const uint32_t* nbr_list, int list_len, const int* part_to_real, int part_map_size,
                       const int* real_to_part)

@@ -25,10 +25,15 @@ namespace masx
 template <typename T>
 void DeviceArray<T>::alloc(size_t count)
 {
-    release();
     if(count == 0) count = 1;  // keep a valid pointer for empty arrays
+    if(ptr && count <= cap)    // a new structure of no larger size reuses the buffer
+    {
+        n = count;
+        return;
+    }
+    release();
     MASX_HIP(hipMalloc((void**)&ptr, count * sizeof(T)));
-    n = count;
+    n = cap = count;
 }
 
 template <typename T>

@@ -19,6 +19,7 @@ struct DeviceArray
 {
     T*     ptr = nullptr;
     size_t n   = 0;
+    size_t cap = 0;   // allocated elements (a smaller alloc() reuses the buffer)
     DeviceArray() = default;
     DeviceArray(const DeviceArray&) = delete;
     DeviceArray& operator=(const DeviceArray&) = delete;
@@ -28,6 +29,7 @@ struct DeviceArray
         if(ptr) (void)hipFree(ptr);
         ptr = nullptr;
         n   = 0;
+        cap = 0;
     }
     void alloc(size_t count);
     void upload(const std::vector<T>& h, hipStream_t st);

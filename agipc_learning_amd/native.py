@@ -84,6 +84,7 @@ def load():
         "masx_create": ([C.POINTER(vp)], C.c_int),
         "masx_destroy": ([vp], C.c_int),
         "masx_add_system": ([vp, C.POINTER(SystemDesc), ip, ip, ip], C.c_int),
+        "masx_set_structure": ([vp, C.c_int, C.c_longlong, ip, ip], C.c_int),
         "masx_set_topology": ([vp, C.c_int, up, up, up, C.c_int, ip, C.c_int, ip], C.c_int),
         "masx_set_topology_morton": ([vp, C.c_int, up, up, up, C.c_int, dp], C.c_int),
         "masx_set_contacts": ([vp, C.c_int, ip, C.c_int, C.c_int], C.c_int),
@@ -295,6 +296,15 @@ class Solver:
         check(self.L.masx_add_system(self.h, C.byref(d), _ptr(rows, C.c_int), _ptr(cols, C.c_int), C.byref(idx)))
         self.systems.append(dict(block_rows=block_rows, nb=len(rows), n_abd=n_abd, n_fem=n_fem, bank=bank))
         return idx.value
+
+    def set_structure(self, sys, rows, cols):
+        """Replace system `sys`'s upper block structure (re-analysed at the next setup)."""
+        r, c = _i32(rows), _i32(cols)
+        if len(r) != len(c):
+            raise ValueError("rows and cols differ in length")
+        check(self.L.masx_set_structure(self.h, sys, len(r), _ptr(r, C.c_int) if len(r) else None,
+                                        _ptr(c, C.c_int) if len(c) else None))
+        self.systems[sys]["nb"] = len(r)
 
     def set_topology(self, sys, nbr_start, nbr_num, nbr_list, part_to_real, real_to_part):
         s, n, l = _u32(nbr_start), _u32(nbr_num), _u32(nbr_list if len(nbr_list) else np.zeros(1))

@@ -87,6 +87,13 @@ int masx_destroy(masx_solver* s);
  * system index in *sys. */
 int masx_add_system(masx_solver* s, const masx_system_desc* desc, const int* rows, const int* cols, int* sys);
 
+/* Replace system `sys`'s upper block structure (nb blocks, host rows/cols as
+ * in masx_add_system), e.g. when a Newton step brings a new contact set: the
+ * structure is analysed again at the next masx_setup, on the same solver, so
+ * its host and device buffers are reused.  No counterpart in the reference,
+ * whose BCOO is rebuilt each step by Converter::convert
+ * (converter.cu:137-284) before MASPreconditioner's set-up. */
+int masx_set_structure(masx_solver* s, int sys, long long nb, const int* rows, const int* cols);
 /* MAS topology: neighbor CSR (load_mesh.cpp:934-1040) and the 32-slot vertex
  * partition (gl_main.cu:1201-1243), host arrays, as uploaded by
  * gl_main.cu:1502-1536 before initPreconditioner_Matrix. */

@@ -109,6 +109,39 @@ def test_new_contact_set_on_the_same_solver_is_reanalyzed(oracle):
         assert st["failure"] == 0 and st["rel_res_true"] <= 1e-4 * (1 + 1e-6)
 
 
+def test_set_structure_on_the_same_solver_equals_a_fresh_solver():
+    # a Newton step with a new contact set changes the upper structure: the
+    # same solver takes it (masx_set_structure + contacts + setup, buffers
+    # reused) and solves exactly as a solver built for it from scratch; then
+    # back to the first structure (smaller), reusing the larger buffers
+    w1 = W.fem_grid(10, n_contacts=100, seed=11, young_sigma=0.5)
+    w2 = W.fem_grid(10, n_contacts=400, seed=12, young_sigma=0.5)
+    assert len(w1["rows"]) != len(w2["rows"])
+
+    def solve_with(s, w):
+        s.set_values(0, dev(w["vals"]))
+        s.build("fp32")
+        b = session.rhs_of(w)
+        x = torch.zeros(len(b), dtype=torch.float64, device="cuda")
+        st = s.solve(dev(b), x, rule=1, tol=1e-4)[0]
+        torch.cuda.synchronize()
+        return x.cpu().numpy(), st["iterations"]
+
+    s = session.solver_for(w1)
+    first = solve_with(s, w1)
+    for w in (w2, w1):
+        s.set_structure(0, w["rows"], w["cols"])
+        s.set_contacts(0, w["pairs"])
+        s.setup()
+        got = solve_with(s, w)
+        fresh = session.solver_for(w)
+        ref = solve_with(fresh, w)
+        fresh.close()
+        assert got[1] == ref[1]
+        assert np.array_equal(got[0], ref[0])
+    assert np.array_equal(got[0], first[0])
+
+
 # ------------------------------------------------------------------- MAS ----
 
 @pytest.mark.parametrize("n,contacts", [(8, 0), (14, 400)])
