@@ -893,8 +893,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
 // corrections they need) are loaded while the r.r partials are reduced.
 __global__ void __launch_bounds__(256) k_precond_finish(DevData d, int force)
 {
+    // rows by XCD as in the SpMV (each XCD a contiguous run of workgroups)
+    const int wgi = xcd_tile(blockIdx.x, gridDim.x);
     __shared__ double smem[8];
-    const int4 w = d.wg_pre[blockIdx.x];
+    const int4 w = d.wg_pre[wgi];
     const int  s = d.single ? 0 : w.x;
     SysState&  st = d.state[s];
     if(!force && st.status != kRunning) return;
@@ -978,7 +980,7 @@ __global__ void __launch_bounds__(256) k_precond_finish(DevData d, int force)
                 else if(st.check_every > 0 && st.k - st.k_check >= st.check_every) chk = 1;
             }
         }
-        if((int)blockIdx.x == wg_range(d, d.wg_pre_range, d.n_wg_pre, s).x && threadIdx.x == 0)
+        if((int)wgi == wg_range(d, d.wg_pre_range, d.n_wg_pre, s).x && threadIdx.x == 0)
         {
             st.rr = rr;
             if(ns != kRunning) st.status = ns;
@@ -1020,7 +1022,7 @@ __global__ void __launch_bounds__(256) k_precond_finish(DevData d, int force)
     }
     if(force) return;
     part = block_sum_last(part, smem);
-    if(threadIdx.x == 0) d.part_pre[blockIdx.x] = part;
+    if(threadIdx.x == 0) d.part_pre[wgi] = part;
 }
 
 // beta from the r.z partials, then Ap = A p_new with p_new = z + beta p_old
@@ -1331,8 +1333,10 @@ __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __re
 // component lists are loaded while the p.Ap partials are reduced.
 __global__ void __launch_bounds__(256) k_update(DevData d, const float* __restrict__ p, int mode)
 {
+    // rows by XCD as in the SpMV (each XCD a contiguous run of workgroups)
+    const int wgi = xcd_tile(blockIdx.x, gridDim.x);
     __shared__ double smem[8];
-    const int4 w = d.wg_upd[blockIdx.x];
+    const int4 w = d.wg_upd[wgi];
     const int  s = d.single ? 0 : w.x;
     SysState&  st = d.state[s];
     bool init = false;
@@ -1344,7 +1348,7 @@ __global__ void __launch_bounds__(256) k_update(DevData d, const float* __restri
             // done if it meets the tolerance, else left to the chunk's check
             // (need_check 2: no second in-iteration attempt)
             if(st.status == kCandidate && st.need_check == 1 &&
-               (int)blockIdx.x == wg_range(d, d.wg_upd_range, d.n_wg_upd, s).x)
+               (int)wgi == wg_range(d, d.wg_upd_range, d.n_wg_upd, s).x)
             {
                 const double tr = sys_sum(d.part_tr, wg_range(d, d.wg_spmv_range, d.n_wg_spmv, s), smem);
                 if(threadIdx.x == 0)
@@ -1370,7 +1374,7 @@ __global__ void __launch_bounds__(256) k_update(DevData d, const float* __restri
         init = st.status == kInit;
     }
     // read before this kernel's first store, so it stays a scalar load
-    const int4 grp = d.wg_upd_grp[blockIdx.x];  // uniform over the workgroup
+    const int4 grp = d.wg_upd_grp[wgi];  // uniform over the workgroup
     const double share = mode == 0 ? partial_share(d.part_spmv, wg_range(d, d.wg_spmv_range, d.n_wg_spmv, s)) : 0.0;
     const int type = w.y >> 28;
     const int aux  = w.y & 0x0fffffff;
@@ -1425,7 +1429,7 @@ __global__ void __launch_bounds__(256) k_update(DevData d, const float* __restri
     {
         const double pAp = block_sum(share, smem);
         const bool bad = !finite_d(pAp) || (st.rule == kRuleTrueRel && pAp <= 0.0);
-        if((int)blockIdx.x == wg_range(d, d.wg_upd_range, d.n_wg_upd, s).x && threadIdx.x == 0)
+        if((int)wgi == wg_range(d, d.wg_upd_range, d.n_wg_upd, s).x && threadIdx.x == 0)
         {
             if(bad && !st.frozen)
             {
@@ -1526,7 +1530,7 @@ __global__ void __launch_bounds__(256) k_update(DevData d, const float* __restri
     {
         double t = 0.0;
         for(int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) t += smem[w];
-        d.part_upd[blockIdx.x] = t;
+        d.part_upd[wgi] = t;
     }
 }
 

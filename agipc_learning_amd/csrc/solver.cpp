@@ -779,7 +779,7 @@ double Solver::time_kernel(int which, int reps)
             default: throw std::invalid_argument("unknown kernel id");
         }
     };
-    launch(nullptr);
+    for(int i = 0; i < std::min(reps, 20); ++i) launch(nullptr);  // untimed warm-up
     MASX_HIP(hipStreamSynchronize(st));
     // each launch carries events stamped by its own dispatch, so the average
     // is of the kernel's execution time (what a kernel trace reports), not of
