@@ -76,7 +76,7 @@ def _busy_median(durations):
 def compare(bench_json, trace_csv):
     line = [l for l in open(bench_json).read().splitlines() if l.strip().startswith("{")][-1]
     bench = json.loads(line)
-    if "batch64" in bench:  # bench.py --batch-only: FP32 kernels, SpMV tiling 4
+    if bench.get("roofline") is None and bench.get("batch64"):  # bench.py --batch-only: FP32, SpMV tiling 4
         us = bench["batch64"]["fp32"]["kernels_us"]
         tiling = 4
     else:
