@@ -259,220 +259,220 @@ __global__ void __launch_bounds__(256) k_gal_gather(const double* fine, double* 
         for(int i = 0; i < 9; ++i) E[9 * (long long)e + i] = acc[i];
 }
 
-// Dense local assembly and FP64 inversion, written diagonal-major in the
-// storage precision T.  One workgroup per bank.  The symmetric matrix is
-// assembled in LDS as a packed lower triangle, then inverted in place by
-// Gauss-Jordan elimination without pivoting on register tiles: the DIM x DIM
-// matrix is spread over a TY x 16 thread grid, thread (ty, tx) holding rows
-// ty + TY r and columns tx + 16 c (TY = 16).  Step k eliminates with pivot
-// k; its pivot row and column pass through LDS (double-buffered, one barrier
-// per step), the pivot row already divided by the pivot and with the
-// reciprocal beside it: the thread holding the pivot computes it once.  A pivot -- the Schur-complement diagonal -- below
-// max(1e-12, 1e-12 max|diag|) is raised to that floor, which inverts the same
-// modified matrix as the pivot-floored Cholesky of
-// pro/.../TraditionalMAS32Preconditioner.cu:1382-1439.  Half-precision storage
-// is scaled by the largest diagonal entry of the inverse (for an SPD inverse,
-// the largest entry in magnitude).
-// Assembles bank `bank` as a packed lower triangle L in LDS (zero diagonal
-// -> 1) and leaves the pivot floor max(1e-12, 1e-12 max|diag|) in red[0].
-template <int BANK>
-__device__ void mas_assemble(const DevData& d, const double* E, const int* ent_ptr, const int* ent_pos,
-                             const int* ent_src, double* L, double* red, int bank)
+// First row of the second assembly pass: the lower triangle's rows [0, P)
+// and [P, DIM) hold about half its elements each.
+__host__ __device__ constexpr int factor_split_row(int dim)
 {
-    constexpr int DIM = 3 * BANK;
-    constexpr int NL  = DIM * (DIM + 1) / 2;
-    const int tid = threadIdx.x;
-    auto at = [](int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; };
-    for(int i = tid; i < NL; i += blockDim.x) L[i] = 0.0;
-    __syncthreads();
-    const bool fine = bank < d.n_fine_banks;
-    for(int k = ent_ptr[bank] + tid; k < ent_ptr[bank + 1]; k += blockDim.x)
-    {
-        const int pos = ent_pos[k] & 0xffff, tr = ent_pos[k] >> 16;
-        const int src = ent_src[k];
-        int I = 0;
-        while(I + 1 < BANK && packed_index(BANK, I + 1, I + 1) <= pos) ++I;
-        const int J = pos - packed_index(BANK, I, I) + I;
-        for(int a = 0; a < 3; ++a)
-            for(int b = 0; b < 3; ++b)
-            {
-                double v;
-                if(fine)
-                    v = tr ? d.vals64[9 * (long long)src + b + 3 * a] : d.vals64[9 * (long long)src + a + 3 * b];
-                else
-                    v = E[9 * (long long)src + 3 * a + b];
-                const int p = 3 * I + a, q = 3 * J + b;
-                if(I == J && p < q) continue;  // diagonal block: its lower half carries it
-                atomicAdd(&L[at(p, q)], v);
-            }
-    }
-    __syncthreads();
-    // zero diagonal -> 1 (padding slots behave as identity)
-    if(tid < DIM && L[at(tid, tid)] == 0.0) L[at(tid, tid)] = 1.0;
-    __syncthreads();
-    if(tid < 64)
-    {
-        double m = 0.0;
-        for(int i = tid; i < DIM; i += 64) m = fmax(m, fabs(L[at(i, i)]));
-#pragma unroll
-        for(int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
-        if(tid == 0) red[0] = fmax(1.0e-12, 1.0e-12 * m);
-    }
+    int p = 0;
+    while(2 * ((p + 1) * (p + 2) / 2) <= dim * (dim + 1) / 2) ++p;
+    return p;
 }
 
-// Writes the inverse (packed lower triangle L) of bank `bank` diagonal-major
-// in T, with the per-bank scale of the half-precision formats.
+// One element of the inverse into the diagonal-major output of bank `bank`:
+// (i, j), i >= j, is the upper element (j, i) of block (j / 3, i / 3); a
+// diagonal block also takes its transpose position.  Half formats store
+// 16-bit halves of 5 words per block (the tenth half is zero).
 template <int BANK, typename T>
-__device__ void mas_store_inverse(const double* L, void* out, float* scale_out, double* red, int bank)
+__device__ __forceinline__ void store_inverse_element(void* out, int bank, int i, int j, double v, double inv_scale)
 {
     constexpr int NBLK = BANK * (BANK + 1) / 2;
-    constexpr int DIM  = 3 * BANK;
-    constexpr bool kHalf = !std::is_same<T, float>::value;
-    const int tid = threadIdx.x;
-    auto at = [](int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; };
-    double scale = 1.0;
-    if(kHalf)
+    const int I = j / 3, J = i / 3, dd = J - I, len = BANK - dd;
+    const int c0 = 3 * (j % 3) + i % 3, c1 = 3 * (i % 3) + j % 3;
+    const int nc = (I == J && i != j) ? 2 : 1;
+    for(int u = 0; u < nc; ++u)
     {
-        double m = 0.0;
-        for(int p = tid; p < DIM; p += blockDim.x) m = fmax(m, L[at(p, p)]);
-#pragma unroll
-        for(int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
-        if((tid & 63) == 0) red[1 + (tid >> 6)] = m;
-        __syncthreads();
-        m = 0.0;
-        for(int w = 0; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, red[1 + w]);
-        scale = m > 0.0 ? m : 1.0;
-    }
-    if(tid == 0) scale_out[bank] = (float)scale;
-    const double inv_scale = 1.0 / scale;
-    if constexpr(!kHalf)
-    {
-        float* dst = reinterpret_cast<float*>(out) + (size_t)bank * NBLK * 9;
-        for(int o = tid; o < NBLK * 9; o += blockDim.x)
+        const int c = u ? c1 : c0;
+        if constexpr(std::is_same<T, float>::value)
+            reinterpret_cast<float*>(out)[(size_t)bank * NBLK * 9 + 9 * diag_off(BANK, dd) + c * len + I] = (float)v;
+        else
         {
-            int dd = 0;
-            while(dd + 1 < BANK && 9 * diag_off(BANK, dd + 1) <= o) ++dd;
-            const int len = BANK - dd, rem = o - 9 * diag_off(BANK, dd);
-            const int c = rem / len, I = rem % len;
-            dst[o] = (float)L[at(3 * I + c / 3, 3 * (I + dd) + c % 3)];
-        }
-    }
-    else
-    {
-        uint32_t* dst = reinterpret_cast<uint32_t*>(out) + (size_t)bank * NBLK * 5;
-        for(int o = tid; o < NBLK * 5; o += blockDim.x)
-        {
-            int dd = 0;
-            while(dd + 1 < BANK && 5 * diag_off(BANK, dd + 1) <= o) ++dd;
-            const int len = BANK - dd, rem = o - 5 * diag_off(BANK, dd);
-            const int q = rem / len, I = rem % len;
-            const int c0 = 2 * q, c1 = 2 * q + 1;
-            const uint32_t lo = half_bits(L[at(3 * I + c0 / 3, 3 * (I + dd) + c0 % 3)] * inv_scale, (T*)nullptr);
-            const uint32_t hi =
-                c1 < 9 ? half_bits(L[at(3 * I + c1 / 3, 3 * (I + dd) + c1 % 3)] * inv_scale, (T*)nullptr) : 0u;
-            dst[o] = lo | (hi << 16);
+            const size_t w = (size_t)bank * NBLK * 5 + 5 * diag_off(BANK, dd) + (c >> 1) * len + I;
+            const uint32_t bits = half_bits(v * inv_scale, (T*)nullptr);
+            if(c == 8)
+                reinterpret_cast<uint32_t*>(out)[w] = bits;  // its word's upper half is the unused tenth value
+            else
+                reinterpret_cast<unsigned short*>(out)[2 * w + (c & 1)] = (unsigned short)bits;
         }
     }
 }
 
-template <int BANK, typename T, int TY>
-__global__ void __launch_bounds__(TY * 16) k_mas_factor(DevData d, const double* E, const int* ent_ptr,
+// Dense local assembly and FP64 inversion of one bank per workgroup, written
+// diagonal-major in the storage precision T.  The inversion is the symmetric
+// sweep: pivot k turns a_ij into a_ij - a_ik a_kj / d (i, j != k), a_ik into
+// a_ik / d and a_kk into -1 / d, so the matrix stays symmetric and, after
+// every pivot, holds -A^-1.  A pivot (the Schur-complement diagonal) below
+// max(1e-12, 1e-12 max|diag|) is raised to that floor, which inverts the
+// same modified matrix as the pivot-floored Cholesky of
+// pro/.../TraditionalMAS32Preconditioner.cu:1382-1439.  Each thread of a
+// 16 x 16 grid keeps only the lower-triangle elements of its tiles
+// (ty + 16 r, tx + 16 c), r >= c, in registers (21 FMAs per thread and pivot
+// for 32-slot banks); the pivot's column passes through LDS, double-buffered,
+// one barrier per pivot.  The bank matrix is assembled in two passes through
+// half the LDS the whole lower triangle would take (zero diagonal -> 1) and
+// the inverse is written from the registers, so five banks fit on a CU.
+// Half-precision storage is scaled by the largest diagonal entry of the
+// inverse (for an SPD inverse, the largest entry in magnitude).
+template <int BANK, typename T>
+__global__ void __launch_bounds__(256) k_mas_factor_sym(DevData d, const double* E, const int* ent_ptr,
                                                         const int* ent_pos, const int* ent_src, void* out,
                                                         float* scale_out, int* status)
 {
-    constexpr int DIM  = 3 * BANK;
-    constexpr int RR   = DIM / TY;  // rows per thread
-    constexpr int RC   = DIM / 16;  // columns per thread
-    constexpr int NL   = DIM * (DIM + 1) / 2;
-    static_assert(DIM % 16 == 0 && DIM % TY == 0 && TY % 16 == 0, "thread grid must tile the bank matrix");
-    __shared__ double L[NL];
-    __shared__ double rowk[2][DIM], colk[2][DIM];
+    constexpr int DIM = 3 * BANK;
+    constexpr int R   = DIM / 16;
+    constexpr int NT  = R * (R + 1) / 2;
+    constexpr int NL  = DIM * (DIM + 1) / 2;
+    constexpr int P   = factor_split_row(DIM);
+    constexpr int N0  = P * (P + 1) / 2;
+    constexpr int NH  = N0 > NL - N0 ? N0 : NL - N0;
+    constexpr bool kHalf = !std::is_same<T, float>::value;
+    static_assert(DIM % 16 == 0, "the 16 x 16 grid must tile the bank matrix");
+    __shared__ double H[NH];
+    __shared__ double vk[2][DIM];
     __shared__ double pinv[2];
     __shared__ int    pflag[2];
-    __shared__ double red[1 + TY * 16 / 64];
-    const int bank = blockIdx.x;
-    const int tid  = threadIdx.x;
-    const int tx = tid & 15, ty = tid >> 4;
-    auto at = [](int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; };
-    mas_assemble<BANK>(d, E, ent_ptr, ent_pos, ent_src, L, red, bank);
-    double a[RR][RC];
+    __shared__ double red[4];
+    const int bank = blockIdx.x, tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    auto tix = [](int r, int c) { return r * (r + 1) / 2 + c; };  // tile (r, c), r >= c
+    double a[NT];
 #pragma unroll
-    for(int r = 0; r < RR; ++r)
-#pragma unroll
-        for(int c = 0; c < RC; ++c) a[r][c] = L[at(ty + TY * r, tx + 16 * c)];
-    __syncthreads();
-    const double floor_value = red[0];
-    int bad = 0;
-    // pivot k = kt + 16 cc: column cc of the threads with tx == kt, row
-    // rr = 16 cc / TY of the threads with ty == k - TY rr (both static in cc)
-#pragma unroll
-    for(int cc = 0; cc < RC; ++cc)
+    for(int t = 0; t < NT; ++t) a[t] = 0.0;
+    // ---- assembly: rows [0, P), then [P, DIM), each into H
+    const bool fine = bank < d.n_fine_banks;
+    double dmax = 0.0;
+    for(int pass = 0; pass < 2; ++pass)
     {
-        const int rr = 16 * cc / TY;
+        const int lo = pass ? P : 0, hi = pass ? DIM : P, base = pass ? N0 : 0;
+        const int n = pass ? NL - N0 : N0;
+        for(int q = tid; q < n; q += 256) H[q] = 0.0;
+        __syncthreads();
+        for(int k = ent_ptr[bank] + tid; k < ent_ptr[bank + 1]; k += 256)
+        {
+            const int pos = ent_pos[k] & 0xffff, tr = ent_pos[k] >> 16;
+            const int src = ent_src[k];
+            int I = 0;
+            while(I + 1 < BANK && packed_index(BANK, I + 1, I + 1) <= pos) ++I;
+            const int J = pos - packed_index(BANK, I, I) + I;
+            for(int aa = 0; aa < 3; ++aa)
+                for(int bb = 0; bb < 3; ++bb)
+                {
+                    const int p = 3 * I + aa, q = 3 * J + bb;
+                    if(I == J && p < q) continue;  // diagonal block: its lower half carries it
+                    const int row = p > q ? p : q, col = p > q ? q : p;
+                    if(row < lo || row >= hi) continue;
+                    double v;
+                    if(fine)
+                        v = tr ? d.vals64[9 * (long long)src + bb + 3 * aa] : d.vals64[9 * (long long)src + aa + 3 * bb];
+                    else
+                        v = E[9 * (long long)src + 3 * aa + bb];
+                    atomicAdd(&H[row * (row + 1) / 2 + col - base], v);
+                }
+        }
+        __syncthreads();
+        // zero diagonal -> 1 (padding slots behave as identity)
+        if(tid < hi - lo)
+        {
+            const int q = (lo + tid) * (lo + tid + 1) / 2 + lo + tid - base;
+            if(H[q] == 0.0) H[q] = 1.0;
+            dmax = fmax(dmax, fabs(H[q]));
+        }
+        __syncthreads();
+#pragma unroll
+        for(int r = 0; r < R; ++r)
+#pragma unroll
+            for(int c = 0; c <= r; ++c)
+            {
+                const int i = ty + 16 * r, j = tx + 16 * c;
+                if(i >= lo && i < hi && i >= j) a[tix(r, c)] = H[i * (i + 1) / 2 + j - base];
+            }
+        __syncthreads();
+    }
+    // pivot floor max(1e-12, 1e-12 max |diag|)
+#pragma unroll
+    for(int o = 32; o > 0; o >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, o, 64));
+    if((tid & 63) == 0) red[tid >> 6] = dmax;
+    __syncthreads();
+    const double floor_value = fmax(1.0e-12, 1.0e-12 * fmax(fmax(red[0], red[1]), fmax(red[2], red[3])));
+    int bad = 0;
+    // ---- sweep: pivot k = kt + 16 cc
+#pragma unroll
+    for(int cc = 0; cc < R; ++cc)
+    {
 #pragma unroll 1
         for(int kt = 0; kt < 16; ++kt)
         {
             const int k = kt + 16 * cc, buf = k & 1;
-            const int ty_k = k - TY * rr;
+            // column k from row k down (lower elements (i, k), i >= k) ...
             if(tx == kt)
             {
 #pragma unroll
-                for(int r = 0; r < RR; ++r) colk[buf][ty + TY * r] = a[r][cc];
+                for(int r = cc; r < R; ++r)
+                    if(r > cc || ty >= kt) vk[buf][ty + 16 * r] = a[tix(r, cc)];
             }
-            if(ty == ty_k)
+            // ... and left of the pivot (lower elements (k, j), j < k); the pivot
+            if(ty == kt)
             {
-                // the pivot sits in this row at column tx == kt; every lane
-                // of the row reads it from its own registers after a shuffle
-                const double pv = __shfl(a[rr][cc], (int)(threadIdx.x & ~15u & 63u) + kt, 64);
-                const bool floored = !(pv >= floor_value) || isinf(pv);  // also catches NaN
-                const double ipv = 1.0 / (floored ? floor_value : pv);
 #pragma unroll
-                for(int c = 0; c < RC; ++c) rowk[buf][tx + 16 * c] = a[rr][c] * ipv;
+                for(int c = 0; c <= cc; ++c)
+                    if(c < cc || tx < kt) vk[buf][tx + 16 * c] = a[tix(cc, c)];
                 if(tx == kt)
                 {
-                    pinv[buf] = ipv;
+                    const double pv = a[tix(cc, cc)];
+                    const bool floored = !(pv >= floor_value) || isinf(pv);  // also catches NaN
+                    pinv[buf]  = 1.0 / (floored ? floor_value : pv);
                     pflag[buf] = floored ? 1 : 0;
                 }
             }
             __syncthreads();
             const double ip = pinv[buf];
             bad += pflag[buf];
-            double rk[RC], f[RR];
+            double f[R], g[R];
 #pragma unroll
-            for(int c = 0; c < RC; ++c) rk[c] = rowk[buf][tx + 16 * c];
+            for(int r = 0; r < R; ++r) f[r] = vk[buf][ty + 16 * r];
 #pragma unroll
-            for(int r = 0; r < RR; ++r) f[r] = colk[buf][ty + TY * r];
-            // rank-1 update of every element, then the pivot column
-            // (a_ik := -a_ik / p), the pivot row (a_kj := a_kj / p) and the
-            // pivot itself (1 / p) are overwritten by the threads holding them
+            for(int c = 0; c < R; ++c) g[c] = vk[buf][tx + 16 * c] * ip;
 #pragma unroll
-            for(int r = 0; r < RR; ++r)
+            for(int r = 0; r < R; ++r)
 #pragma unroll
-                for(int c = 0; c < RC; ++c) a[r][c] -= f[r] * rk[c];
+                for(int c = 0; c <= r; ++c) a[tix(r, c)] -= f[r] * g[c];
+            // column k below the pivot and row k left of it take v / d, the pivot -1 / d
             if(tx == kt)
             {
 #pragma unroll
-                for(int r = 0; r < RR; ++r) a[r][cc] = -(f[r] * ip);
+                for(int r = cc; r < R; ++r)
+                    if(r > cc || ty > kt) a[tix(r, cc)] = f[r] * ip;
             }
-            if(ty == ty_k)
+            if(ty == kt)
             {
 #pragma unroll
-                for(int c = 0; c < RC; ++c) a[rr][c] = rk[c];
-                if(tx == kt) a[rr][cc] = ip;
+                for(int c = 0; c <= cc; ++c)
+                    if(c < cc || tx < kt) a[tix(cc, c)] = g[c];
+                if(tx == kt) a[tix(cc, cc)] = -ip;
             }
         }
     }
-    // the inverse is symmetric: keep its lower triangle in LDS
+    // ---- the inverse (-a) from the registers
+    double scale = 1.0;
+    if constexpr(kHalf)
+    {
+        double m = 0.0;
+        if(ty == tx)
 #pragma unroll
-    for(int r = 0; r < RR; ++r)
+            for(int r = 0; r < R; ++r) m = fmax(m, -a[tix(r, r)]);
 #pragma unroll
-        for(int c = 0; c < RC; ++c)
-        {
-            const int i = ty + TY * r, j = tx + 16 * c;
-            if(i >= j) L[i * (i + 1) / 2 + j] = a[r][c];
-        }
-    __syncthreads();
-    mas_store_inverse<BANK, T>(L, out, scale_out, red, bank);
+        for(int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+        __syncthreads();  // red's floor reads are done
+        if((tid & 63) == 0) red[tid >> 6] = m;
+        __syncthreads();
+        m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        scale = m > 0.0 ? m : 1.0;
+    }
+    if(tid == 0) scale_out[bank] = (float)scale;
+    const double inv_scale = 1.0 / scale;
+#pragma unroll
+    for(int r = 0; r < R; ++r)
+#pragma unroll
+        for(int c = 0; c <= r; ++c)
+            if(r > c || ty >= tx) store_inverse_element<BANK, T>(out, bank, ty + 16 * r, tx + 16 * c, -a[tix(r, c)], inv_scale);
     if(tid == 0 && bad && status) atomicAdd(status, bad);
 }
 
@@ -1706,9 +1706,9 @@ hipError_t launch_mas_factor(const DevData& d, const double* E, const int* ent_p
 {
     if(d.n_banks <= 0) return hipSuccess;
     const dim3 g(d.n_banks);
-#define MASX_FACTOR(BK, T)                                                                                     \
-    hipLaunchKernelGGL((k_mas_factor<BK, T, 16>), g, dim3(256), 0, st, d, E, ent_ptr, ent_pos, ent_src, out, \
-                       scale, status)
+#define MASX_FACTOR(BK, T)                                                                                       \
+    hipLaunchKernelGGL((k_mas_factor_sym<BK, T>), g, dim3(256), 0, st, d, E, ent_ptr, ent_pos, ent_src, out, scale, \
+                       status)
     if(d.bank == 32)
     {
         if(precision == kPrecFp16) MASX_FACTOR(32, __half);
