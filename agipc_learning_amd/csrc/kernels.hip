@@ -1025,15 +1025,6 @@ __global__ void __launch_bounds__(256) k_precond_finish(DevData d, int force)
     if(threadIdx.x == 0) d.part_pre[wgi] = part;
 }
 
-// beta from the r.z partials, then Ap = A p_new with p_new = z + beta p_old
-// formed on the fly (the p update is fused here); partial p.Ap.
-//
-// Row-aligned tiles: a workgroup owns whole rows [r0, r1) and their entries
-// [e0, e1) of the full block CSR.  Thread t takes entries e0 + t + 256 j
-// (j < U), so the column indices, the nine value planes and the z / old-p
-// gathers of a chunk are contiguous and all in flight before the r.z
-// partials are reduced.  Each entry's 3-vector product is staged in LDS and
-// thread t then sums row r0 + t's products in entry order.
 // FP64 true residual r_true = b - A x of rows [r_begin, r_end) (one thread
 // per row, rows strided by the workgroup), from the FP64 input blocks; writes
 // r_true and returns this thread's share of |r_true|^2.
