@@ -130,6 +130,29 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": 5}))
 
 
+def copy_ceiling():
+    """Practical HBM rate of this card: a plain 1 GiB device-to-device copy
+    (read + write bytes over its duration), fastest of 10, in GB/s."""
+    import torch
+
+    n = 1 << 30
+    a = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    best = float("inf")
+    for _ in range(10):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1) * 1e-3)
+    del a, b
+    torch.cuda.empty_cache()
+    return round(2 * n / best / 1e9, 1)
+
+
 def cpu_baseline(w, levels, tol):
     """The FP64 oracle (a scalar C restatement of the reference algorithm, one
     thread) on one step of the same workload: MAS assemble + Cholesky inverse +
@@ -611,6 +634,14 @@ def main():
         del s, vals, b, x
         torch.cuda.empty_cache()
         batch = batch64(args)
+
+    # the copy rate puts the fractions of the nominal 8 TB/s in context
+    ceiling = copy_ceiling()
+    log(f"  device copy: {ceiling:.1f} GB/s")
+    roofline["copy_GBs"] = ceiling
+    if batch is not None and batch.get("roofline"):
+        batch["roofline"]["copy_GBs"] = ceiling
+        batch["roofline"]["frac_of_copy"] = round(batch["roofline"]["achieved"] / ceiling, 4)
 
     iters = plain["st"]["iterations"]
     out = {
