@@ -705,7 +705,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
     constexpr int ND    = BANK / 4;        // block diagonals per wave
     constexpr int TPN   = 256 / BANK;      // threads per node in the descendant sums
     constexpr int CHUNK = 1024;            // descendant entries staged per pass
-    __shared__ float sR[64 * 3];
+    __shared__ float sR[64 * 3 + 3];  // + a zero row for the lanes past a diagonal's end
     __shared__ float sZ[3][64 * 3];
     __shared__ float sD[CHUNK * 3];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -755,6 +755,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
             v = (pick(s_kind, k) == 0 ? d.r : d.Rc)[3 * (long long)pick(s_vec, k) + i % (BANK * 3)];
         sR[i] = v;
     }
+    if(tid < 3) sR[64 * 3 + tid] = 0.f;
     // level >= 2: R(node) = sum of the partials the update kernel formed for
     // it, one per 256-row group holding some of its level-1 descendants.  The
     // lists of a bank's consecutive nodes are contiguous in deep_list; they
@@ -830,24 +831,29 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
     for(int kk = 0; kk < ND; ++kk)
     {
         const int dd  = wave + 4 * kk;
+        // a lane past the diagonal's end (or of a finished system) holds a
+        // finite in-bounds block and multiplies the zero row, adding exact
+        // zeros; its transposed product would go to a lane past the bank's
+        // end, which the shuffle never reads.  Without a select on the
+        // values, each conversion folds into the FMAs that use it.
         const bool act = live && I < BANK - dd;
         float v[9];
         if constexpr(WORDS == 9)
         {
 #pragma unroll
-            for(int c = 0; c < 9; ++c) v[c] = act ? __uint_as_float(raw[kk][c]) : 0.f;
+            for(int c = 0; c < 9; ++c) v[c] = __uint_as_float(raw[kk][c]);
         }
         else
         {
 #pragma unroll
             for(int q = 0; q < 4; ++q)
             {
-                v[2 * q]     = act ? lo_half(raw[kk][q], (T*)nullptr) : 0.f;
-                v[2 * q + 1] = act ? hi_half(raw[kk][q], (T*)nullptr) : 0.f;
+                v[2 * q]     = lo_half(raw[kk][q], (T*)nullptr);
+                v[2 * q + 1] = hi_half(raw[kk][q], (T*)nullptr);
             }
-            v[8] = act ? lo_half(raw[kk][4], (T*)nullptr) : 0.f;
+            v[8] = lo_half(raw[kk][4], (T*)nullptr);
         }
-        const int j = act ? lane + dd : lane;
+        const int j = act ? lane + dd : 64;
         const float q0 = sR[3 * j], q1 = sR[3 * j + 1], q2 = sR[3 * j + 2];
         z0 += v[0] * q0 + v[1] * q1 + v[2] * q2;
         z1 += v[3] * q0 + v[4] * q1 + v[5] * q2;
