@@ -259,15 +259,6 @@ __global__ void __launch_bounds__(256) k_gal_gather(const double* fine, double* 
         for(int i = 0; i < 9; ++i) E[9 * (long long)e + i] = acc[i];
 }
 
-// First row of the second assembly pass: the lower triangle's rows [0, P)
-// and [P, DIM) hold about half its elements each.
-__host__ __device__ constexpr int factor_split_row(int dim)
-{
-    int p = 0;
-    while(2 * ((p + 1) * (p + 2) / 2) <= dim * (dim + 1) / 2) ++p;
-    return p;
-}
-
 // One element of the inverse into the diagonal-major output of bank `bank`:
 // (i, j), i >= j, is the upper element (j, i) of block (j / 3, i / 3); a
 // diagonal block also takes its transpose position.  Half formats store
@@ -296,6 +287,31 @@ __device__ __forceinline__ void store_inverse_element(void* out, int bank, int i
     }
 }
 
+// First row of assembly pass s of `npass` over the lower triangle of a
+// dim x dim matrix (rows [row(s), row(s + 1)) hold about 1/npass of it).
+__host__ __device__ constexpr int factor_pass_row(int dim, int npass, int s)
+{
+    const int target = (int)((long long)s * (dim * (dim + 1) / 2) / npass);
+    int p = 0;
+    while(p < dim && (p + 1) * (p + 2) / 2 <= target) ++p;
+    return p;
+}
+
+__host__ __device__ constexpr int factor_pass_max(int dim, int npass)
+{
+    int m = 0;
+    for(int s = 0; s < npass; ++s)
+    {
+        const int lo = factor_pass_row(dim, npass, s), hi = factor_pass_row(dim, npass, s + 1);
+        const int n = hi * (hi + 1) / 2 - lo * (lo + 1) / 2;
+        m = n > m ? n : m;
+    }
+    return m;
+}
+
+// Register slot of tile (r, c), r >= 2c, of the 8 x 16 grid's lower tiles.
+__host__ __device__ constexpr int tile8x16(int rr, int r, int c) { return c * rr - c * (c - 1) + (r - 2 * c); }
+
 // Dense local assembly and FP64 inversion of one bank per workgroup, written
 // diagonal-major in the storage precision T.  The inversion is the symmetric
 // sweep: pivot k turns a_ij into a_ij - a_ik a_kj / d (i, j != k), a_ik into
@@ -303,49 +319,52 @@ __device__ __forceinline__ void store_inverse_element(void* out, int bank, int i
 // every pivot, holds -A^-1.  A pivot (the Schur-complement diagonal) below
 // max(1e-12, 1e-12 max|diag|) is raised to that floor, which inverts the
 // same modified matrix as the pivot-floored Cholesky of
-// pro/.../TraditionalMAS32Preconditioner.cu:1382-1439.  Each thread of a
-// 16 x 16 grid keeps only the lower-triangle elements of its tiles
-// (ty + 16 r, tx + 16 c), r >= c, in registers (21 FMAs per thread and pivot
-// for 32-slot banks); the pivot's column passes through LDS, double-buffered,
-// one barrier per pivot.  The bank matrix is assembled in two passes through
-// half the LDS the whole lower triangle would take (zero diagonal -> 1) and
-// the inverse is written from the registers, so five banks fit on a CU.
-// Half-precision storage is scaled by the largest diagonal entry of the
-// inverse (for an SPD inverse, the largest entry in magnitude).
+// pro/.../TraditionalMAS32Preconditioner.cu:1382-1439.  Thread (ty, tx) of
+// an 8-row x 16-column grid of 128 threads keeps, in registers, elements
+// (ty + 8 r, tx + 16 c) of the tiles that reach the lower triangle (r >= 2c;
+// 42 for 32-slot banks), so each pivot's barrier, LDS exchange of the pivot
+// column (double-buffered) and bookkeeping serve 42 FMAs per thread; the
+// sweep is bound by instruction issue, not by the FMAs.  The bank matrix is
+// assembled in two passes through half the LDS the whole lower triangle
+// would take (zero diagonal -> 1), that buffer then holds the pivot columns,
+// and the inverse is written from the registers.  Half-precision storage is
+// scaled by the largest diagonal entry of the inverse (for an SPD inverse,
+// the largest entry in magnitude).
 template <int BANK, typename T>
-__global__ void __launch_bounds__(256) k_mas_factor_sym(DevData d, const double* E, const int* ent_ptr,
+__global__ void __launch_bounds__(128) k_mas_factor_sym(DevData d, const double* E, const int* ent_ptr,
                                                         const int* ent_pos, const int* ent_src, void* out,
                                                         float* scale_out, int* status)
 {
-    constexpr int DIM = 3 * BANK;
-    constexpr int R   = DIM / 16;
-    constexpr int NT  = R * (R + 1) / 2;
-    constexpr int NL  = DIM * (DIM + 1) / 2;
-    constexpr int P   = factor_split_row(DIM);
-    constexpr int N0  = P * (P + 1) / 2;
-    constexpr int NH  = N0 > NL - N0 ? N0 : NL - N0;
+    constexpr int DIM   = 3 * BANK;
+    constexpr int RR    = DIM / 8;    // tile rows (8 rows each)
+    constexpr int RC    = DIM / 16;   // tile columns (16 columns each)
+    constexpr int NT    = tile8x16(RR, 2 * RC, RC);  // slots of every tile (r, c), r >= 2c
+    constexpr int NPASS = DIM > 48 ? 2 : 1;
+    constexpr int NH    = factor_pass_max(DIM, NPASS);
+    constexpr int NS    = NH > 2 * DIM + 4 ? NH : 2 * DIM + 4;
     constexpr bool kHalf = !std::is_same<T, float>::value;
-    static_assert(DIM % 16 == 0, "the 16 x 16 grid must tile the bank matrix");
-    __shared__ double H[NH];
-    __shared__ double vk[2][DIM];
-    __shared__ double pinv[2];
-    __shared__ int    pflag[2];
-    __shared__ double red[4];
+    static_assert(DIM % 16 == 0, "the 8 x 16 grid must tile the bank matrix");
+    __shared__ double S[NS];
+    __shared__ double red[2];
+    double* H  = S;
+    double* vk = S;              // during the sweep: two pivot columns ...
+    double* pv = S + 2 * DIM;    // ... the pivots' reciprocals and floored flags
     const int bank = blockIdx.x, tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-    auto tix = [](int r, int c) { return r * (r + 1) / 2 + c; };  // tile (r, c), r >= c
     double a[NT];
 #pragma unroll
     for(int t = 0; t < NT; ++t) a[t] = 0.0;
-    // ---- assembly: rows [0, P), then [P, DIM), each into H
+    // ---- assembly, NPASS row ranges of the lower triangle through H
     const bool fine = bank < d.n_fine_banks;
+    const int  e_begin = ent_ptr[bank], e_end = ent_ptr[bank + 1];
     double dmax = 0.0;
-    for(int pass = 0; pass < 2; ++pass)
+#pragma unroll
+    for(int pass = 0; pass < NPASS; ++pass)
     {
-        const int lo = pass ? P : 0, hi = pass ? DIM : P, base = pass ? N0 : 0;
-        const int n = pass ? NL - N0 : N0;
-        for(int q = tid; q < n; q += 256) H[q] = 0.0;
+        const int lo = factor_pass_row(DIM, NPASS, pass), hi = factor_pass_row(DIM, NPASS, pass + 1);
+        const int base = lo * (lo + 1) / 2, n = hi * (hi + 1) / 2 - base;
+        for(int q = tid; q < n; q += 128) H[q] = 0.0;
         __syncthreads();
-        for(int k = ent_ptr[bank] + tid; k < ent_ptr[bank + 1]; k += 256)
+        for(int k = e_begin + tid; k < e_end; k += 128)
         {
             const int pos = ent_pos[k] & 0xffff, tr = ent_pos[k] >> 16;
             const int src = ent_src[k];
@@ -369,84 +388,91 @@ __global__ void __launch_bounds__(256) k_mas_factor_sym(DevData d, const double*
         }
         __syncthreads();
         // zero diagonal -> 1 (padding slots behave as identity)
-        if(tid < hi - lo)
+        for(int q = lo + tid; q < hi; q += 128)
         {
-            const int q = (lo + tid) * (lo + tid + 1) / 2 + lo + tid - base;
-            if(H[q] == 0.0) H[q] = 1.0;
-            dmax = fmax(dmax, fabs(H[q]));
+            double& h = H[q * (q + 1) / 2 + q - base];
+            if(h == 0.0) h = 1.0;
+            dmax = fmax(dmax, fabs(h));
         }
         __syncthreads();
 #pragma unroll
-        for(int r = 0; r < R; ++r)
+        for(int c = 0; c < RC; ++c)
 #pragma unroll
-            for(int c = 0; c <= r; ++c)
+            for(int r = 2 * c; r < RR; ++r)
             {
-                const int i = ty + 16 * r, j = tx + 16 * c;
-                if(i >= lo && i < hi && i >= j) a[tix(r, c)] = H[i * (i + 1) / 2 + j - base];
+                const int i = ty + 8 * r, j = tx + 16 * c;
+                if(i >= lo && i < hi && i >= j) a[tile8x16(RR, r, c)] = H[i * (i + 1) / 2 + j - base];
             }
-        __syncthreads();
+        __syncthreads();  // H becomes the pivot-column buffer
     }
     // pivot floor max(1e-12, 1e-12 max |diag|)
 #pragma unroll
     for(int o = 32; o > 0; o >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, o, 64));
     if((tid & 63) == 0) red[tid >> 6] = dmax;
     __syncthreads();
-    const double floor_value = fmax(1.0e-12, 1.0e-12 * fmax(fmax(red[0], red[1]), fmax(red[2], red[3])));
+    const double floor_value = fmax(1.0e-12, 1.0e-12 * fmax(red[0], red[1]));
     int bad = 0;
-    // ---- sweep: pivot k = kt + 16 cc
+    // ---- sweep: pivot k = 8 rk + k8 = kt + 16 cc (tile row rk, tile column
+    // cc = rk / 2, kt = k8 + 8 (rk & 1))
 #pragma unroll
-    for(int cc = 0; cc < R; ++cc)
+    for(int rk = 0; rk < RR; ++rk)
     {
-#pragma unroll 1
-        for(int kt = 0; kt < 16; ++kt)
+        const int cc = rk / 2, h = rk & 1;
         {
-            const int k = kt + 16 * cc, buf = k & 1;
-            // column k from row k down (lower elements (i, k), i >= k) ...
-            if(tx == kt)
+#pragma unroll 1
+            for(int k8 = 0; k8 < 8; ++k8)
             {
-#pragma unroll
-                for(int r = cc; r < R; ++r)
-                    if(r > cc || ty >= kt) vk[buf][ty + 16 * r] = a[tix(r, cc)];
-            }
-            // ... and left of the pivot (lower elements (k, j), j < k); the pivot
-            if(ty == kt)
-            {
-#pragma unroll
-                for(int c = 0; c <= cc; ++c)
-                    if(c < cc || tx < kt) vk[buf][tx + 16 * c] = a[tix(cc, c)];
+                const int kt = k8 + 8 * h, k = kt + 16 * cc, buf = k & 1;
+                double* col = vk + buf * DIM;
+                // column k from row k down (lower elements (i, k), i >= k) ...
                 if(tx == kt)
                 {
-                    const double pv = a[tix(cc, cc)];
-                    const bool floored = !(pv >= floor_value) || isinf(pv);  // also catches NaN
-                    pinv[buf]  = 1.0 / (floored ? floor_value : pv);
-                    pflag[buf] = floored ? 1 : 0;
+#pragma unroll
+                    for(int r = rk; r < RR; ++r)
+                        if(r > rk || ty >= k8) col[ty + 8 * r] = a[tile8x16(RR, r, cc)];
                 }
-            }
-            __syncthreads();
-            const double ip = pinv[buf];
-            bad += pflag[buf];
-            double f[R], g[R];
+                // ... and left of the pivot (lower elements (k, j), j < k); the pivot
+                if(ty == k8)
+                {
 #pragma unroll
-            for(int r = 0; r < R; ++r) f[r] = vk[buf][ty + 16 * r];
+                    for(int c = 0; c <= cc; ++c)
+                        if(c < cc || tx < kt) col[tx + 16 * c] = a[tile8x16(RR, rk, c)];
+                    if(tx == kt)
+                    {
+                        const double p = a[tile8x16(RR, rk, cc)];
+                        const bool floored = !(p >= floor_value) || isinf(p);  // also catches NaN
+                        pv[buf]     = 1.0 / (floored ? floor_value : p);
+                        pv[2 + buf] = floored ? 1.0 : 0.0;
+                    }
+                }
+                __syncthreads();
+                const double ip = pv[buf];
+                bad += pv[2 + buf] != 0.0 ? 1 : 0;
+                double g[RC];
 #pragma unroll
-            for(int c = 0; c < R; ++c) g[c] = vk[buf][tx + 16 * c] * ip;
+                for(int c = 0; c < RC; ++c) g[c] = col[tx + 16 * c] * ip;
 #pragma unroll
-            for(int r = 0; r < R; ++r)
+                for(int r = 0; r < RR; ++r)
+                {
+                    const double f = col[ty + 8 * r];
 #pragma unroll
-                for(int c = 0; c <= r; ++c) a[tix(r, c)] -= f[r] * g[c];
-            // column k below the pivot and row k left of it take v / d, the pivot -1 / d
-            if(tx == kt)
-            {
+                    for(int c = 0; c < RC; ++c)
+                        if(2 * c <= r) a[tile8x16(RR, r, c)] -= f * g[c];
+                }
+                // column k below the pivot and row k left of it take v / d, the pivot -1 / d
+                if(tx == kt)
+                {
 #pragma unroll
-                for(int r = cc; r < R; ++r)
-                    if(r > cc || ty > kt) a[tix(r, cc)] = f[r] * ip;
-            }
-            if(ty == kt)
-            {
+                    for(int r = rk; r < RR; ++r)
+                        if(r > rk || ty > k8) a[tile8x16(RR, r, cc)] = col[ty + 8 * r] * ip;
+                }
+                if(ty == k8)
+                {
 #pragma unroll
-                for(int c = 0; c <= cc; ++c)
-                    if(c < cc || tx < kt) a[tix(cc, c)] = g[c];
-                if(tx == kt) a[tix(cc, cc)] = -ip;
+                    for(int c = 0; c <= cc; ++c)
+                        if(c < cc || tx < kt) a[tile8x16(RR, rk, c)] = g[c];
+                    if(tx == kt) a[tile8x16(RR, rk, cc)] = -ip;
+                }
             }
         }
     }
@@ -455,24 +481,29 @@ __global__ void __launch_bounds__(256) k_mas_factor_sym(DevData d, const double*
     if constexpr(kHalf)
     {
         double m = 0.0;
-        if(ty == tx)
 #pragma unroll
-            for(int r = 0; r < R; ++r) m = fmax(m, -a[tix(r, r)]);
+        for(int c = 0; c < RC; ++c)
+#pragma unroll
+            for(int r = 2 * c; r < 2 * c + 2; ++r)
+                if(ty + 8 * r == tx + 16 * c) m = fmax(m, -a[tile8x16(RR, r, c)]);
 #pragma unroll
         for(int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
         __syncthreads();  // red's floor reads are done
         if((tid & 63) == 0) red[tid >> 6] = m;
         __syncthreads();
-        m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        m = fmax(red[0], red[1]);
         scale = m > 0.0 ? m : 1.0;
     }
     if(tid == 0) scale_out[bank] = (float)scale;
     const double inv_scale = 1.0 / scale;
 #pragma unroll
-    for(int r = 0; r < R; ++r)
+    for(int c = 0; c < RC; ++c)
 #pragma unroll
-        for(int c = 0; c <= r; ++c)
-            if(r > c || ty >= tx) store_inverse_element<BANK, T>(out, bank, ty + 16 * r, tx + 16 * c, -a[tix(r, c)], inv_scale);
+        for(int r = 2 * c; r < RR; ++r)
+        {
+            const int i = ty + 8 * r, j = tx + 16 * c;
+            if(i >= j) store_inverse_element<BANK, T>(out, bank, i, j, -a[tile8x16(RR, r, c)], inv_scale);
+        }
     if(tid == 0 && bad && status) atomicAdd(status, bad);
 }
 
@@ -1704,7 +1735,7 @@ hipError_t launch_mas_factor(const DevData& d, const double* E, const int* ent_p
     if(d.n_banks <= 0) return hipSuccess;
     const dim3 g(d.n_banks);
 #define MASX_FACTOR(BK, T)                                                                                       \
-    hipLaunchKernelGGL((k_mas_factor_sym<BK, T>), g, dim3(256), 0, st, d, E, ent_ptr, ent_pos, ent_src, out, scale, \
+    hipLaunchKernelGGL((k_mas_factor_sym<BK, T>), g, dim3(128), 0, st, d, E, ent_ptr, ent_pos, ent_src, out, scale, \
                        status)
     if(d.bank == 32)
     {
