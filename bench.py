@@ -106,13 +106,15 @@ def measured_traffic(kernel, workload_is_default, spmv_entries, summary="fem40c_
 def kernel_times(s, reps):
     """Per-kernel execution time (us) of the four iteration kernels: events
     stamped by each launch's own dispatch, on the library's stream.  Each
-    kernel gets three windows of `reps` launches, taken round-robin over the
-    kernels so the windows of one kernel lie apart in time, and keeps its
-    lowest window mean (a slow episode of a few ms on the box then spoils
-    one window, not the figure)."""
+    kernel gets five windows of `reps` launches, taken round-robin over the
+    kernels with 10 ms between rounds so the windows of one kernel lie apart
+    in time, and keeps its lowest window mean (a slow episode of some ms on
+    the box then spoils a window or two, not the figure)."""
     names = ["mas_local", "precond_finish", "spmv", "update"]
     best = {n: float("inf") for n in names}
-    for _ in range(3):
+    for rnd in range(5):
+        if rnd:
+            time.sleep(0.01)
         for k, n in enumerate(names):
             best[n] = min(best[n], s.time_kernel(k, reps) * 1e3)
     return best
