@@ -647,6 +647,18 @@ __global__ void __launch_bounds__(256) k_solve_prepare(DevData d, const int* int
 // Number of systems not yet done, written to host-visible memory with a
 // sequence number (one workgroup, last node of every captured chunk):
 // host[0] = count, then host[1] = ++(*seq).
+// (unfinished count, sequence number) to host memory, each store made
+// visible to the host before the next
+__device__ __forceinline__ void export_to_host(int unfinished, int* host, int* seq)
+{
+    const int q = *seq + 1;
+    *seq = q;
+    *(volatile int*)host = unfinished;
+    __threadfence_system();
+    *(volatile int*)(host + 1) = q;
+    __threadfence_system();
+}
+
 __global__ void __launch_bounds__(256) k_export_status(DevData d, int* host, int* seq)
 {
     __shared__ int s[256];
@@ -659,15 +671,7 @@ __global__ void __launch_bounds__(256) k_export_status(DevData d, int* host, int
         if((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
         __syncthreads();
     }
-    if(threadIdx.x == 0)
-    {
-        const int q = *seq + 1;
-        *seq = q;
-        *(volatile int*)host = s[0];
-        __threadfence_system();
-        *(volatile int*)(host + 1) = q;
-        __threadfence_system();
-    }
+    if(threadIdx.x == 0) export_to_host(s[0], host, seq);
 }
 
 __global__ void k_init_state(DevData d, int rule, double tol, const int* max_iter, int max_override, int check_every,
@@ -693,27 +697,32 @@ __global__ void k_init_state(DevData d, int rule, double tol, const int* max_ite
 }
 
 // Completes the init / restart phase: bb (init), status -> running.
-__global__ void __launch_bounds__(256) k_phase_finish(DevData d)
+// With `host` (one system only), the block also does k_export_status's
+// work after the transition, saving the chunk a launch.
+__global__ void __launch_bounds__(256) k_phase_finish(DevData d, int* host, int* seq)
 {
     __shared__ double smem[8];
     const int s = blockIdx.x;
     SysState& st = d.state[s];
-    if(st.status != kInit && st.status != kRestarting) return;
-    const double v = sys_sum(d.part_upd, wg_range(d, d.wg_upd_range, d.n_wg_upd, s), smem);
-    if(threadIdx.x == 0)
+    if(st.status == kInit || st.status == kRestarting)  // uniform over the block
     {
-        if(st.status == kInit)
+        const double v = sys_sum(d.part_upd, wg_range(d, d.wg_upd_range, d.n_wg_upd, s), smem);
+        if(threadIdx.x == 0)
         {
-            st.bb = v;
-            st.rr = v;
-            st.status = v > 0.0 ? kRunning : kDone;
-        }
-        else
-        {
-            st.rr = v;
-            st.status = kRunning;
+            if(st.status == kInit)
+            {
+                st.bb = v;
+                st.rr = v;
+                st.status = v > 0.0 ? kRunning : kDone;
+            }
+            else
+            {
+                st.rr = v;
+                st.status = kRunning;
+            }
         }
     }
+    if(host && threadIdx.x == 0) export_to_host(st.status != kDone ? 1 : 0, host, seq);
 }
 
 // ========================================================= iteration kernels
@@ -1811,9 +1820,10 @@ hipError_t launch_export_status(const DevData& d, int* host, int* seq, hipStream
     return hipSuccess;
 }
 
-hipError_t launch_phase_finish(const DevData& d, hipStream_t st)
+hipError_t launch_phase_finish(const DevData& d, hipStream_t st, int* host, int* seq)
 {
-    hipLaunchKernelGGL(k_phase_finish, dim3(d.n_sys), dim3(256), 0, st, d);
+    if(host && d.n_sys != 1) return hipErrorInvalidValue;  // the export counts one system
+    hipLaunchKernelGGL(k_phase_finish, dim3(d.n_sys), dim3(256), 0, st, d, host, seq);
     MASX_CHECK_LAUNCH();
     return hipSuccess;
 }

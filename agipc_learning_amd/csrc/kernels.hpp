@@ -23,7 +23,8 @@ hipError_t launch_init_state(const DevData& d, int rule, double tol, const int* 
 hipError_t launch_solve_prepare(const DevData& d, const int* int_to_ext, const double* b, hipStream_t st);
 // host[0] = systems not done, then host[1] = ++*seq (device counter)
 hipError_t launch_export_status(const DevData& d, int* host, int* seq, hipStream_t st);
-hipError_t launch_phase_finish(const DevData& d, hipStream_t st);
+// host/seq (one system only): also export the chunk status, as launch_export_status
+hipError_t launch_phase_finish(const DevData& d, hipStream_t st, int* host = nullptr, int* seq = nullptr);
 // Events stamped by one kernel's own dispatch (start and end of its
 // execution, via hipExtLaunchKernelGGL): their elapsed time is the kernel's
 // duration alone, without the gaps between launches.  Null: a plain launch.

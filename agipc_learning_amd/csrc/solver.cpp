@@ -537,13 +537,16 @@ void Solver::launch_iteration(hipStream_t st, int i)
 void Solver::run_chunk(hipStream_t st, int chunk, bool with_check)
 {
     for(int i = 0; i < chunk; ++i) launch_iteration(st, i);
+    // one system: the check's last kernel also exports the status
+    const bool fold_export = with_check && plan_.n_sys == 1;
     if(with_check)
     {
         MASX_HIP(launch_true_residual(dd_, n_wg_spmv_, 0, st));
         MASX_HIP(launch_update(dd_, n_wg_upd_, d_p0_.ptr, 1, st));
-        MASX_HIP(launch_phase_finish(dd_, st));
+        MASX_HIP(launch_phase_finish(dd_, st, fold_export ? d_export_ : nullptr,
+                                     fold_export ? d_export_seq_.ptr : nullptr));
     }
-    MASX_HIP(launch_export_status(dd_, d_export_, d_export_seq_.ptr, st));
+    if(!fold_export) MASX_HIP(launch_export_status(dd_, d_export_, d_export_seq_.ptr, st));
 }
 
 void Solver::solve(const double* d_b, double* d_x, const SolveParams& prm, std::vector<SolveStats>* stats,
