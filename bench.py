@@ -475,7 +475,7 @@ def main():
     results = {}
     for prec in precs:
         log(f"step: timed {prec} ({args.warmup} warmup + {args.steps} steps)")
-        ms, st = run(prec, args.steps if prec == "fp32" else max(3, args.steps // 2), args.warmup)
+        ms, st = run(prec, args.steps, args.warmup)
         if st["failure"] != 0 or st["rel_res_true"] > args.tol * (1 + 1e-6):
             raise SystemExit(f"{prec} solve did not reach the tolerance: {st}")
         # split of the step: build alone, solve alone
