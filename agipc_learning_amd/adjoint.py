@@ -104,7 +104,7 @@ class ElasticSystem:
             st = self.solver.solve(rhs, x, rule=self.rule, tol=self.tol, max_iter=self.max_iter)[0]
             st["refinements"] = 1
             self.last_stats = st
-            if st["failure"] != 0:
+            if st["failure"] != 0 or st["status"] != native.STATUS_DONE:
                 raise RuntimeError(f"MAS-PCG failed: {st}")
             return x
         bnorm = float(torch.linalg.vector_norm(rhs))
@@ -115,7 +115,7 @@ class ElasticSystem:
         for k in range(self.max_refine):
             d.zero_()
             st = self.solver.solve(r, d, rule=native.RULE_TRUE_REL, tol=self.inner_tol, max_iter=self.max_iter)[0]
-            if st["failure"] != 0:
+            if st["failure"] != 0 or st["status"] != native.STATUS_DONE:
                 raise RuntimeError(f"MAS-PCG failed in refinement step {k}: {st}")
             iters += st["iterations"]
             x += d

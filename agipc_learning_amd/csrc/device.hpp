@@ -44,6 +44,7 @@ struct alignas(128) SysState
     int    k, status, failure, rule;
     int    max_iter, k_check, check_every, beta_zero;
     int    need_check, restarts, n_true_checks, frozen;
+    int    rejected;  // candidates whose true residual missed the tolerance (each ends in a restart)
 };
 
 enum WgType : int { kWgPlain = 0, kWgFemMas = 1, kWgAbd = 2 };
@@ -117,6 +118,7 @@ struct DevData
     double*     part_spmv;
     double*     part_upd;
     double*     part_tr;
+    double*     part_drift; // per SpMV workgroup: |r_true - r|^2 of its rows at a check
     SysState*   state;
 };
 

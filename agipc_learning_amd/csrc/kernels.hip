@@ -328,8 +328,9 @@ __host__ __device__ constexpr int tile8x16(int rr, int r, int c) { return c * rr
 // assembled in two passes through half the LDS the whole lower triangle
 // would take (zero diagonal -> 1), that buffer then holds the pivot columns,
 // and the inverse is written from the registers.  Half-precision storage is
-// scaled by the largest diagonal entry of the inverse (for an SPD inverse,
-// the largest entry in magnitude).
+// scaled by the largest stored entry of the bank's inverse in magnitude, so
+// every stored value lies in [-1, 1] (also for an inverse that floored
+// pivots left indefinite, whose off-diagonal entries can exceed its diagonal).
 template <int BANK, typename T>
 __global__ void __launch_bounds__(128) k_mas_factor_sym(DevData d, const double* E, const int* ent_ptr,
                                                         const int* ent_pos, const int* ent_src, void* out,
@@ -484,8 +485,8 @@ __global__ void __launch_bounds__(128) k_mas_factor_sym(DevData d, const double*
 #pragma unroll
         for(int c = 0; c < RC; ++c)
 #pragma unroll
-            for(int r = 2 * c; r < 2 * c + 2; ++r)
-                if(ty + 8 * r == tx + 16 * c) m = fmax(m, -a[tile8x16(RR, r, c)]);
+            for(int r = 2 * c; r < RR; ++r)
+                if(ty + 8 * r >= tx + 16 * c) m = fmax(m, fabs(a[tile8x16(RR, r, c)]));
 #pragma unroll
         for(int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
         __syncthreads();  // red's floor reads are done
@@ -694,6 +695,7 @@ __global__ void k_init_state(DevData d, int rule, double tol, const int* max_ite
     st.restarts = 0;
     st.n_true_checks = 0;
     st.frozen = frozen;
+    st.rejected = 0;
 }
 
 // Completes the init / restart phase: bb (init), status -> running.
@@ -709,7 +711,15 @@ __global__ void __launch_bounds__(256) k_phase_finish(DevData d, int* host, int*
         const double v = sys_sum(d.part_upd, wg_range(d, d.wg_upd_range, d.n_wg_upd, s), smem);
         if(threadIdx.x == 0)
         {
-            if(st.status == kInit)
+            if(!isfinite(v))
+            {
+                // a non-finite b (init) or b - A x (restart) ends the solve
+                if(st.status == kInit) st.bb = v;
+                st.rr = v;
+                st.status = kDone;
+                st.failure = kNonFinite;
+            }
+            else if(st.status == kInit)
             {
                 st.bb = v;
                 st.rr = v;
@@ -1073,8 +1083,10 @@ __global__ void __launch_bounds__(256) k_precond_finish(DevData d, int force)
 
 // FP64 true residual r_true = b - A x of rows [r_begin, r_end) (one thread
 // per row, rows strided by the workgroup), from the FP64 input blocks; writes
-// r_true and returns this thread's share of |r_true|^2.
-__device__ double true_residual_rows(const DevData& d, int r_begin, int r_end)
+// r_true and returns this thread's share of |r_true|^2.  With `drift`, also
+// adds this thread's share of |r_true - r|^2 (r the recursive residual of the
+// same iterate) to *drift.
+__device__ double true_residual_rows(const DevData& d, int r_begin, int r_end, double* drift = nullptr)
 {
     double part = 0.0;
     for(int row = r_begin + (int)threadIdx.x; row < r_end; row += blockDim.x)
@@ -1101,6 +1113,11 @@ __device__ double true_residual_rows(const DevData& d, int r_begin, int r_end)
         }
         const long long o = 3 * (long long)row;
         const double t0 = d.b64[o] - a0, t1 = d.b64[o + 1] - a1, t2 = d.b64[o + 2] - a2;
+        if(drift)
+        {
+            const double e0 = t0 - (double)d.r[o], e1 = t1 - (double)d.r[o + 1], e2 = t2 - (double)d.r[o + 2];
+            *drift += e0 * e0 + e1 * e1 + e2 * e2;
+        }
         d.rt[o] = (float)t0; d.rt[o + 1] = (float)t1; d.rt[o + 2] = (float)t2;
         part += t0 * t0 + t1 * t1 + t2 * t2;
     }
@@ -1581,9 +1598,15 @@ __global__ void __launch_bounds__(256) k_true_residual(DevData d, int report_onl
     const SysState& st = d.state[s];
     if(!report_only && !st.need_check) return;
     if(report_only && report_is_current(st)) return;
-    double part = true_residual_rows(d, w.z, w.w);
-    part = block_sum_last(part, smem);
-    if(threadIdx.x == 0) d.part_tr[blockIdx.x] = part;
+    double drift = 0.0;
+    double part = true_residual_rows(d, w.z, w.w, &drift);
+    part = block_sum(part, smem);
+    drift = block_sum_last(drift, smem);
+    if(threadIdx.x == 0)
+    {
+        d.part_tr[blockIdx.x] = part;
+        d.part_drift[blockIdx.x] = drift;
+    }
 }
 
 // One workgroup per system: true residual decision (done / restart on drift).
@@ -1594,7 +1617,10 @@ __global__ void __launch_bounds__(256) k_check_finish(DevData d, int report_only
     SysState& st = d.state[s];
     if(!report_only && !st.need_check) return;
     if(report_only && report_is_current(st)) return;
-    const double tr = sys_sum(d.part_tr, wg_range(d, d.wg_spmv_range, d.n_wg_spmv, s), smem);
+    const int2 tiles = wg_range(d, d.wg_spmv_range, d.n_wg_spmv, s);
+    const double tr = sys_sum(d.part_tr, tiles, smem);
+    // |r_true - r|^2: the true and the recursive residual of the same iterate
+    const double drift2 = (!report_only && st.status == kRunning) ? sys_sum(d.part_drift, tiles, smem) : 0.0;
     if(threadIdx.x != 0) return;
     st.true_rr = tr;
     if(report_only) return;
@@ -1603,15 +1629,25 @@ __global__ void __launch_bounds__(256) k_check_finish(DevData d, int report_only
     st.k_check = st.k;
     const double tol2 = st.tol * st.tol * st.bb;
     bool restart = false;
+    if(!isfinite(tr))
+    {
+        st.status = kDone;
+        st.failure = kNonFinite;
+        return;
+    }
     if(st.status == kCandidate)
     {
         if(tr <= tol2) st.status = kDone;
-        else restart = true;
+        else
+        {
+            restart = true;
+            st.rejected += 1;
+        }
     }
     else if(st.status == kRunning)
     {
-        const double drift = fabs(sqrt(tr) - sqrt(st.rr));
-        restart = drift > 10.0 * st.tol * sqrt(st.bb);
+        // restart when the recursion has drifted from b - A x by more than 10 tol ||b||
+        restart = drift2 > 100.0 * tol2;
     }
     if(restart)
     {

@@ -331,6 +331,7 @@ int masx_solve(masx_solver* s, const double* d_b, double* d_x, const masx_solve_
                 stats[i].rz                = st[i].rz;
                 stats[i].rel_res_recursive = st[i].rel_res_recursive;
                 stats[i].rel_res_true      = st[i].rel_res_true;
+                stats[i].rejected          = st[i].rejected;
             }
     });
 }

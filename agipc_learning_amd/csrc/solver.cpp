@@ -243,6 +243,8 @@ void Solver::setup()
     d_part_deep_.alloc(3 * (size_t)std::max(P.n_deep_slots, 1));
     d_row_anc_.upload(P.row_anc, st);
     d_minv_scale_.alloc(std::max(P.n_banks, 1));
+    // the bank count may have grown with a new structure (alloc keeps a buffer that fits)
+    d_minv_.alloc(std::max<size_t>(inverse_bytes(), 16));
     d_gal_ptr_.upload(P.gal_ptr, st);
     d_gal_src_.upload(P.gal_src, st);
     d_ent_ptr_.upload(P.bank_ent_ptr, st);
@@ -372,6 +374,7 @@ void Solver::build_work_tables()
     d_part_spmv_.alloc(n_wg_spmv_);
     d_part_upd_.alloc(n_wg_upd_);
     d_part_tr_.alloc(n_wg_spmv_);
+    d_part_drift_.alloc(n_wg_spmv_);
     MASX_HIP(hipStreamSynchronize(st));
 }
 
@@ -442,6 +445,7 @@ void Solver::make_devdata()
     d.part_spmv = d_part_spmv_.ptr;
     d.part_upd = d_part_upd_.ptr;
     d.part_tr = d_part_tr_.ptr;
+    d.part_drift = d_part_drift_.ptr;
     d.state = d_state_.ptr;
 }
 
@@ -487,10 +491,14 @@ void Solver::build(int precision, hipStream_t user)
     if(!setup_done_) throw std::runtime_error("setup() must precede build()");
     if(precision < kPrecFp32 || precision > kPrecBf16) throw std::invalid_argument("unknown precision");
     hipStream_t st = enter(user);
-    if(precision != precision_ || !d_minv_.ptr)
+    const bool prec_changed = precision != precision_;
+    precision_ = precision;
+    const char* old_minv = d_minv_.ptr;
+    d_minv_.alloc(std::max<size_t>(inverse_bytes(), 16));  // reuses the buffer when it fits
+    if(inverse_bytes() > d_minv_.cap) throw std::logic_error("inverse buffer smaller than the banks need");
+    if(prec_changed || d_minv_.ptr != old_minv)
     {
-        precision_ = precision;
-        d_minv_.alloc(std::max<size_t>(inverse_bytes(), 16));
+        // the captured chunk holds the old pointer (and the old precision's kernel)
         make_devdata();
         if(graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
         graph_exec_ = nullptr;
@@ -643,6 +651,10 @@ void Solver::solve(const double* d_b, double* d_x, const SolveParams& prm, std::
             if(h.rule == kRuleRefRtz && h.failure == kIterationCap) o.iterations = h.max_iter;
             o.status = h.status;
             o.failure = h.failure;
+            // the host ran out of chunks (repeated restarts) before the system
+            // stopped: report it as capped, not as running without a failure
+            if(h.status != kDone && h.failure == kOk) o.failure = kIterationCap;
+            o.rejected = h.rejected;
             o.restarts = h.restarts;
             o.true_checks = h.n_true_checks;
             o.rz0 = h.rz0;

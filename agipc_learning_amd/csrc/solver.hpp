@@ -55,6 +55,7 @@ struct SolveStats
     double rz0, rz;
     double rel_res_recursive;
     double rel_res_true;
+    int    rejected;  // convergence candidates whose FP64 true residual missed the tolerance
 };
 
 class Solver
@@ -137,7 +138,7 @@ class Solver
     DeviceArray<int4>      d_wg_pre_, d_wg_spmv_, d_wg_upd_, d_wg_upd_grp_;
     DeviceArray<int2>      d_wg_spmv_ent_;
     DeviceArray<int>       d_wg_pre_range_, d_wg_spmv_range_, d_wg_upd_range_;
-    DeviceArray<double>    d_part_pre_, d_part_spmv_, d_part_upd_, d_part_tr_;
+    DeviceArray<double>    d_part_pre_, d_part_spmv_, d_part_upd_, d_part_tr_, d_part_drift_;
     DeviceArray<SysState>  d_state_;
     DeviceArray<int>       d_max_iter_, d_factor_status_;
     // device MAS hierarchy build of setup(): inputs and the builder's scratch

@@ -19,6 +19,8 @@ HEADER = os.path.join(_build.ROOT, "include", "masx.h")
 
 FP32, FP16, BF16 = 0, 1, 2
 RULE_REF_RTZ, RULE_TRUE_REL = 0, 1
+STATUS_RUNNING, STATUS_DONE = 0, 1
+FAIL_OK, FAIL_BAD_RZ, FAIL_BAD_CURVATURE, FAIL_NONFINITE, FAIL_ITERATION_CAP = 0, 1, 2, 3, 4
 PRECISIONS = {"fp32": FP32, "fp16": FP16, "bf16": BF16}
 
 
@@ -35,7 +37,7 @@ class SolveParams(C.Structure):
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("status", C.c_int), ("failure", C.c_int), ("restarts", C.c_int),
                 ("true_checks", C.c_int), ("rz0", C.c_double), ("rz", C.c_double),
-                ("rel_res_recursive", C.c_double), ("rel_res_true", C.c_double)]
+                ("rel_res_recursive", C.c_double), ("rel_res_true", C.c_double), ("rejected", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -177,6 +179,18 @@ def _dev(t):
     if not t.is_contiguous():
         raise ValueError("expected a contiguous tensor")
     return C.c_void_p(t.data_ptr())
+
+
+def _dev_f64(t, n, name):
+    """Device pointer of a contiguous CUDA float64 tensor of exactly n
+    values: the library reads or writes that many through the raw pointer."""
+    import torch
+
+    if t is None or not t.is_cuda or t.dtype != torch.float64:
+        raise ValueError(f"{name}: expected a CUDA float64 tensor")
+    if t.numel() != n:
+        raise ValueError(f"{name}: expected {n} values, got {t.numel()}")
+    return _dev(t)
 
 
 def morton_partition(positions, bank=32):
@@ -329,16 +343,22 @@ class Solver:
     def setup(self):
         check(self.L.masx_setup(self.h))
 
+    def _n_ext(self):
+        return 3 * sum(d["block_rows"] for d in self.systems)
+
     def set_values(self, sys, values, stream=None):
         """values: device float64 tensor (nb, 9) or host numpy array."""
+        n = 9 * self.systems[sys]["nb"]
         if isinstance(values, np.ndarray):
             v = np.ascontiguousarray(values, np.float64)
+            if v.size != n:
+                raise ValueError(f"values: expected {n} values, got {v.size}")
             check(self.L.masx_set_values_host(self.h, sys, _ptr(v, C.c_double), stream))
             self._keep = v
             import torch  # the copy is asynchronous on the library stream
             torch.cuda.synchronize()
         else:
-            check(self.L.masx_set_values(self.h, sys, _dev(values), stream))
+            check(self.L.masx_set_values(self.h, sys, _dev_f64(values, n, "values"), stream))
 
     def build(self, precision="fp32", stream=None):
         check(self.L.masx_build(self.h, PRECISIONS.get(precision, precision), stream))
@@ -347,14 +367,17 @@ class Solver:
               stream=None):
         p = SolveParams(rule, tol, max_iter, check_every, chunk, int(use_graph))
         st = (SolveStats * len(self.systems))()
-        check(self.L.masx_solve(self.h, _dev(b), _dev(x), C.byref(p), st, stream))
+        n = self._n_ext()
+        check(self.L.masx_solve(self.h, _dev_f64(b, n, "b"), _dev_f64(x, n, "x"), C.byref(p), st, stream))
         return [s.as_dict() for s in st]
 
     def spmv(self, x, y, a=1.0, b=0.0, stream=None):
-        check(self.L.masx_spmv(self.h, a, _dev(x), b, _dev(y), stream))
+        n = self._n_ext()
+        check(self.L.masx_spmv(self.h, a, _dev_f64(x, n, "x"), b, _dev_f64(y, n, "y"), stream))
 
     def apply_preconditioner(self, r, z, stream=None):
-        check(self.L.masx_apply_preconditioner(self.h, _dev(r), _dev(z), stream))
+        n = self._n_ext()
+        check(self.L.masx_apply_preconditioner(self.h, _dev_f64(r, n, "r"), _dev_f64(z, n, "z"), stream))
 
     def set_spmv_tiling(self, entries_per_thread):
         check(self.L.masx_set_spmv_tiling(self.h, entries_per_thread))

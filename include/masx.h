@@ -29,6 +29,15 @@ typedef struct masx_mas    masx_mas;
 enum masx_precision { MASX_FP32 = 0, MASX_FP16 = 1, MASX_BF16 = 2 };
 enum masx_rule      { MASX_RULE_REF_RTZ = 0, MASX_RULE_TRUE_REL = 1 };
 enum masx_status    { MASX_RUNNING = 0, MASX_DONE = 1 };
+/* Failure of a stopped solve (cf. the device PCG statuses of
+ * pro/candidate-paper-t57-mas-fused-init/StiffGIPC/agipc/agipc_device_pcg.cuh:12-13):
+ * BAD_RZ: r.z not finite, or not positive under the TRUE rule (InvalidRz);
+ * BAD_CURVATURE: p.Ap not finite, or not positive under the TRUE rule
+ * (NonfiniteCurvature / NonpositiveCurvature); NONFINITE: ||b||, ||r|| or the
+ * true residual not finite (NonfiniteResidual); ITERATION_CAP: max_iter
+ * reached (IterationCap), also reported when the host stops launching work
+ * before a system has stopped.  Vectors are left as the last complete
+ * iteration wrote them. */
 enum masx_failure   { MASX_OK = 0, MASX_BAD_RZ = 1, MASX_BAD_CURVATURE = 2, MASX_NONFINITE = 3,
                       MASX_ITERATION_CAP = 4 };
 
@@ -67,6 +76,8 @@ typedef struct masx_solve_stats
     double rz;
     double rel_res_recursive;
     double rel_res_true; /* ||b - A x|| / ||b|| in FP64 */
+    int    rejected;     /* convergence candidates (recursive residual at the tolerance) whose FP64
+                            true residual missed it; each one restarts the recursion from b - A x */
 } masx_solve_stats;
 
 const char* masx_last_error(void);

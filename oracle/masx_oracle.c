@@ -17,6 +17,32 @@
 /* ------------------------------------------------------------------------ */
 
 /* Eigen column-major 3x3: element (i,j) at [i + 3*j]. */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Threads of the parallel loops (bank inverses, local solves, row SpMV,
+ * vector updates).  Every parallel loop writes disjoint outputs and every
+ * reduction sums fixed-size chunks in a fixed order, so results do not
+ * depend on the thread count. */
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+
+int orc_get_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
 static inline double blk(const double* m, int i, int j) { return m[i + 3 * j]; }
 
 static inline int pad_to(int v, int bank) { return (v + bank - 1) / bank * bank; }
@@ -864,9 +890,12 @@ int orc_mas_invert(orc_mas* m, int method)
     const int    banks = m->total_clusters / bank;
     const size_t bsz   = (size_t)m->nblk * 9;
     free(m->inverse);
-    m->inverse     = (float*)calloc((size_t)banks * bsz, sizeof(float));
-    double* full   = (double*)malloc(sizeof(double) * dim * dim);
-    double* work   = (double*)malloc(sizeof(double) * (dim + dim * dim));
+    m->inverse = (float*)calloc((size_t)banks * bsz, sizeof(float));
+#pragma omp parallel
+    {
+    double* full = (double*)malloc(sizeof(double) * dim * dim);
+    double* work = (double*)malloc(sizeof(double) * (dim + dim * dim));
+#pragma omp for schedule(dynamic, 4)
     for(int b = 0; b < banks; ++b)
     {
         expand_packed(m->local + (size_t)b * bsz, bank, full);
@@ -886,6 +915,7 @@ int orc_mas_invert(orc_mas* m, int method)
     }
     free(full);
     free(work);
+    }
     return 0;
 }
 
@@ -919,6 +949,7 @@ void orc_mas_apply(const orc_mas* m, const double* r, double* z)
     double*      R     = (double*)calloc((size_t)total * 3, sizeof(double));
     double*      Z     = (double*)calloc((size_t)total * 3, sizeof(double));
     /* restriction: level 0 is the slot copy, each coarse slot sums its members */
+#pragma omp parallel for schedule(static)
     for(int s = 0; s < m->n_slots; ++s)
     {
         const int v = m->part_to_real[s];
@@ -932,6 +963,7 @@ void orc_mas_apply(const orc_mas* m, const double* r, double* z)
             for(int d = 0; d < 3; ++d) R[3 * c + d] += r[3 * v + d];
         }
     /* _schwarzLocalXSym6: per bank, Z = Minv_sym * R */
+#pragma omp parallel for schedule(static)
     for(int b = 0; b < total / bank; ++b)
     {
         const float* inv = m->inverse + (size_t)b * bsz;
@@ -950,6 +982,7 @@ void orc_mas_apply(const orc_mas* m, const double* r, double* z)
             }
     }
     /* __collectFinalZ_new */
+#pragma omp parallel for schedule(static)
     for(int v = 0; v < m->n_fem; ++v)
     {
         const int s = m->real_to_part[v];
@@ -1054,11 +1087,98 @@ void orc_diag3_inverse(int block_rows, int nb, const double* vals, const int* ro
 /* PCG                                                                      */
 /* ------------------------------------------------------------------------ */
 
+/* Sum of a[i] b[i]: partial sums over fixed chunks of 4,096 entries (in
+ * parallel), then the partials in chunk order. */
 static double dot(const double* a, const double* b, int n)
 {
+    enum { CH = 4096 };
+    const int nc = (n + CH - 1) / CH;
+    double    stackbuf[64];
+    double*   part = nc <= 64 ? stackbuf : (double*)malloc(sizeof(double) * (size_t)nc);
+#pragma omp parallel for schedule(static) if(nc > 1)
+    for(int c = 0; c < nc; ++c)
+    {
+        double    s  = 0.0;
+        const int e1 = (c + 1) * CH < n ? (c + 1) * CH : n;
+        for(int i = c * CH; i < e1; ++i) s += a[i] * b[i];
+        part[c] = s;
+    }
     double s = 0.0;
-    for(int i = 0; i < n; ++i) s += a[i] * b[i];
+    for(int c = 0; c < nc; ++c) s += part[c];
+    if(part != stackbuf) free(part);
     return s;
+}
+
+/* The symmetric upper-BCOO product of orc_sym_bcoo_spmv (a = 1, b = 0) as a
+ * row-parallel loop: row i's contributions (B x_j of its upper blocks, B^T
+ * x_k of the blocks (k, i) above it) are kept in increasing block index, the
+ * order in which the serial loop adds them, so y is bitwise the same. */
+typedef struct
+{
+    int  n_rows;
+    int* ptr;  /* n_rows + 1 */
+    int* ent;  /* block index << 1 | transposed */
+} row_spmv;
+
+static void row_spmv_build(row_spmv* R, int n_rows, int nb, const int* rows, const int* cols)
+{
+    R->n_rows = n_rows;
+    R->ptr    = (int*)calloc((size_t)n_rows + 1, sizeof(int));
+    for(int k = 0; k < nb; ++k)
+    {
+        R->ptr[rows[k] + 1]++;
+        if(rows[k] != cols[k]) R->ptr[cols[k] + 1]++;
+    }
+    for(int i = 0; i < n_rows; ++i) R->ptr[i + 1] += R->ptr[i];
+    R->ent   = (int*)malloc(sizeof(int) * ((size_t)R->ptr[n_rows] + 1));
+    int* pos = (int*)malloc(sizeof(int) * ((size_t)n_rows + 1));
+    memcpy(pos, R->ptr, sizeof(int) * ((size_t)n_rows + 1));
+    for(int k = 0; k < nb; ++k)
+    {
+        R->ent[pos[rows[k]]++] = k << 1;
+        if(rows[k] != cols[k]) R->ent[pos[cols[k]]++] = (k << 1) | 1;
+    }
+    free(pos);
+}
+
+static void row_spmv_free(row_spmv* R)
+{
+    free(R->ptr);
+    free(R->ent);
+}
+
+static void row_spmv_apply(const row_spmv* R, const double* vals, const int* rows, const int* cols,
+                           const double* x, double* y)
+{
+#pragma omp parallel for schedule(static, 256)
+    for(int i = 0; i < R->n_rows; ++i)
+    {
+        double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+        for(int e = R->ptr[i]; e < R->ptr[i + 1]; ++e)
+        {
+            const int     k = R->ent[e] >> 1;
+            const double* B = vals + 9 * (size_t)k;
+            double        s[3];
+            if(R->ent[e] & 1)
+            {
+                const int c = rows[k];
+                for(int r = 0; r < 3; ++r)
+                    s[r] = blk(B, 0, r) * x[3 * c] + blk(B, 1, r) * x[3 * c + 1] + blk(B, 2, r) * x[3 * c + 2];
+            }
+            else
+            {
+                const int c = cols[k];
+                for(int r = 0; r < 3; ++r)
+                    s[r] = blk(B, r, 0) * x[3 * c] + blk(B, r, 1) * x[3 * c + 1] + blk(B, r, 2) * x[3 * c + 2];
+            }
+            y0 += s[0];
+            y1 += s[1];
+            y2 += s[2];
+        }
+        y[3 * i] = y0;
+        y[3 * i + 1] = y1;
+        y[3 * i + 2] = y2;
+    }
 }
 
 typedef struct
@@ -1075,6 +1195,7 @@ static void apply_precond(const precond_ctx* p, const double* r, double* z)
     const int dof = 3 * p->block_rows;
     if(p->global_diag)
     {
+#pragma omp parallel for schedule(static)
         for(int i = 0; i < p->block_rows; ++i)
             for(int a = 0; a < 3; ++a)
             {
@@ -1122,6 +1243,8 @@ int orc_pcg(int block_rows, int nb, const double* vals, const int* rows, const i
     double* z  = (double*)malloc(sizeof(double) * dof);
     double* p  = (double*)malloc(sizeof(double) * dof);
     double* Ap = (double*)malloc(sizeof(double) * dof);
+    row_spmv A;
+    row_spmv_build(&A, block_rows, nb, rows, cols);
     if(max_iter <= 0) max_iter = (int)(0.3 * (double)dof);
     memset(x, 0, sizeof(double) * dof);
     memcpy(r, b, sizeof(double) * dof);
@@ -1137,9 +1260,10 @@ int orc_pcg(int block_rows, int nb, const double* vals, const int* rows, const i
         s.rz0 = rz0;
         for(k = 1; k < max_iter; ++k)
         {
-            orc_sym_bcoo_spmv(nb, vals, rows, cols, 1.0, p, 0.0, Ap, dof);
+            row_spmv_apply(&A, vals, rows, cols, p, Ap);
             const double pAp   = dot(p, Ap, dof);
             const double alpha = rz / pAp;
+#pragma omp parallel for schedule(static)
             for(int i = 0; i < dof; ++i)
             {
                 x[i] += alpha * p[i];
@@ -1149,6 +1273,7 @@ int orc_pcg(int block_rows, int nb, const double* vals, const int* rows, const i
             apply_precond(&pc, r, z);
             const double rz_new = dot(r, z, dof);
             const double beta   = rz_new / rz;
+#pragma omp parallel for schedule(static)
             for(int i = 0; i < dof; ++i) p[i] = z[i] + beta * p[i];
             rz = rz_new;
         }
@@ -1166,10 +1291,11 @@ int orc_pcg(int block_rows, int nb, const double* vals, const int* rows, const i
         while(k < max_iter && residual2 > target)
         {
             if(!isfinite(rz) || rz <= 0) { s.failure = 1; break; }
-            orc_sym_bcoo_spmv(nb, vals, rows, cols, 1.0, p, 0.0, Ap, dof);
+            row_spmv_apply(&A, vals, rows, cols, p, Ap);
             const double curv = dot(p, Ap, dof);
             if(!isfinite(curv) || curv <= 0) { s.failure = 2; break; }
             const double alpha = rz / curv;
+#pragma omp parallel for schedule(static)
             for(int i = 0; i < dof; ++i)
             {
                 x[i] += alpha * p[i];
@@ -1183,6 +1309,7 @@ int orc_pcg(int block_rows, int nb, const double* vals, const int* rows, const i
             const double next_rz = dot(r, z, dof);
             if(!isfinite(next_rz) || next_rz <= 0 || fabs(rz) < 1e-30) { s.failure = 1; break; }
             const double beta = next_rz / rz;
+#pragma omp parallel for schedule(static)
             for(int i = 0; i < dof; ++i) p[i] = z[i] + beta * p[i];
             rz = next_rz;
         }
@@ -1192,12 +1319,13 @@ int orc_pcg(int block_rows, int nb, const double* vals, const int* rows, const i
     /* residual report */
     const double bb = dot(b, b, dof);
     s.rec_rel_res   = bb > 0 ? sqrt(dot(r, r, dof) / bb) : 0.0;
-    orc_sym_bcoo_spmv(nb, vals, rows, cols, 1.0, x, 0.0, Ap, dof);
-    double t2 = 0.0;
-    for(int i = 0; i < dof; ++i) t2 += (b[i] - Ap[i]) * (b[i] - Ap[i]);
+    row_spmv_apply(&A, vals, rows, cols, x, Ap);
+    for(int i = 0; i < dof; ++i) Ap[i] = b[i] - Ap[i];
+    const double t2 = dot(Ap, Ap, dof);
     s.true_rel_res = bb > 0 ? sqrt(t2 / bb) : sqrt(t2);
     s.iterations   = k;
     if(st) *st = s;
+    row_spmv_free(&A);
     free(r); free(z); free(p); free(Ap); free(diag);
     return k;
 }

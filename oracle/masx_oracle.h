@@ -38,6 +38,11 @@
 extern "C" {
 #endif
 
+/* Threads of the oracle's parallel loops (OpenMP); results do not depend on
+ * the count.  Test infrastructure and the CPU baseline's thread setting. */
+void orc_set_threads(int n);
+int  orc_get_threads(void);
+
 /* y = a*(U + U^T - D) x + b*y over block triplets (any triangle), where each
  * block (i,j) adds B x_j to y_i and, for i != j, B^T x_i to y_j.
  * linear_system/utils/spmv.cu:28-53 (legacy) and :55-118 (SRBK). */

@@ -41,6 +41,8 @@ class PcgStats(C.Structure):
 
 def _declare(L):
     dp, ip, up, fp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_uint), C.POINTER(C.c_float)
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_get_threads.argtypes = []
     L.orc_sym_bcoo_spmv.argtypes = [C.c_int, dp, ip, ip, C.c_double, dp, C.c_double, dp, C.c_int]
     L.orc_morton_partition.argtypes = [C.c_int, dp, C.c_int, ip, ip]
     L.orc_mesh_neighbors.argtypes = [C.c_int, C.c_int, ip, C.c_int, ip, up, up, up, C.c_int]
@@ -77,6 +79,15 @@ def _i32(a):
 
 def _u32(a):
     return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def set_threads(n):
+    """Threads of the oracle's parallel loops (results do not depend on it)."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return lib().orc_get_threads()
 
 
 def sym_spmv(vals, rows, cols, x, a=1.0, b=0.0, y=None):

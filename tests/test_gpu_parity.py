@@ -268,6 +268,40 @@ def test_mixed_precision_converges(precision):
     assert st["iterations"] <= 1.5 * ref["iterations"] + 2
 
 
+def half_ulp(y, precision):
+    """Spacing of the FP16 / BF16 grid at |y| (subnormal FP16 spacing below 2^-14)."""
+    a = np.abs(np.asarray(y, np.float64))
+    if precision == "fp16":
+        return 2.0 ** (np.floor(np.log2(np.maximum(a, 2.0 ** -14))) - 10)
+    return 2.0 ** (np.floor(np.log2(np.maximum(a, 2.0 ** -126))) - 7)
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_half_inverse_and_apply_match_the_oracle(oracle, precision):
+    """Each bank's inverse is stored as half(v / s) with s its largest entry in
+    magnitude: every stored value lies within one half-precision ulp (at the
+    bank scale) of the oracle's inverse, and the half apply (FP32
+    accumulation) matches an FP64 apply of that stored inverse to 1e-4."""
+    w = W.fem_grid(14, n_contacts=400, seed=14, young_sigma=0.5)
+    s = session.solver_for(w)
+    s.build(precision)
+    assert s.factor_status() == 0
+    mas = oracle_mas(oracle, w)
+    got = s.inverse().astype(np.float64)  # banks x blocks x 9, block row-major
+    nbk, nblk = got.shape[0], got.shape[1]
+    ref = mas.inverse().reshape(nbk, nblk, 3, 3).transpose(0, 1, 3, 2).reshape(got.shape).astype(np.float64)
+    scale = np.abs(ref).reshape(nbk, -1).max(axis=1)[:, None, None]
+    tol = half_ulp(ref / scale, precision) * scale
+    bad = np.abs(got - ref) > tol
+    assert not bad.any(), (int(bad.sum()), float(np.abs(got - ref)[bad].max()))
+    mas.set_inverse(got.reshape(nbk, nblk, 3, 3).transpose(0, 1, 3, 2).reshape(nbk, nblk, 9).astype(np.float32))
+    r = np.random.default_rng(6).standard_normal(3 * w["block_rows"])
+    z = torch.zeros_like(dev(r))
+    s.apply_preconditioner(dev(r), z)
+    want = mas.apply(r)
+    assert np.linalg.norm(host(z) - want) <= 1e-4 * np.linalg.norm(want)
+
+
 def test_graph_and_eager_agree_and_are_deterministic():
     w = W.fem_grid(12, n_contacts=200, seed=7, young_sigma=0.5)
     s = session.solver_for(w)
