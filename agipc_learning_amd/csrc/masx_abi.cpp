@@ -516,6 +516,15 @@ int masx_morton_partition_device(masx_morton* m, int n, const double* d_position
     });
 }
 
+int masx_morton_partition_device_noalloc(int n, const double* d_positions, int bank, int* d_part_to_real,
+                                         long long part_cap, int* d_real_to_part, void* stream)
+{
+    return guard([&] {
+        masx::morton_partition_noalloc(n, d_positions, bank, d_part_to_real, part_cap, d_real_to_part,
+                                       (hipStream_t)stream);
+    });
+}
+
 int masx_spmv_layout_check(int n_rows, long long nb, const int* rows, const int* cols, int entries_per_thread,
                            unsigned seed, double* max_rel_err, long long* n_entries, long long* n_in_tile)
 {

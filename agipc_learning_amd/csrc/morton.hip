@@ -11,6 +11,7 @@
 // eight ballots.  Integer work, latency-bound at mesh sizes.
 #include "morton.hpp"
 
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -252,6 +253,111 @@ void grow(T*& p, size_t& cap, size_t need)
     cap = need;
 }
 
+// Where one partition keeps its intermediate arrays.
+struct MtScratch
+{
+    double*   box;      // 6 * (g + 1): per-block partial bounds, then the final 6
+    uint32_t* key[2];   // n each, ping-pong
+    int*      idx[2];   // n each, ping-pong
+    int*      hist;     // kBins * n_tiles, scanned in place
+};
+
+struct MtShape
+{
+    long long padded;
+    int       n_tiles, g;
+};
+
+MtShape mt_shape(int n, const double* pos, int bank, const int* part_to_real, long long part_cap,
+                 const int* real_to_part)
+{
+    if(n < 1 || bank < 1 || !pos || !part_to_real || !real_to_part)
+        throw std::invalid_argument("morton partition: n >= 1, bank >= 1 and device arrays required");
+    MtShape sh;
+    sh.padded = ((long long)n + bank - 1) / bank * bank;
+    if(part_cap < sh.padded) throw std::invalid_argument("part_to_real buffer smaller than the padded vertex count");
+    if(sh.padded > 0x7fffffffLL) throw std::invalid_argument("too many vertices");
+    sh.n_tiles = (int)(((long long)n + kTile - 1) / kTile);
+    sh.g       = (int)std::min<long long>(kBoxMax, ((long long)n + kThreads - 1) / kThreads);
+    return sh;
+}
+
+// The launch sequence: every scratch word a kernel reads was written by an
+// earlier kernel of the same sequence, so scratch needs no clearing.
+void mt_launch(int n, const double* pos, int* part_to_real, int* real_to_part, hipStream_t st, const MtShape& sh,
+               const MtScratch& s)
+{
+    const unsigned blocks = (unsigned)(((long long)n + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(k_mt_box_partial, dim3(sh.g), dim3(kThreads), 0, st, n, pos, s.box);
+    MT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_mt_box_final, dim3(1), dim3(kThreads), 0, st, sh.g, s.box);
+    MT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_mt_codes, dim3(blocks), dim3(kThreads), 0, st, n, pos, s.box + 6 * sh.g, s.key[0], s.idx[0]);
+    MT_HIP(hipGetLastError());
+    int cur = 0;
+    for(int p = 0; p < kPasses; ++p)
+    {
+        const int shift = 8 * p;
+        hipLaunchKernelGGL(k_mt_hist, dim3(sh.n_tiles), dim3(kThreads), 0, st, n, s.key[cur], shift, s.hist,
+                           sh.n_tiles);
+        MT_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_mt_scan, dim3(1), dim3(kScan), 0, st, kBins * sh.n_tiles, s.hist);
+        MT_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_mt_scatter, dim3(sh.n_tiles), dim3(kThreads), 0, st, n, s.key[cur], s.idx[cur],
+                           s.key[1 - cur], s.idx[1 - cur], shift, s.hist, sh.n_tiles);
+        MT_HIP(hipGetLastError());
+        cur = 1 - cur;
+    }
+    hipLaunchKernelGGL(k_mt_maps, dim3((unsigned)((sh.padded + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, n,
+                       (int)sh.padded, s.idx[cur], part_to_real, real_to_part);
+    MT_HIP(hipGetLastError());
+}
+
+// ---- the allocation-free variant's scratch ---------------------------------
+// A static device array holds the scratch of morton_partition_noalloc: the
+// box partials (6 * (kBoxMax + 1) doubles), two key and two index arrays of
+// kMortonNoAllocMax words and the digit counts of kMortonNoAllocMax / kTile
+// tiles, 8.7 MB in all, kept by every GPU the library runs on until the
+// process ends.
+constexpr size_t kAlign256  = 256;
+constexpr size_t round256(size_t b) { return (b + kAlign256 - 1) / kAlign256 * kAlign256; }
+constexpr size_t kNaBoxOff  = 0;
+constexpr size_t kNaKeyOff0 = round256(sizeof(double) * 6 * (kBoxMax + 1));
+constexpr size_t kNaArr     = round256(sizeof(uint32_t) * (size_t)kMortonNoAllocMax);
+constexpr size_t kNaKeyOff1 = kNaKeyOff0 + kNaArr;
+constexpr size_t kNaIdxOff0 = kNaKeyOff1 + kNaArr;
+constexpr size_t kNaIdxOff1 = kNaIdxOff0 + kNaArr;
+constexpr size_t kNaHistOff = kNaIdxOff1 + kNaArr;
+constexpr size_t kNaBytes   = kNaHistOff + round256(sizeof(int) * (size_t)kBins * (kMortonNoAllocMax / kTile));
+static_assert(kMortonNoAllocMax % kTile == 0, "the static scratch holds whole scatter tiles");
+
+static __device__ __align__(256) uint32_t scratch[kNaBytes / sizeof(uint32_t)];
+
+constexpr int kMaxDevices = 64;
+std::mutex    g_scratch_mutex;
+char*         g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up on the
+// first call there.  The array's host-side name is only a handle for the
+// runtime: kernels get this address, never the name.
+char* noalloc_scratch()
+{
+    int dev = -1;
+    MT_HIP(hipGetDevice(&dev));
+    if(dev < 0 || dev >= kMaxDevices) throw std::runtime_error("morton partition: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if(!g_scratch_ptr[dev])
+    {
+        void*            ptr = nullptr;
+        const hipError_t e   = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if(e != hipSuccess || !ptr)
+            throw std::runtime_error(std::string("morton partition: static scratch address: ") +
+                                     hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<char*>(ptr);
+    }
+    return g_scratch_ptr[dev];
+}
+
 }  // namespace
 
 DeviceMorton::~DeviceMorton()
@@ -264,43 +370,30 @@ DeviceMorton::~DeviceMorton()
 void DeviceMorton::partition(int n, const double* pos, int bank, int* part_to_real, long long part_cap,
                              int* real_to_part, hipStream_t st)
 {
-    if(n < 1 || bank < 1 || !pos || !part_to_real || !real_to_part)
-        throw std::invalid_argument("morton partition: n >= 1, bank >= 1 and device arrays required");
-    const long long padded = ((long long)n + bank - 1) / bank * bank;
-    if(part_cap < padded) throw std::invalid_argument("part_to_real buffer smaller than the padded vertex count");
-    if(padded > 0x7fffffffLL) throw std::invalid_argument("too many vertices");
-    const int n_tiles = (int)(((long long)n + kTile - 1) / kTile);
-    const int g = (int)std::min<long long>(kBoxMax, ((long long)n + kThreads - 1) / kThreads);
-    grow(box_, box_cap_, 6 * (size_t)(g + 1));
+    const MtShape sh = mt_shape(n, pos, bank, part_to_real, part_cap, real_to_part);
+    grow(box_, box_cap_, 6 * (size_t)(sh.g + 1));
     for(int b = 0; b < 2; ++b)
     {
         grow(key_[b], key_cap_[b], (size_t)n);
         grow(idx_[b], idx_cap_[b], (size_t)n);
     }
-    grow(hist_, hist_cap_, (size_t)kBins * n_tiles);
-    const unsigned blocks = (unsigned)(((long long)n + kThreads - 1) / kThreads);
-    hipLaunchKernelGGL(k_mt_box_partial, dim3(g), dim3(kThreads), 0, st, n, pos, box_);
-    MT_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_mt_box_final, dim3(1), dim3(kThreads), 0, st, g, box_);
-    MT_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_mt_codes, dim3(blocks), dim3(kThreads), 0, st, n, pos, box_ + 6 * g, key_[0], idx_[0]);
-    MT_HIP(hipGetLastError());
-    int cur = 0;
-    for(int p = 0; p < kPasses; ++p)
-    {
-        const int shift = 8 * p;
-        hipLaunchKernelGGL(k_mt_hist, dim3(n_tiles), dim3(kThreads), 0, st, n, key_[cur], shift, hist_, n_tiles);
-        MT_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_mt_scan, dim3(1), dim3(kScan), 0, st, kBins * n_tiles, hist_);
-        MT_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_mt_scatter, dim3(n_tiles), dim3(kThreads), 0, st, n, key_[cur], idx_[cur], key_[1 - cur],
-                           idx_[1 - cur], shift, hist_, n_tiles);
-        MT_HIP(hipGetLastError());
-        cur = 1 - cur;
-    }
-    hipLaunchKernelGGL(k_mt_maps, dim3((unsigned)((padded + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, n,
-                       (int)padded, idx_[cur], part_to_real, real_to_part);
-    MT_HIP(hipGetLastError());
+    grow(hist_, hist_cap_, (size_t)kBins * sh.n_tiles);
+    mt_launch(n, pos, part_to_real, real_to_part, st, sh, MtScratch{box_, {key_[0], key_[1]}, {idx_[0], idx_[1]}, hist_});
+}
+
+void morton_partition_noalloc(int n, const double* pos, int bank, int* part_to_real, long long part_cap,
+                              int* real_to_part, hipStream_t st)
+{
+    const MtShape sh = mt_shape(n, pos, bank, part_to_real, part_cap, real_to_part);
+    if(n > kMortonNoAllocMax)
+        throw std::invalid_argument("morton partition without allocation: at most " +
+                                    std::to_string(kMortonNoAllocMax) + " vertices (got " + std::to_string(n) + ")");
+    char* base = noalloc_scratch();
+    const MtScratch s{reinterpret_cast<double*>(base + kNaBoxOff),
+                      {reinterpret_cast<uint32_t*>(base + kNaKeyOff0), reinterpret_cast<uint32_t*>(base + kNaKeyOff1)},
+                      {reinterpret_cast<int*>(base + kNaIdxOff0), reinterpret_cast<int*>(base + kNaIdxOff1)},
+                      reinterpret_cast<int*>(base + kNaHistOff)};
+    mt_launch(n, pos, part_to_real, real_to_part, st, sh, s);
 }
 
 }  // namespace masx

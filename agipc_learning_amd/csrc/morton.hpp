@@ -8,6 +8,9 @@
 
 namespace masx
 {
+// Largest vertex count morton_partition_noalloc takes (its static scratch).
+constexpr int kMortonNoAllocMax = 1 << 19;
+
 class DeviceMorton
 {
   public:
@@ -30,4 +33,11 @@ class DeviceMorton
     int*      part_ = nullptr;   // scan partials
     size_t box_cap_ = 0, key_cap_[2] = {0, 0}, idx_cap_[2] = {0, 0}, hist_cap_ = 0, part_cap_ = 0;
 };
+
+// DeviceMorton::partition without allocating: the scratch is a static
+// device array of the library (n <= kMortonNoAllocMax), one per GPU, shared
+// by every call on that GPU, so calls that overlap on one GPU race.  No
+// hipMalloc, hipFree or memset; asynchronous on `st`.
+void morton_partition_noalloc(int n, const double* pos, int bank, int* part_to_real, long long part_cap,
+                              int* real_to_part, hipStream_t st);
 }  // namespace masx

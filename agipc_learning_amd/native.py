@@ -94,6 +94,7 @@ def load():
         "masx_morton_create": ([C.POINTER(vp)], C.c_int),
         "masx_morton_destroy": ([vp], C.c_int),
         "masx_morton_partition_device": ([vp, C.c_int, vp, C.c_int, vp, C.c_longlong, vp, vp], C.c_int),
+        "masx_morton_partition_device_noalloc": ([C.c_int, vp, C.c_int, vp, C.c_longlong, vp, vp], C.c_int),
         "masx_spmv_layout_check": ([C.c_int, C.c_longlong, ip, ip, C.c_int, C.c_uint, dp, C.POINTER(C.c_longlong),
                                     C.POINTER(C.c_longlong)], C.c_int),
         "masx_analyze": ([vp], C.c_int),
@@ -236,6 +237,28 @@ class DeviceMorton:
         check(self.L.masx_morton_partition_device(self.h, n, _dev(pos), bank, _dev(p2r), padded, _dev(r2p),
                                                   stream))
         return p2r, r2p
+
+
+MORTON_NOALLOC_MAX = 524288  # MASX_MORTON_NOALLOC_MAX
+
+
+def morton_partition_noalloc(positions, part_to_real, real_to_part, bank=32, stream=None):
+    """masx_morton_partition_device_noalloc: the device Morton partition into
+    the caller's int32 CUDA tensors part_to_real (>= n rounded up to the bank)
+    and real_to_part (n); the library allocates nothing (static scratch, at
+    most MORTON_NOALLOC_MAX vertices)."""
+    import torch
+
+    pos = positions
+    if pos.dtype != torch.float64 or not pos.is_cuda or pos.dim() != 2 or pos.shape[1] != 3 \
+            or not pos.is_contiguous():
+        raise ValueError("positions must be a contiguous CUDA float64 tensor of shape (n, 3)")
+    n = pos.shape[0]
+    for t, need, name in ((part_to_real, (n + bank - 1) // bank * bank, "part_to_real"), (real_to_part, n, "real_to_part")):
+        if t.dtype != torch.int32 or not t.is_cuda or not t.is_contiguous() or t.numel() < need:
+            raise ValueError(f"{name}: expected a contiguous CUDA int32 tensor of at least {need} values")
+    check(load().masx_morton_partition_device_noalloc(n, _dev(pos), bank, _dev(part_to_real), part_to_real.numel(),
+                                                      _dev(real_to_part), stream))
 
 
 def spmv_layout_check(n_rows, rows, cols, entries_per_thread, seed=0):

@@ -220,6 +220,17 @@ int masx_morton_create(masx_morton** out);
 int masx_morton_destroy(masx_morton* m);
 int masx_morton_partition_device(masx_morton* m, int n, const double* d_positions, int bank, int* d_part_to_real,
                                  long long part_cap, int* d_real_to_part, void* stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): masx_morton_partition_device with the same arguments and
+ * results, minus the object, that allocates nothing.  Its scratch is a static
+ * device array of the library, 8.7 MB on every GPU it runs on, held until the
+ * process ends; n <= MASX_MORTON_NOALLOC_MAX, larger n is an error.  Every
+ * call on one GPU shares that array, so calls that overlap on one GPU (other
+ * streams, other host threads) race. */
+#define MASX_MORTON_NOALLOC_MAX 524288
+int masx_morton_partition_device_noalloc(int n, const double* d_positions, int bank, int* d_part_to_real,
+                                         long long part_cap, int* d_real_to_part, void* stream);
 
 /* Check of the FP32 SpMV's tile layout on the host (no device): from the
  * upper block structure (rows[k] <= cols[k], unique) of one system of n_rows
