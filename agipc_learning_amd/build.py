@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MASX_ARCH", "gfx950")
 
-MASX_SOURCES = ["kernels.hip", "converter.hip", "hierarchy.hip", "fem.hip", "morton.hip", "plan.cpp", "solver.cpp",
+MASX_SOURCES = ["kernels.hip", "converter.hip", "hierarchy.hip", "fem.hip", "morton.hip", "bandwidth.hip", "plan.cpp", "solver.cpp",
                 "masx_abi.cpp"]
 GEN_SOURCES = ["gen/masx_gen.cpp"]
 

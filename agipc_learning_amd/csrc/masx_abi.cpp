@@ -16,6 +16,7 @@
 #include "fem.hpp"
 #include "hierarchy.hpp"
 #include "morton.hpp"
+#include "bandwidth.hpp"
 #include "solver.hpp"
 
 namespace
@@ -513,6 +514,14 @@ int masx_morton_partition_device(masx_morton* m, int n, const double* d_position
     return guard([&] {
         require(m != nullptr, "bad argument");
         m->impl.partition(n, d_positions, bank, d_part_to_real, part_cap, d_real_to_part, (hipStream_t)stream);
+    });
+}
+
+int masx_stream_rate(long long bytes, int kind, int reps, double* gb_per_s)
+{
+    return guard([&] {
+        require(gb_per_s != nullptr, "bad argument");
+        *gb_per_s = masx::stream_rate(bytes, kind, reps);
     });
 }
 

@@ -95,6 +95,7 @@ def load():
         "masx_morton_destroy": ([vp], C.c_int),
         "masx_morton_partition_device": ([vp, C.c_int, vp, C.c_int, vp, C.c_longlong, vp, vp], C.c_int),
         "masx_morton_partition_device_noalloc": ([C.c_int, vp, C.c_int, vp, C.c_longlong, vp, vp], C.c_int),
+        "masx_stream_rate": ([C.c_longlong, C.c_int, C.c_int, C.POINTER(C.c_double)], C.c_int),
         "masx_spmv_layout_check": ([C.c_int, C.c_longlong, ip, ip, C.c_int, C.c_uint, dp, C.POINTER(C.c_longlong),
                                     C.POINTER(C.c_longlong)], C.c_int),
         "masx_analyze": ([vp], C.c_int),
@@ -237,6 +238,15 @@ class DeviceMorton:
         check(self.L.masx_morton_partition_device(self.h, n, _dev(pos), bank, _dev(p2r), padded, _dev(r2p),
                                                   stream))
         return p2r, r2p
+
+
+def stream_rate(nbytes, kind="read", reps=10):
+    """Practical streaming rate of the GPU in GB/s (masx_stream_rate): a plain
+    read-only stream ("read", bytes read) or copy ("copy", bytes read +
+    written) over nbytes of device memory, fastest of reps launches."""
+    out = C.c_double()
+    check(load().masx_stream_rate(int(nbytes), {"read": 0, "copy": 1}[kind], reps, C.byref(out)))
+    return out.value
 
 
 MORTON_NOALLOC_MAX = 524288  # MASX_MORTON_NOALLOC_MAX

@@ -1,17 +1,20 @@
 """Benchmark of the MAS-PCG Newton-step linear solve on one MI355X.
 
-Workload (default): 40x40x40 hex grid (68,921 vertices, 206,763 DOF, 384,000
-tets), stable Neo-Hookean Hessian with lognormal E (median 1e5 Pa, sigma 0.5),
-20,000 point-triangle contact pairs, 32-vertex Morton subdomains, 4 MAS
-levels, tolerance 1e-4 on the FP64 true relative residual.  Synthetic data,
-seed 0, generated in-process.
+Headline workload: 64 independent FEM + contact systems solved together
+(cubes whose vertex counts are drawn uniformly from 5k to 80k, seed 0:
+2,762,169 vertices, 8.29 M DOF; stable Neo-Hookean Hessians with lognormal E,
+5% of each cube's vertices in point-triangle contact pairs, 32-vertex Morton
+subdomains, reference level rule), tolerance 1e-4 on the FP64 true relative
+residual of every system.  Synthetic data, generated in-process.
 
-One step = one Newton step's linear solve with fixed structure: the new
-matrix values (already in HBM, caller layout) are taken in, the MAS
-preconditioner is rebuilt from them (FP64 Galerkin + Cholesky, inverse stored
-in the chosen precision) and PCG runs to the tolerance.  `value` is the plain
-FP32 step time; the mixed-precision (FP16 / BF16 inverse storage) times and
-their speed-up are reported beside it.
+One step = one Newton step's linear solves with fixed structure: the new
+matrix values (already in HBM, caller layout) are taken in, every system's
+MAS preconditioner is rebuilt from them (FP64 Galerkin + factorization,
+inverse stored in the chosen precision) and the batched PCG runs until every
+system meets the tolerance.  `value` is the plain FP32 step time; the
+mixed-precision (FP16 / BF16 inverse storage) times and their speed-ups are
+reported beside it, as are the 40^3 + contacts, hybrid ABD + FEM and 60^3
+adjoint workloads.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -36,26 +39,33 @@ def log(*a):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=20, help="timed steps of the headline (64-system) workload")
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=40, help="hex cells per edge")
-    ap.add_argument("--contacts", type=int, default=20000)
-    ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--tol", type=float, default=1e-4)
-    ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--precisions", default="fp32,fp16,bf16",
                     help="comma list; fp32 is always measured (it is the headline value)")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-batch", action="store_true", help="skip the batched 64-system measurement")
-    ap.add_argument("--batch-steps", type=int, default=5)
-    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-differentiated solve")
-    ap.add_argument("--adjoint-n", type=int, default=60, help="hex cells per edge of the adjoint workload")
+    ap.add_argument("--kernel-reps", type=int, default=20, help="launches per kernel timing window (batch)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines (and the oracle solution errors)")
+    ap.add_argument("--cpu-runs", type=int, default=3, help="timed repetitions of each CPU baseline")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="only the headline workload (no 40^3, hybrid, adjoint or structure-change legs)")
     ap.add_argument("--batch-only", action="store_true",
-                    help="only run the batched 64-system measurement and print its figures (profiling aid)")
+                    help="the headline workload's GPU figures only, no CPU leg (profiling aid)")
+    ap.add_argument("--fem40-only", action="store_true",
+                    help="only the 40^3 + contacts GPU figures (profiling aid)")
+    ap.add_argument("--hybrid-only", action="store_true", help="only the hybrid GPU figures (profiling aid)")
     ap.add_argument("--adjoint-only", action="store_true",
-                    help="only run the adjoint-differentiated solve and print its figures (profiling aid)")
+                    help="only the adjoint-differentiated solve and its figures (profiling aid)")
+    ap.add_argument("--n", type=int, default=40, help="hex cells per edge of the single-system workload")
+    ap.add_argument("--contacts", type=int, default=20000)
+    ap.add_argument("--levels", type=int, default=4)
+    ap.add_argument("--fem40-steps", type=int, default=20)
+    ap.add_argument("--batch-tiling", type=int, default=0, choices=(0, 2, 4, 8),
+                    help="SpMV entries per thread on the batch; 0 times every tiling and keeps the fastest")
     ap.add_argument("--spmv-tiling", type=int, default=0, choices=(0, 2, 4, 8),
-                    help="block entries per SpMV thread; 0 times every tiling and keeps the fastest")
+                    help="SpMV entries per thread on the 40^3 workload; 0 autotunes")
+    ap.add_argument("--adjoint-n", type=int, default=60, help="hex cells per edge of the adjoint workload")
+    ap.add_argument("--side-steps", type=int, default=5, help="timed steps of the hybrid and adjoint legs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -132,58 +142,151 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": 5}))
 
 
-def copy_ceiling():
-    """Practical HBM rate of this card: a plain 1 GiB device-to-device copy
-    (read + write bytes over its duration), fastest of 10, in GB/s."""
-    import torch
+def stream_ceilings():
+    """Practical memory rates of this card from the library's own streaming
+    kernels (masx_stream_rate): a read-only stream and a copy over 4 GiB (far
+    beyond the 256 MB Infinity Cache), fastest of 10 launches, GB/s."""
+    from agipc_learning_amd import native
 
-    n = 1 << 30
-    a = torch.ones(n // 4, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(3):
-        b.copy_(a)
-    best = float("inf")
-    for _ in range(10):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        b.copy_(a)
-        e1.record()
-        e1.synchronize()
-        best = min(best, e0.elapsed_time(e1) * 1e-3)
-    del a, b
-    torch.cuda.empty_cache()
-    return round(2 * n / best / 1e9, 1)
+    n = 4 << 30
+    return {"read_GBs": round(native.stream_rate(n, "read", 10), 1),
+            "copy_GBs": round(native.stream_rate(n, "copy", 10), 1), "bytes": n}
 
 
-def cpu_baseline(w, levels, tol):
-    """The FP64 oracle (a scalar C restatement of the reference algorithm, one
-    thread) on one step of the same workload: MAS assemble + Cholesky inverse +
-    PCG to the tolerance.  The hierarchy is built before the timed region, as
-    the GPU step reuses its structure analysis too."""
+def host_cpu():
+    """What the CPU legs run on: logical CPUs of the host, the CPUs this
+    process may use, and the model name."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "model": model}
+
+
+def oracle_module():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
 
     O.build()
+    return O
+
+
+def oracle_setup(O, w, levels=0):
+    """The oracle's MAS with its hierarchy (outside the timed region, as the
+    GPU step reuses its structure analysis too)."""
+    import numpy as np
+
     nv = w["n_vertices"]
     p2r, r2p = O.morton_partition(w["positions"])
     start, num, lst = O.mesh_neighbors(nv, w["tets"])
     mas = O.Mas(nv, len(p2r), start, num, lst, p2r, r2p, levels=levels)
     mas.reorder(w["pairs"])
-    t0 = time.perf_counter()
-    mas.assemble(w["vals"], w["rows"], w["cols"])
+    fem = (w["rows"] >= w["fem_offset"]) & (w["cols"] >= w["fem_offset"])
+    idx = None if fem.all() else np.nonzero(fem)[0].astype(np.uint32)
+    return mas, idx
+
+
+def oracle_step(O, w, mas, idx, tol):
+    """One step of the oracle (FP64 scalar C restatement of the reference
+    algorithm): MAS assembly + Cholesky-built inverse + PCG to the tolerance
+    (ABD 12x12 block-Jacobi on the bodies when there are any)."""
+    mas.assemble(w["vals"], w["rows"], w["cols"], indices=idx, offset=w["fem_offset"])
     mas.invert(1)
-    _, so = O.pcg(w["block_rows"], w["vals"], w["rows"], w["cols"], w["rhs"], mas=mas, tol=tol, rule=1)
-    dt = time.perf_counter() - t0
-    return {"value": dt * 1e3, "unit": "ms", "cores": 1, "kind": "port",
-            "sample": f"one full step of the same workload (MAS build + PCG, {so['iterations']} iterations, "
-                      f"true rel res {so['true_rel_res']:.2e}), FP64, single thread"}
+    abd = O.abd_block_inverse(w["n_bodies"], w["vals"], w["rows"], w["cols"]) if w["n_bodies"] else None
+    return O.pcg(w["block_rows"], w["vals"], w["rows"], w["cols"], w["rhs"], n_abd=w["n_bodies"], abd_inv=abd,
+                 mas=mas, fem_offset=w["fem_offset"], tol=tol, rule=1)
 
 
-def batch64(args):
-    """64 independent cube systems (vertex counts U[5k, 80k], seed 0; 5% of
-    vertices in contact pairs) solved together in one batched launch
-    sequence: plain FP32 vs FP16 inverse storage.  One step = values in +
-    build + solve of all 64 systems."""
+def stats_of(times_ms):
+    t = sorted(times_ms)
+    return {"median": round(t[len(t) // 2], 2), "min": round(t[0], 2), "max": round(t[-1], 2), "runs": len(t)}
+
+
+def cpu_batch(O, ws, tol, runs):
+    """The oracle on the whole 64-system batch on every thread it has, `runs`
+    times (each run = one step of all 64 systems: assembly + inverse + PCG),
+    and on one thread on a sample of the systems, scaled to the batch by
+    vertex count.  Also returns the oracle's solutions."""
+    threads = O.get_threads()
+    per_run = [0.0] * runs
+    sols, iters = [], []
+    single_sample = list(range(0, len(ws), 16))  # 4 of the 64 systems
+    single = [0.0] * runs
+    for i, w in enumerate(ws):
+        mas, idx = oracle_setup(O, w)
+        for r in range(runs):
+            t0 = time.perf_counter()
+            x, so = oracle_step(O, w, mas, idx, tol)
+            per_run[r] += (time.perf_counter() - t0) * 1e3
+        sols.append(x)
+        iters.append(so["iterations"])
+        if i in single_sample:
+            O.set_threads(1)
+            for r in range(runs):
+                t0 = time.perf_counter()
+                oracle_step(O, w, mas, idx, tol)
+                single[r] += (time.perf_counter() - t0) * 1e3
+            O.set_threads(threads)
+        del mas
+        if i % 16 == 15:
+            log(f"  cpu batch: {i + 1} systems")
+    nv = sum(w["n_vertices"] for w in ws)
+    nv_s = sum(ws[i]["n_vertices"] for i in single_sample)
+    scale = nv / nv_s
+    all_core = stats_of(per_run)
+    one = stats_of([t * scale for t in single])
+    return {"value": all_core["median"], "unit": "ms", "cores": threads, "kind": "port",
+            "sample": f"the whole 64-system step ({nv} vertices): MAS assembly + Cholesky-built inverse + PCG to "
+                      f"{tol:g} per system, FP64 oracle (scalar C restatement of the reference algorithm, OpenMP "
+                      f"loops over banks and rows on {threads} threads), {runs} runs; hierarchy built before the "
+                      "timed region",
+            "all_cores": dict(all_core, threads=threads),
+            "single_core": dict(one, threads=1, sample=f"systems {single_sample} ({nv_s} vertices) timed on one "
+                                f"thread, scaled to the batch by vertex count (x{scale:.2f})"),
+            "host": host_cpu()}, sols, iters
+
+
+def cpu_single(O, w, levels, tol, runs):
+    """The oracle on one step of a single-system workload, every thread and
+    one thread, `runs` times each; returns the oracle's solution too."""
+    mas, idx = oracle_setup(O, w, levels)
+    threads = O.get_threads()
+    times, x, so = [], None, None
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        x, so = oracle_step(O, w, mas, idx, tol)
+        times.append((time.perf_counter() - t0) * 1e3)
+    O.set_threads(1)
+    one = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        oracle_step(O, w, mas, idx, tol)
+        one.append((time.perf_counter() - t0) * 1e3)
+    O.set_threads(threads)
+    all_core = stats_of(times)
+    return {"value": all_core["median"], "unit": "ms", "cores": threads, "kind": "port",
+            "sample": f"one full step (MAS assembly + Cholesky-built inverse + PCG, {so['iterations']} iterations, "
+                      f"true rel res {so['true_rel_res']:.2e}), FP64 oracle, {runs} runs",
+            "all_cores": dict(all_core, threads=threads), "single_core": dict(stats_of(one), threads=1),
+            "oracle_iterations": so["iterations"]}, x
+
+
+def rel_err(a, b):
+    import numpy as np
+
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def batch_leg(args):
+    """The headline: 64 independent cube systems (vertex counts U[5k, 80k],
+    seed 0; 5% of vertices in contact pairs) solved together in one batched
+    launch sequence, plain FP32 vs FP16 / BF16 inverse storage.  One step =
+    values in + build + solve of all 64 systems."""
     import numpy as np
     import torch
     from agipc_learning_amd import session
@@ -193,81 +296,94 @@ def batch64(args):
     ws = W.batch64(seed=0)
     nv = sum(w["n_vertices"] for w in ws)
     log(f"step: batch64 analyze + setup ({nv} vertices, {sum(w['nb'] for w in ws)} upper blocks)")
+    t0 = time.perf_counter()
     s = session.solver_for(ws)
-    batch_tiling = 4  # SpMV entries per thread (the library default), pinned for the traffic lookup
-    s.set_spmv_tiling(batch_tiling)
+    setup_ms = (time.perf_counter() - t0) * 1e3
     vals = [torch.from_numpy(np.ascontiguousarray(w["vals"])).cuda() for w in ws]
     b = torch.from_numpy(session.rhs_of(ws)).cuda()
     x = torch.zeros_like(b)
-    out = {"systems": len(ws), "vertices": nv, "dof": 3 * nv, "upper_blocks": int(s.info()["nb_total"])}
-    for prec in ("fp32", "fp16"):
-        def step():
-            for i, v in enumerate(vals):
-                s.set_values(i, v)
-            s.build(prec)
-            return s.solve(b, x, rule=1, tol=args.tol)
-        step()
+    info = s.info()
+    s.build("fp32")
+    if args.batch_tiling:
+        s.set_spmv_tiling(args.batch_tiling)
+        tiling = {"entries_per_thread": args.batch_tiling, "ms": s.time_kernel(2, 20)}
+    else:
+        log("step: batch64 autotune SpMV tiling")
+        tiling = s.autotune(20)
+    log(f"  tiling {tiling}")
+
+    def step(prec):
+        for i, v in enumerate(vals):
+            s.set_values(i, v)
+        s.build(prec)
+        return s.solve(b, x, rule=1, tol=args.tol)
+
+    precs = ["fp32"] + [p for p in args.precisions.split(",") if p and p != "fp32"]
+    out = {"systems": len(ws), "vertices": nv, "dof": 3 * nv, "upper_blocks": int(info["nb_total"]),
+           "banks": int(info["n_banks"]), "setup_ms": round(setup_ms, 1), "spmv_tiling": tiling}
+    sols = {}
+    for prec in precs:
+        log(f"step: batch64 {prec} ({args.warmup} warmup + {args.steps} steps)")
+        for _ in range(args.warmup):
+            step(prec)
         torch.cuda.synchronize()
         t = time.perf_counter()
-        for _ in range(args.batch_steps):
-            st = step()
+        for _ in range(args.steps):
+            st = step(prec)
         torch.cuda.synchronize()
-        ms = (time.perf_counter() - t) * 1e3 / args.batch_steps
-        worst = max(x["rel_res_true"] for x in st)
-        fails = sum(1 for x in st if x["failure"] != 0)
-        its = [x["iterations"] for x in st]
-        out[prec] = {"ms": round(ms, 3), "max_iterations": max(its), "mean_iterations": round(sum(its) / len(its), 1),
-                     "worst_rel_res_true": worst, "failures": fails}
-        kern = {k: round(v, 2) for k, v in kernel_times(s, 20).items()}
-        alg = algorithmic_bytes(s.info(), nv, 4 if prec == "fp32" else 2)
-        out[prec]["kernels_us"] = kern
-        out[prec]["kernels_GBs"] = {k: round(alg[k] / (kern[k] * 1e-6) / 1e9, 1) for k in kern}
+        ms = (time.perf_counter() - t) * 1e3 / args.steps
+        sols[prec] = x.cpu().numpy().copy()
+        worst = max(q["rel_res_true"] for q in st)
+        fails = sum(1 for q in st if q["failure"] != 0 or q["status"] != 1)
+        if fails or worst > args.tol * (1 + 1e-6):
+            raise SystemExit(f"batch64 {prec}: {fails} systems failed, worst true rel res {worst:.3e}")
+        its = [q["iterations"] for q in st]
+        # split of the step: build alone, solve alone
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(3):
+            s.build(prec)
+        torch.cuda.synchronize()
+        build_ms = (time.perf_counter() - t) * 1e3 / 3
+        t = time.perf_counter()
+        for _ in range(3):
+            s.solve(b, x, rule=1, tol=args.tol)
+        torch.cuda.synchronize()
+        pcg_ms = (time.perf_counter() - t) * 1e3 / 3
+        kern = {k: round(v, 2) for k, v in kernel_times(s, args.kernel_reps).items()}
+        s.build(prec)  # time_kernel leaves the iteration state frozen
+        alg = algorithmic_bytes(info, nv, 4 if prec == "fp32" else 2)
+        out[prec] = {"ms": round(ms, 3), "build_ms": round(build_ms, 3), "pcg_ms": round(pcg_ms, 3),
+                     "max_iterations": max(its), "mean_iterations": round(sum(its) / len(its), 2),
+                     "iterations": its, "worst_rel_res_true": worst,
+                     "dof_iterations_per_s": round(sum(3 * w["n_vertices"] * q["iterations"]
+                                                       for w, q in zip(ws, st)) / (pcg_ms * 1e-3), 1),
+                     "kernels_us": kern,
+                     "kernels_GBs": {k: round(alg[k] / (kern[k] * 1e-6) / 1e9, 1) for k in kern}}
         if prec == "fp32":
-            # the clean HBM case: this working set is far beyond the 256 MB
-            # Infinity Cache, so its dominant kernel's rate is an HBM rate
             dom = max(kern, key=lambda k: kern[k])
             ach = alg[dom] / (kern[dom] * 1e-6) / 1e9
             out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                               "traffic": measured_traffic(dom, True, batch_tiling, "batch64_hbm_traffic.json"),
-                               "algorithmic_bytes": alg[dom], "avg_us": kern[dom]}
-        log(f"  batch64 {prec}: {ms:.2f} ms/step, iterations max {max(its)}, worst true rel res {worst:.2e}, "
-            f"kernels us {kern}, GB/s {out[prec]['kernels_GBs']}")
-    out["fp16_speedup_vs_fp32"] = round(out["fp32"]["ms"] / out["fp16"]["ms"], 3)
-    return out
+                               "traffic": measured_traffic(dom, True, tiling["entries_per_thread"],
+                                                           "batch64_hbm_traffic.json"),
+                               "algorithmic_bytes": alg[dom], "avg_us": kern[dom],
+                               "per_kernel": {k: {"us": kern[k], "GB/s": out[prec]["kernels_GBs"][k],
+                                                  "algorithmic_bytes": alg[k]} for k in kern}}
+        log(f"  batch64 {prec}: {ms:.2f} ms/step (build {build_ms:.2f}, pcg {pcg_ms:.2f}), iterations max "
+            f"{max(its)}, worst true rel res {worst:.2e}, kernels us {kern}, GB/s {out[prec]['kernels_GBs']}")
+    for prec in precs[1:]:
+        out[prec]["speedup_vs_fp32"] = round(out["fp32"]["ms"] / out[prec]["ms"], 3)
+    del s, vals, b, x
+    torch.cuda.empty_cache()
+    return out, ws, sols
 
 
-def hybrid_oracle(w, tol):
-    """The FP64 oracle's solution of the hybrid system (ABD block-Jacobi + MAS,
-    same algorithm), the reference the hybrid leg's error is measured against."""
-    import numpy as np
-    from agipc_learning_amd import session
-
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle as O
-
-    O.build()
-    nv = w["n_vertices"]
-    p2r, r2p = O.morton_partition(w["positions"])
-    st_, nu_, li_ = O.mesh_neighbors(nv, w["tets"])
-    mas = O.Mas(nv, len(p2r), st_, nu_, li_, p2r, r2p)
-    mas.reorder(w["pairs"])
-    fem = (w["rows"] >= w["fem_offset"]) & (w["cols"] >= w["fem_offset"])
-    mas.assemble(w["vals"], w["rows"], w["cols"], indices=np.nonzero(fem)[0].astype(np.uint32),
-                 offset=w["fem_offset"])
-    mas.invert(1)
-    abd = O.abd_block_inverse(w["n_bodies"], w["vals"], w["rows"], w["cols"])
-    xo, so = O.pcg(w["block_rows"], w["vals"], w["rows"], w["cols"], session.rhs_of(w), n_abd=w["n_bodies"],
-                   abd_inv=abd, mas=mas, fem_offset=w["fem_offset"], tol=tol, rule=1)
-    return xo, so["iterations"]
-
-
-def hybrid(args, w, oracle=None):
+def hybrid(args, w):
     """ABD + FEM system: 2,000 affine bodies (dense SPD 12x12 diagonal blocks),
     30,000 body-pair couplings, a 20^3 FEM block (9,261 vertices) coupled to
     200 bodies; ABD 12x12 block-Jacobi + MAS on the FEM part.  Plain FP32 vs
-    FP16 inverse storage; `oracle` = (FP64 oracle solution, its iterations)
-    from the CPU leg gives the solution error."""
+    FP16 / BF16 inverse storage.  Returns the figures and the solutions."""
     import numpy as np
     import torch
     from agipc_learning_amd import session
@@ -280,7 +396,7 @@ def hybrid(args, w, oracle=None):
     out = {"bodies": int(w["n_bodies"]), "fem_vertices": int(w["n_vertices"]), "dof": int(len(b)),
            "upper_blocks": int(s.info()["nb_total"])}
     xs = {}
-    for prec in ("fp32", "fp16"):
+    for prec in ("fp32", "fp16", "bf16"):
         def step():
             s.set_values(0, vals)
             s.build(prec)
@@ -288,22 +404,17 @@ def hybrid(args, w, oracle=None):
         step()
         torch.cuda.synchronize()
         t = time.perf_counter()
-        for _ in range(args.batch_steps):
+        for _ in range(args.side_steps):
             st = step()
         torch.cuda.synchronize()
-        ms = (time.perf_counter() - t) * 1e3 / args.batch_steps
+        ms = (time.perf_counter() - t) * 1e3 / args.side_steps
         xs[prec] = x.cpu().numpy().copy()
         out[prec] = {"ms": round(ms, 4), "iterations": st["iterations"], "rel_res_true": st["rel_res_true"],
                      "failure": st["failure"]}
+        if prec != "fp32":
+            out[prec]["speedup_vs_fp32"] = round(out["fp32"]["ms"] / ms, 3)
         log(f"  hybrid {prec}: {ms:.3f} ms/step, iterations {st['iterations']}, true rel res {st['rel_res_true']:.2e}")
-    out["fp16_speedup_vs_fp32"] = round(out["fp32"]["ms"] / out["fp16"]["ms"], 3)
-    if oracle is not None:
-        xo, it = oracle
-        for prec in xs:
-            out[prec]["solution_rel_err_vs_oracle"] = float(np.linalg.norm(xs[prec] - xo) / np.linalg.norm(xo))
-        out["oracle_iterations"] = it
-        log(f"  hybrid oracle: {it} iterations, errors {[out[p]['solution_rel_err_vs_oracle'] for p in xs]}")
-    return out
+    return out, xs
 
 
 def adjoint_leg(args):
@@ -342,7 +453,7 @@ def adjoint_leg(args):
 
     step()
     torch.cuda.synchronize()
-    reps = max(2, args.batch_steps)
+    reps = max(2, args.side_steps)
     t = time.perf_counter()
     for _ in range(reps):
         gm, gl, loss = step()
@@ -409,29 +520,17 @@ def adjoint_leg(args):
     return out
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path. "
-              "Run with --gpus 1.", file=sys.stderr)
-        sys.exit(2)
-    if args.copy_calibration:
-        copy_calibration()
-        return
-    if args.adjoint_only:
-        print(json.dumps({"adjoint": adjoint_leg(args)}), flush=True)
-        return
-    if args.batch_only:
-        print(json.dumps({"batch64": batch64(args)}), flush=True)
-        return
-
+def fem40_leg(args, structure=True):
+    """The 40^3 grid with 20,000 contacts (68,921 vertices, 206,763 DOF, 4 MAS
+    levels): one system per step, plain FP32 vs FP16 / BF16 inverse storage,
+    the roofline of its dominant kernel and (structure=True) the cost of a
+    structure change (new contact set).  Returns the figures, the workload and
+    the FP32 solution."""
     import numpy as np
     import torch
     from agipc_learning_amd import session
     from agipc_learning_amd import workloads as W
 
-    if not torch.cuda.is_available():
-        raise SystemExit("bench.py needs a GPU")
     log(f"step: generate {args.n}^3 grid, {args.contacts} contacts")
     w = W.fem_grid(args.n, young_sigma=0.5, n_contacts=args.contacts, seed=0)
     nv = w["n_vertices"]
@@ -474,8 +573,8 @@ def main():
     precs = ["fp32"] + [p for p in args.precisions.split(",") if p and p != "fp32"]
     results = {}
     for prec in precs:
-        log(f"step: timed {prec} ({args.warmup} warmup + {args.steps} steps)")
-        ms, st = run(prec, args.steps, args.warmup)
+        log(f"step: fem40c {prec} ({args.warmup} warmup + {args.fem40_steps} steps)")
+        ms, st = run(prec, args.fem40_steps, args.warmup)
         if st["failure"] != 0 or st["rel_res_true"] > args.tol * (1 + 1e-6):
             raise SystemExit(f"{prec} solve did not reach the tolerance: {st}")
         # split of the step: build alone, solve alone
@@ -490,7 +589,9 @@ def main():
             s.solve(b, x, rule=1, tol=args.tol)
         torch.cuda.synchronize()
         pcg_ms = (time.perf_counter() - t) * 1e3 / 5
-        kern = kernel_times(s, max(1, args.kernel_reps // 2))
+        if prec == "fp32":
+            x_fp32 = x.cpu().numpy().copy()
+        kern = kernel_times(s, 100)
         s.build(prec)  # time_kernel leaves the iteration state frozen; rebuild
         results[prec] = dict(ms=ms, st=st, build_ms=build_ms, pcg_ms=pcg_ms, kernels_us=kern)
         log(f"  {prec}: {ms:.3f} ms/step (build {build_ms:.3f}, pcg {pcg_ms:.3f}), iterations {st['iterations']}, "
@@ -509,179 +610,252 @@ def main():
                 "per_kernel": {k: {"us": round(kern[k], 2), "GB/s": round(alg[k] / (kern[k] * 1e-6) / 1e9, 1)}
                                for k in kern}}
 
-    # A Newton step whose contact set changed needs a new structure: the host
-    # analysis (hierarchy, layouts, Galerkin maps) and device set-up.  Timed
-    # on the same mesh with another contact set (mesh neighbours are per mesh).
-    log("step: structure change (new contact set)")
-    from agipc_learning_amd import native
+    if not structure:
+        restructure = None
+    else:
+        # A Newton step whose contact set changed needs a new structure: the host
+        # analysis (hierarchy, layouts, Galerkin maps) and device set-up.  Timed
+        # on the same mesh with another contact set (mesh neighbours are per mesh).
+        log("step: structure change (new contact set)")
+        from agipc_learning_amd import native
 
-    w2 = W.fem_grid(args.n, young_sigma=0.5, n_contacts=args.contacts, seed=1)
-    nbr = W.neighbors(w2["n_vertices"], w2["tets"])
-    vals2 = torch.from_numpy(np.ascontiguousarray(w2["vals"])).cuda()
-    torch.cuda.synchronize()
-    def restructure_once():
-        t0 = time.perf_counter()
-        s2 = native.Solver()
-        s2.add_system(block_rows=w2["block_rows"], rows=w2["rows"], cols=w2["cols"], use_mas=True,
-                      levels=args.levels)
-        s2.set_topology_morton(0, *nbr, w2["positions"])
-        s2.set_contacts(0, w2["pairs"])
-        t1 = time.perf_counter()
-        s2.setup()  # MAS hierarchy on the device, host plan, device upload
-        t2 = time.perf_counter()
-        s2.set_values(0, vals2)
-        s2.build("fp32")
+        w2 = W.fem_grid(args.n, young_sigma=0.5, n_contacts=args.contacts, seed=1)
+        nbr = W.neighbors(w2["n_vertices"], w2["tets"])
+        vals2 = torch.from_numpy(np.ascontiguousarray(w2["vals"])).cuda()
         torch.cuda.synchronize()
-        t3 = time.perf_counter()
-        return (t3 - t0) * 1e3, (t2 - t1) * 1e3, (t3 - t2) * 1e3
+        def restructure_once():
+            t0 = time.perf_counter()
+            s2 = native.Solver()
+            s2.add_system(block_rows=w2["block_rows"], rows=w2["rows"], cols=w2["cols"], use_mas=True,
+                          levels=args.levels)
+            s2.set_topology_morton(0, *nbr, w2["positions"])
+            s2.set_contacts(0, w2["pairs"])
+            t1 = time.perf_counter()
+            s2.setup()  # MAS hierarchy on the device, host plan, device upload
+            t2 = time.perf_counter()
+            s2.set_values(0, vals2)
+            s2.build("fp32")
+            torch.cuda.synchronize()
+            t3 = time.perf_counter()
+            return (t3 - t0) * 1e3, (t2 - t1) * 1e3, (t3 - t2) * 1e3
 
-    restructure_once()
-    times = sorted((restructure_once() for _ in range(3)), key=lambda r: r[0])
-    tot, setup_ms, build_ms = times[1]
-    # the host-only analysis (hierarchy built on the host), for comparison
-    s3 = native.Solver()
-    s3.add_system(block_rows=w2["block_rows"], rows=w2["rows"], cols=w2["cols"], use_mas=True, levels=args.levels)
-    s3.set_topology_morton(0, *nbr, w2["positions"])
-    s3.set_contacts(0, w2["pairs"])
-    t = time.perf_counter()
-    s3.analyze()
-    host_analysis_ms = (time.perf_counter() - t) * 1e3
-    restructure = {"ms": round(tot, 2), "setup_ms": round(setup_ms, 2), "values_and_build_ms": round(build_ms, 2),
-                   "host_only_analysis_ms": round(host_analysis_ms, 2)}
-    del s3
-    # the same solver taking each new structure (masx_set_structure + contacts
-    # + setup + values + build): mesh topology kept, host and device buffers
-    # reused; the two contact sets alternate so every call is a real change
-    s4 = native.Solver()
-    s4.add_system(block_rows=w["block_rows"], rows=w["rows"], cols=w["cols"], use_mas=True, levels=args.levels)
-    s4.set_topology_morton(0, *nbr, w2["positions"])
-    s4.set_contacts(0, w["pairs"])
-    s4.setup()
-    same = []
-    for i in range(6):
-        ww, vv = (w2, vals2) if i % 2 == 0 else (w, vals)
-        torch.cuda.synchronize()
+        restructure_once()
+        times = sorted((restructure_once() for _ in range(3)), key=lambda r: r[0])
+        tot, setup_ms, build_ms = times[1]
+        # the host-only analysis (hierarchy built on the host), for comparison
+        s3 = native.Solver()
+        s3.add_system(block_rows=w2["block_rows"], rows=w2["rows"], cols=w2["cols"], use_mas=True, levels=args.levels)
+        s3.set_topology_morton(0, *nbr, w2["positions"])
+        s3.set_contacts(0, w2["pairs"])
         t = time.perf_counter()
-        s4.set_structure(0, ww["rows"], ww["cols"])
-        s4.set_contacts(0, ww["pairs"])
+        s3.analyze()
+        host_analysis_ms = (time.perf_counter() - t) * 1e3
+        restructure = {"ms": round(tot, 2), "setup_ms": round(setup_ms, 2), "values_and_build_ms": round(build_ms, 2),
+                       "host_only_analysis_ms": round(host_analysis_ms, 2)}
+        del s3
+        # the same solver taking each new structure (masx_set_structure + contacts
+        # + setup + values + build): mesh topology kept, host and device buffers
+        # reused; the two contact sets alternate so every call is a real change
+        s4 = native.Solver()
+        s4.add_system(block_rows=w["block_rows"], rows=w["rows"], cols=w["cols"], use_mas=True, levels=args.levels)
+        s4.set_topology_morton(0, *nbr, w2["positions"])
+        s4.set_contacts(0, w["pairs"])
         s4.setup()
-        s4.set_values(0, vv)
-        s4.build("fp32")
-        torch.cuda.synchronize()
-        same.append((time.perf_counter() - t) * 1e3)
-    restructure["same_solver_ms"] = round(sorted(same[2:])[1], 2)
-    del s4, vals2
-    # the MAS hierarchy alone (the reference's per-step ReorderRealtime): host
-    # build vs the device build, inputs resident where each one runs
-    p2r, r2p = native.morton_partition(w2["positions"])
-    n2, slots2 = w2["n_vertices"], len(p2r)
-    host_ms = []
-    for _ in range(3):
-        t = time.perf_counter()
-        hh = native.hierarchy_host(n2, slots2, *nbr, p2r, r2p, pairs=w2["pairs"])
-        host_ms.append((time.perf_counter() - t) * 1e3)
-    cu = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).cuda()
-    d_in = [cu(nbr[0], np.uint32), cu(nbr[1], np.uint32), cu(nbr[2], np.uint32), cu(p2r, np.int32),
-            cu(r2p, np.int32)]
-    d_pairs = cu(np.asarray(w2["pairs"]).reshape(-1, 4), np.int32)
-    hier = native.DeviceHierarchy()
-    dev_ms = []
-    for _ in range(6):
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        hier.build(n2, slots2, *d_in, pairs=d_pairs)
-        dev_ms.append((time.perf_counter() - t) * 1e3)
-    dh = hier.download()
-    same = all(np.array_equal(dh[k], hh[k]) for k in ("fine_mask", "prefix", "coarse_table", "level_begin"))
-    if not same:
-        raise SystemExit("device hierarchy differs from the host build")
-    restructure["hierarchy_host_ms"] = round(sorted(host_ms)[1], 3)
-    restructure["hierarchy_device_ms"] = round(sorted(dev_ms[1:])[2], 3)
-    # the Morton partition the banks are cut from: host vs device (same arrays)
-    mt_host, mt_dev = [], []
-    dm = native.DeviceMorton()
-    d_pos = torch.from_numpy(np.ascontiguousarray(w["positions"], np.float64)).cuda()
-    for _ in range(5):
-        t = time.perf_counter()
-        native.morton_partition(w["positions"], 32)
-        mt_host.append((time.perf_counter() - t) * 1e3)
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        dm.partition(d_pos, 32)
-        torch.cuda.synchronize()
-        mt_dev.append((time.perf_counter() - t) * 1e3)
-    restructure["morton_host_ms"] = round(sorted(mt_host)[2], 3)
-    restructure["morton_device_ms"] = round(sorted(mt_dev[1:])[1], 3)
-    dm.close()
-    log(f"  structure change: {restructure}")
-    del hier, d_in, d_pairs
-
-    from agipc_learning_amd import workloads as W
-
-    hw = None if args.no_batch else W.hybrid(seed=0)
-    cpu, hyb_oracle = None, None
-    if not args.no_cpu:
-        # CPU leg: the oracle timed on one step of this workload, and the
-        # oracle's solution of the hybrid system (the hybrid leg's checker)
-        log("step: cpu baseline (oracle, one thread)")
-        cpu = cpu_baseline(w, args.levels, args.tol)
-        log(f"  cpu: {cpu['value']:.1f} ms")
-        if hw is not None:
-            hyb_oracle = hybrid_oracle(hw, args.tol)
-
-    hyb = None if hw is None else hybrid(args, hw, hyb_oracle)
-    adj = None if args.no_adjoint else adjoint_leg(args)
-    batch = None
-    if not args.no_batch:
-        del s, vals, b, x
-        torch.cuda.empty_cache()
-        batch = batch64(args)
-
-    # the copy rate puts the fractions of the nominal 8 TB/s in context
-    ceiling = copy_ceiling()
-    log(f"  device copy: {ceiling:.1f} GB/s")
-    roofline["copy_GBs"] = ceiling
-    if batch is not None and batch.get("roofline"):
-        batch["roofline"]["copy_GBs"] = ceiling
-        batch["roofline"]["frac_of_copy"] = round(batch["roofline"]["achieved"] / ceiling, 4)
+        same = []
+        for i in range(6):
+            ww, vv = (w2, vals2) if i % 2 == 0 else (w, vals)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            s4.set_structure(0, ww["rows"], ww["cols"])
+            s4.set_contacts(0, ww["pairs"])
+            s4.setup()
+            s4.set_values(0, vv)
+            s4.build("fp32")
+            torch.cuda.synchronize()
+            same.append((time.perf_counter() - t) * 1e3)
+        restructure["same_solver_ms"] = round(sorted(same[2:])[1], 2)
+        del s4, vals2
+        # the MAS hierarchy alone (the reference's per-step ReorderRealtime): host
+        # build vs the device build, inputs resident where each one runs
+        p2r, r2p = native.morton_partition(w2["positions"])
+        n2, slots2 = w2["n_vertices"], len(p2r)
+        host_ms = []
+        for _ in range(3):
+            t = time.perf_counter()
+            hh = native.hierarchy_host(n2, slots2, *nbr, p2r, r2p, pairs=w2["pairs"])
+            host_ms.append((time.perf_counter() - t) * 1e3)
+        cu = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).cuda()
+        d_in = [cu(nbr[0], np.uint32), cu(nbr[1], np.uint32), cu(nbr[2], np.uint32), cu(p2r, np.int32),
+                cu(r2p, np.int32)]
+        d_pairs = cu(np.asarray(w2["pairs"]).reshape(-1, 4), np.int32)
+        hier = native.DeviceHierarchy()
+        dev_ms = []
+        for _ in range(6):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            hier.build(n2, slots2, *d_in, pairs=d_pairs)
+            dev_ms.append((time.perf_counter() - t) * 1e3)
+        dh = hier.download()
+        same = all(np.array_equal(dh[k], hh[k]) for k in ("fine_mask", "prefix", "coarse_table", "level_begin"))
+        if not same:
+            raise SystemExit("device hierarchy differs from the host build")
+        restructure["hierarchy_host_ms"] = round(sorted(host_ms)[1], 3)
+        restructure["hierarchy_device_ms"] = round(sorted(dev_ms[1:])[2], 3)
+        # the Morton partition the banks are cut from: host vs device (same arrays)
+        mt_host, mt_dev = [], []
+        dm = native.DeviceMorton()
+        d_pos = torch.from_numpy(np.ascontiguousarray(w["positions"], np.float64)).cuda()
+        for _ in range(5):
+            t = time.perf_counter()
+            native.morton_partition(w["positions"], 32)
+            mt_host.append((time.perf_counter() - t) * 1e3)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            dm.partition(d_pos, 32)
+            torch.cuda.synchronize()
+            mt_dev.append((time.perf_counter() - t) * 1e3)
+        restructure["morton_host_ms"] = round(sorted(mt_host)[2], 3)
+        restructure["morton_device_ms"] = round(sorted(mt_dev[1:])[1], 3)
+        dm.close()
+        log(f"  structure change: {restructure}")
+        del hier, d_in, d_pairs
 
     iters = plain["st"]["iterations"]
+    out = {"workload": f"fem{args.n}c: {args.n}^3 hex grid, {nv} vertices, {dof} DOF, {len(w['tets'])} tets, "
+                       f"{args.contacts} contact pairs, lognormal E, 32-vertex Morton banks, {args.levels} MAS levels",
+           "vertices": nv, "dof": dof, "upper_blocks": int(info["nb_total"]), "banks": int(info["n_banks"]),
+           "ms": round(plain["ms"], 4), "iterations": iters, "rel_res_true": plain["st"]["rel_res_true"],
+           "solves_per_s": round(1e3 / plain["ms"], 2),
+           "dof_iterations_per_s": round(dof * iters / (plain["pcg_ms"] * 1e-3), 1),
+           "plain_fp32": {"ms": round(plain["ms"], 4), "build_ms": round(plain["build_ms"], 4),
+                          "pcg_ms": round(plain["pcg_ms"], 4), "iterations": iters},
+           "mixed": {p: {"ms": round(results[p]["ms"], 4), "build_ms": round(results[p]["build_ms"], 4),
+                         "pcg_ms": round(results[p]["pcg_ms"], 4), "iterations": results[p]["st"]["iterations"],
+                         "rel_res_true": results[p]["st"]["rel_res_true"],
+                         "speedup_vs_fp32": round(plain["ms"] / results[p]["ms"], 3)} for p in precs[1:]},
+           "setup_ms": round(setup_ms, 1), "spmv_tiling": tiling, "roofline": roofline}
+    if restructure is not None:
+        out["structure_change"] = restructure
+    return out, w, x_fp32
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path. "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    import torch
+
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py needs a GPU")
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_leg(args)}), flush=True)
+        return
+    if args.fem40_only:
+        print(json.dumps({"fem40c": fem40_leg(args, structure=False)[0]}), flush=True)
+        return
+    if args.hybrid_only:
+        from agipc_learning_amd import workloads as W
+
+        print(json.dumps({"hybrid": hybrid(args, W.hybrid(seed=0))[0]}), flush=True)
+        return
+
+    batch, ws, batch_sols = batch_leg(args)
+    if args.batch_only:
+        print(json.dumps({"batch64": batch}), flush=True)
+        return
+    fem40 = hyb = adj = None
+    x40 = w40 = hw = hyb_sols = None
+    if not args.no_extra:
+        from agipc_learning_amd import workloads as W
+
+        fem40, w40, x40 = fem40_leg(args)
+        hw = W.hybrid(seed=0)
+        hyb, hyb_sols = hybrid(args, hw)
+        adj = adjoint_leg(args)
+    log("step: streaming rates")
+    ceil = stream_ceilings()
+    log(f"  {ceil}")
+    rf = batch["roofline"]
+    rf["read_stream_GBs"], rf["copy_GBs"] = ceil["read_GBs"], ceil["copy_GBs"]
+    rf["frac_of_read_stream"] = round(rf["achieved"] / ceil["read_GBs"], 4)
+
+    cpu = None
+    if not args.no_cpu:
+        O = oracle_module()
+        log(f"step: cpu baseline, 64-system batch ({O.get_threads()} threads, {args.cpu_runs} runs; one thread on "
+            "a sample)")
+        cpu, sols, oracle_its = cpu_batch(O, ws, args.tol, args.cpu_runs)
+        log(f"  cpu batch: {cpu['all_cores']} / one thread {cpu['single_core']}")
+        for prec, xg in batch_sols.items():
+            off, errs = 0, []
+            for w, xo in zip(ws, sols):
+                n = 3 * w["block_rows"]
+                errs.append(rel_err(xg[off:off + n], xo))
+                off += n
+            gi = batch[prec]["iterations"]
+            batch[prec]["solution_rel_err_vs_oracle"] = {"max": max(errs), "median": sorted(errs)[len(errs) // 2]}
+            batch[prec]["iterations_minus_oracle"] = [min(a - b for a, b in zip(gi, oracle_its)),
+                                                     max(a - b for a, b in zip(gi, oracle_its))]
+        batch["oracle_iterations"] = {"max": max(oracle_its), "mean": round(sum(oracle_its) / len(oracle_its), 2)}
+        del sols
+        if fem40 is not None:
+            log("step: cpu baseline, 40^3 + contacts")
+            c40, xo = cpu_single(O, w40, args.levels, args.tol, max(args.cpu_runs, 5))
+            fem40["cpu_baseline"] = c40
+            fem40["solution_rel_err_vs_oracle"] = rel_err(x40, xo)
+            log(f"  cpu 40^3: {c40['all_cores']} / one thread {c40['single_core']}, solution rel err "
+                f"{fem40['solution_rel_err_vs_oracle']:.2e}")
+        if hyb is not None:
+            log("step: oracle, hybrid")
+            ch, xo = cpu_single(O, hw, 0, args.tol, args.cpu_runs)
+            hyb["cpu_baseline"] = ch
+            hyb["oracle_iterations"] = ch["oracle_iterations"]
+            for prec, xg in hyb_sols.items():
+                hyb[prec]["solution_rel_err_vs_oracle"] = rel_err(xg, xo)
+            log(f"  hybrid oracle: {ch['oracle_iterations']} iterations, errors "
+                f"{[hyb[p]['solution_rel_err_vs_oracle'] for p in hyb_sols]}")
+
+    plain = batch["fp32"]
     out = {
         "metric": "MAS-PCG solve ms to 1e-4 rel-res, mixed vs plain FP32; % HBM roofline",
-        "value": round(plain["ms"], 4),
+        "value": plain["ms"],
         "unit": "ms",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(plain["ms"], 4),
+        "ms_per_step": plain["ms"],
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"fem{args.n}c: {args.n}^3 hex grid, {nv} vertices, {dof} DOF, "
-                               f"{len(w['tets'])} tets, {args.contacts} contact pairs, lognormal E, "
-                               f"32-vertex Morton banks, {args.levels} MAS levels, tol {args.tol:g}, "
-                               "step = values in + MAS build + PCG",
-                   "vertices": nv, "dof": dof, "upper_blocks": int(info["nb_total"]), "banks": int(info["n_banks"])},
-        "iterations": iters,
-        "rel_res_true": plain["st"]["rel_res_true"],
-        "solves_per_s": round(1e3 / plain["ms"], 2),
-        "dof_iterations_per_s": round(dof * iters / (plain["pcg_ms"] * 1e-3), 1),
-        "plain_fp32": {"ms": round(plain["ms"], 4), "build_ms": round(plain["build_ms"], 4),
-                       "pcg_ms": round(plain["pcg_ms"], 4), "iterations": iters},
-        "mixed": {p: {"ms": round(results[p]["ms"], 4), "build_ms": round(results[p]["build_ms"], 4),
-                      "pcg_ms": round(results[p]["pcg_ms"], 4), "iterations": results[p]["st"]["iterations"],
-                      "rel_res_true": results[p]["st"]["rel_res_true"],
-                      "speedup_vs_fp32": round(plain["ms"] / results[p]["ms"], 3)} for p in precs[1:]},
-        "setup_ms": round(setup_ms, 1),
-        "structure_change": restructure,
-        "spmv_tiling": tiling,
-        "roofline": roofline,
+        "config": {"workload": f"batch64: 64 independent FEM + contact systems in one batched solve, cubes of "
+                               f"U[5k, 80k] vertices (seed 0), {batch['vertices']} vertices, {batch['dof']} DOF, "
+                               f"{batch['upper_blocks']} upper 3x3 blocks, lognormal E, 5% of vertices in contact "
+                               "pairs, 32-vertex Morton banks, tol 1e-4 (FP64 true residual) per system, step = "
+                               "values in + MAS build + PCG of all 64",
+                   "systems": batch["systems"], "vertices": batch["vertices"], "dof": batch["dof"],
+                   "upper_blocks": batch["upper_blocks"], "banks": batch["banks"]},
+        "iterations": plain["max_iterations"],
+        "rel_res_true": plain["worst_rel_res_true"],
+        "solves_per_s": round(1e3 / plain["ms"] * batch["systems"], 2),
+        "dof_iterations_per_s": plain["dof_iterations_per_s"],
+        "plain_fp32": {k: plain[k] for k in ("ms", "build_ms", "pcg_ms", "max_iterations", "mean_iterations")},
+        "mixed": {p: {k: batch[p][k] for k in ("ms", "build_ms", "pcg_ms", "max_iterations", "mean_iterations",
+                                               "speedup_vs_fp32", "worst_rel_res_true")}
+                  for p in batch if p in ("fp16", "bf16")},
+        "roofline": rf,
         "cpu_baseline": cpu,
+        "stream": ceil,
+        "batch64": batch,
+        "fem40c": fem40,
         "hybrid": hyb,
         "adjoint": adj,
-        "batch64": batch,
     }
     print(json.dumps(out), flush=True)
 

@@ -173,6 +173,13 @@ int masx_autotune(masx_solver* s, int reps, int* entries_per_thread, double* ms)
  * excludes the gaps between launches. */
 int masx_time_kernel(masx_solver* s, int which, int reps, double* ms);
 
+/* Practical streaming rate of the current GPU (no counterpart in the
+ * reference; a measurement aid): the fastest of `reps` launches of a plain
+ * kernel over `bytes` of device memory (>= 1 MiB, a multiple of 16), kind 0 =
+ * read-only stream (bytes read), kind 1 = copy (bytes read + written), in
+ * GB/s.  Allocates and frees its buffers. */
+int masx_stream_rate(long long bytes, int kind, int reps, double* gb_per_s);
+
 /* Introspection (tests, benchmark). */
 typedef struct masx_plan_info
 {

@@ -1,0 +1,200 @@
+"""Parity at the full benchmark sizes: the HIP path through the C ABI against
+the FP64 oracle on the same generated systems (seed 0).
+
+  40^3 grid + 20,000 contact pairs, 4 MAS levels (68,921 vertices)
+  hybrid: 2,000 affine bodies, 30,000 body pairs, 20^3 FEM block
+  64-system batch: cubes of U[5k, 80k] vertices, 5% of vertices in pairs
+  60^3 adjoint (226,981 vertices, 1,296,000 tets)
+
+Bars (floating point, tolerances stated per test): plain FP32 iterations
+within 5% (at least +-1) of the oracle's under both stopping rules, the FP64
+true relative residual at the tolerance, and the solution within 1e-3 of the
+oracle's (both solutions meet ||b - A x|| <= 1e-4 ||b||; the difference is
+what the conditioning lets through, measured at 4e-7 to 2e-5).  The FP16 /
+BF16 inverse storage changes the preconditioner, not the system: those solves
+must meet the same residual, within 10% of the oracle's iterations, and the
+same solution bound.  The adjoint gradient is checked against central finite
+differences of the loss (relative step h = 1e-4) on 100 random tets, each
+difference formed from a solve of the perturbed system for x - x' (no
+cancellation between two large losses)."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from agipc_learning_amd import adjoint
+from agipc_learning_amd import native
+from agipc_learning_amd import session
+from agipc_learning_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def iters_close(a, b, frac=0.05):
+    return abs(a - b) <= max(1, int(frac * b + 0.5))
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def oracle_solve(oracle, w, *, levels=0, rule=1):
+    """The oracle's MAS (Cholesky-built inverse rounded to FP32) and FP64 PCG."""
+    nv = w["n_vertices"]
+    p2r, r2p = oracle.morton_partition(w["positions"])
+    st, nu, li = oracle.mesh_neighbors(nv, w["tets"])
+    mas = oracle.Mas(nv, len(p2r), st, nu, li, p2r, r2p, levels=levels)
+    mas.reorder(w["pairs"])
+    fem = (w["rows"] >= w["fem_offset"]) & (w["cols"] >= w["fem_offset"])
+    mas.assemble(w["vals"], w["rows"], w["cols"], indices=np.nonzero(fem)[0].astype(np.uint32),
+                 offset=w["fem_offset"])
+    mas.invert(1)
+    abd = oracle.abd_block_inverse(w["n_bodies"], w["vals"], w["rows"], w["cols"]) if w["n_bodies"] else None
+    return oracle.pcg(w["block_rows"], w["vals"], w["rows"], w["cols"], w["rhs"], n_abd=w["n_bodies"],
+                      abd_inv=abd, mas=mas, fem_offset=w["fem_offset"], tol=TOL, rule=rule)
+
+
+def gpu_solve(s, b, precision, rule=1):
+    s.build(precision)
+    x = torch.zeros(len(b), dtype=torch.float64, device="cuda")
+    st = s.solve(dev(b), x, rule=rule, tol=TOL)
+    return host(x), st
+
+
+def check_single(s, w, xo, so, name):
+    for prec in ("fp32", "fp16", "bf16"):
+        x, st = gpu_solve(s, w["rhs"], prec)
+        st = st[0]
+        err = rel(x, xo)
+        say(f"{name} {prec}: iterations {st['iterations']} (oracle {so['iterations']}), "
+            f"true rel res {st['rel_res_true']:.3e}, solution rel err {err:.2e}")
+        assert st["failure"] == 0 and st["rel_res_true"] <= TOL * (1 + 1e-6)
+        assert iters_close(st["iterations"], so["iterations"], 0.05 if prec == "fp32" else 0.10), (st, so)
+        assert err <= 1e-3
+
+
+def test_fem40_contacts_4_levels_against_the_oracle(oracle):
+    w = W.fem40c(seed=0)
+    say(f"fem40c: {w['n_vertices']} vertices, {len(w['rows'])} upper blocks, {w['n_pairs']} pairs")
+    s = session.solver_for(w, levels=4)
+    xo, so = oracle_solve(oracle, w, levels=4)
+    assert so["true_rel_res"] <= TOL
+    check_single(s, w, xo, so, "fem40c TRUE rule")
+    # the reference's stopping rule (r.z against r0.z0, one-iteration lag)
+    _, so_ref = oracle_solve(oracle, w, levels=4, rule=0)
+    _, st = gpu_solve(s, w["rhs"], "fp32", rule=0)
+    say(f"fem40c REF rule: iterations {st[0]['iterations']} (oracle {so_ref['iterations']})")
+    assert st[0]["failure"] == 0 and iters_close(st[0]["iterations"], so_ref["iterations"])
+
+
+def test_hybrid_2000_bodies_against_the_oracle(oracle):
+    w = W.hybrid(seed=0)
+    say(f"hybrid: {w['n_bodies']} bodies, {w['n_vertices']} FEM vertices, {len(w['rows'])} upper blocks")
+    s = session.solver_for(w)
+    xo, so = oracle_solve(oracle, w)
+    assert so["true_rel_res"] <= TOL
+    check_single(s, w, xo, so, "hybrid")
+
+
+def test_batch_of_64_systems_against_the_oracle(oracle):
+    ws = W.batch64(seed=0)
+    nv = [w["n_vertices"] for w in ws]
+    say(f"batch64: {len(ws)} systems, {sum(nv)} vertices ({min(nv)} to {max(nv)})")
+    s = session.solver_for(ws)
+    b = session.rhs_of(ws)
+    ref = []
+    for i, w in enumerate(ws):
+        xo, so = oracle_solve(oracle, w)
+        assert so["true_rel_res"] <= TOL
+        ref.append((xo, so))
+        if i % 8 == 7:
+            say(f"  oracle: {i + 1} systems solved")
+    for prec in ("fp32", "fp16"):
+        x, st = gpu_solve(s, b, prec)
+        off, worst_err, worst_res, its = 0, 0.0, 0.0, []
+        for i, w in enumerate(ws):
+            n = 3 * w["block_rows"]
+            xo, so = ref[i]
+            err = rel(x[off:off + n], xo)
+            off += n
+            assert st[i]["failure"] == 0 and st[i]["rel_res_true"] <= TOL * (1 + 1e-6), (i, st[i])
+            assert iters_close(st[i]["iterations"], so["iterations"], 0.05 if prec == "fp32" else 0.10), \
+                (i, st[i]["iterations"], so["iterations"])
+            assert err <= 1e-3, (i, err)
+            worst_err, worst_res = max(worst_err, err), max(worst_res, st[i]["rel_res_true"])
+            its.append((st[i]["iterations"], so["iterations"]))
+        d = [a - o for a, o in its]
+        say(f"batch64 {prec}: iterations gpu - oracle in [{min(d)}, {max(d)}], max {max(a for a, _ in its)}, "
+            f"worst true rel res {worst_res:.3e}, worst solution rel err {worst_err:.2e}")
+
+
+def fd_of_loss(system, mu, lam, x, r0, e, which, h, b):
+    """Central difference of L = 1/2 ||x - x*||^2 in one tet's parameter:
+    for theta' = theta (1 +- h), x' = x - c with H(theta') c = H(theta') x - b
+    = (H(theta') - H(theta)) x, so L' - L = -r0.c + 1/2 ||c||^2 exactly."""
+    out = []
+    for sgn in (1.0, -1.0):
+        m, l_ = mu.clone(), lam.clone()
+        (m if which == 0 else l_)[e] *= 1.0 + sgn * h
+        system.build(m, l_)
+        rhs = torch.zeros_like(b)
+        system.solver.spmv(x, rhs, a=1.0, b=0.0)  # H(theta') x
+        rhs -= b
+        c = system.solve(rhs)
+        out.append(float(-(r0 @ c) + 0.5 * (c @ c)))
+    step = h * float((mu if which == 0 else lam)[e])
+    return (out[0] - out[1]) / (2.0 * step)
+
+
+def test_adjoint_60_cubed_against_finite_differences():
+    w = W.fem60(seed=0)
+    say(f"fem60: {w['n_vertices']} vertices, {len(w['tets'])} tets")
+    rng = np.random.default_rng(0)
+    xstar = dev(rng.standard_normal(3 * w["n_vertices"]))
+    b = dev(w["rhs"])
+    sel = rng.choice(len(w["tets"]), 100, replace=False)
+    mu0, lam0 = dev(w["mu"]), dev(w["lam"])
+
+    system = adjoint.ElasticSystem.from_workload(w, tol=1e-10)
+
+    def gradient(tol):
+        system.tol = tol
+        mu = mu0.clone().requires_grad_(True)
+        lam = lam0.clone().requires_grad_(True)
+        x = adjoint.solve(mu, lam, system, b)
+        (0.5 * ((x - xstar) ** 2).sum()).backward()
+        assert system.forward_stats["rel_res_true"] <= tol * (1 + 1e-6)
+        assert system.adjoint_stats["rel_res_true"] <= tol * (1 + 1e-6)
+        return x.detach(), np.stack([host(mu.grad), host(lam.grad)], 1)
+
+    _, g_prod = gradient(TOL)
+    x, g_ref = gradient(1e-10)
+    r0 = x - xstar
+    fd = np.zeros((len(sel), 2))
+    for k, e in enumerate(sel):
+        for which in (0, 1):
+            fd[k, which] = fd_of_loss(system, mu0, lam0, x, r0, int(e), which, 1e-4, b)
+        if k % 25 == 24:
+            say(f"  finite differences: {k + 1} tets")
+    err_ref = rel(g_ref[sel], fd)
+    err_prod_fd = rel(g_prod[sel], fd)
+    err_prod = rel(g_prod, g_ref)
+    say(f"fem60 adjoint: refined gradient vs FD {err_ref:.2e}; tolerance {TOL:g}: vs FD {err_prod_fd:.2e}, "
+        f"vs the refined gradient (all tets) {err_prod:.2e}")
+    assert err_ref <= 1e-4
+    assert err_prod_fd <= 1e-3 and err_prod <= 1e-3
