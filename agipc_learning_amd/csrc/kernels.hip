@@ -15,6 +15,7 @@
 //  * kernels of a finished system return at once, so a captured chunk of
 //    iterations can run past convergence at negligible cost.
 #include "kernels.hpp"
+#include "mas_tiles.hpp"
 #include "spmv_layout.hpp"
 
 #include <hip/hip_bf16.h>
@@ -263,9 +264,25 @@ __global__ void __launch_bounds__(256) k_gal_gather(const double* fine, double* 
 // (i, j), i >= j, is the upper element (j, i) of block (j / 3, i / 3); a
 // diagonal block also takes its transpose position.  Half formats store
 // 16-bit halves of 5 words per block (the tenth half is zero).
+// Half-precision inverses of 32-slot banks are stored as MFMA tiles (the
+// tile layout of mas_tiles.hpp); 16-slot banks and FP32 keep the block
+// diagonals.
+template <int BANK, typename T>
+constexpr bool kTileLayout = BANK == 32 && !std::is_same<T, float>::value;
+
 template <int BANK, typename T>
 __device__ __forceinline__ void store_inverse_element(void* out, int bank, int i, int j, double v, double inv_scale)
 {
+    if constexpr(kTileLayout<BANK, T>)
+    {
+        // (i, j), i >= j, is the upper element (j, i); a diagonal tile also
+        // takes its mirror (i, j)
+        unsigned short* o = reinterpret_cast<unsigned short*>(out) + (size_t)bank * kTileBankHalves;
+        const unsigned short bits = (unsigned short)half_bits(v * inv_scale, (T*)nullptr);
+        o[tile_half_index(j, i)] = bits;
+        if(i != j && i / kTileDim == j / kTileDim) o[tile_half_index(i, j)] = bits;
+        return;
+    }
     constexpr int NBLK = BANK * (BANK + 1) / 2;
     const int I = j / 3, J = i / 3, dd = J - I, len = BANK - dd;
     const int c0 = 3 * (j % 3) + i % 3, c1 = 3 * (i % 3) + j % 3;
@@ -741,22 +758,26 @@ __global__ void __launch_bounds__(256) k_phase_finish(DevData d, int* host, int*
 // holds 64/BANK banks; each bank is spread over BANK lanes (lane I = block
 // row I) and the four waves take block diagonals d = wave (mod 4).  For the
 // stored block (I, I+d) lane I adds B r_{I+d} to its own row and sends
-// B^T r_I to lane I+d by a shuffle; the four partial results are summed in a
-// fixed order.  Residuals of level >= 2 banks are formed here from the level-1
-// residuals of their descendants.  Workgroups run the coarse banks (the end
-// of the bank list) first, so their longer gathers overlap the fine banks.
-template <int BANK, typename T>
+// B^T r_I to lane I+d by a shuffle; the NW partial results are summed in a
+// fixed order.  NW (4 or 8 waves per workgroup, i.e. the waves that share a
+// bank's block diagonals) is chosen by the autotuner.  Residuals of level >= 2
+// banks are formed here from the level-1 residuals of their descendants.
+// Workgroups run the coarse banks (the end of the bank list) first, so their
+// longer gathers overlap the fine banks.
+template <int BANK, typename T, int NW>
 // four waves per SIMD: the FP32 variant fits 128 VGPRs without spilling
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) k_mas_local(DevData d, int force)
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4, 8))) k_mas_local(DevData d, int force)
 {
+    constexpr int THREADS = 64 * NW;
     constexpr int BPW   = 64 / BANK;
     constexpr int NBLK  = BANK * (BANK + 1) / 2;
     constexpr int WORDS = Store<T>::kWords;
-    constexpr int ND    = BANK / 4;        // block diagonals per wave
-    constexpr int TPN   = 256 / BANK;      // threads per node in the descendant sums
-    constexpr int CHUNK = 1024;            // descendant entries staged per pass
+    constexpr int ND    = BANK / NW;          // block diagonals per wave
+    constexpr int TPN   = THREADS / BANK;     // threads per node in the descendant sums
+    constexpr int CHUNK = 1024;               // descendant entries staged per pass
+    static_assert(BANK % NW == 0 && CHUNK % THREADS == 0, "diagonals and staged entries split evenly");
     __shared__ float sR[64 * 3 + 3];  // + a zero row for the lanes past a diagonal's end
-    __shared__ float sZ[3][64 * 3];
+    __shared__ float sZ[NW - 1][64 * 3];
     __shared__ float sD[CHUNK * 3];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int lb = lane / BANK, I = lane % BANK;
@@ -789,7 +810,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
 #pragma unroll
     for(int kk = 0; kk < ND; ++kk)
     {
-        const int dd = wave + 4 * kk, len = BANK - dd;
+        const int dd = wave + NW * kk, len = BANK - dd;
         const int Ic = I < len ? I : 0;
         const uint32_t* M = reinterpret_cast<const uint32_t*>(d.minv) + (size_t)bl * NBLK * WORDS +
                             WORDS * diag_off(BANK, dd) + Ic;
@@ -824,16 +845,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
         for(int base = lo; base < hi; base += CHUNK)
         {
             const int n = min(CHUNK, hi - base);
-            int   src[CHUNK / 256];
-            float v[CHUNK / 256][3];
+            int   src[CHUNK / THREADS];
+            float v[CHUNK / THREADS][3];
 #pragma unroll
-            for(int j = 0; j < CHUNK / 256; ++j)
+            for(int j = 0; j < CHUNK / THREADS; ++j)
             {
-                const int e = tid + 256 * j;
+                const int e = tid + THREADS * j;
                 src[j] = e < n ? d.deep_list[base + e] : -1;
             }
 #pragma unroll
-            for(int j = 0; j < CHUNK / 256; ++j)
+            for(int j = 0; j < CHUNK / THREADS; ++j)
             {
                 if(src[j] >= 0)
                 {
@@ -843,9 +864,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
             }
             __syncthreads();  // the previous chunk has been consumed
 #pragma unroll
-            for(int j = 0; j < CHUNK / 256; ++j)
+            for(int j = 0; j < CHUNK / THREADS; ++j)
             {
-                const int e = tid + 256 * j;
+                const int e = tid + THREADS * j;
                 if(e < n)
                 {
                     sD[3 * e] = v[j][0]; sD[3 * e + 1] = v[j][1]; sD[3 * e + 2] = v[j][2];
@@ -880,7 +901,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
 #pragma unroll
     for(int kk = 0; kk < ND; ++kk)
     {
-        const int dd  = wave + 4 * kk;
+        const int dd  = wave + NW * kk;
         // a lane past the diagonal's end (or of a finished system) holds a
         // finite in-bounds block and multiplies the zero row, adding exact
         // zeros; its transposed product would go to a lane past the bank's
@@ -928,7 +949,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
     if(wave == 0 && live)
     {
 #pragma unroll
-        for(int w = 0; w < 3; ++w)
+        for(int w = 0; w < NW - 1; ++w)
         {
             z0 += sZ[w][3 * lane]; z1 += sZ[w][3 * lane + 1]; z2 += sZ[w][3 * lane + 2];
         }
@@ -939,6 +960,291 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
         }
         float* dst = (pick(s_kind, lb) == 0 ? d.z : d.Zc) + 3 * (long long)(pick(s_vec, lb) + I);
         dst[0] = z0; dst[1] = z1; dst[2] = z2;
+    }
+}
+
+// ---- MAS local solve on MFMA (32-slot banks, FP16 / BF16 inverse) ---------
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct MfmaHalf;
+template <> struct MfmaHalf<__half>
+{
+    typedef _Float16 elem;
+    typedef _Float16 v8 __attribute__((ext_vector_type(8)));
+    static __device__ __forceinline__ mfma_f4 mma(v8 a, v8 b, mfma_f4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct MfmaHalf<__hip_bfloat16>
+{
+    typedef __bf16 elem;
+    typedef __bf16 v8 __attribute__((ext_vector_type(8)));
+    static __device__ __forceinline__ mfma_f4 mma(v8 a, v8 b, mfma_f4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+
+// LDS copy of a wave's off-diagonal stored tiles (the 15 of tiles 0..17 that
+// are not (0,0), (2,2), (4,4)), for the transposed operands: tile t in slot
+// t - [t > 0] - [t > 10] - [t > 16] at slot * kTrTileStride, its 16-byte
+// lane chunk c at c * 16 + (c / 8) * 64, so that the four 64-byte chunk runs
+// one transposed read takes from a tile fall on alternate halves of the LDS
+// banks.
+constexpr int kTrTileStride = 704;
+constexpr int kTrWaveBytes  = 15 * kTrTileStride;  // 10,560
+__host__ __device__ constexpr bool tr_tile_kept(int t) { return t != 0 && t != 10 && t != 16; }
+__host__ __device__ constexpr int tr_tile_offset(int t)
+{
+    return (t - (t > 0) - (t > 10) - (t > 16)) * kTrTileStride;
+}
+__host__ __device__ constexpr int tr_chunk_offset(int c) { return c * 16 + (c >> 3) * 64; }
+
+typedef short mfma_s4 __attribute__((ext_vector_type(4)));
+
+// The transposed half-operand: lane m (< 32) of the half gets row m % 16,
+// columns 8 (m / 16) .. + 7 of T^T, i.e. T(8 (m / 16) + e, m % 16), e = 0..7.
+// ds_read_b64_tr_b16 hands lane i of each 16-lane group element i % 4 of the
+// 8-byte rows its group's lanes i / 4 + 4 q (q = 0..3) address; lane s of a
+// group therefore addresses T(8 (m / 16) + 4 h + s / 4, 4 (s % 4) .. + 3),
+// one call for e = 4 h .. 4 h + 3 (h = 0, 1).
+template <typename V>
+__device__ __forceinline__ V read_transposed(const char* tile, int m)
+{
+    const int a = m & 3, q = (m & 15) >> 2, ch = m >> 4;
+    const char* base = tile + (a & 1) * 8;
+    const int   c0   = (a >> 1) * 16 + 8 * ch + q;
+    typedef __attribute__((address_space(3))) mfma_s4 lds_s4;
+    const mfma_s4 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s4*)(base + tr_chunk_offset(c0)));
+    const mfma_s4 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s4*)(base + tr_chunk_offset(c0 + 4)));
+    typedef short s8 __attribute__((ext_vector_type(8)));
+    const s8 both = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+    return __builtin_bit_cast(V, both);
+}
+
+// Z = Minv R for every 32-slot bank with the FP16 / BF16 inverse on MFMA.
+// One wave per bank, four banks per workgroup.  z_I (rows 16 I .. 16 I + 15
+// of the bank's 96) = sum over column pairs P of [M(I, 2P) M(I, 2P + 1)]
+// times residual entries 32 P .. 32 P + 31: one v_mfma_f32_16x16x32 per
+// (I, P) and residual part.  M(I, J) is the stored tile T(I, J) for I <= J,
+// loaded straight into operand registers, and T(J, I)^T below the diagonal,
+// read transposed from the wave's LDS copy of its tiles.  The residual
+// enters as two half-precision parts (r' = hi + lo to about 22 bits, r'
+// = r 2^-e with 2^e above the bank's largest |r|) in column 0 of the B
+// operand; FP32 accumulation; z = s 2^e D(:, 0) with s the inverse's scale.
+// Residuals of level >= 2 banks come from the update kernel's partials as
+// in k_mas_local.
+template <typename T>
+__global__ void __launch_bounds__(256) k_mas_local_mfma(DevData d, int force)
+{
+    using Op   = MfmaHalf<T>;
+    using V8   = typename Op::v8;
+    using E    = typename Op::elem;
+    constexpr int BANK  = 32;
+    constexpr int NB    = 4;          // banks per workgroup, one per wave
+    constexpr int TPN   = 256 / BANK;
+    constexpr int CHUNK = 1024;
+    static_assert(CHUNK * 3 * 4 <= NB * kTrWaveBytes, "the deep-residual staging fits the tile buffer");
+    __shared__ float sR[NB * 96];
+    __shared__ __align__(16) char sT[NB * kTrWaveBytes];   // deep-residual staging, then the tiles
+    __shared__ __align__(16) E sB[NB][2][96];               // residual parts hi, lo
+    __shared__ int sMeta[3][NB];                            // live, kind, vector start of each bank
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int b    = (int)(gridDim.x - 1 - blockIdx.x) * NB + wave;
+    int live = 0, kind = 0, vec = 0;
+    if(b < d.n_banks)
+    {
+        kind = d.bank_kind[b];
+        vec  = d.bank_vec[b];
+        live = force || d.state[d.single ? 0 : d.bank_sys[b]].status == kRunning;
+    }
+    // this wave's inverse (a bank of a finished system is not read): nine
+    // tile pairs (lanes 0-31 the first tile, 32-63 the second) and the
+    // diagonal tiles (1,1), (3,3), (5,5) (lanes 32-63; lanes 0-31 load the
+    // same chunks), issued before the residuals are known
+    uint4 pr[kTilePairs], sg[3];
+    if(live)
+    {
+        const uint4* M = reinterpret_cast<const uint4*>(d.minv) + (size_t)b * (kTileBankHalves / 8);
+#pragma unroll
+        for(int q = 0; q < kTilePairs; ++q) pr[q] = M[q * 64 + lane];
+#pragma unroll
+        for(int q = 0; q < 3; ++q) sg[q] = M[(2 * kTilePairs + q) * 32 + (lane & 31)];
+    }
+    else
+    {
+#pragma unroll
+        for(int q = 0; q < kTilePairs; ++q) pr[q] = uint4{0u, 0u, 0u, 0u};
+#pragma unroll
+        for(int q = 0; q < 3; ++q) sg[q] = uint4{0u, 0u, 0u, 0u};
+    }
+    if(lane == 0)
+    {
+        sMeta[0][wave] = live;
+        sMeta[1][wave] = kind;
+        sMeta[2][wave] = vec;
+    }
+    // residuals of level-0 and level-1 banks
+    {
+        float va = 0.f, vb = 0.f;
+        if(live && kind < 2)
+        {
+            const float* src = (kind == 0 ? d.r : d.Rc) + 3 * (long long)vec;
+            va = src[lane];
+            if(lane < 32) vb = src[64 + lane];
+        }
+        sR[wave * 96 + lane] = va;
+        if(lane < 32) sR[wave * 96 + 64 + lane] = vb;
+    }
+    __syncthreads();
+    if(!(sMeta[0][0] | sMeta[0][1] | sMeta[0][2] | sMeta[0][3])) return;  // systems already done
+    // level >= 2: sums of the update kernel's partials, in list order
+    float* sD = reinterpret_cast<float*>(sT);
+#pragma unroll 1
+    for(int k = 0; k < NB; ++k)
+    {
+        if(!(sMeta[0][k] && sMeta[1][k] == 2)) continue;  // uniform over the workgroup
+        const int c0 = sMeta[2][k];
+        const int lo = d.deep_ptr[c0], hi = d.deep_ptr[c0 + BANK];
+        const int node = tid / TPN, sub = tid % TPN;
+        const int nb = d.deep_ptr[c0 + node], ne = d.deep_ptr[c0 + node + 1];
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll 1
+        for(int base = lo; base < hi; base += CHUNK)
+        {
+            const int n = min(CHUNK, hi - base);
+            int   src[CHUNK / 256];
+            float v[CHUNK / 256][3];
+#pragma unroll
+            for(int j = 0; j < CHUNK / 256; ++j)
+            {
+                const int e = tid + 256 * j;
+                src[j] = e < n ? d.deep_list[base + e] : -1;
+            }
+#pragma unroll
+            for(int j = 0; j < CHUNK / 256; ++j)
+            {
+                if(src[j] >= 0)
+                {
+                    const long long c1 = 3 * (long long)src[j];
+                    v[j][0] = d.part_deep[c1]; v[j][1] = d.part_deep[c1 + 1]; v[j][2] = d.part_deep[c1 + 2];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for(int j = 0; j < CHUNK / 256; ++j)
+            {
+                const int e = tid + 256 * j;
+                if(e < n)
+                {
+                    sD[3 * e] = v[j][0]; sD[3 * e + 1] = v[j][1]; sD[3 * e + 2] = v[j][2];
+                }
+            }
+            __syncthreads();
+            const int q0 = max(nb, base), q1 = min(ne, base + n);
+            for(int q = q0 + sub; q < q1; q += TPN)
+            {
+                a0 += sD[3 * (q - base)];
+                a1 += sD[3 * (q - base) + 1];
+                a2 += sD[3 * (q - base) + 2];
+            }
+        }
+#pragma unroll
+        for(int o = TPN / 2; o > 0; o >>= 1)
+        {
+            a0 += __shfl_xor(a0, o, 64);
+            a1 += __shfl_xor(a1, o, 64);
+            a2 += __shfl_xor(a2, o, 64);
+        }
+        if(sub == 0)
+        {
+            const int i = k * 96 + 3 * node;
+            sR[i] = a0; sR[i + 1] = a1; sR[i + 2] = a2;
+        }
+        __syncthreads();  // the staging area is reused by the next bank
+    }
+    __syncthreads();  // residuals complete; the staging area is free
+    // residual parts: r' = r 2^-e, hi = half(r'), lo = half(r' - hi)
+    float ra = sR[wave * 96 + lane], rb = lane < 32 ? sR[wave * 96 + 64 + lane] : 0.f;
+    float mx = fmaxf(fabsf(ra), fabsf(rb));
+#pragma unroll
+    for(int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    int ex = 0;
+    if(mx > 0.f && isfinite(mx)) (void)frexpf(mx, &ex);
+    const float down = ldexpf(1.f, -ex);
+    ra *= down;
+    rb *= down;
+    {
+        const E ha = (E)ra, hb = (E)rb;
+        sB[wave][0][lane] = ha;
+        sB[wave][1][lane] = (E)(ra - (float)ha);
+        if(lane < 32)
+        {
+            sB[wave][0][64 + lane] = hb;
+            sB[wave][1][64 + lane] = (E)(rb - (float)hb);
+        }
+    }
+    // the wave's tile pairs into LDS for the transposed operands
+    char* myT = sT + wave * kTrWaveBytes;
+#pragma unroll
+    for(int q = 0; q < kTilePairs; ++q)
+    {
+        const int t = 2 * q + (lane >> 5);
+        if(tr_tile_kept(t)) *reinterpret_cast<uint4*>(myT + tr_tile_offset(t) + tr_chunk_offset(lane & 31)) = pr[q];
+    }
+    __syncthreads();
+    // B operands: lane l holds B(8 (l / 16) + j, l % 16) of column pair P;
+    // only column 0 is nonzero
+    const bool col0 = (lane & 15) == 0;
+    V8 bh[3], blo[3];
+#pragma unroll
+    for(int P = 0; P < 3; ++P)
+    {
+        const int k0 = 32 * P + 8 * (lane >> 4);
+        const V8 h = *reinterpret_cast<const V8*>(&sB[wave][0][k0]);
+        const V8 l = *reinterpret_cast<const V8*>(&sB[wave][1][k0]);
+        const V8 z = {};
+        bh[P]  = col0 ? h : z;
+        blo[P] = col0 ? l : z;
+    }
+    mfma_f4 acc[6];
+#pragma unroll
+    for(int I = 0; I < 6; ++I)
+    {
+        acc[I] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for(int P = 0; P < 3; ++P)
+        {
+            V8 a;
+            if(I <= 2 * P)
+                a = __builtin_bit_cast(V8, pr[tile_pair_slot(I, P)]);
+            else
+            {
+                // lanes 0-31: T(2P, I)^T; lanes 32-63: T(2P + 1, I)^T, or the
+                // stored diagonal tile when 2P + 1 == I
+                const bool upper_t = 2 * P + 1 < I && lane >= 32;
+                const int  t = upper_t ? tile_pos(2 * P + 1, I) : tile_pos(2 * P, I);
+                a = read_transposed<V8>(myT + tr_tile_offset(t), lane & 31);
+                if(2 * P + 1 == I && lane >= 32) a = __builtin_bit_cast(V8, sg[I / 2]);
+            }
+            acc[I] = Op::mma(a, bh[P], acc[I]);
+            acc[I] = Op::mma(a, blo[P], acc[I]);
+        }
+    }
+    // column 0 of each D: lanes 0, 16, 32, 48 hold rows 4 (l / 16) .. + 3
+    if(live && col0)
+    {
+        const float sc = d.minv_scale[b] * ldexpf(1.f, ex);
+        float* dst = (kind == 0 ? d.z : d.Zc) + 3 * (long long)vec + 4 * (lane >> 4);
+#pragma unroll
+        for(int I = 0; I < 6; ++I)
+        {
+            float4 o;
+            o.x = acc[I][0] * sc; o.y = acc[I][1] * sc; o.z = acc[I][2] * sc; o.w = acc[I][3] * sc;
+            *reinterpret_cast<float4*>(dst + 16 * I) = o;
+        }
     }
 }
 
@@ -1770,6 +2076,7 @@ hipError_t launch_gal_gather(const double* fine, double* E, const int* ptr, cons
 
 size_t mas_bank_bytes(int bank, int precision)
 {
+    if(bank == 32 && precision != kPrecFp32) return (size_t)kTileBankHalves * 2;
     const size_t nblk = (size_t)bank * (bank + 1) / 2;
     return nblk * (precision == kPrecFp32 ? 9 : 5) * 4;
 }
@@ -1875,13 +2182,21 @@ hipError_t launch_phase_finish(const DevData& d, hipStream_t st, int* host, int*
 hipError_t launch_mas_local(const DevData& d, int precision, int force, hipStream_t st, const LaunchEvents* ev)
 {
     if(d.n_banks <= 0) return hipSuccess;
+    if(d.mas_waves != 4 && d.mas_waves != 8) return hipErrorInvalidValue;
     const int bpw = 64 / d.bank;
     const dim3 g(blocks_for(d.n_banks, bpw)), b(256);
-#define MASX_LOCAL(BK, T) MASX_DISPATCH((k_mas_local<BK, T>), g, b, d, force)
+    const dim3 b8(512);
+#define MASX_LOCAL(BK, T)                                                          \
+    do                                                                             \
+    {                                                                              \
+        if(d.mas_waves == 8) MASX_DISPATCH((k_mas_local<BK, T, 8>), g, b8, d, force); \
+        else MASX_DISPATCH((k_mas_local<BK, T, 4>), g, b, d, force);                \
+    } while(0)
     if(d.bank == 32)
     {
-        if(precision == kPrecFp16) MASX_LOCAL(32, __half);
-        else if(precision == kPrecBf16) MASX_LOCAL(32, __hip_bfloat16);
+        const dim3 gm(blocks_for(d.n_banks, 4));
+        if(precision == kPrecFp16) MASX_DISPATCH((k_mas_local_mfma<__half>), gm, b, d, force);
+        else if(precision == kPrecBf16) MASX_DISPATCH((k_mas_local_mfma<__hip_bfloat16>), gm, b, d, force);
         else MASX_LOCAL(32, float);
     }
     else if(d.bank == 16)

@@ -371,6 +371,24 @@ int masx_autotune(masx_solver* s, int reps, int* entries_per_thread, double* ms)
     });
 }
 
+int masx_set_mas_waves(masx_solver* s, int waves)
+{
+    return guard([&] {
+        require(s != nullptr, "null solver");
+        s->impl.set_mas_waves(waves);
+    });
+}
+
+int masx_autotune_mas(masx_solver* s, int reps, int* waves, double* ms)
+{
+    return guard([&] {
+        require(s != nullptr && reps > 0, "bad argument");
+        const auto t = s->impl.autotune_mas(reps);
+        if(waves) *waves = t.entries;
+        if(ms) *ms = t.ms;
+    });
+}
+
 int masx_time_kernel(masx_solver* s, int which, int reps, double* ms)
 {
     return guard([&] {

@@ -1,6 +1,7 @@
 // Host-side structure analysis (see plan.hpp).
 #include "plan.hpp"
 #include "pair_code.hpp"
+#include "phase_timer.hpp"
 
 #include <omp.h>
 
@@ -321,6 +322,7 @@ void build_hierarchy(Hierarchy& h, int n_fem, int n_slots, int bank, int levels,
 
 void build_plan(Plan& P, std::vector<SystemDesc>& sys)
 {
+    PhaseTimer tm("plan");
     const int S = (int)sys.size();
     if(S == 0) throw std::invalid_argument("no systems");
     const int bank = sys[0].bank;
@@ -368,6 +370,7 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
             P.ext_to_int[P.ext_off[s] + d.fem_offset + v] = P.fem_row0[s] + (d.use_mas ? d.real_to_part[v] : v);
         for(int r = P.row_off[s]; r < P.row_off[s + 1]; ++r) P.row_sys[r] = s;
     }
+    tm.mark("rows");
     // ---- full block CSR (both triangles)
     for(int s = 0; s < S; ++s)
     {
@@ -443,6 +446,7 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
             }
         }
     }
+    tm.mark("block CSR");
     // ---- global 3x3 block-Jacobi and ABD bodies
     P.diag_src.assign(P.n_rows, -1);
     P.row_needs_diag.assign(P.n_rows, 0);
@@ -486,6 +490,7 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
     }
     P.n_bodies = (int)P.body_row.size();
 
+    tm.mark("bodies");
     // ---- MAS: hierarchy per system, banks, components, Galerkin maps
     int n_fine_banks = 0, n_coarse = 0;
     std::vector<int> fine_bank_base(S, 0);
@@ -633,6 +638,7 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
         P.sys_level_ptr[s + 1] = (int)P.sys_level_banks.size() / 2;
     }
 
+    tm.mark("banks, ancestors, components");
     // Residuals of levels >= 2, pre-reduced per update group: the update
     // kernel treats 256 rows (256/bank fine banks) of one system per workgroup
     // and sums the level-1 component residuals of those banks.  Each group
@@ -698,6 +704,7 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
                 throw std::logic_error("update-group partials do not cover the level-1 descendants");
     }
 
+    tm.mark("update groups");
     // Galerkin coarse operators A_l = P_l^T A P_l (all entries, not only in-bank),
     // built level by level from the previous level; group g holds level g+1 of
     // every system.
@@ -830,6 +837,7 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
     P.n_gal = (long long)E_ij.size();
     if((long long)P.gal_src.size() >= (1ll << 29)) throw std::invalid_argument("Galerkin map too large");
 
+    tm.mark("Galerkin maps");
     // per-bank entries for the dense local assembly: a fine bank's upper
     // blocks (from its rows of the full CSR, as-is entries with both ends in
     // the bank), then the coarse entries with both ends in one coarse bank.
@@ -889,6 +897,7 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
             P.bank_ent_src[at] = (int)e;
         }
     }
+    tm.mark("bank entries");
 }
 
 

@@ -1,5 +1,7 @@
 // Host orchestrator of the MAS-PCG solve (see solver.hpp).
 #include "solver.hpp"
+#include "mas_tiles.hpp"
+#include "phase_timer.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -66,7 +68,9 @@ void Solver::ensure_stream()
 void Solver::analyze()
 {
     build_plan(plan_, systems_);
+    PhaseTimer tm("analyze");
     validate_plan(plan_);
+    tm.mark("validate");
     analyzed_ = true;
 }
 
@@ -189,6 +193,30 @@ void Solver::set_spmv_tiling(int entries_per_thread)
     }
 }
 
+void Solver::set_mas_waves(int waves)
+{
+    if(waves != 4 && waves != 8) throw std::invalid_argument("MAS local waves per workgroup must be 4 or 8");
+    mas_waves_    = waves;
+    dd_.mas_waves = waves;
+    if(graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    graph_exec_ = nullptr;
+}
+
+Solver::Tiling Solver::autotune_mas(int reps)
+{
+    if(!built_) throw std::runtime_error("build() must precede autotune");
+    if(plan_.n_banks == 0 || (plan_.bank == 32 && precision_ != kPrecFp32)) return Tiling{mas_waves_, 0.0};
+    Tiling best{mas_waves_, 1e30};
+    for(const int w : {4, 8})
+    {
+        set_mas_waves(w);
+        const double ms = time_kernel(0, reps);
+        if(ms < best.ms) best = Tiling{w, ms};
+    }
+    set_mas_waves(best.entries);
+    return best;
+}
+
 Solver::Tiling Solver::autotune_spmv(int reps)
 {
     if(!built_) throw std::runtime_error("build() must precede autotune");
@@ -205,11 +233,14 @@ Solver::Tiling Solver::autotune_spmv(int reps)
 
 void Solver::setup()
 {
+    PhaseTimer tm("setup");
     ensure_stream();
     if(!analyzed_)
     {
         build_hierarchies_on_device();
+        tm.mark("device hierarchy");
         analyze();
+        tm.mark("host plan");
     }
     const Plan& P = plan_;
     hipStream_t st = work_;
@@ -278,9 +309,12 @@ void Solver::setup()
     MASX_HIP(hipMemsetAsync(d_state_.ptr, 0, d_state_.n * sizeof(SysState), st));
     if(h_state_) (void)hipHostFree(h_state_);
     MASX_HIP(hipHostMalloc((void**)&h_state_, sizeof(SysState) * P.n_sys * (kDepth + 1), hipHostMallocDefault));
+    tm.mark("uploads");
     build_work_tables();
+    tm.mark("work tables + SpMV layout");
     make_devdata();
     MASX_HIP(hipStreamSynchronize(st));
+    tm.mark("device sync");
     if(graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     graph_exec_ = nullptr;
     setup_done_ = true;
@@ -390,6 +424,7 @@ void Solver::make_devdata()
     d.n_wg_upd = n_wg_upd_;
     d.bank = P.bank;
     d.n_banks = P.n_banks;
+    d.mas_waves = mas_waves_;
     d.n_fine_banks = P.n_fine_banks;
     d.nnz = P.nnz;
     d.nnz_s = (long long)spmv_layout_.col.size();
@@ -738,6 +773,14 @@ void Solver::download_inverse(std::vector<float>& out)
                     if(precision_ == kPrecFp32)
                     {
                         std::memcpy(&v, &bw[9 * off + c * len + I], 4);
+                    }
+                    else if(B == 32)
+                    {
+                        // MFMA tiles: element (3 I + c / 3, 3 (I + dd) + c % 3) of the upper triangle
+                        const int r = 3 * I + c / 3, col = 3 * (I + dd) + c % 3;
+                        const int hidx = r <= col ? tile_half_index(r, col) : tile_half_index(col, r);
+                        const uint32_t w = bw[hidx / 2];
+                        v = half_to_float(hidx % 2 ? (w >> 16) : (w & 0xffffu)) * scale[b];
                     }
                     else
                     {

@@ -98,6 +98,14 @@ class Solver
     };
     // times every supported SpMV tiling on the built system and keeps the fastest
     Tiling autotune_spmv(int reps);
+    // waves per workgroup of the FP32 MAS local kernel (4 or 8; each workgroup
+    // holds 64 / bank banks, its waves split their block diagonals)
+    void set_mas_waves(int waves);
+    int  mas_waves() const { return mas_waves_; }
+    // times every supported wave count of the MAS local kernel in the built
+    // precision and keeps the fastest (the MFMA kernel of half-precision
+    // 32-slot banks has one wave per bank and nothing to choose)
+    Tiling autotune_mas(int reps);
     // average device time of one kernel of the iteration over `reps` launches
     // (0 MAS local, 1 preconditioner finish, 2 SpMV, 3 update), milliseconds
     double time_kernel(int which, int reps);
@@ -118,6 +126,7 @@ class Solver
     void ensure_stream();
     int  precision_  = kPrecFp32;
     int  entries_    = 4;
+    int  mas_waves_  = 4;
     bool built_      = false;
 
     // device

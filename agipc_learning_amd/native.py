@@ -96,6 +96,8 @@ def load():
         "masx_morton_partition_device": ([vp, C.c_int, vp, C.c_int, vp, C.c_longlong, vp, vp], C.c_int),
         "masx_morton_partition_device_noalloc": ([C.c_int, vp, C.c_int, vp, C.c_longlong, vp, vp], C.c_int),
         "masx_stream_rate": ([C.c_longlong, C.c_int, C.c_int, C.POINTER(C.c_double)], C.c_int),
+        "masx_set_mas_waves": ([vp, C.c_int], C.c_int),
+        "masx_autotune_mas": ([vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)], C.c_int),
         "masx_spmv_layout_check": ([C.c_int, C.c_longlong, ip, ip, C.c_int, C.c_uint, dp, C.POINTER(C.c_longlong),
                                     C.POINTER(C.c_longlong)], C.c_int),
         "masx_analyze": ([vp], C.c_int),
@@ -420,6 +422,16 @@ class Solver:
         e, ms = C.c_int(), C.c_double()
         check(self.L.masx_autotune(self.h, reps, C.byref(e), C.byref(ms)))
         return dict(entries_per_thread=e.value, ms=ms.value)
+
+    def set_mas_waves(self, waves):
+        check(self.L.masx_set_mas_waves(self.h, waves))
+
+    def autotune_mas(self, reps=50):
+        """Times every supported wave count of the MAS local kernel (built
+        precision) and keeps the fastest."""
+        w, ms = C.c_int(), C.c_double()
+        check(self.L.masx_autotune_mas(self.h, reps, C.byref(w), C.byref(ms)))
+        return dict(waves=w.value, ms=ms.value)
 
     def time_kernel(self, which, reps=20):
         ms = C.c_double()

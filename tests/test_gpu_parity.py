@@ -377,6 +377,39 @@ def test_spmv_tilings_agree_and_autotune():
     assert st["failure"] == 0 and abs(st["iterations"] - ref[0]) <= 1
 
 
+@pytest.mark.parametrize("bank", [32, 16])
+def test_mas_waves_agree_and_autotune(oracle, bank):
+    """The FP32 MAS local kernel with 4 or 8 waves per workgroup applies the
+    same preconditioner (the FP32 sums of the waves' partials in another
+    order: within 1e-6 of each other, 1e-4 of the oracle's apply) and gives
+    the same solve; the autotuner keeps one of them."""
+    w = W.fem_grid(12, n_contacts=300, seed=21, young_sigma=0.5)
+    s = session.solver_for(w, bank=bank)
+    s.build("fp32")
+    mas = oracle_mas(oracle, w, bank=bank)
+    r = np.random.default_rng(4).standard_normal(3 * w["block_rows"])
+    want = mas.apply(r)
+    b = dev(session.rhs_of(w))
+    its, zs = [], []
+    for waves in (4, 8):
+        s.set_mas_waves(waves)
+        z = torch.zeros_like(dev(r))
+        s.apply_preconditioner(dev(r), z)
+        zs.append(host(z))
+        assert np.linalg.norm(zs[-1] - want) <= 1e-4 * np.linalg.norm(want), waves
+        x = torch.zeros_like(b)
+        st = s.solve(b, x, rule=1, tol=1e-4)[0]
+        assert st["failure"] == 0 and st["rel_res_true"] <= 1e-4 * (1 + 1e-6), (waves, st)
+        its.append(st["iterations"])
+    assert np.linalg.norm(zs[0] - zs[1]) <= 1e-6 * np.linalg.norm(zs[0])
+    assert abs(its[0] - its[1]) <= 1
+    best = s.autotune_mas(5)
+    assert best["waves"] in (4, 8) and best["ms"] > 0
+    s.build("fp16")
+    half = s.autotune_mas(5)
+    assert (half["ms"] == 0.0) if bank == 32 else (half["waves"] in (4, 8) and half["ms"] > 0)
+
+
 def test_frozen_dump_replay_on_gpu(tmp_path):
     """A frozen system written in the reference's dump layout and read back
     solves with the reference-held counts (MAS32 over the identity partition)."""
