@@ -100,7 +100,8 @@ struct DevData
     const int4*     ugroup;      // {slot base, slot count, component begin, component end}
     const int*      comp_dslot;  // 4 per fine-bank component: group-local slot per deep level, -1 none
     float*          part_deep;   // float3 per slot, written by the update kernel
-    const int*      row_anc;     // 8 per internal row: its level-1 node and that node's ancestors, -1 padded
+    const int*      row_l1;      // per internal row: its level-1 node, -1 none
+    const int*      anc;         // 8 per coarse node (used for level-1 nodes): itself and its ancestors, -1 padded
     // ABD and diagonal block-Jacobi
     const float* abd_inv;   // 144 per body, row-major
     const int*   body_row;

@@ -1294,8 +1294,15 @@ __global__ void __launch_bounds__(256) k_precond_finish(DevData d, int force)
         if(type == kWgFemMas)
         {
             z0 = d.z[o]; z1 = d.z[o + 1]; z2 = d.z[o + 2];
-            const int4 a = reinterpret_cast<const int4*>(d.row_anc)[2 * (long long)row];
-            const int4 e = reinterpret_cast<const int4*>(d.row_anc)[2 * (long long)row + 1];
+            // the row's level-1 node and that node's ancestors (a table of the
+            // coarse nodes, shared by the rows of a component)
+            const int g1 = d.row_l1[row];
+            int4 a = make_int4(-1, -1, -1, -1), e = make_int4(-1, -1, -1, -1);
+            if(g1 >= 0)
+            {
+                a = reinterpret_cast<const int4*>(d.anc)[2 * (long long)g1];
+                e = reinterpret_cast<const int4*>(d.anc)[2 * (long long)g1 + 1];
+            }
             const int ids[8] = {a.x, a.y, a.z, a.w, e.x, e.y, e.z, e.w};
             float g[8][3];
 #pragma unroll

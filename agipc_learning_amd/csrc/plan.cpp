@@ -545,7 +545,7 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
     // level-1 nodes, level-1 parent of each fine row
     P.bank_kind.assign(P.n_banks, 0);
     P.anc.assign(8 * (size_t)std::max(n_coarse, 1), -1);
-    P.row_anc.assign(8 * (size_t)P.n_rows, -1);
+    P.row_l1.assign(P.n_rows, -1);
     std::vector<std::vector<int>> desc(n_coarse);
     for(int s = 0; s < S; ++s)
     {
@@ -577,8 +577,7 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
             const int v = d.part_to_real[slot];
             if(v < 0 || h.levels < 2) continue;
             const int g1 = gcoarse(s, h.going_next[v]);
-            std::copy(P.anc.begin() + 8 * (size_t)g1, P.anc.begin() + 8 * (size_t)g1 + 8,
-                      P.row_anc.begin() + 8 * (size_t)(P.fem_row0[s] + slot));
+            P.row_l1[P.fem_row0[s] + slot] = g1;
         }
     }
     P.desc_ptr.assign(n_coarse + 1, 0);
@@ -961,9 +960,9 @@ void validate_plan(const Plan& P)
         if(v < 0 || v >= P.n_coarse) fail("desc_list");
     for(int v : P.anc)
         if(v < -1 || v >= P.n_coarse) fail("anc");
-    if(P.row_anc.size() != 8 * (size_t)P.n_rows) fail("row_anc");
-    for(int v : P.row_anc)
-        if(v < -1 || v >= P.n_coarse) fail("row_anc range");
+    if(P.row_l1.size() != (size_t)P.n_rows) fail("row_l1");
+    for(int v : P.row_l1)
+        if(v < -1 || v >= P.n_coarse) fail("row_l1 range");
     for(size_t g = 0; g + 1 < P.gal_level_ptr.size(); ++g)
         for(int e = P.gal_level_ptr[g]; e < P.gal_level_ptr[g + 1]; ++e)
             for(int k = P.gal_ptr[e]; k < P.gal_ptr[e + 1]; ++k)

@@ -108,8 +108,8 @@ struct Plan
     std::vector<int> desc_ptr;       // per global coarse node (level >= 2): range of its level-1 descendants
     std::vector<int> desc_list;
     std::vector<int> anc;            // per global coarse node, 8 ints: itself and its ancestors, -1 padded
-    std::vector<int> row_anc;        // per internal row, 8 ints: its level-1 node and that node's
-                                     // ancestors (global coarse indices), -1 padded
+    std::vector<int> row_l1;         // per internal row: its level-1 node (global coarse index), -1 none;
+                                     // the node's ancestors are its anc list
     // level >= 2 residuals pre-reduced per update group (256 rows of one system)
     std::vector<int> bank_ugroup;    // per fine bank: its update group (-1 if none)
     std::vector<int> ugroup;         // per group, 4 ints: slot base, slot count, component begin, end

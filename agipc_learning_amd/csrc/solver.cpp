@@ -272,7 +272,8 @@ void Solver::setup()
     d_ugroup_.upload(P.ugroup, st);
     d_comp_dslot_.upload(P.comp_dslot, st);
     d_part_deep_.alloc(3 * (size_t)std::max(P.n_deep_slots, 1));
-    d_row_anc_.upload(P.row_anc, st);
+    d_row_l1_.upload(P.row_l1, st);
+    d_anc_.upload(P.anc, st);
     d_minv_scale_.alloc(std::max(P.n_banks, 1));
     // the bank count may have grown with a new structure (alloc keeps a buffer that fits)
     d_minv_.alloc(std::max<size_t>(inverse_bytes(), 16));
@@ -463,7 +464,8 @@ void Solver::make_devdata()
     d.ugroup = reinterpret_cast<const int4*>(d_ugroup_.ptr);
     d.comp_dslot = d_comp_dslot_.ptr;
     d.part_deep = d_part_deep_.ptr;
-    d.row_anc = d_row_anc_.ptr;
+    d.row_l1 = d_row_l1_.ptr;
+    d.anc = d_anc_.ptr;
     d.abd_inv = d_abd_inv_.ptr;
     d.body_row = d_body_row_.ptr;
     d.diag_inv = d_diag_inv_.ptr;

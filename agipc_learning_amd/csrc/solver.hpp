@@ -140,7 +140,7 @@ class Solver
     DeviceArray<float>     d_minv_scale_;
     DeviceArray<int>       d_bank_vec_, d_bank_sys_, d_bank_kind_, d_comp_ptr_, d_comp_parent_;
     DeviceArray<uint32_t>  d_comp_mask_;
-    DeviceArray<int>       d_deep_ptr_, d_deep_list_, d_row_anc_, d_bank_ugroup_, d_ugroup_, d_comp_dslot_;
+    DeviceArray<int>       d_deep_ptr_, d_deep_list_, d_row_l1_, d_anc_, d_bank_ugroup_, d_ugroup_, d_comp_dslot_;
     DeviceArray<int>       d_gal_ptr_, d_gal_src_, d_ent_ptr_, d_ent_pos_, d_ent_src_;
     DeviceArray<float>     d_abd_inv_, d_diag_inv_;
     DeviceArray<int>       d_body_row_, d_body_src_, d_diag_src_, d_needs_diag_, d_row_mode_;

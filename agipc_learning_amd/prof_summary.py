@@ -74,15 +74,19 @@ def _busy_median(durations):
 
 
 def compare(bench_json, trace_csv):
+    """bench.py's per-kernel event times (the 64-system batch's FP32 kernels,
+    or the 40^3 workload's for --fem40-only) beside the trace's busy medians."""
     line = [l for l in open(bench_json).read().splitlines() if l.strip().startswith("{")][-1]
     bench = json.loads(line)
-    if bench.get("roofline") is None and bench.get("batch64"):  # bench.py --batch-only: FP32, SpMV tiling 4
-        us = bench["batch64"]["fp32"]["kernels_us"]
-        tiling = 4
+    if bench.get("batch64"):
+        obj = bench["batch64"]
+        us = obj["fp32"]["kernels_us"]
     else:
-        us = {k: v["us"] for k, v in bench["roofline"]["per_kernel"].items()}
-        tiling = bench["spmv_tiling"]["entries_per_thread"]
-    names = {"spmv": f"k_spmv_fused<{tiling}>", "mas_local": "k_mas_local<32, float>",
+        obj = bench["fem40c"]
+        us = {k: v["us"] for k, v in obj["roofline"]["per_kernel"].items()}
+    tiling = obj["spmv_tiling"]["entries_per_thread"]
+    waves = obj["spmv_tiling"].get("mas_local", {}).get("waves", 4)
+    names = {"spmv": f"k_spmv_fused<{tiling}>", "mas_local": f"k_mas_local<32, float, {waves}>",
              "precond_finish": "k_precond_finish", "update": "k_update"}
     d = collections.defaultdict(list)
     for r in csv.DictReader(open(trace_csv)):

@@ -95,7 +95,7 @@ def algorithmic_bytes(info, n_vertices, prec_bytes):
     }
 
 
-KERNEL_PREFIX = {"spmv": "k_spmv_fused", "mas_local": "k_mas_local<32, float>", "update": "k_update",
+KERNEL_PREFIX = {"spmv": "k_spmv_fused", "mas_local": "k_mas_local<32, float,", "update": "k_update",
                  "precond_finish": "k_precond_finish"}
 
 
@@ -310,6 +310,7 @@ def batch_leg(args):
     else:
         log("step: batch64 autotune SpMV tiling")
         tiling = s.autotune(20)
+    tiling["mas_local"] = s.autotune_mas(20)  # waves per workgroup of the FP32 local kernel
     log(f"  tiling {tiling}")
 
     def step(prec):
@@ -569,6 +570,7 @@ def fem40_leg(args, structure=True):
     else:
         log("step: autotune SpMV tiling")
         tiling = s.autotune(50)
+    tiling["mas_local"] = s.autotune_mas(50)
     log(f"  tiling {tiling}")
     precs = ["fp32"] + [p for p in args.precisions.split(",") if p and p != "fp32"]
     results = {}
