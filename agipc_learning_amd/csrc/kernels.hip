@@ -265,23 +265,28 @@ __global__ void __launch_bounds__(256) k_gal_gather(const double* fine, double* 
 // diagonal block also takes its transpose position.  Half formats store
 // 16-bit halves of 5 words per block (the tenth half is zero).
 // Half-precision inverses of 32-slot banks are stored as MFMA tiles (the
-// tile layout of mas_tiles.hpp); 16-slot banks and FP32 keep the block
-// diagonals.
+// tile layout of mas_tiles.hpp) when the MFMA local kernel reads them
+// (DevData::mas_waves == 1); otherwise, and for 16-slot banks and FP32, by
+// block diagonals.
 template <int BANK, typename T>
-constexpr bool kTileLayout = BANK == 32 && !std::is_same<T, float>::value;
+constexpr bool kTileCapable = BANK == 32 && !std::is_same<T, float>::value;
 
 template <int BANK, typename T>
-__device__ __forceinline__ void store_inverse_element(void* out, int bank, int i, int j, double v, double inv_scale)
+__device__ __forceinline__ void store_inverse_element(void* out, int bank, int i, int j, double v, double inv_scale,
+                                                      bool tiles)
 {
-    if constexpr(kTileLayout<BANK, T>)
+    if constexpr(kTileCapable<BANK, T>)
     {
-        // (i, j), i >= j, is the upper element (j, i); a diagonal tile also
-        // takes its mirror (i, j)
-        unsigned short* o = reinterpret_cast<unsigned short*>(out) + (size_t)bank * kTileBankHalves;
-        const unsigned short bits = (unsigned short)half_bits(v * inv_scale, (T*)nullptr);
-        o[tile_half_index(j, i)] = bits;
-        if(i != j && i / kTileDim == j / kTileDim) o[tile_half_index(i, j)] = bits;
-        return;
+        if(tiles)
+        {
+            // (i, j), i >= j, is the upper element (j, i); a diagonal tile
+            // also takes its mirror (i, j)
+            unsigned short* o = reinterpret_cast<unsigned short*>(out) + (size_t)bank * kTileBankHalves;
+            const unsigned short bits = (unsigned short)half_bits(v * inv_scale, (T*)nullptr);
+            o[tile_half_index(j, i)] = bits;
+            if(i != j && i / kTileDim == j / kTileDim) o[tile_half_index(i, j)] = bits;
+            return;
+        }
     }
     constexpr int NBLK = BANK * (BANK + 1) / 2;
     const int I = j / 3, J = i / 3, dd = J - I, len = BANK - dd;
@@ -361,6 +366,7 @@ __global__ void __launch_bounds__(128) k_mas_factor_sym(DevData d, const double*
     constexpr int NH    = factor_pass_max(DIM, NPASS);
     constexpr int NS    = NH > 2 * DIM + 4 ? NH : 2 * DIM + 4;
     constexpr bool kHalf = !std::is_same<T, float>::value;
+    const bool tiles = kTileCapable<BANK, T> && d.mas_waves == 1;  // the MFMA local kernel's layout
     static_assert(DIM % 16 == 0, "the 8 x 16 grid must tile the bank matrix");
     __shared__ double S[NS];
     __shared__ double red[2];
@@ -520,7 +526,7 @@ __global__ void __launch_bounds__(128) k_mas_factor_sym(DevData d, const double*
         for(int r = 2 * c; r < RR; ++r)
         {
             const int i = ty + 8 * r, j = tx + 16 * c;
-            if(i >= j) store_inverse_element<BANK, T>(out, bank, i, j, -a[tile8x16(RR, r, c)], inv_scale);
+            if(i >= j) store_inverse_element<BANK, T>(out, bank, i, j, -a[tile8x16(RR, r, c)], inv_scale, tiles);
         }
     if(tid == 0 && bad && status) atomicAdd(status, bad);
 }
@@ -2083,7 +2089,7 @@ hipError_t launch_gal_gather(const double* fine, double* E, const int* ptr, cons
 
 size_t mas_bank_bytes(int bank, int precision)
 {
-    if(bank == 32 && precision != kPrecFp32) return (size_t)kTileBankHalves * 2;
+    if(bank == 32 && precision != kPrecFp32) return (size_t)kTileBankHalves * 2;  // either layout fits
     const size_t nblk = (size_t)bank * (bank + 1) / 2;
     return nblk * (precision == kPrecFp32 ? 9 : 5) * 4;
 }
@@ -2189,7 +2195,8 @@ hipError_t launch_phase_finish(const DevData& d, hipStream_t st, int* host, int*
 hipError_t launch_mas_local(const DevData& d, int precision, int force, hipStream_t st, const LaunchEvents* ev)
 {
     if(d.n_banks <= 0) return hipSuccess;
-    if(d.mas_waves != 4 && d.mas_waves != 8) return hipErrorInvalidValue;
+    const bool mfma = d.bank == 32 && precision != kPrecFp32 && d.mas_waves == 1;
+    if(!mfma && d.mas_waves != 4 && d.mas_waves != 8) return hipErrorInvalidValue;
     const int bpw = 64 / d.bank;
     const dim3 g(blocks_for(d.n_banks, bpw)), b(256);
     const dim3 b8(512);
@@ -2202,8 +2209,10 @@ hipError_t launch_mas_local(const DevData& d, int precision, int force, hipStrea
     if(d.bank == 32)
     {
         const dim3 gm(blocks_for(d.n_banks, 4));
-        if(precision == kPrecFp16) MASX_DISPATCH((k_mas_local_mfma<__half>), gm, b, d, force);
-        else if(precision == kPrecBf16) MASX_DISPATCH((k_mas_local_mfma<__hip_bfloat16>), gm, b, d, force);
+        if(mfma && precision == kPrecFp16) MASX_DISPATCH((k_mas_local_mfma<__half>), gm, b, d, force);
+        else if(mfma) MASX_DISPATCH((k_mas_local_mfma<__hip_bfloat16>), gm, b, d, force);
+        else if(precision == kPrecFp16) MASX_LOCAL(32, __half);
+        else if(precision == kPrecBf16) MASX_LOCAL(32, __hip_bfloat16);
         else MASX_LOCAL(32, float);
     }
     else if(d.bank == 16)

@@ -193,11 +193,27 @@ void Solver::set_spmv_tiling(int entries_per_thread)
     }
 }
 
+// Half-precision 32-slot banks from this many banks on use the MFMA local
+// kernel by default (four banks per workgroup fill the card; below that the
+// VALU kernel's shorter latency wins, measured on 2,270 and 88,000 banks).
+constexpr int kMfmaAutoBanks = 16384;
+
+int Solver::effective_mas_waves() const
+{
+    const bool mfma_ok = plan_.bank == 32 && precision_ != kPrecFp32;
+    if(mas_waves_ == 1) return mfma_ok ? 1 : 4;
+    if(mas_waves_ == 4 || mas_waves_ == 8) return mas_waves_;
+    return mfma_ok && plan_.n_banks >= kMfmaAutoBanks ? 1 : 4;
+}
+
 void Solver::set_mas_waves(int waves)
 {
-    if(waves != 4 && waves != 8) throw std::invalid_argument("MAS local waves per workgroup must be 4 or 8");
+    if(waves != 0 && waves != 1 && waves != 4 && waves != 8)
+        throw std::invalid_argument("MAS local waves per workgroup must be 0 (automatic), 1 (MFMA), 4 or 8");
+    const bool tiles_before = effective_mas_waves() == 1;
     mas_waves_    = waves;
-    dd_.mas_waves = waves;
+    dd_.mas_waves = effective_mas_waves();
+    if(built_ && (dd_.mas_waves == 1) != tiles_before) built_ = false;  // the inverse is in the other layout
     if(graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     graph_exec_ = nullptr;
 }
@@ -205,15 +221,19 @@ void Solver::set_mas_waves(int waves)
 Solver::Tiling Solver::autotune_mas(int reps)
 {
     if(!built_) throw std::runtime_error("build() must precede autotune");
-    if(plan_.n_banks == 0 || (plan_.bank == 32 && precision_ != kPrecFp32)) return Tiling{mas_waves_, 0.0};
-    Tiling best{mas_waves_, 1e30};
-    for(const int w : {4, 8})
+    if(plan_.n_banks == 0) return Tiling{effective_mas_waves(), 0.0};
+    std::vector<int> cand = {4, 8};
+    if(plan_.bank == 32 && precision_ != kPrecFp32) cand.insert(cand.begin(), 1);
+    Tiling best{effective_mas_waves(), 1e30};
+    for(const int w : cand)
     {
         set_mas_waves(w);
+        if(!built_) build(precision_, nullptr);
         const double ms = time_kernel(0, reps);
         if(ms < best.ms) best = Tiling{w, ms};
     }
     set_mas_waves(best.entries);
+    if(!built_) build(precision_, nullptr);
     return best;
 }
 
@@ -425,7 +445,7 @@ void Solver::make_devdata()
     d.n_wg_upd = n_wg_upd_;
     d.bank = P.bank;
     d.n_banks = P.n_banks;
-    d.mas_waves = mas_waves_;
+    d.mas_waves = effective_mas_waves();
     d.n_fine_banks = P.n_fine_banks;
     d.nnz = P.nnz;
     d.nnz_s = (long long)spmv_layout_.col.size();
@@ -530,6 +550,7 @@ void Solver::build(int precision, hipStream_t user)
     hipStream_t st = enter(user);
     const bool prec_changed = precision != precision_;
     precision_ = precision;
+    dd_.mas_waves = effective_mas_waves();  // the factor writes the layout the local kernel reads
     const char* old_minv = d_minv_.ptr;
     d_minv_.alloc(std::max<size_t>(inverse_bytes(), 16));  // reuses the buffer when it fits
     if(inverse_bytes() > d_minv_.cap) throw std::logic_error("inverse buffer smaller than the banks need");
@@ -742,6 +763,11 @@ void Solver::download_inverse(std::vector<float>& out)
     if(plan_.n_banks)
         MASX_HIP(hipMemcpy(scale.data(), d_minv_scale_.ptr, sizeof(float) * plan_.n_banks, hipMemcpyDeviceToHost));
     const size_t bank_bytes = mas_bank_bytes(B, precision_);
+    // words per bank as the local kernels index them: FP32 9 per block, the
+    // half block-diagonal layout 5 per block, the MFMA tiles kTileBankHalves / 2
+    const size_t stride_words = precision_ == kPrecFp32 ? nblk * 9
+                                : (B == 32 && dd_.mas_waves == 1) ? (size_t)kTileBankHalves / 2
+                                                                   : nblk * 5;
     std::vector<uint32_t> raw((size_t)plan_.n_banks * bank_bytes / 4);
     if(!raw.empty()) MASX_HIP(hipMemcpy(raw.data(), d_minv_.ptr, raw.size() * 4, hipMemcpyDeviceToHost));
     auto half_to_float = [&](uint32_t v16) {
@@ -761,7 +787,7 @@ void Solver::download_inverse(std::vector<float>& out)
     };
     for(int b = 0; b < plan_.n_banks; ++b)
     {
-        const uint32_t* bw = raw.data() + (size_t)b * bank_bytes / 4;
+        const uint32_t* bw = raw.data() + (size_t)b * stride_words;
         float* dst = out.data() + (size_t)b * nblk * 9;
         for(int dd = 0; dd < B; ++dd)
         {
@@ -776,7 +802,7 @@ void Solver::download_inverse(std::vector<float>& out)
                     {
                         std::memcpy(&v, &bw[9 * off + c * len + I], 4);
                     }
-                    else if(B == 32)
+                    else if(B == 32 && dd_.mas_waves == 1)
                     {
                         // MFMA tiles: element (3 I + c / 3, 3 (I + dd) + c % 3) of the upper triangle
                         const int r = 3 * I + c / 3, col = 3 * (I + dd) + c % 3;

@@ -98,13 +98,16 @@ class Solver
     };
     // times every supported SpMV tiling on the built system and keeps the fastest
     Tiling autotune_spmv(int reps);
-    // waves per workgroup of the FP32 MAS local kernel (4 or 8; each workgroup
-    // holds 64 / bank banks, its waves split their block diagonals)
+    // MAS local kernel: 4 or 8 waves per workgroup (VALU; each workgroup holds
+    // 64 / bank banks, its waves split their block diagonals), 1 (the MFMA
+    // kernel, one wave per bank: FP16 / BF16 inverses of 32-slot banks; other
+    // cases fall back to 4), or 0 (automatic: MFMA for half-precision 32-slot
+    // banks from kMfmaAutoBanks banks on, else 4).  A change that moves the
+    // half-precision inverse between the two layouts needs a new build().
     void set_mas_waves(int waves);
-    int  mas_waves() const { return mas_waves_; }
-    // times every supported wave count of the MAS local kernel in the built
-    // precision and keeps the fastest (the MFMA kernel of half-precision
-    // 32-slot banks has one wave per bank and nothing to choose)
+    int  mas_waves() const { return effective_mas_waves(); }
+    // times every MAS local variant the built precision has and keeps the
+    // fastest (rebuilding when the variant's inverse layout differs)
     Tiling autotune_mas(int reps);
     // average device time of one kernel of the iteration over `reps` launches
     // (0 MAS local, 1 preconditioner finish, 2 SpMV, 3 update), milliseconds
@@ -126,7 +129,8 @@ class Solver
     void ensure_stream();
     int  precision_  = kPrecFp32;
     int  entries_    = 4;
-    int  mas_waves_  = 4;
+    int  mas_waves_  = 0;
+    int  effective_mas_waves() const;
     bool built_      = false;
 
     // device

@@ -85,7 +85,7 @@ def compare(bench_json, trace_csv):
         obj = bench["fem40c"]
         us = {k: v["us"] for k, v in obj["roofline"]["per_kernel"].items()}
     tiling = obj["spmv_tiling"]["entries_per_thread"]
-    waves = obj["spmv_tiling"].get("mas_local", {}).get("waves", 4)
+    waves = obj["spmv_tiling"].get("mas_local", {}).get("fp32", {}).get("waves", 4)
     names = {"spmv": f"k_spmv_fused<{tiling}>", "mas_local": f"k_mas_local<32, float, {waves}>",
              "precond_finish": "k_precond_finish", "update": "k_update"}
     d = collections.defaultdict(list)

@@ -310,7 +310,6 @@ def batch_leg(args):
     else:
         log("step: batch64 autotune SpMV tiling")
         tiling = s.autotune(20)
-    tiling["mas_local"] = s.autotune_mas(20)  # waves per workgroup of the FP32 local kernel
     log(f"  tiling {tiling}")
 
     def step(prec):
@@ -323,8 +322,13 @@ def batch_leg(args):
     out = {"systems": len(ws), "vertices": nv, "dof": 3 * nv, "upper_blocks": int(info["nb_total"]),
            "banks": int(info["n_banks"]), "setup_ms": round(setup_ms, 1), "spmv_tiling": tiling}
     sols = {}
+    tiling["mas_local"] = {}
     for prec in precs:
-        log(f"step: batch64 {prec} ({args.warmup} warmup + {args.steps} steps)")
+        # the MAS local variant for this precision (VALU waves per workgroup, or MFMA)
+        s.build(prec)
+        tiling["mas_local"][prec] = s.autotune_mas(20)
+        log(f"step: batch64 {prec} ({args.warmup} warmup + {args.steps} steps), local variant "
+            f"{tiling['mas_local'][prec]}")
         for _ in range(args.warmup):
             step(prec)
         torch.cuda.synchronize()
@@ -570,12 +574,15 @@ def fem40_leg(args, structure=True):
     else:
         log("step: autotune SpMV tiling")
         tiling = s.autotune(50)
-    tiling["mas_local"] = s.autotune_mas(50)
     log(f"  tiling {tiling}")
     precs = ["fp32"] + [p for p in args.precisions.split(",") if p and p != "fp32"]
     results = {}
+    tiling["mas_local"] = {}
     for prec in precs:
-        log(f"step: fem40c {prec} ({args.warmup} warmup + {args.fem40_steps} steps)")
+        s.build(prec)
+        tiling["mas_local"][prec] = s.autotune_mas(50)
+        log(f"step: fem40c {prec} ({args.warmup} warmup + {args.fem40_steps} steps), local variant "
+            f"{tiling['mas_local'][prec]}")
         ms, st = run(prec, args.fem40_steps, args.warmup)
         if st["failure"] != 0 or st["rel_res_true"] > args.tol * (1 + 1e-6):
             raise SystemExit(f"{prec} solve did not reach the tolerance: {st}")

@@ -167,14 +167,19 @@ int masx_set_spmv_tiling(masx_solver* s, int entries_per_thread);
  * each) and keeps the fastest; reports it and its time per launch (ms).
  * Leaves the iteration state frozen: call masx_solve afterwards as usual. */
 int masx_autotune(masx_solver* s, int reps, int* entries_per_thread, double* ms);
-/* MAS local solve of FP32 inverses (and of 16-slot banks): 4 or 8 waves per
- * workgroup share the block diagonals of its banks.  Half-precision inverses
- * of 32-slot banks run one bank per wave on MFMA and ignore this. */
+/* MAS local solve variant.  waves = 4 or 8: the VALU kernel, that many
+ * waves per workgroup sharing the block diagonals of its banks; 1: the MFMA
+ * kernel, one wave per bank (FP16 / BF16 inverses of 32-slot banks; other
+ * cases run 4); 0 (the default): MFMA for half-precision 32-slot banks from
+ * 16,384 banks on, else 4.  The two kernels store a half-precision inverse in
+ * different layouts: a change between them needs a new masx_build before the
+ * next solve (masx_solve reports the missing build). */
 int masx_set_mas_waves(masx_solver* s, int waves);
-/* Times the MAS local kernel of the built precision with every supported
- * wave count (`reps` launches each) and keeps the fastest; reports it and its
- * time per launch (ms; 0 when the built precision has nothing to choose).
- * Leaves the iteration state frozen like masx_autotune. */
+/* Times every MAS local variant of the built precision (`reps` launches
+ * each; for half-precision 32-slot banks 1, 4 and 8, else 4 and 8), keeps the
+ * fastest (rebuilding the preconditioner for a layout change) and reports it
+ * and its time per launch (ms).  Leaves the iteration state frozen like
+ * masx_autotune. */
 int masx_autotune_mas(masx_solver* s, int reps, int* waves, double* ms);
 /* Average execution time of one kernel (0 MAS local, 1 preconditioner finish,
  * 2 fused SpMV, 3 update) over `reps` back-to-back launches, milliseconds:

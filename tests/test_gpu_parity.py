@@ -405,9 +405,42 @@ def test_mas_waves_agree_and_autotune(oracle, bank):
     assert abs(its[0] - its[1]) <= 1
     best = s.autotune_mas(5)
     assert best["waves"] in (4, 8) and best["ms"] > 0
-    s.build("fp16")
-    half = s.autotune_mas(5)
-    assert (half["ms"] == 0.0) if bank == 32 else (half["waves"] in (4, 8) and half["ms"] > 0)
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_half_local_variants_agree_and_autotune(precision):
+    """Half-precision 32-slot banks: the MFMA local kernel (tile layout) and
+    the 4- and 8-wave VALU kernels (block-diagonal layout) apply the same
+    stored inverse (FP32 accumulation in other orders: within 1e-5 of each
+    other) and give the same solve; switching between the layouts asks for a
+    new build; the autotuner keeps one of the three."""
+    w = W.fem_grid(12, n_contacts=300, seed=22, young_sigma=0.5)
+    s = session.solver_for(w)
+    r = dev(np.random.default_rng(5).standard_normal(3 * w["block_rows"]))
+    b = dev(session.rhs_of(w))
+    zs, its = [], []
+    for waves in (1, 4, 8):
+        s.set_mas_waves(waves)
+        s.build(precision)
+        z = torch.zeros_like(r)
+        s.apply_preconditioner(r, z)
+        zs.append(host(z))
+        x = torch.zeros_like(b)
+        st = s.solve(b, x, rule=1, tol=1e-4)[0]
+        assert st["failure"] == 0 and st["rel_res_true"] <= 1e-4 * (1 + 1e-6), (waves, st)
+        its.append(st["iterations"])
+    for z in zs[1:]:
+        assert np.linalg.norm(z - zs[0]) <= 1e-5 * np.linalg.norm(zs[0])
+    assert max(its) - min(its) <= 1
+    s.set_mas_waves(1)  # VALU layout -> MFMA layout: the built inverse no longer fits
+    with pytest.raises(RuntimeError, match="build"):
+        s.solve(b, torch.zeros_like(b), rule=1, tol=1e-4)
+    s.build(precision)
+    best = s.autotune_mas(5)
+    assert best["waves"] in (1, 4, 8) and best["ms"] > 0
+    x = torch.zeros_like(b)
+    st = s.solve(b, x, rule=1, tol=1e-4)[0]
+    assert st["failure"] == 0 and abs(st["iterations"] - its[0]) <= 1
 
 
 def test_frozen_dump_replay_on_gpu(tmp_path):
