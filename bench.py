@@ -99,14 +99,18 @@ KERNEL_PREFIX = {"spmv": "k_spmv_fused", "mas_local": "k_mas_local<32, float,", 
                  "precond_finish": "k_precond_finish"}
 
 
-def measured_traffic(kernel, workload_is_default, spmv_entries, summary="fem40c_hbm_traffic.json"):
-    """HBM bytes per launch of `kernel` from the committed counter summary of
-    this workload (FETCH_SIZE + WRITE_SIZE runs, copy-calibrated), or None."""
+def measured_traffic(kernel, workload_is_default, spmv_entries, summary="fem40c_hbm_traffic.json", waves=None):
+    """HBM bytes per launch of `kernel` (the SpMV at `spmv_entries` per
+    thread, the FP32 MAS local kernel at `waves` waves per workgroup) from the
+    committed counter summary of this workload (FETCH_SIZE + WRITE_SIZE runs,
+    copy-calibrated), or None."""
     path = os.path.join(ROOT, "profiles", summary)
     if not workload_is_default or not os.path.exists(path):
         return None
     data = json.load(open(path))["kernels"]
     want = f"k_spmv_fused<{spmv_entries}>" if kernel == "spmv" else KERNEL_PREFIX[kernel]
+    if kernel == "mas_local" and waves:
+        want = f"{KERNEL_PREFIX[kernel]} {waves}>"
     for name, v in data.items():
         if name.startswith(want):
             return int(v["bytes"])
@@ -371,7 +375,8 @@ def batch_leg(args):
             out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                                "traffic": measured_traffic(dom, True, tiling["entries_per_thread"],
-                                                           "batch64_hbm_traffic.json"),
+                                                           "batch64_hbm_traffic.json",
+                                                           tiling["mas_local"]["fp32"]["waves"]),
                                "algorithmic_bytes": alg[dom], "avg_us": kern[dom],
                                "per_kernel": {k: {"us": kern[k], "GB/s": out[prec]["kernels_GBs"][k],
                                                   "algorithmic_bytes": alg[k]} for k in kern}}
@@ -614,7 +619,8 @@ def fem40_leg(args, structure=True):
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": measured_traffic(dominant, (args.n, args.contacts, args.levels) == (40, 20000, 4),
-                                            tiling["entries_per_thread"]),
+                                            tiling["entries_per_thread"],
+                                            waves=tiling["mas_local"]["fp32"]["waves"]),
                 "algorithmic_bytes": alg[dominant], "avg_us": round(kern[dominant], 2),
                 "per_kernel": {k: {"us": round(kern[k], 2), "GB/s": round(alg[k] / (kern[k] * 1e-6) / 1e9, 1)}
                                for k in kern}}
