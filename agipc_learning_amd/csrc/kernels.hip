@@ -1400,18 +1400,29 @@ __global__ void __launch_bounds__(256) k_precond_finish(DevData d, int force)
     if(threadIdx.x == 0) d.part_pre[wgi] = part;
 }
 
-// FP64 true residual r_true = b - A x of rows [r_begin, r_end) (one thread
-// per row, rows strided by the workgroup), from the FP64 input blocks; writes
+// FP64 true residual r_true = b - A x of rows [r_begin, r_end) (x the FP32
+// iterate), from the FP64 input blocks over the full block CSR; writes
 // r_true and returns this thread's share of |r_true|^2.  With `drift`, also
 // adds this thread's share of |r_true - r|^2 (r the recursive residual of the
-// same iterate) to *drift.
+// same iterate) to *drift.  G lanes share a row, G the largest power of two
+// <= 64 with G * rows <= the workgroup (so the few long rows of a body tile
+// are not one thread's serial loop): each lane sums a strided share of the
+// row's entries and the group adds them by a butterfly, a fixed order for a
+// given range, so the result is deterministic.
 __device__ double true_residual_rows(const DevData& d, int r_begin, int r_end, double* drift = nullptr)
 {
+    const int nr = r_end - r_begin;
+    int G = 1;
+    while(G < 64 && 2 * G * nr <= (int)blockDim.x) G *= 2;
+    const int sub = (int)threadIdx.x % G, per = (int)blockDim.x / G;
     double part = 0.0;
-    for(int row = r_begin + (int)threadIdx.x; row < r_end; row += blockDim.x)
+    for(int base = 0; base < nr; base += per)  // uniform trip count: the butterflies see whole groups
     {
+        const int  row   = r_begin + base + (int)threadIdx.x / G;
+        const bool valid = row < r_end;
         double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-        for(int e = d.row_ptr[row]; e < d.row_ptr[row + 1]; ++e)
+        const int e_end = valid ? d.row_ptr[row + 1] : 0;
+        for(int e = valid ? d.row_ptr[row] + sub : 0; e < e_end; e += G)
         {
             const int code = d.entry_src[e];
             const double* B = d.vals64 + 9 * (long long)(code & 0x7fffffff);
@@ -1430,15 +1441,24 @@ __device__ double true_residual_rows(const DevData& d, int r_begin, int r_end, d
                 a2 += B[6] * x0 + B[7] * x1 + B[8] * x2;
             }
         }
-        const long long o = 3 * (long long)row;
-        const double t0 = d.b64[o] - a0, t1 = d.b64[o + 1] - a1, t2 = d.b64[o + 2] - a2;
-        if(drift)
+        for(int off = G >> 1; off > 0; off >>= 1)
         {
-            const double e0 = t0 - (double)d.r[o], e1 = t1 - (double)d.r[o + 1], e2 = t2 - (double)d.r[o + 2];
-            *drift += e0 * e0 + e1 * e1 + e2 * e2;
+            a0 += __shfl_xor(a0, off, 64);
+            a1 += __shfl_xor(a1, off, 64);
+            a2 += __shfl_xor(a2, off, 64);
         }
-        d.rt[o] = (float)t0; d.rt[o + 1] = (float)t1; d.rt[o + 2] = (float)t2;
-        part += t0 * t0 + t1 * t1 + t2 * t2;
+        if(valid && sub == 0)
+        {
+            const long long o = 3 * (long long)row;
+            const double t0 = d.b64[o] - a0, t1 = d.b64[o + 1] - a1, t2 = d.b64[o + 2] - a2;
+            if(drift)
+            {
+                const double e0 = t0 - (double)d.r[o], e1 = t1 - (double)d.r[o + 1], e2 = t2 - (double)d.r[o + 2];
+                *drift += e0 * e0 + e1 * e1 + e2 * e2;
+            }
+            d.rt[o] = (float)t0; d.rt[o + 1] = (float)t1; d.rt[o + 2] = (float)t2;
+            part += t0 * t0 + t1 * t1 + t2 * t2;
+        }
     }
     return part;
 }
