@@ -19,8 +19,8 @@ CSRC = os.path.join(HERE, "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MASX_ARCH", "gfx950")
 
-MASX_SOURCES = ["kernels.hip", "converter.hip", "hierarchy.hip", "fem.hip", "morton.hip", "bandwidth.hip", "plan.cpp", "solver.cpp",
-                "masx_abi.cpp"]
+MASX_SOURCES = ["kernels.hip", "converter.hip", "hierarchy.hip", "fem.hip", "morton.hip", "bandwidth.hip", "devplan.hip",
+                "plan.cpp", "solver.cpp", "solver_devplan.cpp", "masx_abi.cpp"]
 GEN_SOURCES = ["gen/masx_gen.cpp"]
 
 

@@ -397,6 +397,37 @@ int masx_time_kernel(masx_solver* s, int which, int reps, double* ms)
     });
 }
 
+int masx_set_device_structure(masx_solver* s, int on)
+{
+    return guard([&] {
+        require(s != nullptr, "null solver");
+        s->impl.set_device_structure(on);
+    });
+}
+
+int masx_device_structure_used(masx_solver* s, int* used)
+{
+    return guard([&] {
+        require(s && used, "null argument");
+        *used = s->impl.device_structure_used() ? 1 : 0;
+    });
+}
+
+int masx_plan_array(masx_solver* s, int which, int* out, long long capacity, long long* n)
+{
+    return guard([&] {
+        require(s && n && s->impl.is_setup(), "solver not set up");
+        std::vector<int> v;
+        s->impl.download_plan_array(which, v);
+        *n = (long long)v.size();
+        if(out)
+        {
+            require((long long)v.size() <= capacity, "output buffer too small");
+            std::memcpy(out, v.data(), v.size() * sizeof(int));
+        }
+    });
+}
+
 int masx_plan_info_get(masx_solver* s, masx_plan_info* info)
 {
     return guard([&] {

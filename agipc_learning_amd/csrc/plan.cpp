@@ -320,17 +320,20 @@ void build_hierarchy(Hierarchy& h, int n_fem, int n_slots, int bank, int levels,
 // ----------------------------------------------------------------- plan ----
 
 
-void build_plan(Plan& P, std::vector<SystemDesc>& sys)
+void build_plan(Plan& P, std::vector<SystemDesc>& sys, bool structure_on_device)
 {
     PhaseTimer tm("plan");
     const int S = (int)sys.size();
     if(S == 0) throw std::invalid_argument("no systems");
+    if(structure_on_device && (S != 1 || !sys[0].use_mas || sys[0].n_abd != 0 || !sys[0].hier_ready))
+        throw std::logic_error("the device structure build takes one MAS system without bodies");
     const int bank = sys[0].bank;
     for(auto& s : sys)
         if(s.use_mas && s.bank != bank) throw std::invalid_argument("all systems of a batch share one bank size");
     P     = Plan{};
     P.bank = bank;
     P.n_sys = S;
+    P.structure_on_device = structure_on_device;
     P.row_off.assign(S + 1, 0);
     P.ext_off.assign(S + 1, 0);
     P.fem_row0.assign(S, 0);
@@ -383,6 +386,7 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
             if(d.rows[k] < 0 || d.rows[k] >= d.block_rows || d.cols[k] < 0 || d.cols[k] >= d.block_rows) bad |= 1;
         if(bad) throw std::invalid_argument("block index out of range");
     }
+    if(!structure_on_device)
     {
         // items: the upper blocks of all systems; block k of system s lands in
         // row R as (C, g) and, off the diagonal, in row C as (R, g | T)
@@ -448,12 +452,12 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
     }
     tm.mark("block CSR");
     // ---- global 3x3 block-Jacobi and ABD bodies
-    P.diag_src.assign(P.n_rows, -1);
+    P.diag_src.assign(structure_on_device ? 0 : P.n_rows, -1);
     P.row_needs_diag.assign(P.n_rows, 0);
     for(int s = 0; s < S; ++s)
     {
         auto& d = sys[s];
-        for(long long k = 0; k < d.nb; ++k)
+        for(long long k = 0; !structure_on_device && k < d.nb; ++k)
             if(d.rows[k] == d.cols[k]) P.diag_src[P.ext_to_int[P.ext_off[s] + d.rows[k]]] = (int)(P.src_off[s] + k);
         for(int r = P.row_off[s]; r < P.row_off[s + 1]; ++r)
         {
@@ -704,6 +708,7 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
     }
 
     tm.mark("update groups");
+    if(structure_on_device) return;
     // Galerkin coarse operators A_l = P_l^T A P_l (all entries, not only in-bank),
     // built level by level from the previous level; group g holds level g+1 of
     // every system.
@@ -835,6 +840,7 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys)
     }
     P.n_gal = (long long)E_ij.size();
     if((long long)P.gal_src.size() >= (1ll << 29)) throw std::invalid_argument("Galerkin map too large");
+    for(int e : P.gal_level_ptr) P.gal_level_src.push_back(P.gal_ptr[e]);
 
     tm.mark("Galerkin maps");
     // per-bank entries for the dense local assembly: a fine bank's upper
@@ -905,12 +911,15 @@ void validate_plan(const Plan& P)
     auto fail = [](const char* what) { throw std::logic_error(std::string("plan check failed: ") + what); };
     const long long nb = P.nb_total;
     const int nblk = P.bank * (P.bank + 1) / 2;
-    if((int)P.row_ptr.size() != P.n_rows + 1 || P.row_ptr[0] != 0 || P.row_ptr.back() != P.nnz) fail("row_ptr");
-    for(int r = 0; r < P.n_rows; ++r)
+    const bool host = !P.structure_on_device;  // else the arrays the device builds exist only there
+    if(host && ((int)P.row_ptr.size() != P.n_rows + 1 || P.row_ptr[0] != 0 || P.row_ptr.back() != P.nnz))
+        fail("row_ptr");
+    for(int r = 0; host && r < P.n_rows; ++r)
         if(P.row_ptr[r + 1] < P.row_ptr[r]) fail("row_ptr order");
     int bad_e = 0;
+    const long long nnz_host = host ? P.nnz : 0;
 #pragma omp parallel for reduction(| : bad_e) schedule(static)
-    for(long long e = 0; e < P.nnz; ++e)
+    for(long long e = 0; e < nnz_host; ++e)
     {
         if(P.col[e] < 0 || P.col[e] >= P.n_rows) bad_e |= 1;
         if((P.entry_src[e] & 0x7fffffff) >= nb) bad_e |= 2;
@@ -929,6 +938,7 @@ void validate_plan(const Plan& P)
         if(P.comp_ptr[b + 1] < P.comp_ptr[b]) fail("comp_ptr");
         for(int k = P.comp_ptr[b]; k < P.comp_ptr[b + 1]; ++k)
             if(P.comp_parent[k] < 0 || P.comp_parent[k] >= P.n_coarse) fail("comp_parent");
+        if(!host) continue;
         if(P.bank_ent_ptr[b + 1] < P.bank_ent_ptr[b]) fail("bank_ent_ptr");
         for(int k = P.bank_ent_ptr[b]; k < P.bank_ent_ptr[b + 1]; ++k)
         {
@@ -963,7 +973,7 @@ void validate_plan(const Plan& P)
     if(P.row_l1.size() != (size_t)P.n_rows) fail("row_l1");
     for(int v : P.row_l1)
         if(v < -1 || v >= P.n_coarse) fail("row_l1 range");
-    for(size_t g = 0; g + 1 < P.gal_level_ptr.size(); ++g)
+    for(size_t g = 0; host && g + 1 < P.gal_level_ptr.size(); ++g)
         for(int e = P.gal_level_ptr[g]; e < P.gal_level_ptr[g + 1]; ++e)
             for(int k = P.gal_ptr[e]; k < P.gal_ptr[e + 1]; ++k)
             {
@@ -1042,6 +1052,7 @@ void build_spmv_layout(SpmvLayout& L, int n_sys, const std::vector<int>& row_off
     }
     const long long total = e_base[T] + kSpmvSlack;
     if(total > 0x7fffffffLL || q_base[T] > 0x7fffffffLL) throw std::length_error("SpMV layout too large");
+    L.total     = total;
     L.n_entries = stored;
     L.n_in_tile = q_base[T];
     L.row_ptr.assign(n + 1, 0);

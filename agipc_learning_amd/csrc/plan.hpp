@@ -126,6 +126,7 @@ struct Plan
     std::vector<int> gal_ptr;           // per E entry: range in gal_src
     std::vector<int> gal_src;           // source index << 2 | mode (0 as is, 1 transpose, 2 sym)
     std::vector<int> gal_src_is_fine;   // per launch group: 1 = sources are fine upper blocks
+    std::vector<int> gal_level_src;     // per launch group boundary: gal_ptr there (source offsets)
     long long n_gal = 0;
     // per bank entry lists for the dense local assembly
     std::vector<int> bank_ent_ptr;      // n_banks+1
@@ -140,9 +141,16 @@ struct Plan
     // diagonal 3x3 inverse (global block-Jacobi): per internal row, diagonal source or -1
     std::vector<int> diag_src;
     std::vector<int> row_needs_diag;    // 1 if no local preconditioner covers the row
+    // The block CSR (row_ptr, col, entry_src), diag_src, the Galerkin maps
+    // (gal_ptr, gal_src) and the bank entry lists are built on the device
+    // (devplan.hpp) and exist only there; their sizes (nnz, n_gal,
+    // gal_level_ptr, gal_level_src) are filled in by that build.
+    bool structure_on_device = false;
 };
 
-void build_plan(Plan& plan, std::vector<SystemDesc>& systems);
+// structure_on_device: one MAS system without affine bodies whose hierarchy
+// is ready; the host analysis then skips what the device builds
+void build_plan(Plan& plan, std::vector<SystemDesc>& systems, bool structure_on_device = false);
 // Bounds of every index array the kernels dereference; throws on violation.
 void validate_plan(const Plan& plan);
 
@@ -169,6 +177,7 @@ struct SpmvLayout
     std::vector<int> src;         // per entry: upper block | transposed flag (bit 31), as entry_src;
                                   // kSpmvPadSrc for padding and slack
     std::vector<int> tptr;        // n_rows+1: cumulative transposed products
+    long long total = 0;          // entry slots (col.size(): padding and slack included)
     long long n_entries = 0;      // stored matrix entries (no padding)
     long long n_in_tile = 0;      // off-diagonal pairs stored once
 };

@@ -53,6 +53,7 @@ template struct DeviceArray<uint32_t>;
 template struct DeviceArray<int4>;
 template struct DeviceArray<int2>;
 template struct DeviceArray<SysState>;
+template struct DeviceArray<uint64_t>;
 
 Solver::Solver() = default;
 
@@ -67,6 +68,7 @@ void Solver::ensure_stream()
 
 void Solver::analyze()
 {
+    dev_struct_ = false;
     build_plan(plan_, systems_);
     PhaseTimer tm("analyze");
     validate_plan(plan_);
@@ -77,6 +79,7 @@ void Solver::analyze()
 void Solver::structure_changed(int s)
 {
     systems_.at(s).hier_ready = false;
+    if(s == hier_dev_sys_) hier_dev_sys_ = -1;
     analyzed_   = false;
     setup_done_ = false;
     built_      = false;
@@ -107,8 +110,10 @@ void Solver::build_hierarchies_on_device()
         }
         catch(const std::invalid_argument&)
         {
+            hier_dev_sys_ = -1;
             continue;
         }
+        hier_dev_sys_ = (int)(&d - systems_.data());
         Hierarchy& h = d.hier;
         const DeviceHierarchy& g = *hier_dev_;
         h.bank    = d.bank;
@@ -255,20 +260,39 @@ void Solver::setup()
 {
     PhaseTimer tm("setup");
     ensure_stream();
+    hipStream_t st = work_;
     if(!analyzed_)
     {
         build_hierarchies_on_device();
         tm.mark("device hierarchy");
-        analyze();
-        tm.mark("host plan");
+        if(device_structure_eligible())
+        {
+            build_plan(plan_, systems_, true);
+            validate_plan(plan_);
+            dev_struct_ = true;
+            analyzed_   = true;
+            tm.mark("host plan (MAS tables)");
+        }
+        else
+        {
+            analyze();
+            tm.mark("host plan");
+        }
     }
     const Plan& P = plan_;
-    hipStream_t st = work_;
-    d_row_ptr_.upload(P.row_ptr, st);
-    d_col_.upload(P.col, st);
-    d_entry_src_.upload(P.entry_src, st);
-    d_row_sys_.upload(P.row_sys, st);
     d_e2i_.upload(P.ext_to_int, st);
+    if(dev_struct_)
+    {
+        build_structure_on_device(st);
+        tm.mark("device structure");
+    }
+    else
+    {
+        d_row_ptr_.upload(P.row_ptr, st);
+        d_col_.upload(P.col, st);
+        d_entry_src_.upload(P.entry_src, st);
+    }
+    d_row_sys_.upload(P.row_sys, st);
     d_vals64_.alloc(9 * (size_t)P.nb_total);
     d_E_.alloc(9 * (size_t)std::max<long long>(P.n_gal, 1));
     const size_t nv = 3 * (size_t)P.n_rows;
@@ -297,14 +321,18 @@ void Solver::setup()
     d_minv_scale_.alloc(std::max(P.n_banks, 1));
     // the bank count may have grown with a new structure (alloc keeps a buffer that fits)
     d_minv_.alloc(std::max<size_t>(inverse_bytes(), 16));
-    d_gal_ptr_.upload(P.gal_ptr, st);
-    d_gal_src_.upload(P.gal_src, st);
-    d_ent_ptr_.upload(P.bank_ent_ptr, st);
-    d_ent_pos_.upload(P.bank_ent_pos, st);
-    d_ent_src_.upload(P.bank_ent_src, st);
+    if(!dev_struct_)
+    {
+        d_gal_ptr_.upload(P.gal_ptr, st);
+        d_gal_src_.upload(P.gal_src, st);
+        d_ent_ptr_.upload(P.bank_ent_ptr, st);
+        d_ent_pos_.upload(P.bank_ent_pos, st);
+        d_ent_src_.upload(P.bank_ent_src, st);
+        d_diag_src_.upload(P.diag_src, st);
+        n_bank_ent_ = P.bank_ent_ptr.empty() ? 0 : P.bank_ent_ptr.back();
+    }
     d_body_row_.upload(P.body_row, st);
     d_body_src_.upload(P.body_src, st);
-    d_diag_src_.upload(P.diag_src, st);
     d_needs_diag_.upload(P.row_needs_diag, st);
     d_row_mode_.upload(P.row_needs_diag, st);
     d_abd_inv_.alloc(144 * (size_t)std::max(P.n_bodies, 1));
@@ -388,9 +416,15 @@ void Solver::build_work_tables()
     // SpMV: the layout's tiles (at most 256 rows; several rows only within
     // 256 * entries_ entries, a longer row gets a tile of its own)
     SpmvLayout& L = spmv_layout_;
-    build_spmv_layout(L, S, P.row_off, P.row_ptr, P.col, P.entry_src, 256 * entries_);
-    for(size_t t = 0; t < L.tiles.size() / 4; ++t)
-        spmv.push_back(make_int4(L.tiles[4 * t], kWgPlain << 28, L.tiles[4 * t + 2], L.tiles[4 * t + 3]));
+    hipStream_t st = work_;
+    if(dev_struct_)
+        build_spmv_layout_on_device(256 * entries_, st);  // fills the layout and the SpMV work tables
+    else
+    {
+        build_spmv_layout(L, S, P.row_off, P.row_ptr, P.col, P.entry_src, 256 * entries_);
+        for(size_t t = 0; t < L.tiles.size() / 4; ++t)
+            spmv.push_back(make_int4(L.tiles[4 * t], kWgPlain << 28, L.tiles[4 * t + 2], L.tiles[4 * t + 3]));
+    }
     spmv_r = L.sys_range;
     for(int s = 0; s < S; ++s)
         if(spmv_r[2 * s] == spmv_r[2 * s + 1]) throw std::runtime_error("empty system");
@@ -406,24 +440,26 @@ void Solver::build_work_tables()
             throw std::logic_error("update workgroups do not match the plan's update groups");
         upd_grp[k] = make_int4(P.ugroup[4 * g], P.ugroup[4 * g + 1], P.ugroup[4 * g + 2], P.ugroup[4 * g + 3]);
     }
-    std::vector<int2> spmv_ent(spmv.size());
-    for(size_t k = 0; k < spmv.size(); ++k) spmv_ent[k] = make_int2(L.tile_ent[2 * k], L.tile_ent[2 * k + 1]);
-    hipStream_t st = work_;
-    d_sy_row_ptr_.upload(L.row_ptr, st);
-    d_sy_col_.upload(L.col, st);
-    d_sy_src_.upload(L.src, st);
-    d_sy_tptr_.upload(L.tptr, st);
-    d_vals32_.alloc(9 * (size_t)std::max<long long>(64, ((long long)L.col.size() + 63) / 64 * 64));
+    if(!dev_struct_)
+    {
+        std::vector<int2> spmv_ent(spmv.size());
+        for(size_t k = 0; k < spmv.size(); ++k) spmv_ent[k] = make_int2(L.tile_ent[2 * k], L.tile_ent[2 * k + 1]);
+        d_sy_row_ptr_.upload(L.row_ptr, st);
+        d_sy_col_.upload(L.col, st);
+        d_sy_src_.upload(L.src, st);
+        d_sy_tptr_.upload(L.tptr, st);
+        d_wg_spmv_.upload(spmv, st);
+        d_wg_spmv_ent_.upload(spmv_ent, st);
+        n_wg_spmv_ = (int)spmv.size();
+    }
+    d_vals32_.alloc(9 * (size_t)std::max<long long>(64, (L.total + 63) / 64 * 64));
     d_wg_pre_.upload(pre, st);
-    d_wg_spmv_.upload(spmv, st);
     d_wg_upd_.upload(upd, st);
     d_wg_upd_grp_.upload(upd_grp, st);
-    d_wg_spmv_ent_.upload(spmv_ent, st);
     d_wg_pre_range_.upload(pre_r, st);
     d_wg_spmv_range_.upload(spmv_r, st);
     d_wg_upd_range_.upload(upd_r, st);
     n_wg_pre_  = (int)pre.size();
-    n_wg_spmv_ = (int)spmv.size();
     n_wg_upd_  = (int)upd.size();
     d_part_pre_.alloc(n_wg_pre_);
     d_part_spmv_.alloc(n_wg_spmv_);
@@ -448,7 +484,7 @@ void Solver::make_devdata()
     d.mas_waves = effective_mas_waves();
     d.n_fine_banks = P.n_fine_banks;
     d.nnz = P.nnz;
-    d.nnz_s = (long long)spmv_layout_.col.size();
+    d.nnz_s = spmv_layout_.total;
     d.plane = std::max<long long>(64, (d.nnz_s + 63) / 64 * 64);
     d.row_ptr = d_row_ptr_.ptr;
     d.col = d_col_.ptr;
@@ -567,7 +603,7 @@ void Solver::build(int precision, hipStream_t user)
     {
         // a whole wave per entry where the lists are long (the top levels)
         const int e0 = P.gal_level_ptr[g], e1 = P.gal_level_ptr[g + 1];
-        const long long srcs = e1 > e0 ? (long long)P.gal_ptr[e1] - P.gal_ptr[e0] : 0;
+        const long long srcs = (long long)P.gal_level_src[g + 1] - P.gal_level_src[g];
         const int lanes = srcs > 32ll * (e1 - e0) ? 64 : 16;
         MASX_HIP(launch_gal_gather(d_vals64_.ptr, d_E_.ptr, d_gal_ptr_.ptr, d_gal_src_.ptr, e0, e1,
                                    P.gal_src_is_fine[g], lanes, st));

@@ -113,6 +113,17 @@ class Solver
     // (0 MAS local, 1 preconditioner finish, 2 SpMV, 3 update), milliseconds
     double time_kernel(int which, int reps);
     long long factor_status() const;
+    // Structure analysis of a new structure: 1 (default) builds the block
+    // CSR, Galerkin maps, bank entry lists and SpMV layout on the device when
+    // the solver holds one MAS system without bodies whose hierarchy the
+    // device built; 0 always on the host.  Takes effect at the next analysis.
+    void set_device_structure(int on) { dev_struct_mode_ = on ? 1 : 0; }
+    bool device_structure_used() const { return dev_struct_; }
+    // host copy of a device plan array (tests): 0 row_ptr, 1 col, 2 entry_src,
+    // 3 diag_src, 4 gal_ptr, 5 gal_src, 6 bank_ent_ptr, 7 bank_ent_pos,
+    // 8 bank_ent_src, 9 SpMV row_ptr, 10 SpMV col, 11 SpMV src, 12 SpMV tptr,
+    // 13 SpMV work table (4 ints per tile), 14 SpMV tile entry ranges (2 per tile)
+    void download_plan_array(int which, std::vector<int>& out);
 
   private:
     void build_work_tables();
@@ -156,6 +167,18 @@ class Solver
     DeviceArray<int>       d_max_iter_, d_factor_status_;
     // device MAS hierarchy build of setup(): inputs and the builder's scratch
     void build_hierarchies_on_device();
+    int  hier_dev_sys_    = -1;     // system whose hierarchy hier_dev_ holds
+    int  dev_struct_mode_ = 1;
+    bool dev_struct_      = false;  // the current plan's structure arrays were built on the device
+    bool device_structure_eligible() const;
+    void build_structure_on_device(hipStream_t st);
+    void build_spmv_layout_on_device(int cap, hipStream_t st);
+    int  n_bank_ent_ = 0;
+    DeviceArray<int>       d_dp_rows_, d_dp_cols_, d_dp_vals_, d_dp_vals2_, d_dp_cnt_, d_dp_err_, d_dp_flag_,
+                           d_dp_scan_, d_dp_scan2_, d_dp_bcnt_, d_gal_EI_, d_gal_EJ_, d_dp_nxt_, d_dp_kept_, d_dp_ts_,
+                           d_dp_tk_, d_dp_tq_, d_dp_eb_, d_dp_qb_;
+    DeviceArray<uint64_t>  d_dp_keys_, d_dp_keys2_;
+    DeviceArray<char>      d_dp_tmp_;
     DeviceArray<uint32_t>  d_h_start_, d_h_num_, d_h_list_;
     DeviceArray<int>       d_h_p2r_, d_h_r2p_, d_h_pairs_;
     std::unique_ptr<DeviceHierarchy> hier_dev_;

@@ -1,0 +1,244 @@
+// The solver's structure analysis on the device (devplan.hpp): for one MAS
+// system without affine bodies, the arrays that build_plan and
+// build_spmv_layout make on the host for the same structure.
+#include <algorithm>
+#include <climits>
+#include <stdexcept>
+#include <string>
+
+#include "devplan.hpp"
+#include "solver.hpp"
+#include "spmv_layout.hpp"
+
+namespace masx
+{
+namespace
+{
+void check(hipError_t e, const char* what)
+{
+    if(e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int bits_for(long long n)  // smallest b >= 1 with n <= 2^b
+{
+    int b = 1;
+    while((1ll << b) < n) ++b;
+    return b;
+}
+
+// one int from the device (synchronizes the stream)
+int read_int(const int* p, hipStream_t st)
+{
+    int v = 0;
+    check(hipMemcpyAsync(&v, p, sizeof(int), hipMemcpyDeviceToHost, st), "read back");
+    check(hipStreamSynchronize(st), "stream sync");
+    return v;
+}
+}  // namespace
+
+bool Solver::device_structure_eligible() const
+{
+    if(!dev_struct_mode_ || systems_.size() != 1) return false;
+    const SystemDesc& d = systems_[0];
+    return d.use_mas && d.n_abd == 0 && d.n_fem > 0 && d.hier_ready && hier_dev_sys_ == 0 && hier_dev_;
+}
+
+void Solver::build_structure_on_device(hipStream_t st)
+{
+    Plan&             P = plan_;
+    const SystemDesc& d = systems_[0];
+    const Hierarchy&  h = d.hier;
+    const int         n_rows = P.n_rows, bank = P.bank;
+    const long long   nb = d.nb;
+
+    // ---- full block CSR: stable radix sort of (row, column) keys in block order
+    d_dp_rows_.upload(d.rows, st);
+    d_dp_cols_.upload(d.cols, st);
+    const int       bits    = bits_for((long long)n_rows + 1);  // the sentinel row n_rows fits
+    const long long n_items = 2 * nb;
+    d_dp_keys_.alloc(n_items);
+    d_dp_keys2_.alloc(n_items);
+    d_dp_vals_.alloc(n_items);
+    d_dp_vals2_.alloc(n_items);
+    d_dp_cnt_.alloc((size_t)n_rows + 1);
+    d_diag_src_.alloc(n_rows);
+    d_dp_err_.alloc(1);
+    check(dp_fill(d_dp_cnt_.ptr, 0, (long long)n_rows + 1, st), "fill");
+    check(dp_fill(d_diag_src_.ptr, -1, n_rows, st), "fill");
+    check(dp_fill(d_dp_err_.ptr, 0, 1, st), "fill");
+    check(dp_csr_items(nb, d_dp_rows_.ptr, d_dp_cols_.ptr, d.block_rows, d_e2i_.ptr, n_rows, bits, d_dp_keys_.ptr,
+                       d_dp_vals_.ptr, d_dp_cnt_.ptr, d_diag_src_.ptr, d_dp_err_.ptr, st),
+          "block CSR items");
+    size_t tmp = std::max(dp_sort_bytes(n_items, 2 * bits), dp_scan_bytes((long long)n_rows + 1));
+    d_dp_tmp_.alloc(tmp);
+    check(dp_sort(d_dp_tmp_.ptr, tmp, d_dp_keys_.ptr, d_dp_keys2_.ptr, d_dp_vals_.ptr, d_dp_vals2_.ptr, n_items,
+                  2 * bits, st),
+          "block CSR sort");
+    d_row_ptr_.alloc((size_t)n_rows + 1);
+    check(dp_scan(d_dp_tmp_.ptr, tmp, d_dp_cnt_.ptr, d_row_ptr_.ptr, (long long)n_rows + 1, st), "row scan");
+    if(read_int(d_dp_err_.ptr, st)) throw std::invalid_argument("block index out of range");
+    P.nnz = read_int(d_row_ptr_.ptr + n_rows, st);
+    d_col_.alloc(P.nnz);
+    d_entry_src_.alloc(P.nnz);
+    check(dp_csr_fill(P.nnz, d_dp_keys2_.ptr, d_dp_vals2_.ptr, bits, d_col_.ptr, d_entry_src_.ptr, st), "CSR fill");
+
+    // ---- Galerkin maps, level by level: items sorted by (I, J) stably, runs
+    // of equal keys are the coarse entries
+    const int* going_next = hier_dev_->d_going_next();
+    const long long cap = std::max<long long>(1, nb * std::max(1, h.levels - 1));
+    d_gal_src_.alloc(cap);
+    d_gal_ptr_.alloc(cap + 1);
+    d_gal_EI_.alloc(cap);
+    d_gal_EJ_.alloc(cap);
+    check(dp_fill(d_gal_ptr_.ptr, 0, 1, st), "fill");
+    P.gal_level_ptr.assign(1, 0);
+    P.gal_level_src.assign(1, 0);
+    P.gal_src_is_fine.clear();
+    long long g_base = 0, e_base = 0, e_prev = 0, n_prev = nb;
+    for(int lvl = 1; lvl < h.levels; ++lvl)
+    {
+        const int       lb = h.level_begin[lvl], lc = h.level_count[lvl];
+        const long long n  = lvl == 1 ? nb : n_prev;
+        const int       kb = bits_for(lc);
+        d_dp_flag_.alloc(n + 1);
+        d_dp_scan_.alloc(n + 1);
+        tmp = std::max(dp_sort_bytes(n, 2 * kb), dp_scan_bytes(n + 1));
+        d_dp_tmp_.alloc(tmp);
+        check(dp_gal_items(lvl, n, d_dp_rows_.ptr, d_dp_cols_.ptr, d_gal_EI_.ptr, d_gal_EJ_.ptr, (int)e_prev, going_next,
+                           lb, kb, d_dp_keys_.ptr, d_dp_vals_.ptr, st),
+              "Galerkin items");
+        check(dp_sort(d_dp_tmp_.ptr, tmp, d_dp_keys_.ptr, d_dp_keys2_.ptr, d_dp_vals_.ptr, d_dp_vals2_.ptr, n, 2 * kb, st),
+              "Galerkin sort");
+        check(dp_run_ends(n, d_dp_keys2_.ptr, d_dp_flag_.ptr, st), "run ends");
+        check(dp_scan(d_dp_tmp_.ptr, tmp, d_dp_flag_.ptr, d_dp_scan_.ptr, n + 1, st), "run scan");
+        const int n_new = read_int(d_dp_scan_.ptr + n, st);
+        check(dp_gal_fill(n, d_dp_keys2_.ptr, d_dp_vals2_.ptr, d_dp_flag_.ptr, d_dp_scan_.ptr, (int)g_base,
+                          (int)e_base, lb, kb, d_gal_src_.ptr, d_gal_ptr_.ptr, d_gal_EI_.ptr, d_gal_EJ_.ptr, st),
+              "Galerkin fill");
+        e_prev = e_base;
+        n_prev = n_new;
+        g_base += n;
+        e_base += n_new;
+        P.gal_level_ptr.push_back((int)e_base);
+        P.gal_level_src.push_back((int)g_base);
+        P.gal_src_is_fine.push_back(lvl == 1 ? 1 : 0);
+    }
+    if(g_base >= (1ll << 29)) throw std::invalid_argument("Galerkin map too large");
+    P.n_gal = e_base;
+
+    // ---- bank entry lists: fine banks from their rows, then the coarse
+    // entries with both ends in one bank, in entry order
+    const int n_slots = h.n_slots, n_fine = P.n_fine_banks, n_banks = P.n_banks, n_gal = (int)P.n_gal;
+    d_dp_cnt_.alloc((size_t)std::max(n_rows, n_slots) + 1);
+    d_dp_scan_.alloc((size_t)n_slots + 1);
+    d_dp_flag_.alloc((size_t)n_gal + 1);
+    d_dp_scan2_.alloc((size_t)n_gal + 1);
+    d_dp_bcnt_.alloc((size_t)n_banks + 1);
+    d_ent_ptr_.alloc((size_t)n_banks + 1);
+    tmp = std::max({dp_scan_bytes((long long)n_slots + 1), dp_scan_bytes((long long)n_gal + 1),
+                    dp_scan_bytes((long long)n_banks + 1)});
+    d_dp_tmp_.alloc(tmp);
+    check(dp_fill(d_dp_bcnt_.ptr, 0, (long long)n_banks + 1, st), "fill");
+    check(dp_ent_count_rows(n_slots, bank, d_row_ptr_.ptr, d_col_.ptr, d_entry_src_.ptr, d_dp_cnt_.ptr, st),
+          "entry counts");
+    check(dp_scan(d_dp_tmp_.ptr, tmp, d_dp_cnt_.ptr, d_dp_scan_.ptr, (long long)n_slots + 1, st), "entry scan");
+    check(dp_ent_bank_counts(n_fine, bank, d_dp_scan_.ptr, d_dp_bcnt_.ptr, st), "bank counts");
+    check(dp_ent_flag_coarse(n_gal, bank, n_slots, n_fine, d_gal_EI_.ptr, d_gal_EJ_.ptr, d_dp_flag_.ptr,
+                             d_dp_bcnt_.ptr, st),
+          "coarse entry flags");
+    check(dp_scan(d_dp_tmp_.ptr, tmp, d_dp_flag_.ptr, d_dp_scan2_.ptr, (long long)n_gal + 1, st), "coarse scan");
+    check(dp_scan(d_dp_tmp_.ptr, tmp, d_dp_bcnt_.ptr, d_ent_ptr_.ptr, (long long)n_banks + 1, st), "bank scan");
+    const int fine_total = read_int(d_dp_scan_.ptr + n_slots, st);
+    n_bank_ent_          = read_int(d_ent_ptr_.ptr + n_banks, st);
+    d_ent_pos_.alloc(n_bank_ent_);
+    d_ent_src_.alloc(n_bank_ent_);
+    check(dp_ent_fill_rows(n_slots, bank, d_row_ptr_.ptr, d_col_.ptr, d_entry_src_.ptr, d_dp_scan_.ptr,
+                           d_ent_pos_.ptr, d_ent_src_.ptr, st),
+          "fine entries");
+    check(dp_ent_fill_coarse(n_gal, bank, d_gal_EI_.ptr, d_gal_EJ_.ptr, d_dp_flag_.ptr, d_dp_scan2_.ptr, fine_total,
+                             d_ent_pos_.ptr, d_ent_src_.ptr, st),
+          "coarse entries");
+    d_gal_src_.n = (size_t)std::max<long long>(g_base, 1);
+    d_gal_ptr_.n = (size_t)P.n_gal + 1;
+}
+
+void Solver::build_spmv_layout_on_device(int cap, hipStream_t st)
+{
+    if(cap < 1 || cap > kSpmvMaxCap) throw std::invalid_argument("SpMV tile capacity out of range");
+    const int n = plan_.n_rows;
+    if((unsigned)n > kSpmvColMask) throw std::length_error("too many rows for the SpMV layout");
+    d_dp_nxt_.alloc(n);
+    d_dp_kept_.alloc(n);
+    d_dp_ts_.alloc(n);
+    d_dp_err_.alloc(2);
+    check(dp_fill(d_dp_err_.ptr, 0, 2, st), "fill");
+    check(dp_sl_next(n, d_row_ptr_.ptr, d_col_.ptr, cap, d_dp_nxt_.ptr, d_dp_kept_.ptr, st), "tile ends");
+    check(dp_sl_chain(n, d_dp_nxt_.ptr, d_dp_ts_.ptr, d_dp_err_.ptr + 1, st), "tile chain");
+    const int T = read_int(d_dp_err_.ptr + 1, st);
+    d_dp_tk_.alloc((size_t)T + 1);
+    d_dp_tq_.alloc((size_t)T + 1);
+    d_dp_eb_.alloc((size_t)T + 1);
+    d_dp_qb_.alloc((size_t)T + 1);
+    const size_t tmp = dp_scan_bytes((long long)T + 1);
+    d_dp_tmp_.alloc(tmp);
+    check(dp_sl_tile_counts(T, n, d_dp_ts_.ptr, d_dp_kept_.ptr, d_row_ptr_.ptr, d_col_.ptr, d_dp_tk_.ptr, d_dp_tq_.ptr,
+                            st),
+          "tile counts");
+    check(dp_scan(d_dp_tmp_.ptr, tmp, d_dp_tk_.ptr, d_dp_eb_.ptr, (long long)T + 1, st), "entry scan");
+    check(dp_scan(d_dp_tmp_.ptr, tmp, d_dp_tq_.ptr, d_dp_qb_.ptr, (long long)T + 1, st), "product scan");
+    const long long aligned = read_int(d_dp_eb_.ptr + T, st);
+    const long long n_trans = read_int(d_dp_qb_.ptr + T, st);
+    const long long total   = aligned + kSpmvSlack;
+    if(total > INT_MAX || n_trans < 0) throw std::length_error("SpMV layout too large");
+    d_sy_col_.alloc(total);
+    d_sy_src_.alloc(total);
+    d_sy_row_ptr_.alloc((size_t)n + 1);
+    d_sy_tptr_.alloc((size_t)n + 1);
+    d_wg_spmv_.alloc(T);
+    d_wg_spmv_ent_.alloc(T);
+    check(dp_sl_pad(total, d_sy_col_.ptr, d_sy_src_.ptr, st), "layout padding");
+    check(dp_sl_fill(T, n, d_dp_ts_.ptr, d_dp_kept_.ptr, d_dp_eb_.ptr, d_dp_qb_.ptr, d_row_ptr_.ptr, d_col_.ptr,
+                     d_entry_src_.ptr, d_sy_row_ptr_.ptr, d_sy_tptr_.ptr, d_sy_col_.ptr, d_sy_src_.ptr,
+                     d_wg_spmv_.ptr, d_wg_spmv_ent_.ptr, d_dp_err_.ptr, st),
+          "layout fill");
+    if(read_int(d_dp_err_.ptr, st))
+        throw std::logic_error("SpMV layout: a tile's transposed products exceed the column word's field");
+    SpmvLayout& L = spmv_layout_;
+    L           = SpmvLayout{};
+    L.cap       = cap;
+    L.total     = total;
+    L.n_in_tile = n_trans;
+    L.sys_range = {0, T};
+    n_wg_spmv_  = T;
+}
+
+void Solver::download_plan_array(int which, std::vector<int>& out)
+{
+    if(!setup_done_) throw std::runtime_error("setup() must come first");
+    const Plan& P = plan_;
+    const int   T = n_wg_spmv_;
+    const int*  src = nullptr;
+    size_t      n   = 0;
+    switch(which)
+    {
+    case 0: src = d_row_ptr_.ptr; n = (size_t)P.n_rows + 1; break;
+    case 1: src = d_col_.ptr; n = (size_t)P.nnz; break;
+    case 2: src = d_entry_src_.ptr; n = (size_t)P.nnz; break;
+    case 3: src = d_diag_src_.ptr; n = (size_t)P.n_rows; break;
+    case 4: src = d_gal_ptr_.ptr; n = (size_t)P.n_gal + 1; break;
+    case 5: src = d_gal_src_.ptr; n = (size_t)P.gal_level_src.back(); break;
+    case 6: src = d_ent_ptr_.ptr; n = (size_t)P.n_banks + 1; break;
+    case 7: src = d_ent_pos_.ptr; n = (size_t)n_bank_ent_; break;
+    case 8: src = d_ent_src_.ptr; n = (size_t)n_bank_ent_; break;
+    case 9: src = d_sy_row_ptr_.ptr; n = (size_t)P.n_rows + 1; break;
+    case 10: src = d_sy_col_.ptr; n = (size_t)spmv_layout_.total; break;
+    case 11: src = d_sy_src_.ptr; n = (size_t)spmv_layout_.total; break;
+    case 12: src = d_sy_tptr_.ptr; n = (size_t)P.n_rows + 1; break;
+    case 13: src = reinterpret_cast<const int*>(d_wg_spmv_.ptr); n = 4 * (size_t)T; break;
+    case 14: src = reinterpret_cast<const int*>(d_wg_spmv_ent_.ptr); n = 2 * (size_t)T; break;
+    default: throw std::invalid_argument("unknown plan array");
+    }
+    out.assign(n, 0);
+    if(n) check(hipMemcpy(out.data(), src, n * sizeof(int), hipMemcpyDeviceToHost), "plan array copy");
+}
+}  // namespace masx

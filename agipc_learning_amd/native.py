@@ -97,6 +97,9 @@ def load():
         "masx_morton_partition_device_noalloc": ([C.c_int, vp, C.c_int, vp, C.c_longlong, vp, vp], C.c_int),
         "masx_stream_rate": ([C.c_longlong, C.c_int, C.c_int, C.POINTER(C.c_double)], C.c_int),
         "masx_set_mas_waves": ([vp, C.c_int], C.c_int),
+        "masx_set_device_structure": ([vp, C.c_int], C.c_int),
+        "masx_device_structure_used": ([vp, ip], C.c_int),
+        "masx_plan_array": ([vp, C.c_int, ip, C.c_longlong, C.POINTER(C.c_longlong)], C.c_int),
         "masx_autotune_mas": ([vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)], C.c_int),
         "masx_spmv_layout_check": ([C.c_int, C.c_longlong, ip, ip, C.c_int, C.c_uint, dp, C.POINTER(C.c_longlong),
                                     C.POINTER(C.c_longlong)], C.c_int),
@@ -425,6 +428,29 @@ class Solver:
 
     def set_mas_waves(self, waves):
         check(self.L.masx_set_mas_waves(self.h, waves))
+
+    def set_device_structure(self, on):
+        """1: a new structure of one MAS system without bodies is analyzed on
+        the device at the next setup (the default); 0: on the host."""
+        check(self.L.masx_set_device_structure(self.h, 1 if on else 0))
+
+    def device_structure_used(self):
+        v = C.c_int()
+        check(self.L.masx_device_structure_used(self.h, C.byref(v)))
+        return bool(v.value)
+
+    PLAN_ARRAYS = ("row_ptr", "col", "entry_src", "diag_src", "gal_ptr", "gal_src", "bank_ent_ptr", "bank_ent_pos",
+                   "bank_ent_src", "spmv_row_ptr", "spmv_col", "spmv_src", "spmv_tptr", "spmv_tiles",
+                   "spmv_tile_entries")
+
+    def plan_array(self, name):
+        """Host copy of an internal structure array (int32)."""
+        which = self.PLAN_ARRAYS.index(name)
+        n = C.c_longlong()
+        check(self.L.masx_plan_array(self.h, which, None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1), np.int32)
+        check(self.L.masx_plan_array(self.h, which, _ptr(out, C.c_int), out.size, C.byref(n)))
+        return out[:n.value]
 
     def autotune_mas(self, reps=50):
         """Times every supported wave count of the MAS local kernel (built

@@ -187,6 +187,25 @@ int masx_autotune_mas(masx_solver* s, int reps, int* waves, double* ms);
  * excludes the gaps between launches. */
 int masx_time_kernel(masx_solver* s, int which, int reps, double* ms);
 
+/* Where a new structure is analyzed (ReorderRealtime + PrepareHessian_bcoo,
+ * TraditionalMAS32Preconditioner.cu:2189-2230, 2234-2525, which rebuild it on
+ * the GPU every Newton step): on = 1 (the default) builds the block CSR, the
+ * diagonal sources, the Galerkin maps, the bank entry lists and the SpMV tile
+ * layout on the device when the solver holds one MAS system without affine
+ * bodies (the hierarchy is always built there); 0 builds them on the host.
+ * Takes effect at the next masx_setup after a structure change.  The two give
+ * the same arrays bit for bit.  *used: whether the current set-up built them
+ * on the device. */
+int masx_set_device_structure(masx_solver* s, int on);
+int masx_device_structure_used(masx_solver* s, int* used);
+/* Copy of an internal structure array (for tests; no reference counterpart):
+ * which = 0 row_ptr, 1 col, 2 entry_src, 3 diag_src, 4 gal_ptr, 5 gal_src,
+ * 6 bank_ent_ptr, 7 bank_ent_pos, 8 bank_ent_src, 9..12 the SpMV layout's
+ * row_ptr, col, src, tptr, 13 its work table (4 ints per tile), 14 its tile
+ * entry ranges (2 ints per tile).  *n receives the length; out may be null
+ * to query it. */
+int masx_plan_array(masx_solver* s, int which, int* out, long long capacity, long long* n);
+
 /* Practical streaming rate of the current GPU (no counterpart in the
  * reference; a measurement aid): the fastest of `reps` launches of a plain
  * kernel over `bytes` of device memory (>= 1 MiB, a multiple of 16), kind 0 =

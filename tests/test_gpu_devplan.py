@@ -1,0 +1,147 @@
+"""The structure analysis on the device equals the host analysis bit for bit.
+
+For one MAS system without affine bodies, masx_setup builds the block CSR,
+the diagonal sources, the Galerkin maps of every coarse level, the bank entry
+lists and the SpMV tile layout on the device (the reference rebuilds the same
+information on the GPU each Newton step: ReorderRealtime and
+PrepareHessian_bcoo, TraditionalMAS32Preconditioner.cu:2189-2230,
+2234-2525).  The host analysis (build_plan, build_spmv_layout) is pinned by
+the oracle tests; here every array of the device build is compared with the
+host build of the same structure, for every SpMV tiling, and the two solvers
+must then solve bit for bit alike.  Structures: grids with and without
+contacts, bank 16, a level override, duplicate blocks and blocks given in
+the lower triangle, and a structure change on the same solver."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from agipc_learning_amd import native
+from agipc_learning_amd import session
+from agipc_learning_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+ARRAYS = native.Solver.PLAN_ARRAYS
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def with_duplicates_and_lower(w, seed):
+    """The same matrix with some blocks split into two duplicates (halves) and
+    some off-diagonal blocks given as their transpose in the lower triangle."""
+    rng = np.random.default_rng(seed)
+    rows, cols = w["rows"].copy(), w["cols"].copy()
+    vals = w["vals"].reshape(-1, 3, 3).copy()  # column-major 3x3 per block: vals[k][c][r]
+    nb = len(rows)
+    dup = rng.choice(nb, nb // 20, replace=False)
+    half = vals[dup] * 0.5
+    vals[dup] = half
+    off = np.flatnonzero(rows != cols)
+    low = rng.choice(off, len(off) // 10, replace=False)
+    rows[low], cols[low] = cols[low].copy(), rows[low].copy()
+    vals[low] = vals[low].transpose(0, 2, 1)
+    out = dict(w)
+    out["rows"] = np.concatenate([rows, w["rows"][dup]]).astype(np.int32)
+    out["cols"] = np.concatenate([cols, w["cols"][dup]]).astype(np.int32)
+    out["vals"] = np.ascontiguousarray(np.concatenate([vals, half]).reshape(-1, 9))
+    return out
+
+
+def solve(s, w):
+    s.set_values(0, dev(w["vals"]))
+    s.build("fp32")
+    b = session.rhs_of(w)
+    x = torch.zeros(len(b), dtype=torch.float64, device="cuda")
+    st = s.solve(dev(b), x, rule=1, tol=1e-4)[0]
+    torch.cuda.synchronize()
+    return x.cpu().numpy(), st
+
+
+def compare(sd, sh, label):
+    assert sd.device_structure_used() and not sh.device_structure_used()
+    assert sd.info() == sh.info()
+    for u in (2, 4, 8):
+        sd.set_spmv_tiling(u)
+        sh.set_spmv_tiling(u)
+        for name in ARRAYS:
+            if not name.startswith("spmv") and u != 2:
+                continue
+            a, b = sd.plan_array(name), sh.plan_array(name)
+            assert a.shape == b.shape, (label, u, name, a.shape, b.shape)
+            assert np.array_equal(a, b), (label, u, name, int(np.flatnonzero(a != b)[0]))
+    say(f"{label}: {len(ARRAYS)} arrays equal at 2/4/8 entries per thread "
+        f"(nnz {sd.info()['nnz']}, coarse entries {sd.info()['n_gal']})")
+
+
+CASES = [
+    ("8^3", dict(n=8, contacts=0, seed=0), {}),
+    ("16^3 + 600 contacts", dict(n=16, contacts=600, seed=3), {}),
+    ("12^3 + 300 contacts, bank 16", dict(n=12, contacts=300, seed=4), dict(bank=16)),
+    ("14^3 + 200 contacts, 2 levels", dict(n=14, contacts=200, seed=5), dict(levels=2)),
+    ("10^3 + 100 contacts, duplicates and lower blocks", dict(n=10, contacts=100, seed=6, dup=True), {}),
+]
+
+
+@pytest.mark.parametrize("label,spec,kw", CASES, ids=[c[0] for c in CASES])
+def test_device_structure_equals_host(label, spec, kw):
+    w = W.fem_grid(spec["n"], n_contacts=spec["contacts"], seed=spec["seed"], young_sigma=0.5)
+    if spec.get("dup"):
+        w = with_duplicates_and_lower(w, spec["seed"])
+    sh = session.describe(w, **kw)
+    sh.set_device_structure(0)
+    sh.setup()
+    sd = session.describe(w, **kw)
+    sd.setup()
+    compare(sd, sh, label)
+    xd, std = solve(sd, w)
+    xh, sth = solve(sh, w)
+    assert std["iterations"] == sth["iterations"]
+    assert np.array_equal(xd, xh)
+    assert std["rel_res_true"] <= 1e-4 * (1 + 1e-9)
+    sd.close()
+    sh.close()
+
+
+def test_device_structure_at_the_40_cubed_size_and_after_a_contact_change():
+    # the 40^3 grid with 20,000 contacts, then a new contact set on the same
+    # solver (buffers reused), each against a fresh host analysis
+    w1 = W.fem40c(seed=0)
+    w2 = W.fem_grid(40, n_contacts=20000, seed=1, young_sigma=0.5)
+    sd = session.describe(w1)
+    sd.setup()
+    for k, w in enumerate((w1, w2)):
+        if k:
+            sd.set_structure(0, w["rows"], w["cols"])
+            sd.set_contacts(0, w["pairs"])
+            sd.setup()
+        sh = session.describe(w)
+        sh.set_device_structure(0)
+        sh.setup()
+        compare(sd, sh, f"40^3 + 20k contacts (structure {k})")
+        xd, std = solve(sd, w)
+        xh, sth = solve(sh, w)
+        assert std["iterations"] == sth["iterations"]
+        assert np.array_equal(xd, xh)
+        sh.close()
+    sd.close()
+
+
+def test_batches_and_bodies_keep_the_host_analysis():
+    a = W.fem_grid(6, n_contacts=20, seed=1)
+    b = W.fem_grid(7, n_contacts=20, seed=2)
+    s = session.describe([a, b])
+    s.setup()
+    assert not s.device_structure_used()
+    s.close()
+    h = W.hybrid(n_bodies=20, n_body_pairs=60, fem_n=4, n_coupled=5, seed=0)
+    s = session.describe(h)
+    s.setup()
+    assert not s.device_structure_used()
+    s.close()
