@@ -285,7 +285,7 @@ __global__ void __launch_bounds__(1024) k_dp_sl_chain(int n, const int* nxt, int
 
 __global__ void __launch_bounds__(kThreads) k_dp_sl_tile_counts(int n_tiles, int n, const int* ts, const int* kept,
                                                                 const int* row_ptr, const int* col, int* kept_al,
-                                                                int* trans)
+                                                                int* trans, int* err)
 {
     __shared__ int s[kThreads];
     const int t = blockIdx.x;
@@ -304,6 +304,7 @@ __global__ void __launch_bounds__(kThreads) k_dp_sl_tile_counts(int n_tiles, int
     {
         trans[t]   = total;
         kept_al[t] = (kept[r0] + kSpmvAlign - 1) / kSpmvAlign * kSpmvAlign;
+        if(total > kSpmvMaxCap) atomicOr(err, 4);  // tpos has 14 bits
     }
 }
 
@@ -516,10 +517,10 @@ hipError_t dp_sl_chain(int n, const int* nxt, int* ts, int* n_tiles, hipStream_t
 }
 
 hipError_t dp_sl_tile_counts(int n_tiles, int n, const int* ts, const int* kept, const int* row_ptr, const int* col,
-                             int* kept_al, int* trans, hipStream_t st)
+                             int* kept_al, int* trans, int* err, hipStream_t st)
 {
     hipLaunchKernelGGL(k_dp_sl_tile_counts, dim3(n_tiles + 1), dim3(kThreads), 0, st, n_tiles, n, ts, kept, row_ptr,
-                       col, kept_al, trans);
+                       col, kept_al, trans, err);
     return hipGetLastError();
 }
 

@@ -78,12 +78,12 @@ hipError_t dp_ent_bank_counts(int n_fine_banks, int bank, const int* row_off, in
 // from row 0 lists the tile starts (ts, *n_tiles).
 hipError_t dp_sl_next(int n, const int* row_ptr, const int* col, int cap, int* nxt, int* kept, hipStream_t st);
 hipError_t dp_sl_chain(int n, const int* nxt, int* ts, int* n_tiles, hipStream_t st);
-// per tile: aligned kept count (for the entry scan) and transposed products
+// per tile: aligned kept count (for the entry scan) and transposed products;
+// err bit 2 when a tile's transposed products exceed the column word's field
 hipError_t dp_sl_tile_counts(int n_tiles, int n, const int* ts, const int* kept, const int* row_ptr, const int* col,
-                             int* kept_al, int* trans, hipStream_t st);
+                             int* kept_al, int* trans, int* err, hipStream_t st);
 // entries, row pointers, transposed-product pointers, SpMV work table and
-// tile entry ranges; err bit 2 when a tile's transposed products exceed
-// the column word's field
+// tile entry ranges (a tile over the field's limit writes no entries)
 hipError_t dp_sl_fill(int n_tiles, int n, const int* ts, const int* kept, const int* e_base, const int* q_base,
                       const int* row_ptr, const int* col, const int* entry_src, int* L_row_ptr, int* L_tptr,
                       int* L_col, int* L_src, int4* wg, int2* wg_ent, int* err, hipStream_t st);

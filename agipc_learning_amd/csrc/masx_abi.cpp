@@ -200,6 +200,7 @@ int masx_set_topology(masx_solver* s, int sys, const uint32_t* nbr_start, const 
             const int slot = real_to_part[v];
             require(slot >= 0 && slot < part_map_size && part_to_real[slot] == v, "inconsistent partition");
         }
+        ++d.topo_version;
         s->impl.structure_changed(sys);
     });
 }
@@ -453,8 +454,8 @@ int masx_get_hierarchy(masx_solver* s, int sys, int* levels, int* total, int* le
 {
     return guard([&] {
         require(s && s->impl.is_analyzed() && sys >= 0 && sys < s->impl.n_systems(), "bad system index");
-        const auto& h = s->impl.system(sys).hier;
         require(s->impl.system(sys).use_mas, "system has no MAS hierarchy");
+        const auto& h = s->impl.hierarchy(sys);
         if(levels) *levels = h.levels;
         if(total) *total = h.total;
         if(level_count)
@@ -746,7 +747,22 @@ int masx_mas_set_preconditioner_bcoo(masx_mas* m, const double* d_values, const 
                 throw std::runtime_error("pair copy failed");
         }
         const bool same = m->solver && rows == m->rows && cols == m->cols && pairs == m->pairs;
-        if(!same)
+        if(!same && m->solver)
+        {
+            // a new structure on the same mesh (a new contact set): the solver
+            // keeps its topology and device buffers (ReorderRealtime runs again)
+            auto& d = m->solver->impl.system(0);
+            d.nb    = triplet_num;
+            d.rows  = rows;
+            d.cols  = cols;
+            d.pairs = pairs;
+            m->solver->impl.structure_changed(0);
+            m->solver->impl.setup();
+            m->rows  = std::move(rows);
+            m->cols  = std::move(cols);
+            m->pairs = std::move(pairs);
+        }
+        else if(!same)
         {
             m->solver = std::make_unique<masx_solver>();
             masx::SystemDesc d;

@@ -39,6 +39,7 @@ struct Hierarchy
     std::vector<int>      comp_ptr;      // total/bank + 1
     std::vector<uint32_t> comp_mask;
     std::vector<int>      comp_parent;   // slot at the next level
+    bool                  tables_pending = false;  // fine_mask, prefix, coarse_table still only on the device
 };
 
 int  reference_level_count(int vert_num, int part_map_size, int bank);
@@ -70,6 +71,7 @@ struct SystemDesc
     std::vector<int>      part_to_real, real_to_part;
     std::vector<int>      pairs;        // int4 per pair
     int                   node_offset = 0;
+    long long             topo_version = 0;  // bumped by every new topology (neighbours, partition)
     Hierarchy             hier;
     bool                  hier_ready = false;  // hier already built (on the device) for this structure
 };

@@ -95,11 +95,18 @@ void Solver::build_hierarchies_on_device()
     {
         if(!d.use_mas || d.n_fem <= 0 || d.hier_ready) continue;
         if(!hier_dev_) hier_dev_ = std::make_unique<DeviceHierarchy>();
-        d_h_start_.upload(d.nbr_start, st);
-        d_h_num_.upload(d.nbr_num, st);
-        d_h_list_.upload(d.nbr_list, st);
-        d_h_p2r_.upload(d.part_to_real, st);
-        d_h_r2p_.upload(d.real_to_part, st);
+        const int sys = (int)(&d - systems_.data());
+        if(hier_in_sys_ != sys || hier_in_version_ != d.topo_version)
+        {
+            // the mesh topology and partition (unchanged by a new contact set)
+            d_h_start_.upload(d.nbr_start, st);
+            d_h_num_.upload(d.nbr_num, st);
+            d_h_list_.upload(d.nbr_list, st);
+            d_h_p2r_.upload(d.part_to_real, st);
+            d_h_r2p_.upload(d.real_to_part, st);
+            hier_in_sys_     = sys;
+            hier_in_version_ = d.topo_version;
+        }
         d_h_pairs_.upload(d.pairs, st);
         const int n_slots = (int)d.part_to_real.size(), n_pairs = (int)d.pairs.size() / 4;
         try
@@ -113,7 +120,7 @@ void Solver::build_hierarchies_on_device()
             hier_dev_sys_ = -1;
             continue;
         }
-        hier_dev_sys_ = (int)(&d - systems_.data());
+        hier_dev_sys_ = sys;
         Hierarchy& h = d.hier;
         const DeviceHierarchy& g = *hier_dev_;
         h.bank    = d.bank;
@@ -123,18 +130,41 @@ void Solver::build_hierarchies_on_device()
         h.total   = g.total();
         h.level_count.assign(h.levels + 1, 0);
         h.level_begin.assign(h.levels + 1, 0);
-        h.fine_mask.assign(d.n_fem, 0u);
-        h.prefix.assign(n_slots / d.bank, 0);
         h.going_next.assign(std::max(h.total, d.n_fem), -1);
-        h.coarse_table.assign(6 * (size_t)d.n_fem, 0);
         h.comp_ptr.assign(h.total / d.bank + 1, 0);
         h.comp_mask.assign(g.n_components(), 0u);
         h.comp_parent.assign(g.n_components(), 0);
-        g.download(h.level_count.data(), h.level_begin.data(), h.fine_mask.data(), h.prefix.data(),
-                   h.going_next.data(), h.coarse_table.data(), h.comp_ptr.data(), h.comp_mask.data(),
+        // the plan reads the levels, going_next and the components; with one
+        // system the per-vertex tables stay on the device until asked for
+        // (the builder keeps them until the next structure)
+        h.tables_pending = systems_.size() == 1;
+        h.fine_mask.assign(h.tables_pending ? 0 : d.n_fem, 0u);
+        h.prefix.assign(h.tables_pending ? 0 : n_slots / d.bank, 0);
+        h.coarse_table.assign(h.tables_pending ? 0 : 6 * (size_t)d.n_fem, 0);
+        g.download(h.level_count.data(), h.level_begin.data(), h.tables_pending ? nullptr : h.fine_mask.data(),
+                   h.tables_pending ? nullptr : h.prefix.data(), h.going_next.data(),
+                   h.tables_pending ? nullptr : h.coarse_table.data(), h.comp_ptr.data(), h.comp_mask.data(),
                    h.comp_parent.data(), st);
         d.hier_ready = true;
     }
+}
+
+const Hierarchy& Solver::hierarchy(int s)
+{
+    SystemDesc& d = systems_.at(s);
+    Hierarchy&  h = d.hier;
+    if(h.tables_pending)
+    {
+        if(hier_dev_sys_ != s || !hier_dev_) throw std::logic_error("hierarchy tables are no longer on the device");
+        h.fine_mask.assign(d.n_fem, 0u);
+        h.prefix.assign(h.n_slots / d.bank, 0);
+        h.coarse_table.assign(6 * (size_t)d.n_fem, 0);
+        ensure_stream();
+        hier_dev_->download(nullptr, nullptr, h.fine_mask.data(), h.prefix.data(), nullptr, h.coarse_table.data(),
+                            nullptr, nullptr, nullptr, work_);
+        h.tables_pending = false;
+    }
+    return h;
 }
 
 Solver::~Solver()
@@ -142,6 +172,7 @@ Solver::~Solver()
     if(graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if(h_state_) (void)hipHostFree(h_state_);
     if(h_export_) (void)hipHostFree(h_export_);
+    if(h_dp_) (void)hipHostFree(h_dp_);
     if(ev_a_) (void)hipEventDestroy(ev_a_);
     if(ev_b_) (void)hipEventDestroy(ev_b_);
     for(auto e : timing_ev_) (void)hipEventDestroy(e);

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <initializer_list>
 #include <memory>
 #include <string>
 #include <vector>
@@ -66,6 +67,8 @@ class Solver
 
     int  add_system(const SystemDesc& d);
     SystemDesc& system(int s) { return systems_.at(s); }
+    // system s's MAS hierarchy with every table on the host
+    const Hierarchy& hierarchy(int s);
     int  n_systems() const { return (int)systems_.size(); }
     void analyze();                    // host-only structure analysis (no device needed)
     // analyze (if needed; the MAS hierarchies are then built on the device)
@@ -168,12 +171,16 @@ class Solver
     // device MAS hierarchy build of setup(): inputs and the builder's scratch
     void build_hierarchies_on_device();
     int  hier_dev_sys_    = -1;     // system whose hierarchy hier_dev_ holds
+    int  hier_in_sys_     = -1;     // system whose topology the d_h_* inputs hold ...
+    long long hier_in_version_ = -1;  // ... at this topology version
     int  dev_struct_mode_ = 1;
     bool dev_struct_      = false;  // the current plan's structure arrays were built on the device
     bool device_structure_eligible() const;
     void build_structure_on_device(hipStream_t st);
     void build_spmv_layout_on_device(int cap, hipStream_t st);
     int  n_bank_ent_ = 0;
+    int* h_dp_ = nullptr;  // pinned read-back slots of the device structure build
+    const int* read_ints(std::initializer_list<const int*> ptrs, hipStream_t st);
     DeviceArray<int>       d_dp_rows_, d_dp_cols_, d_dp_vals_, d_dp_vals2_, d_dp_cnt_, d_dp_err_, d_dp_flag_,
                            d_dp_scan_, d_dp_scan2_, d_dp_bcnt_, d_gal_EI_, d_gal_EJ_, d_dp_nxt_, d_dp_kept_, d_dp_ts_,
                            d_dp_tk_, d_dp_tq_, d_dp_eb_, d_dp_qb_;

@@ -26,15 +26,18 @@ int bits_for(long long n)  // smallest b >= 1 with n <= 2^b
     return b;
 }
 
-// one int from the device (synchronizes the stream)
-int read_int(const int* p, hipStream_t st)
-{
-    int v = 0;
-    check(hipMemcpyAsync(&v, p, sizeof(int), hipMemcpyDeviceToHost, st), "read back");
-    check(hipStreamSynchronize(st), "stream sync");
-    return v;
-}
 }  // namespace
+
+// device ints into the pinned read-back slots 0..n-1, one synchronization
+const int* Solver::read_ints(std::initializer_list<const int*> ptrs, hipStream_t st)
+{
+    if(!h_dp_) check(hipHostMalloc((void**)&h_dp_, 16 * sizeof(int), hipHostMallocDefault), "pinned read-back buffer");
+    if(ptrs.size() > 16) throw std::logic_error("too many read-backs");
+    int k = 0;
+    for(const int* p : ptrs) check(hipMemcpyAsync(h_dp_ + k++, p, sizeof(int), hipMemcpyDeviceToHost, st), "read back");
+    check(hipStreamSynchronize(st), "stream sync");
+    return h_dp_;
+}
 
 bool Solver::device_structure_eligible() const
 {
@@ -76,8 +79,9 @@ void Solver::build_structure_on_device(hipStream_t st)
           "block CSR sort");
     d_row_ptr_.alloc((size_t)n_rows + 1);
     check(dp_scan(d_dp_tmp_.ptr, tmp, d_dp_cnt_.ptr, d_row_ptr_.ptr, (long long)n_rows + 1, st), "row scan");
-    if(read_int(d_dp_err_.ptr, st)) throw std::invalid_argument("block index out of range");
-    P.nnz = read_int(d_row_ptr_.ptr + n_rows, st);
+    const int* rb = read_ints({d_dp_err_.ptr, d_row_ptr_.ptr + n_rows}, st);
+    if(rb[0]) throw std::invalid_argument("block index out of range");
+    P.nnz = rb[1];
     d_col_.alloc(P.nnz);
     d_entry_src_.alloc(P.nnz);
     check(dp_csr_fill(P.nnz, d_dp_keys2_.ptr, d_dp_vals2_.ptr, bits, d_col_.ptr, d_entry_src_.ptr, st), "CSR fill");
@@ -111,7 +115,7 @@ void Solver::build_structure_on_device(hipStream_t st)
               "Galerkin sort");
         check(dp_run_ends(n, d_dp_keys2_.ptr, d_dp_flag_.ptr, st), "run ends");
         check(dp_scan(d_dp_tmp_.ptr, tmp, d_dp_flag_.ptr, d_dp_scan_.ptr, n + 1, st), "run scan");
-        const int n_new = read_int(d_dp_scan_.ptr + n, st);
+        const int n_new = read_ints({d_dp_scan_.ptr + n}, st)[0];
         check(dp_gal_fill(n, d_dp_keys2_.ptr, d_dp_vals2_.ptr, d_dp_flag_.ptr, d_dp_scan_.ptr, (int)g_base,
                           (int)e_base, lb, kb, d_gal_src_.ptr, d_gal_ptr_.ptr, d_gal_EI_.ptr, d_gal_EJ_.ptr, st),
               "Galerkin fill");
@@ -125,6 +129,8 @@ void Solver::build_structure_on_device(hipStream_t st)
     }
     if(g_base >= (1ll << 29)) throw std::invalid_argument("Galerkin map too large");
     P.n_gal = e_base;
+    // the last level's fill still reads the scan buffer that may be replaced below
+    check(hipStreamSynchronize(st), "stream sync");
 
     // ---- bank entry lists: fine banks from their rows, then the coarse
     // entries with both ends in one bank, in entry order
@@ -148,8 +154,9 @@ void Solver::build_structure_on_device(hipStream_t st)
           "coarse entry flags");
     check(dp_scan(d_dp_tmp_.ptr, tmp, d_dp_flag_.ptr, d_dp_scan2_.ptr, (long long)n_gal + 1, st), "coarse scan");
     check(dp_scan(d_dp_tmp_.ptr, tmp, d_dp_bcnt_.ptr, d_ent_ptr_.ptr, (long long)n_banks + 1, st), "bank scan");
-    const int fine_total = read_int(d_dp_scan_.ptr + n_slots, st);
-    n_bank_ent_          = read_int(d_ent_ptr_.ptr + n_banks, st);
+    rb = read_ints({d_dp_scan_.ptr + n_slots, d_ent_ptr_.ptr + n_banks}, st);
+    const int fine_total = rb[0];
+    n_bank_ent_          = rb[1];
     d_ent_pos_.alloc(n_bank_ent_);
     d_ent_src_.alloc(n_bank_ent_);
     check(dp_ent_fill_rows(n_slots, bank, d_row_ptr_.ptr, d_col_.ptr, d_entry_src_.ptr, d_dp_scan_.ptr,
@@ -174,7 +181,7 @@ void Solver::build_spmv_layout_on_device(int cap, hipStream_t st)
     check(dp_fill(d_dp_err_.ptr, 0, 2, st), "fill");
     check(dp_sl_next(n, d_row_ptr_.ptr, d_col_.ptr, cap, d_dp_nxt_.ptr, d_dp_kept_.ptr, st), "tile ends");
     check(dp_sl_chain(n, d_dp_nxt_.ptr, d_dp_ts_.ptr, d_dp_err_.ptr + 1, st), "tile chain");
-    const int T = read_int(d_dp_err_.ptr + 1, st);
+    const int T = read_ints({d_dp_err_.ptr + 1}, st)[0];
     d_dp_tk_.alloc((size_t)T + 1);
     d_dp_tq_.alloc((size_t)T + 1);
     d_dp_eb_.alloc((size_t)T + 1);
@@ -182,14 +189,15 @@ void Solver::build_spmv_layout_on_device(int cap, hipStream_t st)
     const size_t tmp = dp_scan_bytes((long long)T + 1);
     d_dp_tmp_.alloc(tmp);
     check(dp_sl_tile_counts(T, n, d_dp_ts_.ptr, d_dp_kept_.ptr, d_row_ptr_.ptr, d_col_.ptr, d_dp_tk_.ptr, d_dp_tq_.ptr,
-                            st),
+                            d_dp_err_.ptr, st),
           "tile counts");
     check(dp_scan(d_dp_tmp_.ptr, tmp, d_dp_tk_.ptr, d_dp_eb_.ptr, (long long)T + 1, st), "entry scan");
     check(dp_scan(d_dp_tmp_.ptr, tmp, d_dp_tq_.ptr, d_dp_qb_.ptr, (long long)T + 1, st), "product scan");
-    const long long aligned = read_int(d_dp_eb_.ptr + T, st);
-    const long long n_trans = read_int(d_dp_qb_.ptr + T, st);
+    const int* rb = read_ints({d_dp_eb_.ptr + T, d_dp_qb_.ptr + T, d_dp_err_.ptr}, st);
+    if(rb[2]) throw std::logic_error("SpMV layout: a tile's transposed products exceed the column word's field");
+    const long long aligned = rb[0], n_trans = rb[1];
     const long long total   = aligned + kSpmvSlack;
-    if(total > INT_MAX || n_trans < 0) throw std::length_error("SpMV layout too large");
+    if(total > INT_MAX || aligned < 0 || n_trans < 0) throw std::length_error("SpMV layout too large");
     d_sy_col_.alloc(total);
     d_sy_src_.alloc(total);
     d_sy_row_ptr_.alloc((size_t)n + 1);
@@ -201,8 +209,6 @@ void Solver::build_spmv_layout_on_device(int cap, hipStream_t st)
                      d_entry_src_.ptr, d_sy_row_ptr_.ptr, d_sy_tptr_.ptr, d_sy_col_.ptr, d_sy_src_.ptr,
                      d_wg_spmv_.ptr, d_wg_spmv_ent_.ptr, d_dp_err_.ptr, st),
           "layout fill");
-    if(read_int(d_dp_err_.ptr, st))
-        throw std::logic_error("SpMV layout: a tile's transposed products exceed the column word's field");
     SpmvLayout& L = spmv_layout_;
     L           = SpmvLayout{};
     L.cap       = cap;

@@ -145,3 +145,42 @@ def test_batches_and_bodies_keep_the_host_analysis():
     s.setup()
     assert not s.device_structure_used()
     s.close()
+
+
+def test_mas_object_takes_a_new_contact_set_on_the_same_solver():
+    # the reference's per-step call (setPreconditioner_bcoo with the new
+    # structure and contact pairs) on one MAS object: the object keeps its
+    # solver and re-analyzes on the device; the apply equals that of a fresh
+    # object given the new structure directly, bit for bit
+    wa = W.fem_grid(12, n_contacts=200, seed=21, young_sigma=0.5)
+    wb = W.fem_grid(12, n_contacts=500, seed=22, young_sigma=0.5)
+    nv = wa["n_vertices"]
+    p2r, r2p = native.morton_partition(wa["positions"])
+    st, nu, li = W.neighbors(nv, wa["tets"])
+    R = dev(np.random.default_rng(4).standard_normal(3 * nv))
+
+    def fresh():
+        m = native.MasPreconditioner(32)
+        m.init_neighbor(nv, 0, len(li), len(p2r))
+        m.upload(li, st, nu, p2r, r2p)
+        m.init_matrix()
+        return m
+
+    def apply(m, w):
+        n = len(w["rows"])
+        pairs = dev(np.asarray(w["pairs"], np.int32).reshape(-1, 4))
+        m.set_preconditioner_bcoo(dev(w["vals"]), dev(w["rows"]), dev(w["cols"]), dev(np.arange(n, dtype=np.int32)),
+                                  0, n, pairs=pairs, cp_num=len(pairs))
+        Z = torch.zeros_like(R)
+        m.preconditioning(R, Z)
+        torch.cuda.synchronize()
+        return Z.cpu().numpy()
+
+    m = fresh()
+    za = apply(m, wa)
+    zb = apply(m, wb)    # same object, new contact set
+    zb_ref = apply(fresh(), wb)
+    za2 = apply(m, wa)   # and back
+    assert np.array_equal(zb, zb_ref)
+    assert np.array_equal(za2, za)
+    assert not np.array_equal(za, zb)
