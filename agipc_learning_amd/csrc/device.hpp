@@ -105,6 +105,13 @@ struct DevData
     // ABD and diagonal block-Jacobi
     const float* abd_inv;   // 144 per body, row-major
     const int*   body_row;
+    // 12 x 12 body couplings of the SpMV's body tiles: per body its couplings
+    // [abd_cp_ptr[i], abd_cp_ptr[i+1]), column body abd_cp_col, values 144
+    // floats per coupling (row-major over the 12 scalar rows and columns)
+    const int*   abd_cp_ptr;
+    const int*   abd_cp_col;
+    const int*   abd_cp_row;   // per coupling: first internal row of its column body
+    const float* vals_abd;
     const float* diag_inv;  // 9 per internal row, row-major (identity where unused)
     const int*   row_mode;  // per internal row: 0 identity, 1 diag inverse
     // work distribution: int4 {sys, aux, begin, end}

@@ -175,6 +175,36 @@ __device__ __forceinline__ bool finite_d(double v) { return isfinite(v); }
 
 // ===================================================================== build
 
+// 12 x 12 body couplings of the SpMV's body tiles from the FP64 upper blocks:
+// one thread per coupling and 3 x 3 sub-block (q, t), summing the sub-block's
+// entries of the full CSR (duplicate blocks follow the first, transposed
+// copies read their upper block transposed) in FP64 before rounding.
+__global__ void __launch_bounds__(256) k_pack_abd(DevData d, const int* __restrict__ cp_body,
+                                                  const int* __restrict__ cp_sub, long long n_sub, float* vals_abd)
+{
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if(g >= n_sub) return;
+    const long long k = g / 16;
+    const int sub = (int)(g % 16), q = sub / 4, t = sub % 4;
+    const int r = d.body_row[cp_body[k]] + q, target = d.body_row[d.abd_cp_col[k]] + t;
+    double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int e_end = d.row_ptr[r + 1];
+    for(int e = cp_sub[g]; e >= 0 && e < e_end && d.col[e] == target; ++e)
+    {
+        const int code = d.entry_src[e];
+        const double* B = d.vals64 + 9 * (long long)(code & 0x7fffffff);
+#pragma unroll
+        for(int i = 0; i < 3; ++i)
+#pragma unroll
+            for(int j = 0; j < 3; ++j) acc[3 * i + j] += code >= 0 ? B[i + 3 * j] : B[j + 3 * i];
+    }
+    float* out = vals_abd + 144 * k + 12 * (3 * q) + 3 * t;
+#pragma unroll
+    for(int i = 0; i < 3; ++i)
+#pragma unroll
+        for(int j = 0; j < 3; ++j) out[12 * i + j] = (float)acc[3 * i + j];
+}
+
 // Upper FP64 blocks (col-major) -> FP32 SoA planes of the SpMV layout, one
 // thread per entry so the plane writes are contiguous; an entry marked
 // transposed takes its upper block's transpose, a padding entry zeros.
@@ -1708,6 +1738,92 @@ __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __re
         spmv_load_chunk<U>(d, p_old, base, n, c);
         spmv_reduce_chunk<U>(c, beta, base, n, has_row, rb, re, 0, 0, sX, sP, sQ, acc);
     }
+    if((w.y >> 28) == kWgAbd)  // uniform: a tile of affine bodies
+    {
+        // their 12 x 12 couplings: 48 threads per body, thread 48 b + 12 g + q
+        // takes scalar row q of the tile's body b over every fourth of its
+        // couplings (g, g + 4, ...), one coupling block row (three float4
+        // loads) each, x from the tile's own p or gathered as z + beta p_old
+        // (float4 loads: a body's 12 values start 16-byte aligned); the four
+        // partial sums of a row are added in group order, after the row's
+        // 3 x 3 entries
+        constexpr int G = 4, BATCH = 2;
+        const int b0 = w.y & 0x0fffffff, nbod = (r1 - r0) / 4;
+        const int bl = (int)threadIdx.x / (12 * G), g = ((int)threadIdx.x / 12) % G, q = (int)threadIdx.x % 12;
+        float a = 0.f;
+        if(bl < nbod)
+        {
+            const int b = b0 + bl;
+            const int k1 = d.abd_cp_ptr[b + 1];
+            // BATCH couplings at a time: their column rows, then every value and
+            // x load, then the products (the loads of a batch overlap)
+#pragma unroll 1
+            for(int kb = d.abd_cp_ptr[b] + g; kb < k1; kb += G * BATCH)
+            {
+                int    jr[BATCH];
+                float4 v[BATCH][3], zz[BATCH][3], pp[BATCH][3];
+#pragma unroll
+                for(int u = 0; u < BATCH; ++u)
+                {
+                    const int k = kb + G * u;
+                    jr[u] = k < k1 ? d.abd_cp_row[k] : -1;
+                }
+#pragma unroll
+                for(int u = 0; u < BATCH; ++u)
+                {
+                    const int k = kb + G * u;
+                    if(jr[u] < 0) continue;
+                    const float4* V = reinterpret_cast<const float4*>(d.vals_abd + 144 * (long long)k + 12 * q);
+                    v[u][0] = V[0]; v[u][1] = V[1]; v[u][2] = V[2];
+                    if(jr[u] < r0 || jr[u] >= r1)
+                    {
+                        const float4* Zg = reinterpret_cast<const float4*>(d.z + 3 * (long long)jr[u]);
+                        const float4* Pg = reinterpret_cast<const float4*>(p_old + 3 * (long long)jr[u]);
+                        zz[u][0] = Zg[0]; zz[u][1] = Zg[1]; zz[u][2] = Zg[2];
+                        pp[u][0] = Pg[0]; pp[u][1] = Pg[1]; pp[u][2] = Pg[2];
+                    }
+                }
+#pragma unroll
+                for(int u = 0; u < BATCH; ++u)
+                {
+                    if(jr[u] < 0) continue;
+                    float x[12];
+                    if(jr[u] >= r0 && jr[u] < r1)
+                    {
+#pragma unroll
+                        for(int c = 0; c < 12; ++c) x[c] = sX[3 * (jr[u] - r0) + c];
+                    }
+                    else
+                    {
+#pragma unroll
+                        for(int c = 0; c < 3; ++c)
+                        {
+                            x[4 * c]     = zz[u][c].x + beta * pp[u][c].x;
+                            x[4 * c + 1] = zz[u][c].y + beta * pp[u][c].y;
+                            x[4 * c + 2] = zz[u][c].z + beta * pp[u][c].z;
+                            x[4 * c + 3] = zz[u][c].w + beta * pp[u][c].w;
+                        }
+                    }
+                    a += v[u][0].x * x[0] + v[u][0].y * x[1] + v[u][0].z * x[2] + v[u][0].w * x[3] +
+                         v[u][1].x * x[4] + v[u][1].y * x[5] + v[u][1].z * x[6] + v[u][1].w * x[7] +
+                         v[u][2].x * x[8] + v[u][2].y * x[9] + v[u][2].z * x[10] + v[u][2].w * x[11];
+                }
+            }
+        }
+        sP[threadIdx.x] = a;  // the chunk sums above are consumed
+        __syncthreads();
+        if(has_row)
+        {
+            const int lr = row - r0, base = 12 * G * (lr / 4) + 3 * (lr % 4);
+#pragma unroll
+            for(int gg = 0; gg < G; ++gg)
+            {
+                acc[0] += sP[base + 12 * gg];
+                acc[1] += sP[base + 12 * gg + 1];
+                acc[2] += sP[base + 12 * gg + 2];
+            }
+        }
+    }
     double part = 0.0;
     if(has_row)
     {
@@ -2087,6 +2203,16 @@ hipError_t launch_pack_values(const DevData& d, hipStream_t st)
 {
     if(d.nnz_s <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_pack_values, dim3(blocks_for(d.nnz_s, 256)), dim3(256), 0, st, d);
+    MASX_CHECK_LAUNCH();
+    return hipSuccess;
+}
+
+hipError_t launch_pack_abd(const DevData& d, const int* cp_body, const int* cp_sub, long long n_couplings,
+                           float* vals_abd, hipStream_t st)
+{
+    if(n_couplings <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_abd, dim3(blocks_for(16 * n_couplings, 256)), dim3(256), 0, st, d, cp_body, cp_sub,
+                       16 * n_couplings, vals_abd);
     MASX_CHECK_LAUNCH();
     return hipSuccess;
 }

@@ -9,6 +9,8 @@ size_t mas_bank_bytes(int bank, int precision);
 bool   spmv_tiling_supported(int entries_per_thread);
 
 hipError_t launch_pack_values(const DevData& d, hipStream_t st);
+hipError_t launch_pack_abd(const DevData& d, const int* cp_body, const int* cp_sub, long long n_couplings,
+                           float* vals_abd, hipStream_t st);
 // lanes per coarse entry: 16 or 64
 hipError_t launch_gal_gather(const double* fine, double* E, const int* ptr, const int* src, int e0, int e1,
                              int src_is_fine, int lanes, hipStream_t st);

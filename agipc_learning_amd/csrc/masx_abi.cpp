@@ -584,11 +584,13 @@ int masx_morton_partition_device_noalloc(int n, const double* d_positions, int b
     });
 }
 
-int masx_spmv_layout_check(int n_rows, long long nb, const int* rows, const int* cols, int entries_per_thread,
-                           unsigned seed, double* max_rel_err, long long* n_entries, long long* n_in_tile)
+int masx_spmv_layout_check_bodies(int n_rows, int n_bodies, long long nb, const int* rows, const int* cols,
+                                  int entries_per_thread, unsigned seed, double* max_rel_err, long long* n_entries,
+                                  long long* n_in_tile)
 {
     return guard([&] {
         require(n_rows >= 1 && nb >= 1 && rows && cols && max_rel_err, "bad argument");
+        require(n_bodies >= 0 && 4LL * n_bodies <= n_rows, "bodies must fit the rows");
         require(entries_per_thread >= 1 && 256LL * entries_per_thread <= masx::kSpmvMaxCap, "bad entries per thread");
         // full CSR: upper (r, c) -> row r as is, row c transposed
         std::vector<std::array<int, 3>> ent;
@@ -613,7 +615,13 @@ int masx_spmv_layout_check(int n_rows, long long nb, const int* rows, const int*
         }
         for(int r = 0; r < n_rows; ++r) row_ptr[r + 1] += row_ptr[r];
         masx::SpmvLayout L;
-        masx::build_spmv_layout(L, 1, {0, n_rows}, row_ptr, col, src, 256 * entries_per_thread);
+        masx::SpmvBodies bodies;  // bodies occupy rows [0, 4 n_bodies)
+        bodies.abd_row0  = {0};
+        bodies.n_abd     = {n_bodies};
+        bodies.body_base = {0};
+        std::vector<int> body_row(n_bodies);
+        for(int b = 0; b < n_bodies; ++b) body_row[b] = 4 * b;
+        masx::build_spmv_layout(L, 1, {0, n_rows}, row_ptr, col, src, 256 * entries_per_thread, &bodies);
         // seeded values (64-bit LCG, uniform in [-1, 1))
         uint64_t z = 0x9e3779b97f4a7c15ULL ^ seed;
         auto uni = [&z]() {
@@ -624,7 +632,8 @@ int masx_spmv_layout_check(int n_rows, long long nb, const int* rows, const int*
             yf(3 * (size_t)n_rows, 0.0);
         for(auto& v : upper) v = uni();
         for(auto& v : x) v = uni();
-        masx::spmv_layout_apply(L, upper.data(), x.data(), y.data());
+        masx::spmv_layout_apply(L, upper.data(), x.data(), y.data(), row_ptr.data(), col.data(), src.data(),
+                                body_row.data());
         double scale = 0.0, err = 0.0;
         for(int r = 0; r < n_rows; ++r)
             for(int e = row_ptr[r]; e < row_ptr[r + 1]; ++e)
@@ -643,6 +652,13 @@ int masx_spmv_layout_check(int n_rows, long long nb, const int* rows, const int*
         if(n_entries) *n_entries = L.n_entries;
         if(n_in_tile) *n_in_tile = L.n_in_tile;
     });
+}
+
+int masx_spmv_layout_check(int n_rows, long long nb, const int* rows, const int* cols, int entries_per_thread,
+                           unsigned seed, double* max_rel_err, long long* n_entries, long long* n_in_tile)
+{
+    return masx_spmv_layout_check_bodies(n_rows, 0, nb, rows, cols, entries_per_thread, seed, max_rel_err, n_entries,
+                                         n_in_tile);
 }
 
 // ------------------------------------------------------------------ MAS ----

@@ -993,7 +993,8 @@ void validate_plan(const Plan& P)
 // ------------------------------------------------------------ SpMV layout --
 
 void build_spmv_layout(SpmvLayout& L, int n_sys, const std::vector<int>& row_off, const std::vector<int>& row_ptr,
-                       const std::vector<int>& col, const std::vector<int>& entry_src, int cap)
+                       const std::vector<int>& col, const std::vector<int>& entry_src, int cap,
+                       const SpmvBodies* bodies)
 {
     if(cap < 1 || cap > kSpmvMaxCap) throw std::invalid_argument("SpMV tile capacity out of range");
     const int n = (int)row_ptr.size() - 1;
@@ -1004,14 +1005,60 @@ void build_spmv_layout(SpmvLayout& L, int n_sys, const std::vector<int>& row_off
     };
     // entries row r keeps in a tile starting at r0: columns < r0 or >= r
     auto kept = [&](int r, int r0) { return (row_ptr[r + 1] - first_ge(r, r)) + (first_ge(r, r0) - row_ptr[r]); };
+    // body rows [a0, a1) of system s (empty without bodies); a body row keeps
+    // its entries outside the bodies' columns (the 12 x 12 couplings take those)
+    auto body_rows = [&](int s, int& a0, int& a1) {
+        a0 = a1 = row_off[s];
+        if(bodies && bodies->n_abd[s] > 0)
+        {
+            a0 = bodies->abd_row0[s];
+            a1 = a0 + 4 * bodies->n_abd[s];
+        }
+    };
+    auto kept_body = [&](int r, int a0, int a1) {
+        return (row_ptr[r + 1] - row_ptr[r]) - (first_ge(r, a1) - first_ge(r, a0));
+    };
     L = SpmvLayout{};
     L.cap = cap;
     L.sys_range.resize(2 * (size_t)n_sys);
     std::vector<long long> t_kept;
+    std::vector<int>       t_body;  // per tile: first row of its system's bodies, -1 for a row tile
+    std::vector<int>       t_body_end;
     for(int s = 0; s < n_sys; ++s)
     {
         L.sys_range[2 * s] = (int)(L.tiles.size() / 4);
         int r0 = row_off[s];
+        int a0, a1;
+        body_rows(s, a0, a1);
+        if(a1 > a0)
+        {
+            if(a0 != row_off[s] || a1 > row_off[s + 1]) throw std::logic_error("SpMV layout: bodies must lead the rows");
+            const int nab = bodies->n_abd[s];
+            auto body_kept = [&](int b) {
+                long long k = 0;
+                for(int q = 0; q < 4; ++q) k += kept_body(a0 + 4 * b + q, a0, a1);
+                return k;
+            };
+            for(int b0 = 0; b0 < nab;)
+            {
+                long long k = body_kept(b0);
+                int b1 = b0 + 1;
+                while(b1 < nab && b1 - b0 < kBodiesPerTile)
+                {
+                    const long long q = body_kept(b1);
+                    if(k + q > cap) break;
+                    k += q;
+                    ++b1;
+                }
+                L.tiles.insert(L.tiles.end(), {s, kSpmvBodyTile << 28 | (bodies->body_base[s] + b0), a0 + 4 * b0,
+                                               a0 + 4 * b1});
+                t_kept.push_back(k);
+                t_body.push_back(a0);
+                t_body_end.push_back(a1);
+                b0 = b1;
+            }
+            r0 = a1;
+        }
         while(r0 < row_off[s + 1])
         {
             long long k = row_ptr[r0 + 1] - row_ptr[r0];
@@ -1025,19 +1072,22 @@ void build_spmv_layout(SpmvLayout& L, int n_sys, const std::vector<int>& row_off
             }
             L.tiles.insert(L.tiles.end(), {s, 0, r0, r1});
             t_kept.push_back(k);
+            t_body.push_back(-1);
+            t_body_end.push_back(-1);
             r0 = r1;
         }
         L.sys_range[2 * s + 1] = (int)(L.tiles.size() / 4);
     }
     const int T = (int)t_kept.size();
-    // transposed products per tile: pairs r < c inside it
+    // transposed products per tile: pairs r < c inside it (a body tile's
+    // entries all lie outside it)
     std::vector<long long> t_trans(T);
 #pragma omp parallel for schedule(dynamic, 64)
     for(int t = 0; t < T; ++t)
     {
         const int r0 = L.tiles[4 * t + 2], r1 = L.tiles[4 * t + 3];
         long long q = 0;
-        for(int r = r0; r < r1; ++r) q += first_ge(r, r1) - first_ge(r, r + 1);
+        for(int r = r0; t_body[t] < 0 && r < r1; ++r) q += first_ge(r, r1) - first_ge(r, r + 1);
         t_trans[t] = q;
     }
     // tile starts aligned to kSpmvAlign entries; the gaps are padding
@@ -1067,11 +1117,11 @@ void build_spmv_layout(SpmvLayout& L, int n_sys, const std::vector<int>& row_off
         const int r0 = L.tiles[4 * t + 2], r1 = L.tiles[4 * t + 3], nr = r1 - r0;
         int tstart[kSpmvTileRows + 1];
         std::fill(tstart, tstart + nr + 1, 0);
-        for(int r = r0; r < r1; ++r)
+        for(int r = r0; t_body[t] < 0 && r < r1; ++r)
             for(int e = first_ge(r, r + 1), e_end = first_ge(r, r1); e < e_end; ++e) ++tstart[col[e] - r0 + 1];
         for(int j = 0; j < nr; ++j) tstart[j + 1] += tstart[j];
         if(t_trans[t] != tstart[nr]) bad |= 1;
-        if(nr > 1 && t_kept[t] > cap) bad |= 2;
+        if(nr > (t_body[t] >= 0 ? 4 : 1) && t_kept[t] > cap) bad |= 2;  // a body may take several chunks
         if(tstart[nr] > kSpmvMaxCap) bad |= 4;
         int tcur[kSpmvTileRows];
         std::copy(tstart, tstart + nr, tcur);
@@ -1085,7 +1135,11 @@ void build_spmv_layout(SpmvLayout& L, int n_sys, const std::vector<int>& row_off
             {
                 const int c = col[e];
                 unsigned word = (unsigned)c;
-                if(c >= r0 && c < r1)
+                if(t_body[t] >= 0)
+                {
+                    if(c >= t_body[t] && c < t_body_end[t]) continue;  // in a 12 x 12 coupling
+                }
+                else if(c >= r0 && c < r1)
                 {
                     if(c < r) continue;  // the tile's lower copy: its upper twin covers it
                     const int tpos = c == r ? 0 : tcur[c - r0]++;
@@ -1104,9 +1158,41 @@ void build_spmv_layout(SpmvLayout& L, int n_sys, const std::vector<int>& row_off
     if(bad) throw std::logic_error("SpMV layout: tile codes inconsistent (check mask " + std::to_string(bad) + ")");
     L.row_ptr[n] = (int)(T ? e_end[T - 1] : 0);
     L.tptr[n]    = (int)q_base[T];
+    // 12 x 12 couplings of every body, in increasing column body
+    L.cp_ptr.assign(1, 0);
+    for(int s = 0; bodies && s < n_sys; ++s)
+    {
+        int a0, a1;
+        body_rows(s, a0, a1);
+        for(int b = 0; b < (a1 - a0) / 4; ++b)
+        {
+            std::vector<int> js;
+            for(int q = 0; q < 4; ++q)
+            {
+                const int r = a0 + 4 * b + q;
+                for(int e = first_ge(r, a0), e1 = first_ge(r, a1); e < e1; ++e) js.push_back((col[e] - a0) / 4);
+            }
+            std::sort(js.begin(), js.end());
+            js.erase(std::unique(js.begin(), js.end()), js.end());
+            for(const int j : js)
+            {
+                L.cp_col.push_back(bodies->body_base[s] + j);
+                L.cp_body.push_back(bodies->body_base[s] + b);
+                for(int q = 0; q < 4; ++q)
+                    for(int t = 0; t < 4; ++t)
+                    {
+                        const int r = a0 + 4 * b + q, target = a0 + 4 * j + t;
+                        const int e = first_ge(r, target);
+                        L.cp_sub.push_back(e < row_ptr[r + 1] && col[e] == target ? e : -1);
+                    }
+            }
+            L.cp_ptr.push_back((int)L.cp_col.size());
+        }
+    }
 }
 
-void spmv_layout_apply(const SpmvLayout& L, const double* upper, const double* x, double* y)
+void spmv_layout_apply(const SpmvLayout& L, const double* upper, const double* x, double* y, const int* full_row_ptr,
+                       const int* full_col, const int* full_src, const int* body_row)
 {
     const int T = (int)L.tiles.size() / 4;
     const int U = std::max(1, L.cap / 256);
@@ -1181,6 +1267,32 @@ void spmv_layout_apply(const SpmvLayout& L, const double* upper, const double* x
                 for(int i = 0; i < 3; ++i) acc[3 * (r - r0) + i] += Q[3 * q + i];
             for(int i = 0; i < 3; ++i) y[3 * (long long)r + i] = acc[3 * (r - r0) + i];
         }
+        // a body tile: then its bodies' 12 x 12 couplings, sub-blocks summed
+        // over their duplicate entries in the full CSR
+        if((L.tiles[4 * t + 1] >> 28) != kSpmvBodyTile) continue;
+        if(!full_row_ptr || !full_col || !full_src || !body_row)
+            throw std::invalid_argument("body couplings need the full CSR and the body rows");
+        const int b0 = L.tiles[4 * t + 1] & 0x0fffffff, b1 = b0 + (r1 - r0) / 4;
+        for(int b = b0; b < b1; ++b)
+            for(int k = L.cp_ptr[b]; k < L.cp_ptr[b + 1]; ++k)
+                for(int q = 0; q < 4; ++q)
+                    for(int tt = 0; tt < 4; ++tt)
+                    {
+                        const int r = body_row[b] + q, target = body_row[L.cp_col[k]] + tt;
+                        const double* xc = x + 3 * (long long)target;
+                        for(int e = L.cp_sub[16 * k + 4 * q + tt]; e >= 0 && e < full_row_ptr[r + 1] &&
+                                                                   full_col[e] == target;
+                            ++e)
+                        {
+                            double B[9];
+                            const int code = full_src[e];
+                            const double* Ub = upper + 9 * (long long)(code & 0x7fffffff);
+                            for(int i = 0; i < 3; ++i)
+                                for(int j = 0; j < 3; ++j) B[3 * i + j] = code >= 0 ? Ub[i + 3 * j] : Ub[j + 3 * i];
+                            for(int i = 0; i < 3; ++i)
+                                y[3 * (long long)r + i] += B[3 * i] * xc[0] + B[3 * i + 1] * xc[1] + B[3 * i + 2] * xc[2];
+                        }
+                    }
     }
 }
 

@@ -168,10 +168,23 @@ void validate_plan(const Plan& plan);
 //
 // Entry column words and tile limits: spmv_layout.hpp.
 
+// Rows of affine bodies get tiles of their own (up to kBodiesPerTile bodies):
+// their couplings to bodies of the same system are dense 12 x 12 blocks
+// (cp_*), the rest of their entries (couplings to FEM vertices) 3 x 3 entries
+// of the layout as usual.  Per system: first internal row of its bodies,
+// their count and the global index of its first body.
+struct SpmvBodies
+{
+    std::vector<int> abd_row0, n_abd, body_base;
+};
+constexpr int kBodiesPerTile = 5;   // 5 bodies x 12 scalar rows x 4 coupling groups = 240 threads
+constexpr int kSpmvBodyTile  = 2;   // the work-table type of a body tile (the kernels' kWgAbd)
+
 struct SpmvLayout
 {
     int cap = 0;
-    std::vector<int> tiles;       // 4 per tile: system, 0, first row, end row
+    std::vector<int> tiles;       // 4 per tile: system, type << 28 | aux (body tiles: kWgAbd, first body),
+                                  // first row, end row
     std::vector<int> tile_ent;    // 2 per tile: entry range [e0, e1)
     std::vector<int> sys_range;   // 2 per system: tile range
     std::vector<int> row_ptr;     // n_rows+1
@@ -182,17 +195,28 @@ struct SpmvLayout
     long long total = 0;          // entry slots (col.size(): padding and slack included)
     long long n_entries = 0;      // stored matrix entries (no padding)
     long long n_in_tile = 0;      // off-diagonal pairs stored once
+    // 12 x 12 body couplings: body i (global) has couplings [cp_ptr[i],
+    // cp_ptr[i+1]) in increasing column body cp_col (global), each with, per
+    // 3 x 3 sub-block (q, t) (q row, t column within the bodies), the first
+    // full-CSR entry of row 4i+q with column 4j+t (-1 if none; duplicate
+    // blocks follow it in the row)
+    std::vector<int> cp_ptr, cp_col, cp_body, cp_sub;
 };
 
 // From the full block CSR (rows sorted by column) and its entry sources.
 // Throws if a tile's codes would not fit their fields.
 void build_spmv_layout(SpmvLayout& L, int n_sys, const std::vector<int>& row_off, const std::vector<int>& row_ptr,
-                       const std::vector<int>& col, const std::vector<int>& entry_src, int cap);
+                       const std::vector<int>& col, const std::vector<int>& entry_src, int cap,
+                       const SpmvBodies* bodies = nullptr);
 // y = A x with the layout (FP64, host), from FP64 upper blocks (col-major),
 // read the way the kernel reads it: each thread's run of cap/256 consecutive
 // entries summed per row (restarting at row-start flags), the runs' ends
 // summed by the owning row, then its transposed products.  For tests of the
 // layout.
-void spmv_layout_apply(const SpmvLayout& L, const double* upper, const double* x, double* y);
+// Body tiles add their 12 x 12 couplings, read through the full CSR
+// (row pointers, columns, entry sources) and the first row of every body.
+void spmv_layout_apply(const SpmvLayout& L, const double* upper, const double* x, double* y,
+                       const int* full_row_ptr = nullptr, const int* full_col = nullptr,
+                       const int* full_src = nullptr, const int* body_row = nullptr);
 
 }  // namespace masx

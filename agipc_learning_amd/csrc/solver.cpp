@@ -224,6 +224,8 @@ void Solver::set_spmv_tiling(int entries_per_thread)
         if(built_)
         {
             MASX_HIP(launch_pack_values(dd_, work_));
+            MASX_HIP(launch_pack_abd(dd_, d_cp_body_.ptr, d_cp_sub_.ptr, (long long)spmv_layout_.cp_col.size(),
+                                     d_vals_abd_.ptr, work_));
             MASX_HIP(hipStreamSynchronize(work_));
         }
     }
@@ -452,10 +454,30 @@ void Solver::build_work_tables()
         build_spmv_layout_on_device(256 * entries_, st);  // fills the layout and the SpMV work tables
     else
     {
-        build_spmv_layout(L, S, P.row_off, P.row_ptr, P.col, P.entry_src, 256 * entries_);
+        // rows of affine bodies get body tiles with 12 x 12 couplings
+        static_assert(kSpmvBodyTile == kWgAbd, "body tiles carry the ABD work type");
+        SpmvBodies bodies;
+        for(int s = 0, base = 0; s < S; ++s)
+        {
+            bodies.abd_row0.push_back(P.abd_row0[s]);
+            bodies.n_abd.push_back(systems_[s].n_abd);
+            bodies.body_base.push_back(base);
+            base += systems_[s].n_abd;
+        }
+        build_spmv_layout(L, S, P.row_off, P.row_ptr, P.col, P.entry_src, 256 * entries_, &bodies);
         for(size_t t = 0; t < L.tiles.size() / 4; ++t)
-            spmv.push_back(make_int4(L.tiles[4 * t], kWgPlain << 28, L.tiles[4 * t + 2], L.tiles[4 * t + 3]));
+            spmv.push_back(make_int4(L.tiles[4 * t], L.tiles[4 * t + 1], L.tiles[4 * t + 2], L.tiles[4 * t + 3]));
     }
+    d_cp_ptr_.upload(L.cp_ptr, st);
+    d_cp_col_.upload(L.cp_col, st);
+    {
+        std::vector<int> cp_row(L.cp_col.size());
+        for(size_t k = 0; k < cp_row.size(); ++k) cp_row[k] = P.body_row[L.cp_col[k]];
+        d_cp_row_.upload(cp_row, st);
+    }
+    d_cp_body_.upload(L.cp_body, st);
+    d_cp_sub_.upload(L.cp_sub, st);
+    d_vals_abd_.alloc(144 * std::max<size_t>(L.cp_col.size(), 1));
     spmv_r = L.sys_range;
     for(int s = 0; s < S; ++s)
         if(spmv_r[2 * s] == spmv_r[2 * s + 1]) throw std::runtime_error("empty system");
@@ -524,6 +546,10 @@ void Solver::make_devdata()
     d.sy_col = d_sy_col_.ptr;
     d.sy_src = d_sy_src_.ptr;
     d.sy_tptr = d_sy_tptr_.ptr;
+    d.abd_cp_ptr = d_cp_ptr_.ptr;
+    d.abd_cp_col = d_cp_col_.ptr;
+    d.abd_cp_row = d_cp_row_.ptr;
+    d.vals_abd = d_vals_abd_.ptr;
     d.row_sys = d_row_sys_.ptr;
     d.vals32 = d_vals32_.ptr;
     d.vals64 = d_vals64_.ptr;
@@ -630,6 +656,8 @@ void Solver::build(int precision, hipStream_t user)
     }
     const Plan& P = plan_;
     MASX_HIP(launch_pack_values(dd_, st));
+    MASX_HIP(launch_pack_abd(dd_, d_cp_body_.ptr, d_cp_sub_.ptr, (long long)spmv_layout_.cp_col.size(),
+                             d_vals_abd_.ptr, st));
     for(size_t g = 0; g + 1 < P.gal_level_ptr.size(); ++g)
     {
         // a whole wave per entry where the lists are long (the top levels)

@@ -103,6 +103,8 @@ def load():
         "masx_autotune_mas": ([vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)], C.c_int),
         "masx_spmv_layout_check": ([C.c_int, C.c_longlong, ip, ip, C.c_int, C.c_uint, dp, C.POINTER(C.c_longlong),
                                     C.POINTER(C.c_longlong)], C.c_int),
+        "masx_spmv_layout_check_bodies": ([C.c_int, C.c_int, C.c_longlong, ip, ip, C.c_int, C.c_uint, dp,
+                                           C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], C.c_int),
         "masx_analyze": ([vp], C.c_int),
         "masx_setup": ([vp], C.c_int),
         "masx_set_values": ([vp, C.c_int, vp, vp], C.c_int),
@@ -276,15 +278,16 @@ def morton_partition_noalloc(positions, part_to_real, real_to_part, bank=32, str
                                                       _dev(real_to_part), stream))
 
 
-def spmv_layout_check(n_rows, rows, cols, entries_per_thread, seed=0):
+def spmv_layout_check(n_rows, rows, cols, entries_per_thread, seed=0, n_bodies=0):
     """Host check of the FP32 SpMV's tile layout for one system's upper block
-    structure: (max |y_layout - y_full| / max |y_full| in FP64 on seeded
-    random blocks and x, layout entries, pairs stored once)."""
+    structure (n_bodies affine bodies in rows [0, 4 n_bodies), whose couplings
+    to bodies become 12 x 12 blocks): (max |y_layout - y_full| / max |y_full|
+    in FP64 on seeded random blocks and x, layout entries, pairs stored once)."""
     L = load()
     r, c = _i32(rows), _i32(cols)
     err, ne, nt = C.c_double(), C.c_longlong(), C.c_longlong()
-    check(L.masx_spmv_layout_check(n_rows, len(r), _ptr(r, C.c_int), _ptr(c, C.c_int), entries_per_thread, seed,
-                                   C.byref(err), C.byref(ne), C.byref(nt)))
+    check(L.masx_spmv_layout_check_bodies(n_rows, n_bodies, len(r), _ptr(r, C.c_int), _ptr(c, C.c_int),
+                                          entries_per_thread, seed, C.byref(err), C.byref(ne), C.byref(nt)))
     return err.value, ne.value, nt.value
 
 

@@ -283,6 +283,12 @@ int masx_morton_partition_device_noalloc(int n, const double* d_positions, int b
  * pcg_solver.cu store and multiply the full matrix). */
 int masx_spmv_layout_check(int n_rows, long long nb, const int* rows, const int* cols, int entries_per_thread,
                            unsigned seed, double* max_rel_err, long long* n_entries, long long* n_in_tile);
+/* The same check with n_bodies affine bodies in rows [0, 4 n_bodies): their
+ * rows get body tiles whose couplings to bodies are 12 x 12 blocks (the
+ * layout's ABD path; rows 4i..4i+3 are body i, GIPC.cu:10271-10272). */
+int masx_spmv_layout_check_bodies(int n_rows, int n_bodies, long long nb, const int* rows, const int* cols,
+                                  int entries_per_thread, unsigned seed, double* max_rel_err, long long* n_entries,
+                                  long long* n_in_tile);
 
 /* ---- Stable Neo-Hookean element Hessians and their parameter gradient ---
  * The FEM part of the Newton matrix (femEnergy.cu: first Piola stress

@@ -158,3 +158,29 @@ def test_spmv_layout_rejects_bad_structure():
         native.spmv_layout_check(4, np.array([0, 0]), np.array([1, 1]), 2)
     with pytest.raises(RuntimeError, match="entries per thread"):
         native.spmv_layout_check(4, np.array([0]), np.array([0]), 512)
+
+
+@pytest.mark.parametrize("entries", [1, 2, 4, 8])
+def test_spmv_layout_with_body_couplings(entries):
+    # affine bodies lead the rows (4 per body): their couplings to bodies are
+    # 12 x 12 blocks read through the full CSR, their other entries (to FEM
+    # vertices) 3 x 3 entries of body tiles; the product equals the full CSR
+    # product.  Also: a body coupled to many vertices (its tile spans several
+    # chunks) and a body coupled to nothing but itself.
+    w = W.hybrid(seed=4, n_bodies=60, n_body_pairs=300, fem_n=5, n_coupled=12)
+    nb_ = w["n_bodies"]
+    rows, cols = _upper_of(w["rows"], w["cols"])
+    n = w["block_rows"]
+    hub = 4 * 7  # body 7's first row coupled to every FEM vertex
+    extra_c = np.arange(4 * nb_, n)
+    rows = np.concatenate([rows, np.full(len(extra_c), hub)])
+    cols = np.concatenate([cols, extra_c])
+    keep = ~((rows >= 4 * 59) & (rows < 4 * 60) & (cols >= 0) & (cols < 4 * 59))  # body 59: self only
+    keep &= ~((cols >= 4 * 59) & (cols < 4 * 60) & (rows < 4 * 59))
+    rows, cols = _upper_of(rows[keep], cols[keep])
+    err, n_ent, n_once = native.spmv_layout_check(n, rows, cols, entries, seed=entries, n_bodies=nb_)
+    off = int((rows != cols).sum())
+    in_bodies = (rows < 4 * nb_) & (cols < 4 * nb_)
+    body_full = int(in_bodies.sum() + (in_bodies & (rows != cols)).sum())
+    assert err <= 1e-13
+    assert n_ent == n + 2 * off - body_full - n_once
