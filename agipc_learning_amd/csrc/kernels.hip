@@ -800,10 +800,16 @@ __global__ void __launch_bounds__(256) k_phase_finish(DevData d, int* host, int*
 // banks are formed here from the level-1 residuals of their descendants.
 // Workgroups run the coarse banks (the end of the bank list) first, so their
 // longer gathers overlap the fine banks.
-template <int BANK, typename T, int NW>
+//
+// STAGED (FP32): the workgroup's inverses (contiguous in memory) are copied to
+// LDS with 16-byte loads, fully coalesced, and each lane then reads its
+// blocks' values from LDS; otherwise every lane loads its own values (4-byte
+// loads, a diagonal's values of two banks per wave instruction).
+template <int BANK, typename T, int NW, bool STAGED = false>
 // four waves per SIMD: the FP32 variant fits 128 VGPRs without spilling
 __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4, 8))) k_mas_local(DevData d, int force)
 {
+    static_assert(!STAGED || std::is_same<T, float>::value, "LDS staging is the FP32 layout's");
     constexpr int THREADS = 64 * NW;
     constexpr int BPW   = 64 / BANK;
     constexpr int NBLK  = BANK * (BANK + 1) / 2;
@@ -814,7 +820,13 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4,
     static_assert(BANK % NW == 0 && CHUNK % THREADS == 0, "diagonals and staged entries split evenly");
     __shared__ float sR[64 * 3 + 3];  // + a zero row for the lanes past a diagonal's end
     __shared__ float sZ[NW - 1][64 * 3];
-    __shared__ float sD[CHUNK * 3];
+    constexpr int kM4    = NBLK * 9 / 4;                                   // float4 per FP32 bank
+    constexpr int kStage = STAGED ? (BPW * kM4 + THREADS - 1) / THREADS : 1;  // float4 per thread
+    static_assert(!STAGED || (NBLK * 9 % 4 == 0 && BPW * NBLK * 9 >= CHUNK * 3),
+                  "a bank is whole float4s; the staged inverses' area holds the descendant chunk");
+    __shared__ __align__(16) float sM[STAGED ? BPW * NBLK * 9 : 4];
+    __shared__ float sD0[STAGED ? 1 : CHUNK * 3];
+    float* sD = STAGED ? sM : sD0;  // the descendant sums run before the inverses reach LDS
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int lb = lane / BANK, I = lane % BANK;
     const int bank0 = (int)(gridDim.x - 1 - blockIdx.x) * BPW;
@@ -842,16 +854,32 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4,
     }
     // this wave's inverse blocks: their addresses depend on the bank only, so
     // every load is issued before the residuals are known
-    uint32_t raw[ND][WORDS];
-#pragma unroll
-    for(int kk = 0; kk < ND; ++kk)
+    uint32_t raw[STAGED ? 1 : ND][WORDS];
+    float4   stage[kStage];
+    if constexpr(STAGED)
     {
-        const int dd = wave + NW * kk, len = BANK - dd;
-        const int Ic = I < len ? I : 0;
-        const uint32_t* M = reinterpret_cast<const uint32_t*>(d.minv) + (size_t)bl * NBLK * WORDS +
-                            WORDS * diag_off(BANK, dd) + Ic;
+        // the workgroup's live banks, 16 bytes per lane and load
 #pragma unroll
-        for(int c = 0; c < WORDS; ++c) raw[kk][c] = M[c * len];
+        for(int u = 0; u < kStage; ++u)
+        {
+            const int i = tid + THREADS * u, k = i / kM4;
+            stage[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if(i < BPW * kM4 && pick(s_live, k))
+                stage[u] = reinterpret_cast<const float4*>(d.minv)[(size_t)(bank0 + k) * kM4 + i % kM4];
+        }
+    }
+    else
+    {
+#pragma unroll
+        for(int kk = 0; kk < ND; ++kk)
+        {
+            const int dd = wave + NW * kk, len = BANK - dd;
+            const int Ic = I < len ? I : 0;
+            const uint32_t* M = reinterpret_cast<const uint32_t*>(d.minv) + (size_t)bl * NBLK * WORDS +
+                                WORDS * diag_off(BANK, dd) + Ic;
+#pragma unroll
+            for(int c = 0; c < WORDS; ++c) raw[kk][c] = M[c * len];
+        }
     }
     // residuals of level-0 and level-1 banks
     for(int i = tid; i < 64 * 3; i += blockDim.x)
@@ -930,6 +958,16 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4,
             sR[3 * i] = a0; sR[3 * i + 1] = a1; sR[3 * i + 2] = a2;
         }
     }
+    if constexpr(STAGED)
+    {
+        __syncthreads();  // the descendant chunk in the same area is consumed
+#pragma unroll
+        for(int u = 0; u < kStage; ++u)
+        {
+            const int i = tid + THREADS * u;
+            if(i < BPW * kM4) reinterpret_cast<float4*>(sM)[i] = stage[u];
+        }
+    }
     __syncthreads();
     const bool live = b < d.n_banks && pick(s_live, lb);
     const float r0 = sR[3 * lane], r1 = sR[3 * lane + 1], r2 = sR[3 * lane + 2];
@@ -945,7 +983,14 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4,
         // values, each conversion folds into the FMAs that use it.
         const bool act = live && I < BANK - dd;
         float v[9];
-        if constexpr(WORDS == 9)
+        if constexpr(STAGED)
+        {
+            const int len = BANK - dd, Ic = I < len ? I : 0;
+            const float* Ms = sM + lb * NBLK * 9 + 9 * diag_off(BANK, dd) + Ic;
+#pragma unroll
+            for(int c = 0; c < 9; ++c) v[c] = Ms[c * len];
+        }
+        else if constexpr(WORDS == 9)
         {
 #pragma unroll
             for(int c = 0; c < 9; ++c) v[c] = __uint_as_float(raw[kk][c]);
@@ -2342,7 +2387,8 @@ hipError_t launch_mas_local(const DevData& d, int precision, int force, hipStrea
 {
     if(d.n_banks <= 0) return hipSuccess;
     const bool mfma = d.bank == 32 && precision != kPrecFp32 && d.mas_waves == 1;
-    if(!mfma && d.mas_waves != 4 && d.mas_waves != 8) return hipErrorInvalidValue;
+    const bool staged = precision == kPrecFp32 && (d.mas_waves == 5 || d.mas_waves == 9);
+    if(!mfma && !staged && d.mas_waves != 4 && d.mas_waves != 8) return hipErrorInvalidValue;
     const int bpw = 64 / d.bank;
     const dim3 g(blocks_for(d.n_banks, bpw)), b(256);
     const dim3 b8(512);
@@ -2352,6 +2398,13 @@ hipError_t launch_mas_local(const DevData& d, int precision, int force, hipStrea
         if(d.mas_waves == 8) MASX_DISPATCH((k_mas_local<BK, T, 8>), g, b8, d, force); \
         else MASX_DISPATCH((k_mas_local<BK, T, 4>), g, b, d, force);                \
     } while(0)
+#define MASX_LOCAL_F32(BK)                                                                      \
+    do                                                                                          \
+    {                                                                                           \
+        if(d.mas_waves == 9) MASX_DISPATCH((k_mas_local<BK, float, 8, true>), g, b8, d, force);    \
+        else if(d.mas_waves == 5) MASX_DISPATCH((k_mas_local<BK, float, 4, true>), g, b, d, force); \
+        else MASX_LOCAL(BK, float);                                                             \
+    } while(0)
     if(d.bank == 32)
     {
         const dim3 gm(blocks_for(d.n_banks, 4));
@@ -2359,17 +2412,18 @@ hipError_t launch_mas_local(const DevData& d, int precision, int force, hipStrea
         else if(mfma) MASX_DISPATCH((k_mas_local_mfma<__hip_bfloat16>), gm, b, d, force);
         else if(precision == kPrecFp16) MASX_LOCAL(32, __half);
         else if(precision == kPrecBf16) MASX_LOCAL(32, __hip_bfloat16);
-        else MASX_LOCAL(32, float);
+        else MASX_LOCAL_F32(32);
     }
     else if(d.bank == 16)
     {
         if(precision == kPrecFp16) MASX_LOCAL(16, __half);
         else if(precision == kPrecBf16) MASX_LOCAL(16, __hip_bfloat16);
-        else MASX_LOCAL(16, float);
+        else MASX_LOCAL_F32(16);
     }
     else
         return hipErrorInvalidValue;
 #undef MASX_LOCAL
+#undef MASX_LOCAL_F32
     MASX_CHECK_LAUNCH();
     return hipSuccess;
 }

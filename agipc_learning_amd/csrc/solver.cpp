@@ -231,23 +231,32 @@ void Solver::set_spmv_tiling(int entries_per_thread)
     }
 }
 
-// Half-precision 32-slot banks from this many banks on use the MFMA local
-// kernel by default (four banks per workgroup fill the card; below that the
-// VALU kernel's shorter latency wins, measured on 2,270 and 88,000 banks).
+// From this many banks on the MAS local kernel defaults to the MFMA kernel
+// (half-precision 32-slot banks) or the LDS-staged FP32 kernel; below, the
+// register-fed VALU kernel's shorter latency wins (measured on 2,270 and
+// 91,066 banks).
 constexpr int kMfmaAutoBanks = 16384;
 
 int Solver::effective_mas_waves() const
 {
     const bool mfma_ok = plan_.bank == 32 && precision_ != kPrecFp32;
+    const bool fp32    = precision_ == kPrecFp32;
     if(mas_waves_ == 1) return mfma_ok ? 1 : 4;
     if(mas_waves_ == 4 || mas_waves_ == 8) return mas_waves_;
-    return mfma_ok && plan_.n_banks >= kMfmaAutoBanks ? 1 : 4;
+    if(mas_waves_ == 5 || mas_waves_ == 9) return fp32 ? mas_waves_ : mas_waves_ - 1;  // LDS staging is FP32's
+    // automatic: on large batches the MFMA kernel (half) or the LDS-staged
+    // FP32 kernel stream the inverse fastest; on a few thousand banks the
+    // register-fed 4-wave kernel's shorter latency wins
+    if(plan_.n_banks >= kMfmaAutoBanks) return mfma_ok ? 1 : (fp32 ? 9 : 4);
+    return 4;
 }
 
 void Solver::set_mas_waves(int waves)
 {
-    if(waves != 0 && waves != 1 && waves != 4 && waves != 8)
-        throw std::invalid_argument("MAS local waves per workgroup must be 0 (automatic), 1 (MFMA), 4 or 8");
+    if(waves != 0 && waves != 1 && waves != 4 && waves != 5 && waves != 8 && waves != 9)
+        throw std::invalid_argument(
+            "MAS local variant must be 0 (automatic), 1 (MFMA), 4 or 8 (waves per workgroup), 5 or 9 (4 or 8 waves, "
+            "FP32 inverse staged through LDS)");
     const bool tiles_before = effective_mas_waves() == 1;
     mas_waves_    = waves;
     dd_.mas_waves = effective_mas_waves();
@@ -262,6 +271,7 @@ Solver::Tiling Solver::autotune_mas(int reps)
     if(plan_.n_banks == 0) return Tiling{effective_mas_waves(), 0.0};
     std::vector<int> cand = {4, 8};
     if(plan_.bank == 32 && precision_ != kPrecFp32) cand.insert(cand.begin(), 1);
+    if(precision_ == kPrecFp32) cand.insert(cand.end(), {5, 9});
     Tiling best{effective_mas_waves(), 1e30};
     for(const int w : cand)
     {

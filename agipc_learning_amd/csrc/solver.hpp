@@ -102,10 +102,12 @@ class Solver
     // times every supported SpMV tiling on the built system and keeps the fastest
     Tiling autotune_spmv(int reps);
     // MAS local kernel: 4 or 8 waves per workgroup (VALU; each workgroup holds
-    // 64 / bank banks, its waves split their block diagonals), 1 (the MFMA
+    // 64 / bank banks, its waves split their block diagonals), 5 or 9 (the
+    // same with 4 or 8 waves, the FP32 inverse staged through LDS with
+    // 16-byte loads; half precision falls back to 4 or 8), 1 (the MFMA
     // kernel, one wave per bank: FP16 / BF16 inverses of 32-slot banks; other
-    // cases fall back to 4), or 0 (automatic: MFMA for half-precision 32-slot
-    // banks from kMfmaAutoBanks banks on, else 4).  A change that moves the
+    // cases fall back to 4), or 0 (automatic: from kMfmaAutoBanks banks on
+    // MFMA for half-precision 32-slot banks and 9 for FP32, else 4).  A change that moves the
     // half-precision inverse between the two layouts needs a new build().
     void set_mas_waves(int waves);
     int  mas_waves() const { return effective_mas_waves(); }

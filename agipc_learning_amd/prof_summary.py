@@ -73,6 +73,13 @@ def _busy_median(durations):
     return b[len(b) // 2]
 
 
+def mas_local_variant(waves):
+    """Template arguments after the element type of the FP32 MAS local kernel
+    the variant code runs: 4 / 8 waves, 5 / 9 the same with the inverse
+    staged through LDS."""
+    return {4: "4, false", 8: "8, false", 5: "4, true", 9: "8, true"}[waves]
+
+
 def compare(bench_json, trace_csv):
     """bench.py's per-kernel event times (the 64-system batch's FP32 kernels,
     or the 40^3 workload's for --fem40-only) beside the trace's busy medians."""
@@ -86,7 +93,7 @@ def compare(bench_json, trace_csv):
         us = {k: v["us"] for k, v in obj["roofline"]["per_kernel"].items()}
     tiling = obj["spmv_tiling"]["entries_per_thread"]
     waves = obj["spmv_tiling"].get("mas_local", {}).get("fp32", {}).get("waves", 4)
-    names = {"spmv": f"k_spmv_fused<{tiling}>", "mas_local": f"k_mas_local<32, float, {waves}>",
+    names = {"spmv": f"k_spmv_fused<{tiling}>", "mas_local": f"k_mas_local<32, float, {mas_local_variant(waves)}>",
              "precond_finish": "k_precond_finish", "update": "k_update"}
     d = collections.defaultdict(list)
     for r in csv.DictReader(open(trace_csv)):

@@ -110,7 +110,8 @@ def measured_traffic(kernel, workload_is_default, spmv_entries, summary="fem40c_
     data = json.load(open(path))["kernels"]
     want = f"k_spmv_fused<{spmv_entries}>" if kernel == "spmv" else KERNEL_PREFIX[kernel]
     if kernel == "mas_local" and waves:
-        want = f"{KERNEL_PREFIX[kernel]} {waves}>"
+        from agipc_learning_amd.prof_summary import mas_local_variant
+        want = f"{KERNEL_PREFIX[kernel]} {mas_local_variant(waves)}>"
     for name, v in data.items():
         if name.startswith(want):
             return int(v["bytes"])

@@ -168,10 +168,12 @@ int masx_set_spmv_tiling(masx_solver* s, int entries_per_thread);
  * Leaves the iteration state frozen: call masx_solve afterwards as usual. */
 int masx_autotune(masx_solver* s, int reps, int* entries_per_thread, double* ms);
 /* MAS local solve variant.  waves = 4 or 8: the VALU kernel, that many
- * waves per workgroup sharing the block diagonals of its banks; 1: the MFMA
- * kernel, one wave per bank (FP16 / BF16 inverses of 32-slot banks; other
- * cases run 4); 0 (the default): MFMA for half-precision 32-slot banks from
- * 16,384 banks on, else 4.  The two kernels store a half-precision inverse in
+ * waves per workgroup sharing the block diagonals of its banks; 5 or 9: the
+ * same with 4 or 8 waves, the workgroup's FP32 inverses staged through LDS
+ * with 16-byte loads (half precision runs 4 or 8); 1: the MFMA kernel, one
+ * wave per bank (FP16 / BF16 inverses of 32-slot banks; other cases run 4);
+ * 0 (the default): from 16,384 banks on MFMA for half-precision 32-slot
+ * banks and 9 for FP32, else 4.  The two kernels store a half-precision inverse in
  * different layouts: a change between them needs a new masx_build before the
  * next solve (masx_solve reports the missing build). */
 int masx_set_mas_waves(masx_solver* s, int waves);
