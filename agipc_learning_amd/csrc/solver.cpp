@@ -269,16 +269,24 @@ Solver::Tiling Solver::autotune_mas(int reps)
 {
     if(!built_) throw std::runtime_error("build() must precede autotune");
     if(plan_.n_banks == 0) return Tiling{effective_mas_waves(), 0.0};
-    std::vector<int> cand = {4, 8};
-    if(plan_.bank == 32 && precision_ != kPrecFp32) cand.insert(cand.begin(), 1);
-    if(precision_ == kPrecFp32) cand.insert(cand.end(), {5, 9});
-    Tiling best{effective_mas_waves(), 1e30};
+    // the automatic choice first; another variant replaces the best only when
+    // it is more than 2 % faster, so near-ties keep one choice from run to run
+    const int first = effective_mas_waves();
+    std::vector<int> cand = {first};
+    for(const int w : {1, 4, 8, 5, 9})
+    {
+        if(w == first || (w == 1 && !(plan_.bank == 32 && precision_ != kPrecFp32)) ||
+           ((w == 5 || w == 9) && precision_ != kPrecFp32))
+            continue;
+        cand.push_back(w);
+    }
+    Tiling best{first, 1e30};
     for(const int w : cand)
     {
         set_mas_waves(w);
         if(!built_) build(precision_, nullptr);
         const double ms = time_kernel(0, reps);
-        if(ms < best.ms) best = Tiling{w, ms};
+        if(ms < 0.98 * best.ms) best = Tiling{w, ms};
     }
     set_mas_waves(best.entries);
     if(!built_) build(precision_, nullptr);
@@ -288,12 +296,16 @@ Solver::Tiling Solver::autotune_mas(int reps)
 Solver::Tiling Solver::autotune_spmv(int reps)
 {
     if(!built_) throw std::runtime_error("build() must precede autotune");
-    Tiling best{entries_, 1e30};
-    for(const int u : {2, 4, 8})
+    // the current tiling first; another replaces the best only when more than
+    // 2 % faster (near-ties keep one choice from run to run)
+    const int first = entries_;
+    Tiling best{first, 1e30};
+    for(const int u : {first, 2, 4, 8})
     {
+        if(u == first && best.ms < 1e30) continue;
         set_spmv_tiling(u);
         const double ms = time_kernel(2, reps);
-        if(ms < best.ms) best = Tiling{u, ms};
+        if(ms < 0.98 * best.ms) best = Tiling{u, ms};
     }
     set_spmv_tiling(best.entries);
     return best;
