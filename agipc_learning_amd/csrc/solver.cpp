@@ -411,8 +411,13 @@ void Solver::setup()
     MASX_HIP(hipMemsetAsync(d_export_seq_.ptr, 0, sizeof(int), st));
     d_factor_status_.alloc(1);
     MASX_HIP(hipMemsetAsync(d_state_.ptr, 0, d_state_.n * sizeof(SysState), st));
-    if(h_state_) (void)hipHostFree(h_state_);
-    MASX_HIP(hipHostMalloc((void**)&h_state_, sizeof(SysState) * P.n_sys * (kDepth + 1), hipHostMallocDefault));
+    if(h_state_cap_ < P.n_sys)  // pinned allocations are slow: kept across structures that fit
+    {
+        if(h_state_) (void)hipHostFree(h_state_);
+        h_state_ = nullptr;
+        MASX_HIP(hipHostMalloc((void**)&h_state_, sizeof(SysState) * P.n_sys * (kDepth + 1), hipHostMallocDefault));
+        h_state_cap_ = P.n_sys;
+    }
     tm.mark("uploads");
     build_work_tables();
     tm.mark("work tables + SpMV layout");

@@ -202,6 +202,7 @@ class Solver
     static constexpr int kDepth = 2;       // graph chunks in flight during a solve
     hipEvent_t  chunk_ev_[kDepth] = {};
     SysState*   h_state_ = nullptr;        // pinned: kDepth + 1 slots of n_sys states
+    int         h_state_cap_ = 0;          // systems it holds
     int*        h_export_ = nullptr;       // coherent pinned: unfinished-system count after each chunk
     int*        d_export_ = nullptr;       // its device address
     long long   exports_ = 0;              // status exports launched so far (host mirror of the sequence)
