@@ -95,7 +95,10 @@ class ElasticSystem:
         (which FP32 iterates cannot reach: x itself is FP32 inside the PCG)
         use FP64 iterative refinement: x and the residual b - A x stay FP64
         (the library's FP64 SpMV), each correction is an FP32 MAS-PCG solve
-        to inner_tol of the current residual."""
+        of the current residual to what is still missing (tol over the
+        current relative residual, with a margin of 2), but never tighter
+        than inner_tol: a tolerance of 1e-5 then costs one full solve and a
+        short correction."""
         import torch
 
         rhs = rhs.detach().to(torch.float64).contiguous()
@@ -114,7 +117,8 @@ class ElasticSystem:
         iters, rel, st = 0, 1.0, None
         for k in range(self.max_refine):
             d.zero_()
-            st = self.solver.solve(r, d, rule=native.RULE_TRUE_REL, tol=self.inner_tol, max_iter=self.max_iter)[0]
+            step_tol = min(0.5, max(self.inner_tol, 0.5 * self.tol / rel)) if rel > 0 else self.inner_tol
+            st = self.solver.solve(r, d, rule=native.RULE_TRUE_REL, tol=step_tol, max_iter=self.max_iter)[0]
             if st["failure"] != 0 or st["status"] != native.STATUS_DONE:
                 raise RuntimeError(f"MAS-PCG failed in refinement step {k}: {st}")
             iters += st["iterations"]

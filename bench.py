@@ -517,17 +517,35 @@ def adjoint_leg(args):
     solve_ms = (time.perf_counter() - t) * 1e3 / 3
     y = system.solve(x - xstar)
     grad_ms = timed(lambda: system.param_grad(mu_d, lam_d, x, y))
-    # refined reference gradient
+    # refined reference gradient (both solves to 1e-9 by FP64 refinement; the
+    # faster of two steps)
     system.tol = 1e-9
-    t = time.perf_counter()
-    gm_r, gl_r, _ = step()
-    torch.cuda.synchronize()
-    refined_ms = (time.perf_counter() - t) * 1e3
+    refined_ms = 1e30
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        gm_r, gl_r, _ = step()
+        torch.cuda.synchronize()
+        refined_ms = min(refined_ms, (time.perf_counter() - t) * 1e3)
     fwd_r, adj_r = dict(system.forward_stats), dict(system.adjoint_stats)
     g = torch.cat([gm, gl])
     g_r = torch.cat([gm_r, gl_r])
-    err = float(torch.linalg.vector_norm(g - g_r) / torch.linalg.vector_norm(g_r))
+    rel_to_refined = lambda a: float(torch.linalg.vector_norm(a - g_r) / torch.linalg.vector_norm(g_r))
+    err = rel_to_refined(g)
     err_mu = float(torch.linalg.vector_norm(gm - gm_r) / torch.linalg.vector_norm(gm_r))
+    # the tolerance whose gradient meets 1e-4: both solves to 1e-5 (one full
+    # FP32 solve and a short FP64-refined correction each)
+    system.tol = 1e-5
+    step()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        gm5, gl5, _ = step()
+    torch.cuda.synchronize()
+    tight_ms = (time.perf_counter() - t) * 1e3 / reps
+    fwd5, adj5 = dict(system.forward_stats), dict(system.adjoint_stats)
+    err5 = rel_to_refined(torch.cat([gm5, gl5]))
+    system.tol = args.tol
     out = {"workload": f"fem{n}: {n}^3 hex grid, {w['n_vertices']} vertices, {len(w['tets'])} tets, lognormal "
                        "Lame parameters, L = 1/2 ||x - x*||^2",
            "dof": 3 * w["n_vertices"], "upper_blocks": int(system.nb), "setup_ms": round(setup_ms, 1),
@@ -540,11 +558,14 @@ def adjoint_leg(args):
            "refined": {"tol": 1e-9, "step_ms": round(refined_ms, 3),
                        "forward_iterations": fwd_r["iterations"], "forward_refinements": fwd_r["refinements"],
                        "adjoint_iterations": adj_r["iterations"], "adjoint_refinements": adj_r["refinements"]},
-           "grad_rel_err_vs_refined": err, "grad_mu_rel_err_vs_refined": err_mu}
+           "grad_rel_err_vs_refined": err, "grad_mu_rel_err_vs_refined": err_mu,
+           "tol_1e-5": {"step_ms": round(tight_ms, 3), "forward_iterations": fwd5["iterations"],
+                        "forward_refinements": fwd5["refinements"], "adjoint_iterations": adj5["iterations"],
+                        "grad_rel_err_vs_refined": err5}}
     log(f"  adjoint: step {step_ms:.2f} ms (assemble {assemble_ms:.3f}, assemble + build {build_ms:.3f}, "
         f"one solve {solve_ms:.3f} (two per step), "
         f"grad {grad_ms:.3f}), iterations {fwd['iterations']}/{adj['iterations']}, refined step {refined_ms:.1f} ms, "
-        f"grad rel err vs refined {err:.2e}")
+        f"grad rel err vs refined {err:.2e}; tol 1e-5: step {tight_ms:.2f} ms, grad rel err {err5:.2e}")
     return out
 
 
