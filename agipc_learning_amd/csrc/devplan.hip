@@ -39,6 +39,23 @@ __device__ __forceinline__ int first_ge(const int* row_ptr, const int* col, int 
     return lo;
 }
 
+// the system whose range [off[s], off[s+1]) holds x (off increasing, n_sys >= 1)
+__device__ __forceinline__ int system_of(const int* off, int n_sys, long long x)
+{
+    int lo = 0, hi = n_sys - 1;
+    while(lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if(off[mid] <= x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+constexpr uint64_t kDpSentinel = ~(uint64_t)0;
+
 // exclusive scan of v over a 256-thread workgroup (s: 256 ints of LDS);
 // returns the exclusive prefix, *total the sum
 __device__ int block_exclusive_scan(int v, int* s, int* total)
@@ -65,12 +82,14 @@ __global__ void k_dp_fill(int* p, int value, long long n)
         p[i] = value;
 }
 
-__global__ void k_dp_csr_items(long long nb, const int* rows, const int* cols, int n_ext, const int* e2i, int n_rows,
-                               int bits, uint64_t* keys, int* vals, int* row_cnt, int* diag_src, int* err)
+__global__ void k_dp_csr_items(long long nb, const int* rows, const int* cols, DpSystems sys, const int* e2i,
+                               int n_rows, int bits, uint64_t* keys, int* vals, int* row_cnt, int* diag_src, int* err)
 {
     const uint64_t sentinel = (uint64_t)n_rows << bits;
     for(long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < nb; k += (long long)gridDim.x * blockDim.x)
     {
+        const int s = system_of(sys.src_off, sys.n_sys, k);
+        const int e0 = sys.ext_off[s], n_ext = sys.ext_off[s + 1] - e0;
         const int r = rows[k], c = cols[k];
         if((unsigned)r >= (unsigned)n_ext || (unsigned)c >= (unsigned)n_ext)
         {
@@ -79,7 +98,7 @@ __global__ void k_dp_csr_items(long long nb, const int* rows, const int* cols, i
             vals[2 * k] = vals[2 * k + 1] = -1;
             continue;
         }
-        const int iR = e2i[r];
+        const int iR = e2i[e0 + r];
         if(r == c)
         {
             keys[2 * k]     = (uint64_t)iR << bits | (uint64_t)iR;
@@ -91,7 +110,7 @@ __global__ void k_dp_csr_items(long long nb, const int* rows, const int* cols, i
         }
         else
         {
-            const int iC    = e2i[c];
+            const int iC    = e2i[e0 + c];
             keys[2 * k]     = (uint64_t)iR << bits | (uint64_t)iC;
             vals[2 * k]     = (int)k;
             keys[2 * k + 1] = (uint64_t)iC << bits | (uint64_t)iR;
@@ -113,25 +132,35 @@ __global__ void k_dp_csr_fill(long long nnz, const uint64_t* keys, const int* va
 }
 
 __global__ void k_dp_gal_items(int lvl, long long n, const int* rows, const int* cols, const int* EI, const int* EJ,
-                               int e_prev, const int* going_next, int lb, int bits, uint64_t* keys, int* vals)
+                               const int* ES, int e_prev, DpSystems sys, const int* gn_all, const int* lvl_begin,
+                               int bits, uint64_t* keys, int* vals, int* n_valid)
 {
+    int valid = 0;
     for(long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
     {
+        const int s = lvl == 1 ? system_of(sys.src_off, sys.n_sys, k) : ES[e_prev + k];
+        if(sys.levels[s] <= lvl)
+        {
+            keys[k] = kDpSentinel;
+            vals[k] = -1;
+            continue;
+        }
+        const int* gn = gn_all + sys.gn_off[s];
         int a, b, src;
         bool diag;
         if(lvl == 1)
         {
             const int r = rows[k], c = cols[k];
-            a    = going_next[r];
-            b    = going_next[c];
+            a    = gn[r];
+            b    = gn[c];
             diag = r == c;
             src  = (int)k;
         }
         else
         {
             const int pi = EI[e_prev + k], pj = EJ[e_prev + k];
-            a    = going_next[pi];
-            b    = going_next[pj];
+            a    = gn[pi];
+            b    = gn[pj];
             diag = pi == pj;
             src  = e_prev + (int)k;
         }
@@ -139,30 +168,39 @@ __global__ void k_dp_gal_items(int lvl, long long n, const int* rows, const int*
         if(a < b) { I = a; J = b; mode = 0; }
         else if(a > b) { I = b; J = a; mode = 1; }
         else { I = a; J = a; mode = diag ? 0 : 2; }
-        keys[k] = (uint64_t)(I - lb) << bits | (uint64_t)(J - lb);
+        const int lb = lvl_begin[s];
+        keys[k] = (uint64_t)s << (2 * bits) | (uint64_t)(I - lb) << bits | (uint64_t)(J - lb);
         vals[k] = src << 2 | mode;
+        ++valid;
     }
+    if(valid) atomicAdd(n_valid, valid);
 }
 
 __global__ void k_dp_run_ends(long long n, const uint64_t* keys, int* flag)
 {
     for(long long x = blockIdx.x * (long long)blockDim.x + threadIdx.x; x <= n; x += (long long)gridDim.x * blockDim.x)
-        flag[x] = x == n ? 0 : (x + 1 == n || keys[x + 1] != keys[x] ? 1 : 0);
+        flag[x] = x == n || keys[x] == kDpSentinel ? 0 : (x + 1 == n || keys[x + 1] != keys[x] ? 1 : 0);
 }
 
 __global__ void k_dp_gal_fill(long long n, const uint64_t* keys, const int* vals, const int* flag, const int* qpos,
-                              int g_base, int e_base, int lb, int bits, int* gal_src, int* gal_ptr, int* EI, int* EJ)
+                              int g_base, int e_base, const int* lvl_begin, int bits, int* gal_src, int* gal_ptr,
+                              int* EI, int* EJ, int* ES)
 {
     const uint64_t mask = ((uint64_t)1 << bits) - 1;
     for(long long x = blockIdx.x * (long long)blockDim.x + threadIdx.x; x < n; x += (long long)gridDim.x * blockDim.x)
     {
+        const uint64_t key = keys[x];
+        if(key == kDpSentinel) continue;  // sorted last: x is past the level's items
         gal_src[g_base + x] = vals[x];
         if(flag[x])
         {
-            const int q          = e_base + qpos[x];
-            EI[q]                = lb + (int)(keys[x] >> bits);
-            EJ[q]                = lb + (int)(keys[x] & mask);
-            gal_ptr[q + 1]       = g_base + (int)x + 1;
+            const int s  = (int)(key >> (2 * bits));
+            const int lb = lvl_begin[s];
+            const int q  = e_base + qpos[x];
+            EI[q]          = lb + (int)((key >> bits) & mask);
+            EJ[q]          = lb + (int)(key & mask);
+            ES[q]          = s;
+            gal_ptr[q + 1] = g_base + (int)x + 1;
         }
     }
 }
@@ -199,8 +237,13 @@ __global__ void k_dp_ent_fill_rows(int n_slots, int bank, const int* row_ptr, co
     }
 }
 
-__global__ void k_dp_ent_flag_coarse(int n_gal, int bank, int n_slots, int n_fine_banks, const int* EI, const int* EJ,
-                                     int* flag, int* bank_cnt)
+__device__ __forceinline__ int coarse_bank_of(const DpSystems& sys, int n_fine_banks, int bank, int s, int I)
+{
+    return n_fine_banks + (sys.coarse_base[s] + I - sys.n_slots[s]) / bank;
+}
+
+__global__ void k_dp_ent_flag_coarse(int n_gal, int bank, int n_fine_banks, DpSystems sys, const int* EI, const int* EJ,
+                                     const int* ES, int* flag, int* bank_cnt)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if(e > n_gal) return;
@@ -209,17 +252,27 @@ __global__ void k_dp_ent_flag_coarse(int n_gal, int bank, int n_slots, int n_fin
     {
         const int I = EI[e], J = EJ[e];
         f = I / bank == J / bank;
-        if(f) atomicAdd(&bank_cnt[n_fine_banks + (I - n_slots) / bank], 1);
+        if(f) atomicAdd(&bank_cnt[coarse_bank_of(sys, n_fine_banks, bank, ES[e], I)], 1);
     }
     flag[e] = f;
 }
 
-__global__ void k_dp_ent_fill_coarse(int n_gal, int bank, const int* EI, const int* EJ, const int* flag,
-                                     const int* fpos, int fine_total, int* pos, int* src)
+__global__ void k_dp_ent_first_coarse(int n_gal, int bank, int n_fine_banks, DpSystems sys, const int* EI,
+                                      const int* ES, const int* flag, const int* fpos, int* bank_first)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if(e >= n_gal || !flag[e]) return;
-    const int at = fine_total + fpos[e];
+    atomicMin(&bank_first[coarse_bank_of(sys, n_fine_banks, bank, ES[e], EI[e])], fpos[e]);
+}
+
+__global__ void k_dp_ent_fill_coarse(int n_gal, int bank, int n_fine_banks, DpSystems sys, const int* EI,
+                                     const int* EJ, const int* ES, const int* flag, const int* fpos,
+                                     const int* bank_first, const int* bank_ptr, int* pos, int* src)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if(e >= n_gal || !flag[e]) return;
+    const int gb = coarse_bank_of(sys, n_fine_banks, bank, ES[e], EI[e]);
+    const int at = bank_ptr[gb] + fpos[e] - bank_first[gb];
     pos[at] = packed_index(bank, EI[e] % bank, EJ[e] % bank);
     src[at] = e;
 }
@@ -231,13 +284,15 @@ __global__ void k_dp_ent_bank_counts(int n_fine_banks, int bank, const int* row_
 }
 
 // the greedy tile that starts at r0 (build_spmv_layout's cut, from any r0)
-__global__ void k_dp_sl_next(int n, const int* row_ptr, const int* col, int cap, int* nxt, int* kept)
+__global__ void k_dp_sl_next(int n, const int* row_ptr, const int* col, const int* row_sys, const int* row_off,
+                             int cap, int* nxt, int* kept)
 {
     const int r0 = blockIdx.x * blockDim.x + threadIdx.x;
     if(r0 >= n) return;
+    const int r_end = row_off[row_sys[r0] + 1];  // the system's end
     int k = row_ptr[r0 + 1] - row_ptr[r0];
     int r1 = r0 + 1;
-    while(r1 < n && r1 - r0 < kSpmvTileRows)
+    while(r1 < r_end && r1 - r0 < kSpmvTileRows)
     {
         const int q = (row_ptr[r1 + 1] - first_ge(row_ptr, col, r1, r1)) + (first_ge(row_ptr, col, r1, r0) - row_ptr[r1]);
         if(k + q > cap) break;
@@ -310,9 +365,9 @@ __global__ void __launch_bounds__(kThreads) k_dp_sl_tile_counts(int n_tiles, int
 
 __global__ void __launch_bounds__(kThreads) k_dp_sl_fill(int n_tiles, int n, const int* ts, const int* kept,
                                                          const int* e_base, const int* q_base, const int* row_ptr,
-                                                         const int* col, const int* entry_src, int* L_row_ptr,
-                                                         int* L_tptr, int* L_col, int* L_src, int4* wg, int2* wg_ent,
-                                                         int* err)
+                                                         const int* col, const int* entry_src, const int* row_sys,
+                                                         int* L_row_ptr, int* L_tptr, int* L_col, int* L_src, int4* wg,
+                                                         int2* wg_ent, int* err)
 {
     __shared__ int s_scan[kThreads];
     __shared__ int s_tstart[kThreads + 1];
@@ -360,7 +415,7 @@ __global__ void __launch_bounds__(kThreads) k_dp_sl_fill(int n_tiles, int n, con
     }
     if(j == 0)
     {
-        wg[t]     = make_int4(0, kWgPlain << 28, r0, r1);
+        wg[t]     = make_int4(row_sys[r0], kWgPlain << 28, r0, r1);
         wg_ent[t] = make_int2(e_base[t], e_base[t] + kept[r0]);
         if(t == n_tiles - 1)
         {
@@ -417,11 +472,11 @@ hipError_t dp_fill(int* p, int value, long long n, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t dp_csr_items(long long nb, const int* rows, const int* cols, int n_ext, const int* e2i, int n_rows,
+hipError_t dp_csr_items(long long nb, const int* rows, const int* cols, DpSystems sys, const int* e2i, int n_rows,
                         int bits, uint64_t* keys, int* vals, int* row_cnt, int* diag_src, int* err, hipStream_t st)
 {
     if(nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_dp_csr_items, dim3(grid_for(nb)), dim3(kThreads), 0, st, nb, rows, cols, n_ext, e2i, n_rows,
+    hipLaunchKernelGGL(k_dp_csr_items, dim3(grid_for(nb)), dim3(kThreads), 0, st, nb, rows, cols, sys, e2i, n_rows,
                        bits, keys, vals, row_cnt, diag_src, err);
     return hipGetLastError();
 }
@@ -435,12 +490,12 @@ hipError_t dp_csr_fill(long long nnz, const uint64_t* keys, const int* vals, int
 }
 
 hipError_t dp_gal_items(int lvl, long long n, const int* rows, const int* cols, const int* EI, const int* EJ,
-                        int e_prev, const int* going_next, int lb, int bits, uint64_t* keys, int* vals,
-                        hipStream_t st)
+                        const int* ES, int e_prev, DpSystems sys, const int* gn_all, const int* lvl_begin, int bits,
+                        uint64_t* keys, int* vals, int* n_valid, hipStream_t st)
 {
     if(n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_dp_gal_items, dim3(grid_for(n)), dim3(kThreads), 0, st, lvl, n, rows, cols, EI, EJ, e_prev,
-                       going_next, lb, bits, keys, vals);
+    hipLaunchKernelGGL(k_dp_gal_items, dim3(grid_for(n)), dim3(kThreads), 0, st, lvl, n, rows, cols, EI, EJ, ES,
+                       e_prev, sys, gn_all, lvl_begin, bits, keys, vals, n_valid);
     return hipGetLastError();
 }
 
@@ -451,12 +506,12 @@ hipError_t dp_run_ends(long long n, const uint64_t* keys, int* flag, hipStream_t
 }
 
 hipError_t dp_gal_fill(long long n, const uint64_t* keys, const int* vals, const int* flag, const int* qpos,
-                       int g_base, int e_base, int lb, int bits, int* gal_src, int* gal_ptr, int* EI, int* EJ,
-                       hipStream_t st)
+                       int g_base, int e_base, const int* lvl_begin, int bits, int* gal_src, int* gal_ptr, int* EI,
+                       int* EJ, int* ES, hipStream_t st)
 {
     if(n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_dp_gal_fill, dim3(grid_for(n)), dim3(kThreads), 0, st, n, keys, vals, flag, qpos, g_base,
-                       e_base, lb, bits, gal_src, gal_ptr, EI, EJ);
+                       e_base, lvl_begin, bits, gal_src, gal_ptr, EI, EJ, ES);
     return hipGetLastError();
 }
 
@@ -477,20 +532,30 @@ hipError_t dp_ent_fill_rows(int n_slots, int bank, const int* row_ptr, const int
     return hipGetLastError();
 }
 
-hipError_t dp_ent_flag_coarse(int n_gal, int bank, int n_slots, int n_fine_banks, const int* EI, const int* EJ,
-                              int* flag, int* bank_cnt, hipStream_t st)
+hipError_t dp_ent_flag_coarse(int n_gal, int bank, int n_fine_banks, DpSystems sys, const int* EI, const int* EJ,
+                              const int* ES, int* flag, int* bank_cnt, hipStream_t st)
 {
     hipLaunchKernelGGL(k_dp_ent_flag_coarse, dim3((n_gal + kThreads) / kThreads), dim3(kThreads), 0, st, n_gal, bank,
-                       n_slots, n_fine_banks, EI, EJ, flag, bank_cnt);
+                       n_fine_banks, sys, EI, EJ, ES, flag, bank_cnt);
     return hipGetLastError();
 }
 
-hipError_t dp_ent_fill_coarse(int n_gal, int bank, const int* EI, const int* EJ, const int* flag,
-                              const int* fpos, int fine_total, int* pos, int* src, hipStream_t st)
+hipError_t dp_ent_first_coarse(int n_gal, int bank, int n_fine_banks, DpSystems sys, const int* EI, const int* ES,
+                               const int* flag, const int* fpos, int* bank_first, hipStream_t st)
+{
+    if(n_gal <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dp_ent_first_coarse, dim3((n_gal + kThreads - 1) / kThreads), dim3(kThreads), 0, st, n_gal,
+                       bank, n_fine_banks, sys, EI, ES, flag, fpos, bank_first);
+    return hipGetLastError();
+}
+
+hipError_t dp_ent_fill_coarse(int n_gal, int bank, int n_fine_banks, DpSystems sys, const int* EI, const int* EJ,
+                              const int* ES, const int* flag, const int* fpos, const int* bank_first,
+                              const int* bank_ptr, int* pos, int* src, hipStream_t st)
 {
     if(n_gal <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_dp_ent_fill_coarse, dim3((n_gal + kThreads - 1) / kThreads), dim3(kThreads), 0, st, n_gal,
-                       bank, EI, EJ, flag, fpos, fine_total, pos, src);
+                       bank, n_fine_banks, sys, EI, EJ, ES, flag, fpos, bank_first, bank_ptr, pos, src);
     return hipGetLastError();
 }
 
@@ -502,11 +567,12 @@ hipError_t dp_ent_bank_counts(int n_fine_banks, int bank, const int* row_off, in
     return hipGetLastError();
 }
 
-hipError_t dp_sl_next(int n, const int* row_ptr, const int* col, int cap, int* nxt, int* kept, hipStream_t st)
+hipError_t dp_sl_next(int n, const int* row_ptr, const int* col, const int* row_sys, const int* row_off, int cap,
+                      int* nxt, int* kept, hipStream_t st)
 {
     if(n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_dp_sl_next, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, st, n, row_ptr, col, cap,
-                       nxt, kept);
+    hipLaunchKernelGGL(k_dp_sl_next, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, st, n, row_ptr, col,
+                       row_sys, row_off, cap, nxt, kept);
     return hipGetLastError();
 }
 
@@ -525,12 +591,12 @@ hipError_t dp_sl_tile_counts(int n_tiles, int n, const int* ts, const int* kept,
 }
 
 hipError_t dp_sl_fill(int n_tiles, int n, const int* ts, const int* kept, const int* e_base, const int* q_base,
-                      const int* row_ptr, const int* col, const int* entry_src, int* L_row_ptr, int* L_tptr,
-                      int* L_col, int* L_src, int4* wg, int2* wg_ent, int* err, hipStream_t st)
+                      const int* row_ptr, const int* col, const int* entry_src, const int* row_sys, int* L_row_ptr,
+                      int* L_tptr, int* L_col, int* L_src, int4* wg, int2* wg_ent, int* err, hipStream_t st)
 {
     if(n_tiles <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_dp_sl_fill, dim3(n_tiles), dim3(kThreads), 0, st, n_tiles, n, ts, kept, e_base, q_base,
-                       row_ptr, col, entry_src, L_row_ptr, L_tptr, L_col, L_src, wg, wg_ent, err);
+                       row_ptr, col, entry_src, row_sys, L_row_ptr, L_tptr, L_col, L_src, wg, wg_ent, err);
     return hipGetLastError();
 }
 

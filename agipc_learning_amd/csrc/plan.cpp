@@ -325,8 +325,9 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys, bool structure_on_device)
     PhaseTimer tm("plan");
     const int S = (int)sys.size();
     if(S == 0) throw std::invalid_argument("no systems");
-    if(structure_on_device && (S != 1 || !sys[0].use_mas || sys[0].n_abd != 0 || !sys[0].hier_ready))
-        throw std::logic_error("the device structure build takes one MAS system without bodies");
+    for(int s = 0; structure_on_device && s < S; ++s)
+        if(!sys[s].use_mas || sys[s].n_abd != 0 || !sys[s].hier_ready)
+            throw std::logic_error("the device structure build takes MAS systems without bodies");
     const int bank = sys[0].bank;
     for(auto& s : sys)
         if(s.use_mas && s.bank != bank) throw std::invalid_argument("all systems of a batch share one bank size");

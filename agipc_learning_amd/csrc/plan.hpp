@@ -150,8 +150,8 @@ struct Plan
     bool structure_on_device = false;
 };
 
-// structure_on_device: one MAS system without affine bodies whose hierarchy
-// is ready; the host analysis then skips what the device builds
+// structure_on_device: MAS systems without affine bodies whose hierarchies
+// are ready; the host analysis then skips what the device builds
 void build_plan(Plan& plan, std::vector<SystemDesc>& systems, bool structure_on_device = false);
 // Bounds of every index array the kernels dereference; throws on violation.
 void validate_plan(const Plan& plan);

@@ -120,8 +120,8 @@ class Solver
     long long factor_status() const;
     // Structure analysis of a new structure: 1 (default) builds the block
     // CSR, Galerkin maps, bank entry lists and SpMV layout on the device when
-    // the solver holds one MAS system without bodies whose hierarchy the
-    // device built; 0 always on the host.  Takes effect at the next analysis.
+    // every system of the solver is a MAS system without bodies; 0 always on
+    // the host.  Takes effect at the next analysis.
     void set_device_structure(int on) { dev_struct_mode_ = on ? 1 : 0; }
     bool device_structure_used() const { return dev_struct_; }
     // host copy of a device plan array (tests): 0 row_ptr, 1 col, 2 entry_src,
@@ -186,8 +186,8 @@ class Solver
     int* h_dp_ = nullptr;  // pinned read-back slots of the device structure build
     const int* read_ints(std::initializer_list<const int*> ptrs, hipStream_t st);
     DeviceArray<int>       d_dp_rows_, d_dp_cols_, d_dp_vals_, d_dp_vals2_, d_dp_cnt_, d_dp_err_, d_dp_flag_,
-                           d_dp_scan_, d_dp_scan2_, d_dp_bcnt_, d_gal_EI_, d_gal_EJ_, d_dp_nxt_, d_dp_kept_, d_dp_ts_,
-                           d_dp_tk_, d_dp_tq_, d_dp_eb_, d_dp_qb_;
+                           d_dp_scan_, d_dp_scan2_, d_dp_bcnt_, d_dp_bfirst_, d_gal_EI_, d_gal_EJ_, d_gal_ES_, d_dp_nxt_,
+                           d_dp_kept_, d_dp_ts_, d_dp_tk_, d_dp_tq_, d_dp_eb_, d_dp_qb_, d_dp_sys_, d_dp_gn_;
     DeviceArray<uint64_t>  d_dp_keys_, d_dp_keys2_;
     DeviceArray<char>      d_dp_tmp_;
     DeviceArray<uint32_t>  d_h_start_, d_h_num_, d_h_list_;

@@ -402,6 +402,28 @@ def batch_leg(args):
             f"{max(its)}, worst true rel res {worst:.2e}, kernels us {kern}, GB/s {out[prec]['kernels_GBs']}")
     for prec in precs[1:]:
         out[prec]["speedup_vs_fp32"] = round(out["fp32"]["ms"] / out[prec]["ms"], 3)
+    if not args.no_extra:
+        # a structure change of every system (new rows/cols and contact pairs,
+        # here the same ones: the analysis reruns in full): MAS hierarchies,
+        # block CSR, Galerkin maps, bank entries and SpMV layout on the device,
+        # and for comparison the host analysis
+        log("step: batch64 structure change")
+        restr = {}
+        for key, on_device in (("ms", True), ("host_structure_ms", False)):
+            s.set_device_structure(on_device)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for i, w in enumerate(ws):
+                s.set_structure(i, w["rows"], w["cols"])
+                s.set_contacts(i, w["pairs"])
+            s.setup()
+            torch.cuda.synchronize()
+            restr[key] = round((time.perf_counter() - t) * 1e3, 1)
+            if s.device_structure_used() != on_device:
+                raise SystemExit("batch structure analysis ran on the wrong side")
+        s.set_device_structure(True)
+        out["structure_change"] = restr
+        log(f"  structure change: {restr}")
     del s, vals, b, x
     torch.cuda.empty_cache()
     return out, ws, sols

@@ -1,6 +1,6 @@
 """The structure analysis on the device equals the host analysis bit for bit.
 
-For one MAS system without affine bodies, masx_setup builds the block CSR,
+For MAS systems without affine bodies, masx_setup builds the block CSR,
 the diagonal sources, the Galerkin maps of every coarse level, the bank entry
 lists and the SpMV tile layout on the device (the reference rebuilds the same
 information on the GPU each Newton step: ReorderRealtime and
@@ -10,7 +10,8 @@ the oracle tests; here every array of the device build is compared with the
 host build of the same structure, for every SpMV tiling, and the two solvers
 must then solve bit for bit alike.  Structures: grids with and without
 contacts, bank 16, a level override, duplicate blocks and blocks given in
-the lower triangle, and a structure change on the same solver."""
+the lower triangle, a structure change on the same solver, and a batch of
+systems of different sizes."""
 import numpy as np
 import pytest
 
@@ -133,13 +134,38 @@ def test_device_structure_at_the_40_cubed_size_and_after_a_contact_change():
     sd.close()
 
 
-def test_batches_and_bodies_keep_the_host_analysis():
-    a = W.fem_grid(6, n_contacts=20, seed=1)
-    b = W.fem_grid(7, n_contacts=20, seed=2)
-    s = session.describe([a, b])
-    s.setup()
-    assert not s.device_structure_used()
-    s.close()
+def test_batch_device_structure_equals_host():
+    # a batch of systems of different sizes (so different level counts and
+    # coarse sizes; one without contacts), as one solver: the device analysis
+    # keys the coarse levels by system and ranks each bank's entries, the
+    # SpMV tiles stop at every system's end
+    ws = [W.fem_grid(n, n_contacts=c, seed=30 + n, young_sigma=0.5)
+          for n, c in ((3, 0), (12, 300), (7, 40), (16, 600), (5, 10))]
+    sh = session.describe(ws)
+    sh.set_device_structure(0)
+    sh.setup()
+    sd = session.describe(ws)
+    sd.setup()
+    compare(sd, sh, "batch of 5 systems")
+    for s in (sd, sh):
+        for i, w in enumerate(ws):
+            s.set_values(i, dev(w["vals"]))
+        s.build("fp32")
+    b = session.rhs_of(ws)
+    xs = []
+    for s in (sd, sh):
+        x = torch.zeros(len(b), dtype=torch.float64, device="cuda")
+        st = s.solve(dev(b), x, rule=1, tol=1e-4)
+        torch.cuda.synchronize()
+        assert all(q["rel_res_true"] <= 1e-4 * (1 + 1e-9) for q in st)
+        xs.append((x.cpu().numpy(), [q["iterations"] for q in st]))
+    assert xs[0][1] == xs[1][1]
+    assert np.array_equal(xs[0][0], xs[1][0])
+    sd.close()
+    sh.close()
+
+
+def test_bodies_keep_the_host_analysis():
     h = W.hybrid(n_bodies=20, n_body_pairs=60, fem_n=4, n_coupled=5, seed=0)
     s = session.describe(h)
     s.setup()
