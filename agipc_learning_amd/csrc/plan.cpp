@@ -551,7 +551,8 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys, bool structure_on_device)
     P.bank_kind.assign(P.n_banks, 0);
     P.anc.assign(8 * (size_t)std::max(n_coarse, 1), -1);
     P.row_l1.assign(P.n_rows, -1);
-    std::vector<std::vector<int>> desc(n_coarse);
+    // systems are independent here (disjoint banks, coarse nodes and rows)
+#pragma omp parallel for schedule(dynamic, 1)
     for(int s = 0; s < S; ++s)
     {
         auto& d = sys[s];
@@ -572,7 +573,6 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys, bool structure_on_device)
             while(cur >= 0 && k < 8)
             {
                 P.anc[8 * (size_t)g1 + k++] = gcoarse(s, cur);
-                if(cur != slot1) desc[gcoarse(s, cur)].push_back(g1);
                 if(cur >= top_begin) break;
                 cur = h.going_next[cur];
             }
@@ -585,10 +585,28 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys, bool structure_on_device)
             P.row_l1[P.fem_row0[s] + slot] = g1;
         }
     }
+    // level-1 descendants of every deeper node, in level-1 node order (a
+    // counting pass over the ancestor lists, then the fill)
+    std::vector<std::pair<int, int>> l1_ranges;  // per system: its level-1 nodes [first, end)
+    for(int s = 0; s < S; ++s)
+    {
+        const auto& h = sys[s].hier;
+        if(sys[s].use_mas && h.levels > 1)
+            l1_ranges.push_back({gcoarse(s, h.level_begin[1]), gcoarse(s, h.level_begin[1]) + h.level_count[1]});
+    }
     P.desc_ptr.assign(n_coarse + 1, 0);
-    for(int c = 0; c < n_coarse; ++c) P.desc_ptr[c + 1] = P.desc_ptr[c] + (int)desc[c].size();
+    for(const auto& rg : l1_ranges)
+        for(int g1 = rg.first; g1 < rg.second; ++g1)
+            for(int k = 1; k < 8 && P.anc[8 * (size_t)g1 + k] >= 0; ++k) ++P.desc_ptr[P.anc[8 * (size_t)g1 + k] + 1];
+    for(int c = 0; c < n_coarse; ++c) P.desc_ptr[c + 1] += P.desc_ptr[c];
     P.desc_list.resize(P.desc_ptr[n_coarse]);
-    for(int c = 0; c < n_coarse; ++c) std::copy(desc[c].begin(), desc[c].end(), P.desc_list.begin() + P.desc_ptr[c]);
+    {
+        std::vector<int> at(P.desc_ptr.begin(), P.desc_ptr.end() - 1);
+        for(const auto& rg : l1_ranges)
+            for(int g1 = rg.first; g1 < rg.second; ++g1)
+                for(int k = 1; k < 8 && P.anc[8 * (size_t)g1 + k] >= 0; ++k)
+                    P.desc_list[at[P.anc[8 * (size_t)g1 + k]]++] = g1;
+    }
     // self-check: every level-1 node lists itself first and appears exactly
     // once among the descendants of each deeper level
     for(int s = 0; s < S; ++s)
@@ -615,21 +633,28 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys, bool structure_on_device)
     P.comp_mask.resize(P.comp_ptr[P.n_banks]);
     P.comp_parent.resize(P.comp_ptr[P.n_banks]);
     P.sys_level_ptr.assign(S + 1, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for(int s = 0; s < S; ++s)
+    {
+        const auto& d = sys[s];
+        if(!d.use_mas) continue;
+        const auto& h = d.hier;
+        for(int b = 0; b < h.total / bank; ++b)
+        {
+            const int gb = gbank_of_slot(s, b * bank);
+            for(int k = h.comp_ptr[b]; k < h.comp_ptr[b + 1]; ++k)
+            {
+                P.comp_mask[P.comp_ptr[gb] + k - h.comp_ptr[b]]   = h.comp_mask[k];
+                P.comp_parent[P.comp_ptr[gb] + k - h.comp_ptr[b]] = gcoarse(s, h.comp_parent[k]);
+            }
+        }
+    }
     for(int s = 0; s < S; ++s)
     {
         auto& d = sys[s];
         if(d.use_mas)
         {
             const auto& h = d.hier;
-            for(int b = 0; b < h.total / bank; ++b)
-            {
-                const int gb = gbank_of_slot(s, b * bank);
-                for(int k = h.comp_ptr[b]; k < h.comp_ptr[b + 1]; ++k)
-                {
-                    P.comp_mask[P.comp_ptr[gb] + k - h.comp_ptr[b]]   = h.comp_mask[k];
-                    P.comp_parent[P.comp_ptr[gb] + k - h.comp_ptr[b]] = gcoarse(s, h.comp_parent[k]);
-                }
-            }
             // coarse levels 1..L-1 as global coarse bank ranges
             for(int l = 1; l < h.levels; ++l)
             {

@@ -96,23 +96,25 @@ void Solver::build_hierarchies_on_device()
         if(!d.use_mas || d.n_fem <= 0 || d.hier_ready) continue;
         if(!hier_dev_) hier_dev_ = std::make_unique<DeviceHierarchy>();
         const int sys = (int)(&d - systems_.data());
-        if(hier_in_sys_ != sys || hier_in_version_ != d.topo_version)
+        // the mesh topology and partition stay on the device per system (a new
+        // contact set leaves them unchanged)
+        while((int)topo_.size() <= sys) topo_.push_back(std::make_unique<TopoCache>());
+        TopoCache& tc = *topo_[sys];
+        if(tc.version != d.topo_version)
         {
-            // the mesh topology and partition (unchanged by a new contact set)
-            d_h_start_.upload(d.nbr_start, st);
-            d_h_num_.upload(d.nbr_num, st);
-            d_h_list_.upload(d.nbr_list, st);
-            d_h_p2r_.upload(d.part_to_real, st);
-            d_h_r2p_.upload(d.real_to_part, st);
-            hier_in_sys_     = sys;
-            hier_in_version_ = d.topo_version;
+            tc.start.upload(d.nbr_start, st);
+            tc.num.upload(d.nbr_num, st);
+            tc.list.upload(d.nbr_list, st);
+            tc.p2r.upload(d.part_to_real, st);
+            tc.r2p.upload(d.real_to_part, st);
+            tc.version = d.topo_version;
         }
         d_h_pairs_.upload(d.pairs, st);
         const int n_slots = (int)d.part_to_real.size(), n_pairs = (int)d.pairs.size() / 4;
         try
         {
-            hier_dev_->build(d.n_fem, n_slots, d.bank, d.levels, d_h_start_.ptr, d_h_num_.ptr, d_h_list_.ptr,
-                             (long long)d.nbr_list.size(), d_h_p2r_.ptr, d_h_r2p_.ptr, n_pairs > 0 ? d_h_pairs_.ptr : nullptr,
+            hier_dev_->build(d.n_fem, n_slots, d.bank, d.levels, tc.start.ptr, tc.num.ptr, tc.list.ptr,
+                             (long long)d.nbr_list.size(), tc.p2r.ptr, tc.r2p.ptr, n_pairs > 0 ? d_h_pairs_.ptr : nullptr,
                              n_pairs, d.node_offset, st);
         }
         catch(const std::invalid_argument&)

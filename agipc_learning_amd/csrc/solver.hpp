@@ -175,8 +175,13 @@ class Solver
     // device MAS hierarchy build of setup(): inputs and the builder's scratch
     void build_hierarchies_on_device();
     int  hier_dev_sys_    = -1;     // system whose hierarchy hier_dev_ holds
-    int  hier_in_sys_     = -1;     // system whose topology the d_h_* inputs hold ...
-    long long hier_in_version_ = -1;  // ... at this topology version
+    struct TopoCache  // one system's mesh topology and partition on the device
+    {
+        DeviceArray<uint32_t> start, num, list;
+        DeviceArray<int>      p2r, r2p;
+        long long             version = -1;  // the system's topo_version they hold
+    };
+    std::vector<std::unique_ptr<TopoCache>> topo_;
     int  dev_struct_mode_ = 1;
     bool dev_struct_      = false;  // the current plan's structure arrays were built on the device
     bool device_structure_eligible() const;
@@ -190,8 +195,7 @@ class Solver
                            d_dp_kept_, d_dp_ts_, d_dp_tk_, d_dp_tq_, d_dp_eb_, d_dp_qb_, d_dp_sys_, d_dp_gn_;
     DeviceArray<uint64_t>  d_dp_keys_, d_dp_keys2_;
     DeviceArray<char>      d_dp_tmp_;
-    DeviceArray<uint32_t>  d_h_start_, d_h_num_, d_h_list_;
-    DeviceArray<int>       d_h_p2r_, d_h_r2p_, d_h_pairs_;
+    DeviceArray<int>       d_h_pairs_;
     std::unique_ptr<DeviceHierarchy> hier_dev_;
     int n_wg_pre_ = 0, n_wg_spmv_ = 0, n_wg_upd_ = 0;
     DevData dd_{};

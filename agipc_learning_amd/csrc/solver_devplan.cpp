@@ -97,24 +97,18 @@ void Solver::build_structure_on_device(hipStream_t st)
     }
 
     // ---- full block CSR: stable radix sort of (row, column) keys in block order
+    d_dp_rows_.alloc(std::max<long long>(nb, 1));
+    d_dp_cols_.alloc(std::max<long long>(nb, 1));
+    for(int s = 0; s < S; ++s)  // each system's structure at its block offset
     {
-        std::vector<int> all_rows, all_cols;
-        const std::vector<int>* rows = &systems_[0].rows;
-        const std::vector<int>* cols = &systems_[0].cols;
-        if(S > 1)
-        {
-            all_rows.reserve(nb);
-            all_cols.reserve(nb);
-            for(const SystemDesc& d : systems_)
-            {
-                all_rows.insert(all_rows.end(), d.rows.begin(), d.rows.end());
-                all_cols.insert(all_cols.end(), d.cols.begin(), d.cols.end());
-            }
-            rows = &all_rows;
-            cols = &all_cols;
-        }
-        d_dp_rows_.upload(*rows, st);
-        d_dp_cols_.upload(*cols, st);
+        const SystemDesc& d = systems_[s];
+        if(d.nb <= 0) continue;
+        check(hipMemcpyAsync(d_dp_rows_.ptr + P.src_off[s], d.rows.data(), sizeof(int) * d.nb, hipMemcpyHostToDevice,
+                             st),
+              "structure upload");
+        check(hipMemcpyAsync(d_dp_cols_.ptr + P.src_off[s], d.cols.data(), sizeof(int) * d.nb, hipMemcpyHostToDevice,
+                             st),
+              "structure upload");
     }
     const int       bits    = bits_for((long long)n_rows + 1);  // the sentinel row n_rows fits
     const long long n_items = 2 * nb;
