@@ -193,8 +193,9 @@ int masx_time_kernel(masx_solver* s, int which, int reps, double* ms);
  * TraditionalMAS32Preconditioner.cu:2189-2230, 2234-2525, which rebuild it on
  * the GPU every Newton step): on = 1 (the default) builds the block CSR, the
  * diagonal sources, the Galerkin maps, the bank entry lists and the SpMV tile
- * layout on the device when the solver holds one MAS system without affine
- * bodies (the hierarchy is always built there); 0 builds them on the host.
+ * layout on the device when every system of the solver is a MAS system
+ * without affine bodies (one system or a batch; the hierarchies are always
+ * built there); 0 builds them on the host.
  * Takes effect at the next masx_setup after a structure change.  The two give
  * the same arrays bit for bit.  *used: whether the current set-up built them
  * on the device. */
