@@ -54,6 +54,44 @@ __device__ __forceinline__ uint32_t half_bits(double v, __hip_bfloat16*)
     return (uint32_t)u;
 }
 
+// Loads of operands each iteration streams once (bank inverses, SpMV values
+// and columns): nontemporal, so they pass through the caches without
+// evicting the vectors that the gathers reuse.  On this card a read stream
+// of nontemporal 16-byte loads runs ~15% faster than one of ordinary loads.
+typedef float    nt_f4 __attribute__((ext_vector_type(4)));
+typedef uint32_t nt_u4 __attribute__((ext_vector_type(4)));
+typedef int      nt_i4 __attribute__((ext_vector_type(4)));
+typedef float    nt_f2 __attribute__((ext_vector_type(2)));
+typedef int      nt_i2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float    ld_nt(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ int      ld_nt(const int* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ uint32_t ld_nt(const uint32_t* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ float4 ld_nt(const float4* p)
+{
+    const nt_f4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint4 ld_nt(const uint4* p)
+{
+    const nt_u4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_u4*>(p));
+    return uint4{v.x, v.y, v.z, v.w};
+}
+__device__ __forceinline__ int4 ld_nt(const int4* p)
+{
+    const nt_i4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_i4*>(p));
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float2 ld_nt(const float2* p)
+{
+    const nt_f2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f2*>(p));
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ int2 ld_nt(const int2* p)
+{
+    const nt_i2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_i2*>(p));
+    return make_int2(v.x, v.y);
+}
+
 // Loads the nine values of block (I, I+d) of one bank (row-major within the block).
 template <int BANK>
 __device__ __forceinline__ void load_block(const float* M, int d, int I, float v[9])
@@ -819,7 +857,6 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4,
     constexpr int CHUNK = 1024;               // descendant entries staged per pass
     static_assert(BANK % NW == 0 && CHUNK % THREADS == 0, "diagonals and staged entries split evenly");
     __shared__ float sR[64 * 3 + 3];  // + a zero row for the lanes past a diagonal's end
-    __shared__ float sZ[NW - 1][64 * 3];
     constexpr int kM4    = NBLK * 9 / 4;                                   // float4 per FP32 bank
     constexpr int kStage = STAGED ? (BPW * kM4 + THREADS - 1) / THREADS : 1;  // float4 per thread
     static_assert(!STAGED || (NBLK * 9 % 4 == 0 && BPW * NBLK * 9 >= CHUNK * 3),
@@ -827,6 +864,11 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4,
     __shared__ __align__(16) float sM[STAGED ? BPW * NBLK * 9 : 4];
     __shared__ float sD0[STAGED ? 1 : CHUNK * 3];
     float* sD = STAGED ? sM : sD0;  // the descendant sums run before the inverses reach LDS
+    // the waves' partial results (NW - 1 rows of 64 x 3) reuse the staging
+    // area once the products are done (staged 32-slot banks: 38.8 KB of LDS
+    // per workgroup, so four workgroups share a CU)
+    float* sZ = STAGED ? sM : sD0;
+    static_assert((NW - 1) * 64 * 3 <= (STAGED ? BPW * NBLK * 9 : CHUNK * 3), "partials fit the staging area");
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int lb = lane / BANK, I = lane % BANK;
     const int bank0 = (int)(gridDim.x - 1 - blockIdx.x) * BPW;
@@ -865,7 +907,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4,
             const int i = tid + THREADS * u, k = i / kM4;
             stage[u] = make_float4(0.f, 0.f, 0.f, 0.f);
             if(i < BPW * kM4 && pick(s_live, k))
-                stage[u] = reinterpret_cast<const float4*>(d.minv)[(size_t)(bank0 + k) * kM4 + i % kM4];
+                stage[u] = ld_nt(reinterpret_cast<const float4*>(d.minv) + (size_t)(bank0 + k) * kM4 + i % kM4);
         }
     }
     else
@@ -1022,9 +1064,11 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4,
             z0 += t0; z1 += t1; z2 += t2;
         }
     }
+    if constexpr(STAGED) __syncthreads();  // every wave has read its blocks from the staged inverses
     if(wave > 0)
     {
-        sZ[wave - 1][3 * lane] = z0; sZ[wave - 1][3 * lane + 1] = z1; sZ[wave - 1][3 * lane + 2] = z2;
+        float* o = sZ + (wave - 1) * 64 * 3 + 3 * lane;
+        o[0] = z0; o[1] = z1; o[2] = z2;
     }
     __syncthreads();
     if(wave == 0 && live)
@@ -1032,7 +1076,8 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4,
 #pragma unroll
         for(int w = 0; w < NW - 1; ++w)
         {
-            z0 += sZ[w][3 * lane]; z1 += sZ[w][3 * lane + 1]; z2 += sZ[w][3 * lane + 2];
+            const float* o = sZ + w * 64 * 3 + 3 * lane;
+            z0 += o[0]; z1 += o[1]; z2 += o[2];
         }
         if(WORDS != 9)
         {
@@ -1150,9 +1195,9 @@ __global__ void __launch_bounds__(256) k_mas_local_mfma(DevData d, int force)
     {
         const uint4* M = reinterpret_cast<const uint4*>(d.minv) + (size_t)b * (kTileBankHalves / 8);
 #pragma unroll
-        for(int q = 0; q < kTilePairs; ++q) pr[q] = M[q * 64 + lane];
+        for(int q = 0; q < kTilePairs; ++q) pr[q] = ld_nt(M + q * 64 + lane);
 #pragma unroll
-        for(int q = 0; q < 3; ++q) sg[q] = M[(2 * kTilePairs + q) * 32 + (lane & 31)];
+        for(int q = 0; q < 3; ++q) sg[q] = ld_nt(M + (2 * kTilePairs + q) * 32 + (lane & 31));
     }
     else
     {
@@ -1554,16 +1599,27 @@ struct SpmvChunk
     float g[U][6];   // z and old p of a column outside the tile
 };
 
-// U consecutive floats (ints) from a run start aligned to min(U, 4) entries
-template <int U, typename T>
+// a load of streamed data: nontemporal (NT) or ordinary
+template <bool NT, typename V>
+__device__ __forceinline__ V ld_stream(const V* p)
+{
+    if constexpr(NT)
+        return ld_nt(p);
+    else
+        return *p;
+}
+
+// U consecutive floats (ints) of the streamed layout from a run start aligned
+// to min(U, 4) entries
+template <int U, bool NT, typename T>
 __device__ __forceinline__ void load_run(const T* __restrict__ p, T (&o)[U])
 {
     if constexpr(U == 1)
-        o[0] = p[0];
+        o[0] = ld_stream<NT>(p);
     else if constexpr(U == 2)
     {
         using V2 = std::conditional_t<std::is_same_v<T, int>, int2, float2>;
-        const V2 t = *reinterpret_cast<const V2*>(p);
+        const V2 t = ld_stream<NT>(reinterpret_cast<const V2*>(p));
         o[0] = t.x; o[1] = t.y;
     }
     else
@@ -1572,7 +1628,7 @@ __device__ __forceinline__ void load_run(const T* __restrict__ p, T (&o)[U])
 #pragma unroll
         for(int h = 0; h < U / 4; ++h)
         {
-            const V4 t = reinterpret_cast<const V4*>(p)[h];
+            const V4 t = ld_stream<NT>(reinterpret_cast<const V4*>(p) + h);
             o[4 * h] = t.x; o[4 * h + 1] = t.y; o[4 * h + 2] = t.z; o[4 * h + 3] = t.w;
         }
     }
@@ -1581,7 +1637,7 @@ __device__ __forceinline__ void load_run(const T* __restrict__ p, T (&o)[U])
 // This thread's run: entries [base + U t, base + U t + U) of the chunk (those
 // at or past n are read -- the layout's spare entries keep that in bounds --
 // and ignored)
-template <int U>
+template <int U, bool NT>
 __device__ __forceinline__ void spmv_load_chunk(const DevData& d, const float* __restrict__ p_old, int base, int n,
                                                 SpmvChunk<U>& c)
 {
@@ -1593,12 +1649,12 @@ __device__ __forceinline__ void spmv_load_chunk(const DevData& d, const float* _
         return;
     }
     const int e = base + k0;
-    load_run<U>(d.sy_col + e, c.cix);
+    load_run<U, NT>(d.sy_col + e, c.cix);
 #pragma unroll
     for(int q = 0; q < 9; ++q)
     {
         float t[U];
-        load_run<U>(d.vals32 + q * d.plane + e, t);
+        load_run<U, NT>(d.vals32 + q * d.plane + e, t);
 #pragma unroll
         for(int j = 0; j < U; ++j) c.v[j][q] = t[j];
     }
@@ -1698,7 +1754,9 @@ __device__ __forceinline__ void spmv_reduce_chunk(const SpmvChunk<U>& c, float b
 // tile of the SpMV layout: at most 256 rows, one thread per row; a tile of
 // several rows fits one chunk of 256 * U entries, a longer single row takes
 // several chunks (it has no transposed products).  Also the p.Ap partial.
-template <int U>
+// NT: the matrix is read with nontemporal loads (the choice for matrices far
+// larger than the caches).
+template <int U, bool NT>
 __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __restrict__ p_old,
                                                     float* __restrict__ p_new)
 {
@@ -1748,7 +1806,7 @@ __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __re
     }
     SpmvChunk<U> c;
     const int n0 = min(CH, e1 - e0);
-    if(n0 > 0) spmv_load_chunk<U>(d, p_old, e0, n0, c);
+    if(n0 > 0) spmv_load_chunk<U, NT>(d, p_old, e0, n0, c);
     const double rz = block_sum(share, smem);
     const bool bad = !finite_d(rz) || (st_rule == kRuleTrueRel && rz <= 0.0);
     if(wg == wg_range(d, d.wg_spmv_range, d.n_wg_spmv, s).x && threadIdx.x == 0)
@@ -1780,7 +1838,7 @@ __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __re
     for(int base = e0 + CH; base < e1; base += CH)
     {
         const int n = min(CH, e1 - base);
-        spmv_load_chunk<U>(d, p_old, base, n, c);
+        spmv_load_chunk<U, NT>(d, p_old, base, n, c);
         spmv_reduce_chunk<U>(c, beta, base, n, has_row, rb, re, 0, 0, sX, sP, sQ, acc);
     }
     if((w.y >> 28) == kWgAbd)  // uniform: a tile of affine bodies
@@ -1819,7 +1877,7 @@ __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __re
                     const int k = kb + G * u;
                     if(jr[u] < 0) continue;
                     const float4* V = reinterpret_cast<const float4*>(d.vals_abd + 144 * (long long)k + 12 * q);
-                    v[u][0] = V[0]; v[u][1] = V[1]; v[u][2] = V[2];
+                    v[u][0] = ld_stream<NT>(V); v[u][1] = ld_stream<NT>(V + 1); v[u][2] = ld_stream<NT>(V + 2);
                     if(jr[u] < r0 || jr[u] >= r1)
                     {
                         const float4* Zg = reinterpret_cast<const float4*>(d.z + 3 * (long long)jr[u]);
@@ -2445,13 +2503,15 @@ hipError_t launch_spmv_fused(const DevData& d, int n_wg, int entries_per_thread,
                              hipStream_t st, const LaunchEvents* ev)
 {
     if(n_wg <= 0) return hipSuccess;
+    void (*kern)(DevData, const float*, float*) = nullptr;
     switch(entries_per_thread)
     {
-        case 2: MASX_DISPATCH(k_spmv_fused<2>, dim3(n_wg), dim3(256), d, p_old, p_new); break;
-        case 4: MASX_DISPATCH(k_spmv_fused<4>, dim3(n_wg), dim3(256), d, p_old, p_new); break;
-        case 8: MASX_DISPATCH(k_spmv_fused<8>, dim3(n_wg), dim3(256), d, p_old, p_new); break;
+        case 2: kern = d.spmv_nt ? k_spmv_fused<2, true> : k_spmv_fused<2, false>; break;
+        case 4: kern = d.spmv_nt ? k_spmv_fused<4, true> : k_spmv_fused<4, false>; break;
+        case 8: kern = d.spmv_nt ? k_spmv_fused<8, true> : k_spmv_fused<8, false>; break;
         default: return hipErrorInvalidValue;
     }
+    MASX_DISPATCH(kern, dim3(n_wg), dim3(256), d, p_old, p_new);
     MASX_CHECK_LAUNCH();
     return hipSuccess;
 }

@@ -372,6 +372,22 @@ int masx_autotune(masx_solver* s, int reps, int* entries_per_thread, double* ms)
     });
 }
 
+int masx_set_spmv_loads(masx_solver* s, int mode)
+{
+    return guard([&] {
+        require(s != nullptr, "null solver");
+        s->impl.set_spmv_loads(mode);
+    });
+}
+
+int masx_spmv_loads_used(masx_solver* s, int* nontemporal)
+{
+    return guard([&] {
+        require(s != nullptr && nontemporal != nullptr, "null argument");
+        *nontemporal = s->impl.spmv_nontemporal() ? 1 : 0;
+    });
+}
+
 int masx_set_mas_waves(masx_solver* s, int waves)
 {
     return guard([&] {

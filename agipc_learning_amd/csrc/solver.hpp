@@ -99,8 +99,13 @@ class Solver
         int    entries;  // SpMV entries per thread per chunk
         double ms;
     };
-    // times every supported SpMV tiling on the built system and keeps the fastest
+    // times every supported SpMV tiling on the built system and keeps the
+    // fastest, then the other load policy on it
     Tiling autotune_spmv(int reps);
+    // SpMV matrix loads: 1 nontemporal, 2 ordinary, 0 automatic (nontemporal
+    // once the streamed matrix outgrows kSpmvCachedBytes)
+    void set_spmv_loads(int mode);
+    bool spmv_nontemporal() const { return effective_spmv_nt(); }
     // MAS local kernel: 4 or 8 waves per workgroup (VALU; each workgroup holds
     // 64 / bank banks, its waves split their block diagonals), 5 or 9 (the
     // same with 4 or 8 waves, the FP32 inverse staged through LDS with
@@ -147,6 +152,8 @@ class Solver
     int  entries_    = 4;
     int  mas_waves_  = 0;
     int  effective_mas_waves() const;
+    int  spmv_loads_ = 0;
+    bool effective_spmv_nt() const;
     bool built_      = false;
 
     // device

@@ -97,6 +97,8 @@ def load():
         "masx_morton_partition_device_noalloc": ([C.c_int, vp, C.c_int, vp, C.c_longlong, vp, vp], C.c_int),
         "masx_stream_rate": ([C.c_longlong, C.c_int, C.c_int, C.POINTER(C.c_double)], C.c_int),
         "masx_set_mas_waves": ([vp, C.c_int], C.c_int),
+        "masx_set_spmv_loads": ([vp, C.c_int], C.c_int),
+        "masx_spmv_loads_used": ([vp, ip], C.c_int),
         "masx_set_device_structure": ([vp, C.c_int], C.c_int),
         "masx_device_structure_used": ([vp, ip], C.c_int),
         "masx_plan_array": ([vp, C.c_int, ip, C.c_longlong, C.POINTER(C.c_longlong)], C.c_int),
@@ -424,13 +426,23 @@ class Solver:
         check(self.L.masx_set_spmv_tiling(self.h, entries_per_thread))
 
     def autotune(self, reps=50):
-        """Times every supported SpMV tiling and keeps the fastest."""
+        """Times every supported SpMV tiling and keeps the fastest, then the
+        other matrix load policy on it."""
         e, ms = C.c_int(), C.c_double()
         check(self.L.masx_autotune(self.h, reps, C.byref(e), C.byref(ms)))
-        return dict(entries_per_thread=e.value, ms=ms.value)
+        return dict(entries_per_thread=e.value, ms=ms.value, nontemporal=self.spmv_nontemporal())
 
     def set_mas_waves(self, waves):
         check(self.L.masx_set_mas_waves(self.h, waves))
+
+    def set_spmv_loads(self, mode):
+        """SpMV matrix loads: 1 nontemporal, 2 ordinary, 0 automatic."""
+        check(self.L.masx_set_spmv_loads(self.h, mode))
+
+    def spmv_nontemporal(self):
+        v = C.c_int()
+        check(self.L.masx_spmv_loads_used(self.h, C.byref(v)))
+        return bool(v.value)
 
     def set_device_structure(self, on):
         """1: a new structure of one MAS system without bodies is analyzed on

@@ -93,7 +93,8 @@ def compare(bench_json, trace_csv):
         us = {k: v["us"] for k, v in obj["roofline"]["per_kernel"].items()}
     tiling = obj["spmv_tiling"]["entries_per_thread"]
     waves = obj["spmv_tiling"].get("mas_local", {}).get("fp32", {}).get("waves", 4)
-    names = {"spmv": f"k_spmv_fused<{tiling}>", "mas_local": f"k_mas_local<32, float, {mas_local_variant(waves)}>",
+    nt = "true" if obj["spmv_tiling"].get("nontemporal") else "false"
+    names = {"spmv": f"k_spmv_fused<{tiling}, {nt}>", "mas_local": f"k_mas_local<32, float, {mas_local_variant(waves)}>",
              "precond_finish": "k_precond_finish", "update": "k_update"}
     d = collections.defaultdict(list)
     for r in csv.DictReader(open(trace_csv)):

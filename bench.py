@@ -99,16 +99,19 @@ KERNEL_PREFIX = {"spmv": "k_spmv_fused", "mas_local": "k_mas_local<32, float,", 
                  "precond_finish": "k_precond_finish"}
 
 
-def measured_traffic(kernel, workload_is_default, spmv_entries, summary="fem40c_hbm_traffic.json", waves=None):
+def measured_traffic(kernel, workload_is_default, spmv_entries, summary="fem40c_hbm_traffic.json", waves=None,
+                     spmv_nt=False):
     """HBM bytes per launch of `kernel` (the SpMV at `spmv_entries` per
-    thread, the FP32 MAS local kernel at `waves` waves per workgroup) from the
+    thread with nontemporal matrix loads or not, the FP32 MAS local kernel at
+    `waves` waves per workgroup) from the
     committed counter summary of this workload (FETCH_SIZE + WRITE_SIZE runs,
     copy-calibrated), or None."""
     path = os.path.join(ROOT, "profiles", summary)
     if not workload_is_default or not os.path.exists(path):
         return None
     data = json.load(open(path))["kernels"]
-    want = f"k_spmv_fused<{spmv_entries}>" if kernel == "spmv" else KERNEL_PREFIX[kernel]
+    want = (f"k_spmv_fused<{spmv_entries}, {'true' if spmv_nt else 'false'}>" if kernel == "spmv"
+            else KERNEL_PREFIX[kernel])
     if kernel == "mas_local" and waves:
         from agipc_learning_amd.prof_summary import mas_local_variant
         want = f"{KERNEL_PREFIX[kernel]} {mas_local_variant(waves)}>"
@@ -325,7 +328,8 @@ def batch_leg(args):
     s.build("fp32")
     if args.batch_tiling:
         s.set_spmv_tiling(args.batch_tiling)
-        tiling = {"entries_per_thread": args.batch_tiling, "ms": s.time_kernel(2, 20)}
+        tiling = {"entries_per_thread": args.batch_tiling, "ms": s.time_kernel(2, 20),
+                  "nontemporal": s.spmv_nontemporal()}
     else:
         log("step: batch64 autotune SpMV tiling")
         tiling = s.autotune(20)
@@ -394,7 +398,8 @@ def batch_leg(args):
                                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                                "traffic": measured_traffic(dom, True, tiling["entries_per_thread"],
                                                            "batch64_hbm_traffic.json",
-                                                           tiling["mas_local"]["fp32"]["waves"]),
+                                                           tiling["mas_local"]["fp32"]["waves"],
+                                                           tiling["nontemporal"]),
                                "algorithmic_bytes": alg[dom], "avg_us": kern[dom],
                                "per_kernel": {k: {"us": kern[k], "GB/s": out[prec]["kernels_GBs"][k],
                                                   "algorithmic_bytes": alg[k]} for k in kern}}
@@ -637,7 +642,8 @@ def fem40_leg(args, structure=True):
     s.build("fp32")
     if args.spmv_tiling:
         s.set_spmv_tiling(args.spmv_tiling)
-        tiling = {"entries_per_thread": args.spmv_tiling, "ms": s.time_kernel(2, 50)}
+        tiling = {"entries_per_thread": args.spmv_tiling, "ms": s.time_kernel(2, 50),
+                  "nontemporal": s.spmv_nontemporal()}
     else:
         log("step: autotune SpMV tiling")
         tiling = s.autotune(50)
@@ -682,7 +688,8 @@ def fem40_leg(args, structure=True):
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": measured_traffic(dominant, (args.n, args.contacts, args.levels) == (40, 20000, 4),
                                             tiling["entries_per_thread"],
-                                            waves=tiling["mas_local"]["fp32"]["waves"]),
+                                            waves=tiling["mas_local"]["fp32"]["waves"],
+                                            spmv_nt=tiling["nontemporal"]),
                 "algorithmic_bytes": alg[dominant], "avg_us": round(kern[dominant], 2),
                 "per_kernel": {k: {"us": round(kern[k], 2), "GB/s": round(alg[k] / (kern[k] * 1e-6) / 1e9, 1)}
                                for k in kern}}

@@ -164,9 +164,21 @@ int masx_apply_preconditioner(masx_solver* s, const double* d_r, double* d_z, vo
  * workgroup of its own. */
 int masx_set_spmv_tiling(masx_solver* s, int entries_per_thread);
 /* Times every supported SpMV tiling on the built system (`reps` launches
- * each) and keeps the fastest; reports it and its time per launch (ms).
- * Leaves the iteration state frozen: call masx_solve afterwards as usual. */
+ * each) and keeps the fastest, then times the other load policy
+ * (masx_set_spmv_loads) on it and keeps it if faster; reports the tiling and
+ * the time per launch (ms).  Leaves the iteration state frozen: call
+ * masx_solve afterwards as usual. */
 int masx_autotune(masx_solver* s, int reps, int* entries_per_thread, double* ms);
+/* How the SpMV reads the matrix (value planes, column words, body
+ * couplings), which it streams once per iteration: mode 1 nontemporal loads
+ * (they pass the caches without evicting the vectors), 2 ordinary loads (a
+ * matrix the 256 MB Infinity Cache holds is then re-read from it in the next
+ * iteration), 0 (the default) nontemporal once the matrix exceeds 256 MiB.
+ * Results are identical under every mode.  No counterpart in the reference
+ * (its SpMV uses ordinary loads). */
+int masx_set_spmv_loads(masx_solver* s, int mode);
+/* *nontemporal: whether the SpMV currently uses nontemporal loads. */
+int masx_spmv_loads_used(masx_solver* s, int* nontemporal);
 /* MAS local solve variant.  waves = 4 or 8: the VALU kernel, that many
  * waves per workgroup sharing the block diagonals of its banks; 5 or 9: the
  * same with 4 or 8 waves, the workgroup's FP32 inverses staged through LDS
@@ -212,8 +224,10 @@ int masx_plan_array(masx_solver* s, int which, int* out, long long capacity, lon
 /* Practical streaming rate of the current GPU (no counterpart in the
  * reference; a measurement aid): the fastest of `reps` launches of a plain
  * kernel over `bytes` of device memory (>= 1 MiB, a multiple of 16), kind 0 =
- * read-only stream (bytes read), kind 1 = copy (bytes read + written), in
- * GB/s.  Allocates and frees its buffers. */
+ * read-only stream (bytes read; nontemporal 16-byte loads, a contiguous span
+ * per workgroup), kind 1 = copy (bytes read + written; a grid-stride loop of
+ * ordinary 16-byte loads and stores), in GB/s.  Allocates and frees its
+ * buffers. */
 int masx_stream_rate(long long bytes, int kind, int reps, double* gb_per_s);
 
 /* Introspection (tests, benchmark). */

@@ -377,6 +377,35 @@ def test_spmv_tilings_agree_and_autotune():
     assert st["failure"] == 0 and abs(st["iterations"] - ref[0]) <= 1
 
 
+@pytest.mark.parametrize("kind", ["fem", "hybrid"])
+def test_spmv_load_policies_identical(kind):
+    """Nontemporal and ordinary SpMV matrix loads (masx_set_spmv_loads) give
+    bit-identical solves at every tiling, FEM rows and affine-body couplings
+    alike; the automatic policy keeps a matrix this small on ordinary loads."""
+    w = (W.fem_grid(10, n_contacts=600, seed=5) if kind == "fem"
+         else W.hybrid(seed=4, n_bodies=60, n_body_pairs=400, fem_n=7, n_coupled=12))
+    s = session.solver_for(w)
+    s.build("fp32")
+    assert not s.spmv_nontemporal()
+    b = dev(session.rhs_of(w))
+    for t in TILINGS:
+        s.set_spmv_tiling(t)
+        out = {}
+        for mode in (1, 2):
+            s.set_spmv_loads(mode)
+            assert s.spmv_nontemporal() == (mode == 1)
+            x = torch.zeros_like(b)
+            st = s.solve(b, x, rule=1, tol=1e-6)[0]
+            assert st["failure"] == 0 and st["rel_res_true"] <= 1e-6 * (1 + 1e-6), (t, mode, st)
+            out[mode] = (st["iterations"], host(x))
+        assert out[1][0] == out[2][0], t
+        assert np.array_equal(out[1][1], out[2][1]), t
+    s.set_spmv_loads(0)
+    assert not s.spmv_nontemporal()
+    with pytest.raises(RuntimeError):
+        s.set_spmv_loads(3)
+
+
 @pytest.mark.parametrize("bank", [32, 16])
 def test_mas_waves_agree_and_autotune(oracle, bank):
     """The FP32 MAS local kernel with 4 or 8 waves per workgroup, its inverse
