@@ -424,7 +424,7 @@ __host__ __device__ constexpr int tile8x16(int rr, int r, int c) { return c * rr
 template <int BANK, typename T>
 __global__ void __launch_bounds__(128) k_mas_factor_sym(DevData d, const double* E, const int* ent_ptr,
                                                         const int* ent_pos, const int* ent_src, void* out,
-                                                        float* scale_out, int* status)
+                                                        float* scale_out, int* status, int bank_begin)
 {
     constexpr int DIM   = 3 * BANK;
     constexpr int RR    = DIM / 8;    // tile rows (8 rows each)
@@ -441,7 +441,7 @@ __global__ void __launch_bounds__(128) k_mas_factor_sym(DevData d, const double*
     double* H  = S;
     double* vk = S;              // during the sweep: two pivot columns ...
     double* pv = S + 2 * DIM;    // ... the pivots' reciprocals and floored flags
-    const int bank = blockIdx.x, tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const int bank = bank_begin + (int)blockIdx.x, tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
     double a[NT];
 #pragma unroll
     for(int t = 0; t < NT; ++t) a[t] = 0.0;
@@ -595,6 +595,189 @@ __global__ void __launch_bounds__(128) k_mas_factor_sym(DevData d, const double*
         {
             const int i = ty + 8 * r, j = tx + 16 * c;
             if(i >= j) store_inverse_element<BANK, T>(out, bank, i, j, -a[tile8x16(RR, r, c)], inv_scale, tiles);
+        }
+    if(tid == 0 && bad && status) atomicAdd(status, bad);
+}
+
+// The same sweep by one wave per bank: lane (ty, tx) of an 8 x 8 grid keeps
+// elements (ty + 8 r, tx + 8 c) of the 8 x 8 tiles r >= c (78 for 32-slot
+// banks) in registers.  Each pivot's exchange of the pivot column stays
+// within the wave (no barrier between waves on the pivot chain), and two
+// banks share a SIMD.
+__host__ __device__ constexpr int tile8(int r, int c) { return r * (r + 1) / 2 + c; }
+
+// value of v in lane `lane` (uniform over the wave)
+__device__ __forceinline__ double readlane_f64(double v, int lane)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+template <int BANK, typename T>
+__global__ void __launch_bounds__(64) k_mas_factor_wave(DevData d, const double* E, const int* ent_ptr,
+                                                        const int* ent_pos, const int* ent_src, void* out,
+                                                        float* scale_out, int* status, int bank_begin)
+{
+    constexpr int DIM   = 3 * BANK;
+    constexpr int NB    = DIM / 8;              // tile rows = tile columns
+    constexpr int NT    = NB * (NB + 1) / 2;    // lower tiles
+    constexpr int NPASS = DIM > 48 ? 2 : 1;
+    constexpr int NH    = factor_pass_max(DIM, NPASS);
+    constexpr int NS    = NH > 2 * DIM ? NH : 2 * DIM;
+    constexpr bool kHalf = !std::is_same<T, float>::value;
+    const bool tiles = kTileCapable<BANK, T> && d.mas_waves == 1;  // the MFMA local kernel's layout
+    static_assert(DIM % 8 == 0, "the 8 x 8 grid must tile the bank matrix");
+    __shared__ double S[NS];
+    double* H  = S;
+    double* vk = S;              // during the sweep: two pivot columns
+    const int bank = bank_begin + (int)blockIdx.x, tid = threadIdx.x, tx = tid & 7, ty = tid >> 3;
+    double a[NT];
+#pragma unroll
+    for(int t = 0; t < NT; ++t) a[t] = 0.0;
+    // ---- assembly, NPASS row ranges of the lower triangle through H
+    const bool fine = bank < d.n_fine_banks;
+    const int  e_begin = ent_ptr[bank], e_end = ent_ptr[bank + 1];
+    double dmax = 0.0;
+#pragma unroll
+    for(int pass = 0; pass < NPASS; ++pass)
+    {
+        const int lo = factor_pass_row(DIM, NPASS, pass), hi = factor_pass_row(DIM, NPASS, pass + 1);
+        const int base = lo * (lo + 1) / 2, n = hi * (hi + 1) / 2 - base;
+        for(int q = tid; q < n; q += 64) H[q] = 0.0;
+        __syncthreads();
+        for(int k = e_begin + tid; k < e_end; k += 64)
+        {
+            const int pos = ent_pos[k] & 0xffff, tr = ent_pos[k] >> 16;
+            const int src = ent_src[k];
+            int I = 0;
+            while(I + 1 < BANK && packed_index(BANK, I + 1, I + 1) <= pos) ++I;
+            const int J = pos - packed_index(BANK, I, I) + I;
+            for(int aa = 0; aa < 3; ++aa)
+                for(int bb = 0; bb < 3; ++bb)
+                {
+                    const int p = 3 * I + aa, q = 3 * J + bb;
+                    if(I == J && p < q) continue;  // diagonal block: its lower half carries it
+                    const int row = p > q ? p : q, col = p > q ? q : p;
+                    if(row < lo || row >= hi) continue;
+                    double v;
+                    if(fine)
+                        v = tr ? d.vals64[9 * (long long)src + bb + 3 * aa] : d.vals64[9 * (long long)src + aa + 3 * bb];
+                    else
+                        v = E[9 * (long long)src + 3 * aa + bb];
+                    atomicAdd(&H[row * (row + 1) / 2 + col - base], v);
+                }
+        }
+        __syncthreads();
+        // zero diagonal -> 1 (padding slots behave as identity)
+        for(int q = lo + tid; q < hi; q += 64)
+        {
+            double& h = H[q * (q + 1) / 2 + q - base];
+            if(h == 0.0) h = 1.0;
+            dmax = fmax(dmax, fabs(h));
+        }
+        __syncthreads();
+#pragma unroll
+        for(int r = 0; r < NB; ++r)
+#pragma unroll
+            for(int c = 0; c <= r; ++c)
+            {
+                const int i = ty + 8 * r, j = tx + 8 * c;
+                if(i >= lo && i < hi && i >= j) a[tile8(r, c)] = H[i * (i + 1) / 2 + j - base];
+            }
+        __syncthreads();  // H becomes the pivot-column buffer
+    }
+    // pivot floor max(1e-12, 1e-12 max |diag|)
+#pragma unroll
+    for(int o = 32; o > 0; o >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, o, 64));
+    const double floor_value = fmax(1.0e-12, 1.0e-12 * dmax);
+    int bad = 0;
+    // ---- sweep: pivot k = 8 rk + k8 (tile row and column rk).  Each pivot's
+    // value is read from its lane (readlane) as soon as the previous pivot's
+    // update has formed it, and every lane takes its reciprocal: the division
+    // overlaps the exchange of the pivot column instead of preceding it.
+    double pk = readlane_f64(a[tile8(0, 0)], 0);
+#pragma unroll
+    for(int rk = 0; rk < NB; ++rk)
+    {
+#pragma unroll 1
+        for(int k8 = 0; k8 < 8; ++k8)
+        {
+            const int k = k8 + 8 * rk, buf = k & 1;
+            double* col = vk + buf * DIM;
+            const bool floored = !(pk >= floor_value) || isinf(pk);  // also catches NaN
+            const double ip = 1.0 / (floored ? floor_value : pk);
+            bad += floored ? 1 : 0;
+            // column k from row k down (lower elements (i, k), i >= k) ...
+            if(tx == k8)
+            {
+#pragma unroll
+                for(int r = rk; r < NB; ++r)
+                    if(r > rk || ty >= k8) col[ty + 8 * r] = a[tile8(r, rk)];
+            }
+            // ... and left of the pivot (lower elements (k, j), j < k)
+            if(ty == k8)
+            {
+#pragma unroll
+                for(int c = 0; c <= rk; ++c)
+                    if(c < rk || tx < k8) col[tx + 8 * c] = a[tile8(rk, c)];
+            }
+            __syncthreads();
+            double g[NB];
+#pragma unroll
+            for(int c = 0; c < NB; ++c) g[c] = col[tx + 8 * c] * ip;
+#pragma unroll
+            for(int r = 0; r < NB; ++r)
+            {
+                const double f = col[ty + 8 * r];
+#pragma unroll
+                for(int c = 0; c <= r; ++c) a[tile8(r, c)] -= f * g[c];
+            }
+            // column k below the pivot and row k left of it take v / d, the pivot -1 / d
+            if(tx == k8)
+            {
+#pragma unroll
+                for(int r = rk; r < NB; ++r)
+                    if(r > rk || ty > k8) a[tile8(r, rk)] = col[ty + 8 * r] * ip;
+            }
+            if(ty == k8)
+            {
+#pragma unroll
+                for(int c = 0; c <= rk; ++c)
+                    if(c < rk || tx < k8) a[tile8(rk, c)] = g[c];
+                if(tx == k8) a[tile8(rk, rk)] = -ip;
+            }
+            // the next pivot (k + 1, k + 1), untouched by this pivot's row and column
+            if(k8 < 7)
+                pk = readlane_f64(a[tile8(rk, rk)], 9 * (k8 + 1));
+            else if(rk + 1 < NB)
+                pk = readlane_f64(a[tile8(rk + 1 < NB ? rk + 1 : rk, rk + 1 < NB ? rk + 1 : rk)], 0);
+        }
+    }
+    // ---- the inverse (-a) from the registers
+    double scale = 1.0;
+    if constexpr(kHalf)
+    {
+        double m = 0.0;
+#pragma unroll
+        for(int r = 0; r < NB; ++r)
+#pragma unroll
+            for(int c = 0; c <= r; ++c)
+                if(ty + 8 * r >= tx + 8 * c) m = fmax(m, fabs(a[tile8(r, c)]));
+#pragma unroll
+        for(int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+        scale = m > 0.0 ? m : 1.0;
+    }
+    if(tid == 0) scale_out[bank] = (float)scale;
+    const double inv_scale = 1.0 / scale;
+#pragma unroll
+    for(int r = 0; r < NB; ++r)
+#pragma unroll
+        for(int c = 0; c <= r; ++c)
+        {
+            const int i = ty + 8 * r, j = tx + 8 * c;
+            if(i >= j) store_inverse_element<BANK, T>(out, bank, i, j, -a[tile8(r, c)], inv_scale, tiles);
         }
     if(tid == 0 && bad && status) atomicAdd(status, bad);
 }
@@ -2344,13 +2527,23 @@ size_t mas_bank_bytes(int bank, int precision)
 }
 
 hipError_t launch_mas_factor(const DevData& d, const double* E, const int* ent_ptr, const int* ent_pos,
-                             const int* ent_src, void* out, float* scale, int precision, int* status, hipStream_t st)
+                             const int* ent_src, void* out, float* scale, int precision, int* status, int bank_begin,
+                             int bank_end, hipStream_t st)
 {
-    if(d.n_banks <= 0) return hipSuccess;
-    const dim3 g(d.n_banks);
-#define MASX_FACTOR(BK, T)                                                                                       \
-    hipLaunchKernelGGL((k_mas_factor_sym<BK, T>), g, dim3(128), 0, st, d, E, ent_ptr, ent_pos, ent_src, out, scale, \
-                       status)
+    if(bank_begin < 0 || bank_end > d.n_banks || bank_begin > bank_end) return hipErrorInvalidValue;
+    if(bank_end == bank_begin) return hipSuccess;
+    const dim3 g(bank_end - bank_begin);
+    const bool wave = d.factor_waves == 1;  // one wave per bank, else two
+#define MASX_FACTOR(BK, T)                                                                                          \
+    do                                                                                                             \
+    {                                                                                                              \
+        if(wave)                                                                                                   \
+            hipLaunchKernelGGL((k_mas_factor_wave<BK, T>), g, dim3(64), 0, st, d, E, ent_ptr, ent_pos, ent_src,    \
+                               out, scale, status, bank_begin);                                                    \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_mas_factor_sym<BK, T>), g, dim3(128), 0, st, d, E, ent_ptr, ent_pos, ent_src,    \
+                               out, scale, status, bank_begin);                                                    \
+    } while(0)
     if(d.bank == 32)
     {
         if(precision == kPrecFp16) MASX_FACTOR(32, __half);

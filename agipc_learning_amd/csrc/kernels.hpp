@@ -14,8 +14,10 @@ hipError_t launch_pack_abd(const DevData& d, const int* cp_body, const int* cp_s
 // lanes per coarse entry: 16 or 64
 hipError_t launch_gal_gather(const double* fine, double* E, const int* ptr, const int* src, int e0, int e1,
                              int src_is_fine, int lanes, hipStream_t st);
+// banks [bank_begin, bank_end) of the bank list
 hipError_t launch_mas_factor(const DevData& d, const double* E, const int* ent_ptr, const int* ent_pos,
-                             const int* ent_src, void* out, float* scale, int precision, int* status, hipStream_t st);
+                             const int* ent_src, void* out, float* scale, int precision, int* status, int bank_begin,
+                             int bank_end, hipStream_t st);
 hipError_t launch_abd_factor(const DevData& d, const int* body_src, float* inv, int n_bodies, hipStream_t st);
 hipError_t launch_diag_factor(const DevData& d, const int* diag_src, const int* needs, float* inv, hipStream_t st);
 hipError_t launch_load_rhs(const DevData& d, const int* e2i, const double* b, int n_ext, hipStream_t st);

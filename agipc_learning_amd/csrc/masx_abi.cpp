@@ -372,6 +372,14 @@ int masx_autotune(masx_solver* s, int reps, int* entries_per_thread, double* ms)
     });
 }
 
+int masx_set_factor_waves(masx_solver* s, int waves)
+{
+    return guard([&] {
+        require(s != nullptr, "null solver");
+        s->impl.set_factor_waves(waves);
+    });
+}
+
 int masx_set_spmv_loads(masx_solver* s, int mode)
 {
     return guard([&] {

@@ -61,8 +61,11 @@ void Solver::ensure_stream()
 {
     if(work_) return;
     MASX_HIP(hipStreamCreateWithFlags(&work_, hipStreamNonBlocking));
+    MASX_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
     MASX_HIP(hipEventCreate(&ev_a_));
     MASX_HIP(hipEventCreate(&ev_b_));
+    MASX_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+    MASX_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
     for(auto& e : chunk_ev_) MASX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
 
@@ -177,9 +180,12 @@ Solver::~Solver()
     if(h_dp_) (void)hipHostFree(h_dp_);
     if(ev_a_) (void)hipEventDestroy(ev_a_);
     if(ev_b_) (void)hipEventDestroy(ev_b_);
+    if(ev_fork_) (void)hipEventDestroy(ev_fork_);
+    if(ev_join_) (void)hipEventDestroy(ev_join_);
     for(auto e : timing_ev_) (void)hipEventDestroy(e);
     for(auto e : chunk_ev_)
         if(e) (void)hipEventDestroy(e);
+    if(side_) (void)hipStreamDestroy(side_);
     if(work_) (void)hipStreamDestroy(work_);
 }
 
@@ -600,6 +606,7 @@ void Solver::make_devdata()
     d.nnz = P.nnz;
     d.nnz_s = spmv_layout_.total;
     d.spmv_nt = effective_spmv_nt() ? 1 : 0;
+    d.factor_waves = effective_factor_waves();
     d.plane = std::max<long long>(64, (d.nnz_s + 63) / 64 * 64);
     d.row_ptr = d_row_ptr_.ptr;
     d.col = d_col_.ptr;
@@ -698,6 +705,31 @@ size_t Solver::inverse_bytes() const
     return (size_t)plan_.n_banks * mas_bank_bytes(plan_.bank, precision_);
 }
 
+// From this many banks the MAS factor runs one wave per bank: more banks
+// per SIMD and no barrier between waves on the pivot chain (measured on
+// 91,066 banks: 4.3 against 5.5 ms); on a few thousand banks the build waits
+// on one bank's latency, which two waves per bank shorten (2,270 banks: 0.16
+// against 0.19 ms).
+constexpr int kFactorWaveBanks = 16384;
+
+int Solver::effective_factor_waves() const
+{
+    if(factor_waves_ == 1 || factor_waves_ == 2) return factor_waves_;
+    return plan_.n_banks >= kFactorWaveBanks ? 1 : 2;
+}
+
+void Solver::set_factor_waves(int waves)
+{
+    if(waves < 0 || waves > 2) throw std::invalid_argument("MAS factor waves per bank must be 0 (automatic), 1 or 2");
+    factor_waves_    = waves;
+    dd_.factor_waves = effective_factor_waves();
+}
+
+// From this many banks build() overlaps its memory-bound kernels with the
+// MAS factor (a factor of milliseconds); measured: 91,066 banks gain, 2,270
+// banks lose the ~50 us the extra launch and the stream waits cost.
+constexpr int kOverlapBanks = 16384;
+
 void Solver::build(int precision, hipStream_t user)
 {
     if(!setup_done_) throw std::runtime_error("setup() must precede build()");
@@ -705,7 +737,8 @@ void Solver::build(int precision, hipStream_t user)
     hipStream_t st = enter(user);
     const bool prec_changed = precision != precision_;
     precision_ = precision;
-    dd_.mas_waves = effective_mas_waves();  // the factor writes the layout the local kernel reads
+    dd_.mas_waves    = effective_mas_waves();  // the factor writes the layout the local kernel reads
+    dd_.factor_waves = effective_factor_waves();
     const char* old_minv = d_minv_.ptr;
     d_minv_.alloc(std::max<size_t>(inverse_bytes(), 16));  // reuses the buffer when it fits
     if(inverse_bytes() > d_minv_.cap) throw std::logic_error("inverse buffer smaller than the banks need");
@@ -717,9 +750,21 @@ void Solver::build(int precision, hipStream_t user)
         graph_exec_ = nullptr;
     }
     const Plan& P = plan_;
-    MASX_HIP(launch_pack_values(dd_, st));
-    MASX_HIP(launch_pack_abd(dd_, d_cp_body_.ptr, d_cp_sub_.ptr, (long long)spmv_layout_.cp_col.size(),
-                             d_vals_abd_.ptr, st));
+    // On large systems the MAS factor of the fine banks (compute-bound,
+    // milliseconds) runs on the work stream while the side stream forms the
+    // coarse levels' Galerkin sums and factors the coarse banks, then packs
+    // the SpMV value planes and factors the block-Jacobi blocks (which only
+    // the solve needs): the side stream's kernels fill the CUs the factor's
+    // register-limited waves leave free.  On small systems the cross-stream
+    // waits would cost more than they hide: one stream, in that order.
+    const bool overlap = P.n_banks >= kOverlapBanks;
+    hipStream_t aux = overlap ? side_ : st;
+    MASX_HIP(hipMemsetAsync(d_factor_status_.ptr, 0, sizeof(int), st));
+    if(overlap)
+    {
+        MASX_HIP(hipEventRecord(ev_fork_, st));
+        MASX_HIP(hipStreamWaitEvent(side_, ev_fork_, 0));
+    }
     for(size_t g = 0; g + 1 < P.gal_level_ptr.size(); ++g)
     {
         // a whole wave per entry where the lists are long (the top levels)
@@ -727,13 +772,21 @@ void Solver::build(int precision, hipStream_t user)
         const long long srcs = (long long)P.gal_level_src[g + 1] - P.gal_level_src[g];
         const int lanes = srcs > 32ll * (e1 - e0) ? 64 : 16;
         MASX_HIP(launch_gal_gather(d_vals64_.ptr, d_E_.ptr, d_gal_ptr_.ptr, d_gal_src_.ptr, e0, e1,
-                                   P.gal_src_is_fine[g], lanes, st));
+                                   P.gal_src_is_fine[g], lanes, aux));
     }
-    MASX_HIP(hipMemsetAsync(d_factor_status_.ptr, 0, sizeof(int), st));
+    const int split = overlap ? P.n_fine_banks : P.n_banks;
+    if(overlap)
+        MASX_HIP(launch_mas_factor(dd_, d_E_.ptr, d_ent_ptr_.ptr, d_ent_pos_.ptr, d_ent_src_.ptr, d_minv_.ptr,
+                                   d_minv_scale_.ptr, precision_, d_factor_status_.ptr, split, P.n_banks, side_));
+    MASX_HIP(launch_pack_values(dd_, aux));
+    MASX_HIP(launch_pack_abd(dd_, d_cp_body_.ptr, d_cp_sub_.ptr, (long long)spmv_layout_.cp_col.size(),
+                             d_vals_abd_.ptr, aux));
+    MASX_HIP(launch_abd_factor(dd_, d_body_src_.ptr, d_abd_inv_.ptr, P.n_bodies, aux));
+    MASX_HIP(launch_diag_factor(dd_, d_diag_src_.ptr, d_needs_diag_.ptr, d_diag_inv_.ptr, aux));
+    if(overlap) MASX_HIP(hipEventRecord(ev_join_, side_));
     MASX_HIP(launch_mas_factor(dd_, d_E_.ptr, d_ent_ptr_.ptr, d_ent_pos_.ptr, d_ent_src_.ptr, d_minv_.ptr,
-                               d_minv_scale_.ptr, precision_, d_factor_status_.ptr, st));
-    MASX_HIP(launch_abd_factor(dd_, d_body_src_.ptr, d_abd_inv_.ptr, P.n_bodies, st));
-    MASX_HIP(launch_diag_factor(dd_, d_diag_src_.ptr, d_needs_diag_.ptr, d_diag_inv_.ptr, st));
+                               d_minv_scale_.ptr, precision_, d_factor_status_.ptr, 0, split, st));
+    if(overlap) MASX_HIP(hipStreamWaitEvent(st, ev_join_, 0));
     leave(user);
     built_ = true;
 }

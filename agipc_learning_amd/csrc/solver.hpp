@@ -106,6 +106,10 @@ class Solver
     // once the streamed matrix outgrows kSpmvCachedBytes)
     void set_spmv_loads(int mode);
     bool spmv_nontemporal() const { return effective_spmv_nt(); }
+    // MAS factor: 1 or 2 waves per bank, 0 automatic (1 from kFactorWaveBanks
+    // banks on); takes effect at the next build()
+    void set_factor_waves(int waves);
+    int  factor_waves() const { return effective_factor_waves(); }
     // MAS local kernel: 4 or 8 waves per workgroup (VALU; each workgroup holds
     // 64 / bank banks, its waves split their block diagonals), 5 or 9 (the
     // same with 4 or 8 waves, the FP32 inverse staged through LDS with
@@ -154,6 +158,8 @@ class Solver
     int  effective_mas_waves() const;
     int  spmv_loads_ = 0;
     bool effective_spmv_nt() const;
+    int  factor_waves_ = 0;
+    int  effective_factor_waves() const;
     bool built_      = false;
 
     // device
@@ -209,6 +215,11 @@ class Solver
 
     hipStream_t work_ = nullptr;
     hipEvent_t  ev_a_ = nullptr, ev_b_ = nullptr;
+    // build(): the Galerkin sums, the coarse banks' factor, the SpMV value
+    // packing and the block-Jacobi factors run on side_ (default priority)
+    // beside the fine banks' MAS factor
+    hipStream_t side_ = nullptr;
+    hipEvent_t  ev_fork_ = nullptr, ev_join_ = nullptr;
     std::vector<hipEvent_t> timing_ev_;    // time_kernel: one start/stop pair per launch
     static constexpr int kDepth = 2;       // graph chunks in flight during a solve
     hipEvent_t  chunk_ev_[kDepth] = {};

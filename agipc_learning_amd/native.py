@@ -98,6 +98,7 @@ def load():
         "masx_stream_rate": ([C.c_longlong, C.c_int, C.c_int, C.POINTER(C.c_double)], C.c_int),
         "masx_set_mas_waves": ([vp, C.c_int], C.c_int),
         "masx_set_spmv_loads": ([vp, C.c_int], C.c_int),
+        "masx_set_factor_waves": ([vp, C.c_int], C.c_int),
         "masx_spmv_loads_used": ([vp, ip], C.c_int),
         "masx_set_device_structure": ([vp, C.c_int], C.c_int),
         "masx_device_structure_used": ([vp, ip], C.c_int),
@@ -434,6 +435,10 @@ class Solver:
 
     def set_mas_waves(self, waves):
         check(self.L.masx_set_mas_waves(self.h, waves))
+
+    def set_factor_waves(self, waves):
+        """MAS factor: 1 or 2 waves per bank, 0 automatic (next build)."""
+        check(self.L.masx_set_factor_waves(self.h, waves))
 
     def set_spmv_loads(self, mode):
         """SpMV matrix loads: 1 nontemporal, 2 ordinary, 0 automatic."""

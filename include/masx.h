@@ -189,6 +189,14 @@ int masx_spmv_loads_used(masx_solver* s, int* nontemporal);
  * different layouts: a change between them needs a new masx_build before the
  * next solve (masx_solve reports the missing build). */
 int masx_set_mas_waves(masx_solver* s, int waves);
+/* MAS factor (the dense assembly and FP64 inverse of every bank at
+ * masx_build): waves = 1, one wave per bank (more banks per SIMD; the faster
+ * on many banks), 2, two waves per bank (a bank's inverse sooner; the faster
+ * on a few thousand), 0 (the default): 1 from 16,384 banks, else 2.  Takes
+ * effect at the next masx_build; the inverses agree to FP64 rounding.  No
+ * counterpart in the reference (__inverse6_P96x96,
+ * TraditionalMAS32Preconditioner.cu:1099-1147, has one form). */
+int masx_set_factor_waves(masx_solver* s, int waves);
 /* Times every MAS local variant of the built precision (`reps` launches
  * each; for half-precision 32-slot banks 1, 4 and 8, else 4 and 8), keeps the
  * fastest (rebuilding the preconditioner for a layout change) and reports it

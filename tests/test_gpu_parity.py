@@ -144,10 +144,14 @@ def test_set_structure_on_the_same_solver_equals_a_fresh_solver():
 
 # ------------------------------------------------------------------- MAS ----
 
+@pytest.mark.parametrize("factor_waves", [1, 2])
 @pytest.mark.parametrize("n,contacts", [(8, 0), (14, 400)])
-def test_mas_inverse_and_apply_parity(oracle, n, contacts):
+def test_mas_inverse_and_apply_parity(oracle, n, contacts, factor_waves):
+    """Both MAS factor kernels (one or two waves per bank) invert every bank
+    to the oracle's inverse."""
     w = W.fem_grid(n, n_contacts=contacts, seed=n, young_sigma=0.5)
     s = session.solver_for(w)
+    s.set_factor_waves(factor_waves)
     s.build("fp32")
     assert s.factor_status() == 0
     mas = oracle_mas(oracle, w)
@@ -163,9 +167,11 @@ def test_mas_inverse_and_apply_parity(oracle, n, contacts):
     assert np.linalg.norm(got - want) <= 1e-4 * np.linalg.norm(want)
 
 
-def test_mas_bank16_apply_parity(oracle):
+@pytest.mark.parametrize("factor_waves", [1, 2])
+def test_mas_bank16_apply_parity(oracle, factor_waves):
     w = W.fem_grid(9, n_contacts=50, seed=2)
     s = session.solver_for(w, bank=16)
+    s.set_factor_waves(factor_waves)
     s.build("fp32")
     mas = oracle_mas(oracle, w, bank=16)
     r = np.random.default_rng(1).standard_normal(3 * w["block_rows"])
@@ -276,14 +282,19 @@ def half_ulp(y, precision):
     return 2.0 ** (np.floor(np.log2(np.maximum(a, 2.0 ** -126))) - 7)
 
 
+@pytest.mark.parametrize("factor_waves,local", [(2, 0), (1, 0), (1, 1)])
 @pytest.mark.parametrize("precision", ["fp16", "bf16"])
-def test_half_inverse_and_apply_match_the_oracle(oracle, precision):
+def test_half_inverse_and_apply_match_the_oracle(oracle, precision, factor_waves, local):
     """Each bank's inverse is stored as half(v / s) with s its largest entry in
     magnitude: every stored value lies within one half-precision ulp (at the
     bank scale) of the oracle's inverse, and the half apply (FP32
-    accumulation) matches an FP64 apply of that stored inverse to 1e-4."""
+    accumulation) matches an FP64 apply of that stored inverse to 1e-4.  Both
+    MAS factor kernels, in the block-diagonal layout and (local = 1) the MFMA
+    local kernel's tile layout."""
     w = W.fem_grid(14, n_contacts=400, seed=14, young_sigma=0.5)
     s = session.solver_for(w)
+    s.set_factor_waves(factor_waves)
+    s.set_mas_waves(local)
     s.build(precision)
     assert s.factor_status() == 0
     mas = oracle_mas(oracle, w)
