@@ -58,6 +58,7 @@ struct DevData
     int n_rows, n_sys, bank, n_banks, n_fine_banks;
     int mas_waves;                      // waves per workgroup of the FP32 MAS local kernel (4 or 8)
     int spmv_nt;                        // 1: the SpMV reads the matrix with nontemporal loads
+    int spmv_bodies;                    // 1: the SpMV layout has tiles of affine bodies
     int factor_waves;                   // waves per bank of the MAS factor (1 or 2)
     int single;                         // 1: one system (system index 0, whole work tables)
     int n_wg_pre, n_wg_spmv, n_wg_upd;  // workgroups of the iteration kernels' tables

@@ -1938,8 +1938,10 @@ __device__ __forceinline__ void spmv_reduce_chunk(const SpmvChunk<U>& c, float b
 // several rows fits one chunk of 256 * U entries, a longer single row takes
 // several chunks (it has no transposed products).  Also the p.Ap partial.
 // NT: the matrix is read with nontemporal loads (the choice for matrices far
-// larger than the caches).
-template <int U, bool NT>
+// larger than the caches).  BODIES: the layout has tiles of affine bodies
+// (their 12 x 12 couplings' registers would otherwise set every tile's
+// register count and occupancy).
+template <int U, bool NT, bool BODIES>
 __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __restrict__ p_old,
                                                     float* __restrict__ p_new)
 {
@@ -2024,7 +2026,7 @@ __global__ void __launch_bounds__(256) k_spmv_fused(DevData d, const float* __re
         spmv_load_chunk<U, NT>(d, p_old, base, n, c);
         spmv_reduce_chunk<U>(c, beta, base, n, has_row, rb, re, 0, 0, sX, sP, sQ, acc);
     }
-    if((w.y >> 28) == kWgAbd)  // uniform: a tile of affine bodies
+    if(BODIES && (w.y >> 28) == kWgAbd)  // uniform: a tile of affine bodies
     {
         // their 12 x 12 couplings: 48 threads per body, thread 48 b + 12 g + q
         // takes scalar row q of the tile's body b over every fourth of its
@@ -2697,13 +2699,18 @@ hipError_t launch_spmv_fused(const DevData& d, int n_wg, int entries_per_thread,
 {
     if(n_wg <= 0) return hipSuccess;
     void (*kern)(DevData, const float*, float*) = nullptr;
+    const bool nt = d.spmv_nt != 0, bodies = d.spmv_bodies != 0;
+#define MASX_SPMV_PICK(UU)                                                                 \
+    kern = bodies ? (nt ? k_spmv_fused<UU, true, true> : k_spmv_fused<UU, false, true>) \
+                  : (nt ? k_spmv_fused<UU, true, false> : k_spmv_fused<UU, false, false>)
     switch(entries_per_thread)
     {
-        case 2: kern = d.spmv_nt ? k_spmv_fused<2, true> : k_spmv_fused<2, false>; break;
-        case 4: kern = d.spmv_nt ? k_spmv_fused<4, true> : k_spmv_fused<4, false>; break;
-        case 8: kern = d.spmv_nt ? k_spmv_fused<8, true> : k_spmv_fused<8, false>; break;
+        case 2: MASX_SPMV_PICK(2); break;
+        case 4: MASX_SPMV_PICK(4); break;
+        case 8: MASX_SPMV_PICK(8); break;
         default: return hipErrorInvalidValue;
     }
+#undef MASX_SPMV_PICK
     MASX_DISPATCH(kern, dim3(n_wg), dim3(256), d, p_old, p_new);
     MASX_CHECK_LAUNCH();
     return hipSuccess;

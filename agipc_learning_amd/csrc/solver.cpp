@@ -606,6 +606,7 @@ void Solver::make_devdata()
     d.nnz = P.nnz;
     d.nnz_s = spmv_layout_.total;
     d.spmv_nt = effective_spmv_nt() ? 1 : 0;
+    d.spmv_bodies = P.n_bodies > 0 ? 1 : 0;
     d.factor_waves = effective_factor_waves();
     d.plane = std::max<long long>(64, (d.nnz_s + 63) / 64 * 64);
     d.row_ptr = d_row_ptr_.ptr;

@@ -94,7 +94,7 @@ def compare(bench_json, trace_csv):
     tiling = obj["spmv_tiling"]["entries_per_thread"]
     waves = obj["spmv_tiling"].get("mas_local", {}).get("fp32", {}).get("waves", 4)
     nt = "true" if obj["spmv_tiling"].get("nontemporal") else "false"
-    names = {"spmv": f"k_spmv_fused<{tiling}, {nt}>", "mas_local": f"k_mas_local<32, float, {mas_local_variant(waves)}>",
+    names = {"spmv": f"k_spmv_fused<{tiling}, {nt},", "mas_local": f"k_mas_local<32, float, {mas_local_variant(waves)}>",
              "precond_finish": "k_precond_finish", "update": "k_update"}
     d = collections.defaultdict(list)
     for r in csv.DictReader(open(trace_csv)):
@@ -103,9 +103,10 @@ def compare(bench_json, trace_csv):
            "|---|---|---|---|"]
     for k, n in names.items():
         e = us[k]
-        if n in d:
-            m = _busy_median(d[n])
-            out.append(f"| {n} | {e:.2f} | {m:.2f} | {m / e:.3f} |")
+        full = [t for t in d if t == n or (n.endswith(",") and t.startswith(n))]
+        if full:
+            m = _busy_median(d[full[0]])
+            out.append(f"| {full[0]} | {e:.2f} | {m:.2f} | {m / e:.3f} |")
     return "\n".join(out)
 
 

@@ -110,7 +110,7 @@ def measured_traffic(kernel, workload_is_default, spmv_entries, summary="fem40c_
     if not workload_is_default or not os.path.exists(path):
         return None
     data = json.load(open(path))["kernels"]
-    want = (f"k_spmv_fused<{spmv_entries}, {'true' if spmv_nt else 'false'}>" if kernel == "spmv"
+    want = (f"k_spmv_fused<{spmv_entries}, {'true' if spmv_nt else 'false'}," if kernel == "spmv"
             else KERNEL_PREFIX[kernel])
     if kernel == "mas_local" and waves:
         from agipc_learning_amd.prof_summary import mas_local_variant
