@@ -311,12 +311,14 @@ def one_bank_system(diag0=1.0, nan=False):
     return s
 
 
+@pytest.mark.parametrize("factor_waves", [1, 2])
 @pytest.mark.parametrize("precision", ["fp32", "fp16", "bf16"])
-def test_local_identity_factors_cleanly(precision):
+def test_local_identity_factors_cleanly(precision, factor_waves):
     # the fine bank's local matrix is I; the one coarse node above it (the
     # chain is one component) aggregates 32 identity blocks: 32 I, padded
-    # slots identity
+    # slots identity (both MAS factor kernels)
     s = one_bank_system()
+    s.set_factor_waves(factor_waves)
     s.build(precision)
     assert s.factor_status() == 0
     inv = s.inverse()
@@ -328,12 +330,15 @@ def test_local_identity_factors_cleanly(precision):
     assert np.array_equal(inv, want)
 
 
+@pytest.mark.parametrize("factor_waves", [1, 2])
 @pytest.mark.parametrize("nan", [False, True])
-def test_local_not_spd_or_not_finite_is_counted(nan):
+def test_local_not_spd_or_not_finite_is_counted(nan, factor_waves):
     s = one_bank_system(diag0=-1.0, nan=nan)
+    s.set_factor_waves(factor_waves)
     s.build("fp32")
     assert s.factor_status() >= 1
     s2 = one_bank_system(diag0=-1.0, nan=nan)
+    s2.set_factor_waves(factor_waves)
     s2.build("fp16")
     assert s2.factor_status() >= 1
     if not nan:
