@@ -377,6 +377,30 @@ __device__ __forceinline__ void store_inverse_element(void* out, int bank, int i
     }
 }
 
+// Bytes of one bank's stored inverse (the MFMA tile layout, the half
+// block-diagonal layout of five words per block, or FP32 nine floats per
+// block); every byte of it is written by store_inverse_element.
+template <int BANK, typename T>
+__device__ __forceinline__ int bank_inverse_bytes(bool tiles)
+{
+    constexpr int NBLK = BANK * (BANK + 1) / 2;
+    if constexpr(kTileCapable<BANK, T>)
+        if(tiles) return kTileBankHalves * 2;
+    return std::is_same<T, float>::value ? NBLK * 9 * 4 : NBLK * 5 * 4;
+}
+
+// The bank's inverse is first laid out in LDS (`stage`, bank-relative, in the
+// stored layout) and then copied to HBM with 16-byte stores: one coalesced
+// stream instead of a scattered 2- or 4-byte store per element.
+template <int BANK, typename T>
+__device__ __forceinline__ void copy_bank_inverse(const void* stage, void* out, int bank, bool tiles)
+{
+    const int n16 = bank_inverse_bytes<BANK, T>(tiles) / 16;
+    const float4* src = static_cast<const float4*>(stage);
+    float4* dst = reinterpret_cast<float4*>(static_cast<char*>(out) + (size_t)bank * (n16 * 16));
+    for(int q = (int)threadIdx.x; q < n16; q += (int)blockDim.x) dst[q] = src[q];
+}
+
 // First row of assembly pass s of `npass` over the lower triangle of a
 // dim x dim matrix (rows [row(s), row(s + 1)) hold about 1/npass of it).
 __host__ __device__ constexpr int factor_pass_row(int dim, int npass, int s)
@@ -416,8 +440,8 @@ __host__ __device__ constexpr int tile8x16(int rr, int r, int c) { return c * rr
 // column (double-buffered) and bookkeeping serve 42 FMAs per thread; the
 // sweep is bound by instruction issue, not by the FMAs.  The bank matrix is
 // assembled in two passes through half the LDS the whole lower triangle
-// would take (zero diagonal -> 1), that buffer then holds the pivot columns,
-// and the inverse is written from the registers.  Half-precision storage is
+// would take (zero diagonal -> 1), that buffer then holds the pivot columns
+// and at the end the bank's stored inverse, copied out in 16-byte stores.  Half-precision storage is
 // scaled by the largest stored entry of the bank's inverse in magnitude, so
 // every stored value lies in [-1, 1] (also for an inverse that floored
 // pivots left indefinite, whose off-diagonal entries can exceed its diagonal).
@@ -436,7 +460,7 @@ __global__ void __launch_bounds__(128) k_mas_factor_sym(DevData d, const double*
     constexpr bool kHalf = !std::is_same<T, float>::value;
     const bool tiles = kTileCapable<BANK, T> && d.mas_waves == 1;  // the MFMA local kernel's layout
     static_assert(DIM % 16 == 0, "the 8 x 16 grid must tile the bank matrix");
-    __shared__ double S[NS];
+    __shared__ __align__(16) double S[NS];
     __shared__ double red[2];
     double* H  = S;
     double* vk = S;              // during the sweep: two pivot columns ...
@@ -588,14 +612,18 @@ __global__ void __launch_bounds__(128) k_mas_factor_sym(DevData d, const double*
     }
     if(tid == 0) scale_out[bank] = (float)scale;
     const double inv_scale = 1.0 / scale;
+    static_assert(sizeof(S) >= (size_t)BANK * (BANK + 1) / 2 * 9 * 4, "the stored inverse fits the LDS buffer");
+    __syncthreads();  // the sweep's pivot columns are read
 #pragma unroll
     for(int c = 0; c < RC; ++c)
 #pragma unroll
         for(int r = 2 * c; r < RR; ++r)
         {
             const int i = ty + 8 * r, j = tx + 16 * c;
-            if(i >= j) store_inverse_element<BANK, T>(out, bank, i, j, -a[tile8x16(RR, r, c)], inv_scale, tiles);
+            if(i >= j) store_inverse_element<BANK, T>(S, 0, i, j, -a[tile8x16(RR, r, c)], inv_scale, tiles);
         }
+    __syncthreads();
+    copy_bank_inverse<BANK, T>(S, out, bank, tiles);
     if(tid == 0 && bad && status) atomicAdd(status, bad);
 }
 
@@ -629,7 +657,7 @@ __global__ void __launch_bounds__(64) k_mas_factor_wave(DevData d, const double*
     constexpr bool kHalf = !std::is_same<T, float>::value;
     const bool tiles = kTileCapable<BANK, T> && d.mas_waves == 1;  // the MFMA local kernel's layout
     static_assert(DIM % 8 == 0, "the 8 x 8 grid must tile the bank matrix");
-    __shared__ double S[NS];
+    __shared__ __align__(16) double S[NS];
     double* H  = S;
     double* vk = S;              // during the sweep: two pivot columns
     const int bank = bank_begin + (int)blockIdx.x, tid = threadIdx.x, tx = tid & 7, ty = tid >> 3;
@@ -771,14 +799,18 @@ __global__ void __launch_bounds__(64) k_mas_factor_wave(DevData d, const double*
     }
     if(tid == 0) scale_out[bank] = (float)scale;
     const double inv_scale = 1.0 / scale;
+    static_assert(sizeof(S) >= (size_t)BANK * (BANK + 1) / 2 * 9 * 4, "the stored inverse fits the LDS buffer");
+    __syncthreads();  // the sweep's pivot columns are read
 #pragma unroll
     for(int r = 0; r < NB; ++r)
 #pragma unroll
         for(int c = 0; c <= r; ++c)
         {
             const int i = ty + 8 * r, j = tx + 8 * c;
-            if(i >= j) store_inverse_element<BANK, T>(out, bank, i, j, -a[tile8(r, c)], inv_scale, tiles);
+            if(i >= j) store_inverse_element<BANK, T>(S, 0, i, j, -a[tile8(r, c)], inv_scale, tiles);
         }
+    __syncthreads();
+    copy_bank_inverse<BANK, T>(S, out, bank, tiles);
     if(tid == 0 && bad && status) atomicAdd(status, bad);
 }
 
