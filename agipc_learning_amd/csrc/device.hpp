@@ -60,6 +60,7 @@ struct DevData
     int spmv_nt;                        // 1: the SpMV reads the matrix with nontemporal loads
     int spmv_bodies;                    // 1: the SpMV layout has tiles of affine bodies
     int factor_waves;                   // waves per bank of the MAS factor (1 or 2)
+    const unsigned short* inv_map;      // stored position of a bank inverse -> element (launch_inverse_map)
     int single;                         // 1: one system (system index 0, whole work tables)
     int n_wg_pre, n_wg_spmv, n_wg_upd;  // workgroups of the iteration kernels' tables
     long long nnz;          // entries of the full block CSR

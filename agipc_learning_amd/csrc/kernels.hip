@@ -328,9 +328,9 @@ __global__ void __launch_bounds__(256) k_gal_gather(const double* fine, double* 
         for(int i = 0; i < 9; ++i) E[9 * (long long)e + i] = acc[i];
 }
 
-// One element of the inverse into the diagonal-major output of bank `bank`:
-// (i, j), i >= j, is the upper element (j, i) of block (j / 3, i / 3); a
-// diagonal block also takes its transpose position.  Half formats store
+// Where one element of a bank's inverse is stored, diagonal-major: (i, j),
+// i >= j, is the upper element (j, i) of block (j / 3, i / 3); a diagonal
+// block also takes its transpose position.  Half formats store
 // 16-bit halves of 5 words per block (the tenth half is zero).
 // Half-precision inverses of 32-slot banks are stored as MFMA tiles (the
 // tile layout of mas_tiles.hpp) when the MFMA local kernel reads them
@@ -340,23 +340,26 @@ template <int BANK, typename T>
 constexpr bool kTileCapable = BANK == 32 && !std::is_same<T, float>::value;
 
 template <int BANK, typename T>
-__device__ __forceinline__ void store_inverse_element(void* out, int bank, int i, int j, double v, double inv_scale,
-                                                      bool tiles)
+__device__ __forceinline__ int inverse_slots(int i, int j, bool tiles, int (&slot)[2])
 {
+    // Stored positions of element (i, j), i >= j, of a bank's inverse,
+    // bank-relative, in units of the stored value (an FP32 float, or a
+    // 16-bit half of the half-precision layouts): one, or two where the
+    // element has a mirror in the same diagonal block or tile.
     if constexpr(kTileCapable<BANK, T>)
     {
         if(tiles)
         {
-            // (i, j), i >= j, is the upper element (j, i); a diagonal tile
-            // also takes its mirror (i, j)
-            unsigned short* o = reinterpret_cast<unsigned short*>(out) + (size_t)bank * kTileBankHalves;
-            const unsigned short bits = (unsigned short)half_bits(v * inv_scale, (T*)nullptr);
-            o[tile_half_index(j, i)] = bits;
-            if(i != j && i / kTileDim == j / kTileDim) o[tile_half_index(i, j)] = bits;
-            return;
+            // (i, j) is the upper element (j, i); a diagonal tile also takes its mirror (i, j)
+            slot[0] = tile_half_index(j, i);
+            if(i != j && i / kTileDim == j / kTileDim)
+            {
+                slot[1] = tile_half_index(i, j);
+                return 2;
+            }
+            return 1;
         }
     }
-    constexpr int NBLK = BANK * (BANK + 1) / 2;
     const int I = j / 3, J = i / 3, dd = J - I, len = BANK - dd;
     const int c0 = 3 * (j % 3) + i % 3, c1 = 3 * (i % 3) + j % 3;
     const int nc = (I == J && i != j) ? 2 : 1;
@@ -364,22 +367,18 @@ __device__ __forceinline__ void store_inverse_element(void* out, int bank, int i
     {
         const int c = u ? c1 : c0;
         if constexpr(std::is_same<T, float>::value)
-            reinterpret_cast<float*>(out)[(size_t)bank * NBLK * 9 + 9 * diag_off(BANK, dd) + c * len + I] = (float)v;
+            slot[u] = 9 * diag_off(BANK, dd) + c * len + I;
         else
-        {
-            const size_t w = (size_t)bank * NBLK * 5 + 5 * diag_off(BANK, dd) + (c >> 1) * len + I;
-            const uint32_t bits = half_bits(v * inv_scale, (T*)nullptr);
-            if(c == 8)
-                reinterpret_cast<uint32_t*>(out)[w] = bits;  // its word's upper half is the unused tenth value
-            else
-                reinterpret_cast<unsigned short*>(out)[2 * w + (c & 1)] = (unsigned short)bits;
-        }
+            // value c in half c & 1 of word c / 2 (value 8's word keeps the
+            // unused tenth value, zero, in its upper half)
+            slot[u] = 2 * (5 * diag_off(BANK, dd) + (c >> 1) * len + I) + (c & 1);
     }
+    return nc;
 }
 
 // Bytes of one bank's stored inverse (the MFMA tile layout, the half
 // block-diagonal layout of five words per block, or FP32 nine floats per
-// block); every byte of it is written by store_inverse_element.
+// block); every position is either an element's (inverse_slots) or a zero.
 template <int BANK, typename T>
 __device__ __forceinline__ int bank_inverse_bytes(bool tiles)
 {
@@ -389,16 +388,63 @@ __device__ __forceinline__ int bank_inverse_bytes(bool tiles)
     return std::is_same<T, float>::value ? NBLK * 9 * 4 : NBLK * 5 * 4;
 }
 
-// The bank's inverse is first laid out in LDS (`stage`, bank-relative, in the
-// stored layout) and then copied to HBM with 16-byte stores: one coalesced
-// stream instead of a scattered 2- or 4-byte store per element.
+// The bank's inverse leaves the factor kernels as a dense lower triangle of
+// floats in LDS (element (i, j), i >= j, at i (i + 1) / 2 + j; the half
+// layouts' values already scaled); `map` (one entry per stored value, from
+// k_inverse_map: the element each position holds, 0xffff for a zero) turns
+// it into the stored layout, written in 16-byte stores.
 template <int BANK, typename T>
-__device__ __forceinline__ void copy_bank_inverse(const void* stage, void* out, int bank, bool tiles)
+__device__ __forceinline__ void write_bank_inverse(const float* L, const unsigned short* __restrict__ map, void* out,
+                                                   int bank, bool tiles)
 {
     const int n16 = bank_inverse_bytes<BANK, T>(tiles) / 16;
-    const float4* src = static_cast<const float4*>(stage);
-    float4* dst = reinterpret_cast<float4*>(static_cast<char*>(out) + (size_t)bank * (n16 * 16));
-    for(int q = (int)threadIdx.x; q < n16; q += (int)blockDim.x) dst[q] = src[q];
+    auto val = [&](unsigned short e) { return e != 0xffffu ? L[e] : 0.f; };
+    if constexpr(std::is_same<T, float>::value)
+    {
+        float4* dst = reinterpret_cast<float4*>(static_cast<char*>(out) + (size_t)bank * (n16 * 16));
+        for(int q = (int)threadIdx.x; q < n16; q += (int)blockDim.x)
+        {
+            const ushort4 m = reinterpret_cast<const ushort4*>(map)[q];
+            dst[q] = make_float4(val(m.x), val(m.y), val(m.z), val(m.w));
+        }
+    }
+    else
+    {
+        uint4* dst = reinterpret_cast<uint4*>(static_cast<char*>(out) + (size_t)bank * (n16 * 16));
+        for(int q = (int)threadIdx.x; q < n16; q += (int)blockDim.x)
+        {
+            const uint4 mw = reinterpret_cast<const uint4*>(map)[q];  // eight 16-bit entries
+            const uint32_t m[4] = {mw.x, mw.y, mw.z, mw.w};
+            uint32_t o[4];
+#pragma unroll
+            for(int h = 0; h < 4; ++h)
+            {
+                const unsigned short e0 = (unsigned short)(m[h] & 0xffffu), e1 = (unsigned short)(m[h] >> 16);
+                const uint32_t b0 = e0 != 0xffffu ? half_bits((double)L[e0], (T*)nullptr) : 0u;
+                const uint32_t b1 = e1 != 0xffffu ? half_bits((double)L[e1], (T*)nullptr) : 0u;
+                o[h] = (b0 & 0xffffu) | (b1 << 16);
+            }
+            dst[q] = uint4{o[0], o[1], o[2], o[3]};
+        }
+    }
+}
+
+// map[position] = the dense lower-triangle element a bank's stored inverse
+// keeps there (the map was filled with 0xffff: positions no element takes
+// hold zero).  One thread per element.
+template <int BANK, typename T>
+__global__ void __launch_bounds__(256) k_inverse_map(unsigned short* map, int tiles)
+{
+    constexpr int DIM = 3 * BANK, NE = DIM * (DIM + 1) / 2;
+    static_assert(NE < 0xffff, "element indices fit 16 bits");
+    const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if(e >= NE) return;
+    int i = 0;
+    while((i + 1) * (i + 2) / 2 <= e) ++i;
+    const int j = e - i * (i + 1) / 2;
+    int slot[2];
+    const int n = inverse_slots<BANK, T>(i, j, tiles != 0, slot);
+    for(int u = 0; u < n; ++u) map[slot[u]] = (unsigned short)e;
 }
 
 // First row of assembly pass s of `npass` over the lower triangle of a
@@ -612,7 +658,8 @@ __global__ void __launch_bounds__(128) k_mas_factor_sym(DevData d, const double*
     }
     if(tid == 0) scale_out[bank] = (float)scale;
     const double inv_scale = 1.0 / scale;
-    static_assert(sizeof(S) >= (size_t)BANK * (BANK + 1) / 2 * 9 * 4, "the stored inverse fits the LDS buffer");
+    static_assert(sizeof(S) >= (size_t)(3 * BANK) * (3 * BANK + 1) / 2 * 4, "the dense inverse fits the LDS buffer");
+    float* Lf = reinterpret_cast<float*>(S);
     __syncthreads();  // the sweep's pivot columns are read
 #pragma unroll
     for(int c = 0; c < RC; ++c)
@@ -620,10 +667,11 @@ __global__ void __launch_bounds__(128) k_mas_factor_sym(DevData d, const double*
         for(int r = 2 * c; r < RR; ++r)
         {
             const int i = ty + 8 * r, j = tx + 16 * c;
-            if(i >= j) store_inverse_element<BANK, T>(S, 0, i, j, -a[tile8x16(RR, r, c)], inv_scale, tiles);
+            const double v = -a[tile8x16(RR, r, c)];
+            if(i >= j) Lf[i * (i + 1) / 2 + j] = (float)(kHalf ? v * inv_scale : v);
         }
     __syncthreads();
-    copy_bank_inverse<BANK, T>(S, out, bank, tiles);
+    write_bank_inverse<BANK, T>(Lf, d.inv_map, out, bank, tiles);
     if(tid == 0 && bad && status) atomicAdd(status, bad);
 }
 
@@ -799,7 +847,8 @@ __global__ void __launch_bounds__(64) k_mas_factor_wave(DevData d, const double*
     }
     if(tid == 0) scale_out[bank] = (float)scale;
     const double inv_scale = 1.0 / scale;
-    static_assert(sizeof(S) >= (size_t)BANK * (BANK + 1) / 2 * 9 * 4, "the stored inverse fits the LDS buffer");
+    static_assert(sizeof(S) >= (size_t)(3 * BANK) * (3 * BANK + 1) / 2 * 4, "the dense inverse fits the LDS buffer");
+    float* Lf = reinterpret_cast<float*>(S);
     __syncthreads();  // the sweep's pivot columns are read
 #pragma unroll
     for(int r = 0; r < NB; ++r)
@@ -807,10 +856,11 @@ __global__ void __launch_bounds__(64) k_mas_factor_wave(DevData d, const double*
         for(int c = 0; c <= r; ++c)
         {
             const int i = ty + 8 * r, j = tx + 8 * c;
-            if(i >= j) store_inverse_element<BANK, T>(S, 0, i, j, -a[tile8(r, c)], inv_scale, tiles);
+            const double v = -a[tile8(r, c)];
+            if(i >= j) Lf[i * (i + 1) / 2 + j] = (float)(kHalf ? v * inv_scale : v);
         }
     __syncthreads();
-    copy_bank_inverse<BANK, T>(S, out, bank, tiles);
+    write_bank_inverse<BANK, T>(Lf, d.inv_map, out, bank, tiles);
     if(tid == 0 && bad && status) atomicAdd(status, bad);
 }
 
@@ -2593,6 +2643,41 @@ hipError_t launch_mas_factor(const DevData& d, const double* E, const int* ent_p
     else
         return hipErrorInvalidValue;
 #undef MASX_FACTOR
+    MASX_CHECK_LAUNCH();
+    return hipSuccess;
+}
+
+int inverse_map_entries(int bank, int precision, bool tiles)
+{
+    const int nblk = bank * (bank + 1) / 2;
+    if(precision == kPrecFp32) return nblk * 9;
+    if(tiles && bank == 32) return kTileBankHalves;
+    return nblk * 10;
+}
+
+hipError_t launch_inverse_map(int bank, int precision, bool tiles, unsigned short* map, hipStream_t st)
+{
+    const int n = inverse_map_entries(bank, precision, tiles);
+    hipError_t e = hipMemsetAsync(map, 0xff, sizeof(unsigned short) * (size_t)n, st);
+    if(e != hipSuccess) return e;
+    const int ne = 3 * bank * (3 * bank + 1) / 2;
+    const dim3 g((ne + 255) / 256);
+#define MASX_MAP(BK, T) hipLaunchKernelGGL((k_inverse_map<BK, T>), g, dim3(256), 0, st, map, tiles ? 1 : 0)
+    if(bank == 32)
+    {
+        if(precision == kPrecFp16) MASX_MAP(32, __half);
+        else if(precision == kPrecBf16) MASX_MAP(32, __hip_bfloat16);
+        else MASX_MAP(32, float);
+    }
+    else if(bank == 16)
+    {
+        if(precision == kPrecFp16) MASX_MAP(16, __half);
+        else if(precision == kPrecBf16) MASX_MAP(16, __hip_bfloat16);
+        else MASX_MAP(16, float);
+    }
+    else
+        return hipErrorInvalidValue;
+#undef MASX_MAP
     MASX_CHECK_LAUNCH();
     return hipSuccess;
 }

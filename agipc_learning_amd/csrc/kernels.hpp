@@ -14,6 +14,12 @@ hipError_t launch_pack_abd(const DevData& d, const int* cp_body, const int* cp_s
 // lanes per coarse entry: 16 or 64
 hipError_t launch_gal_gather(const double* fine, double* E, const int* ptr, const int* src, int e0, int e1,
                              int src_is_fine, int lanes, hipStream_t st);
+// the map from stored position of a bank's inverse to its element (one
+// entry per stored value: FP32 floats or 16-bit halves), for the layout of
+// this bank size, precision and (half, 32-slot banks) MFMA tiles or not;
+// the factor kernels write through it (DevData::inv_map)
+int inverse_map_entries(int bank, int precision, bool tiles);
+hipError_t launch_inverse_map(int bank, int precision, bool tiles, unsigned short* map, hipStream_t st);
 // banks [bank_begin, bank_end) of the bank list
 hipError_t launch_mas_factor(const DevData& d, const double* E, const int* ent_ptr, const int* ent_pos,
                              const int* ent_src, void* out, float* scale, int precision, int* status, int bank_begin,

@@ -608,6 +608,7 @@ void Solver::make_devdata()
     d.spmv_nt = effective_spmv_nt() ? 1 : 0;
     d.spmv_bodies = P.n_bodies > 0 ? 1 : 0;
     d.factor_waves = effective_factor_waves();
+    d.inv_map = d_inv_map_.ptr;
     d.plane = std::max<long long>(64, (d.nnz_s + 63) / 64 * 64);
     d.row_ptr = d_row_ptr_.ptr;
     d.col = d_col_.ptr;
@@ -751,6 +752,18 @@ void Solver::build(int precision, hipStream_t user)
         graph_exec_ = nullptr;
     }
     const Plan& P = plan_;
+    {
+        // the factor writes the inverse through this layout's map
+        const bool tiles = P.bank == 32 && precision_ != kPrecFp32 && dd_.mas_waves == 1;
+        const int  key   = (P.bank * 4 + precision_) * 2 + (tiles ? 1 : 0);
+        if(key != inv_map_key_ && P.n_banks > 0)
+        {
+            d_inv_map_.alloc((size_t)inverse_map_entries(P.bank, precision_, tiles));
+            MASX_HIP(launch_inverse_map(P.bank, precision_, tiles, d_inv_map_.ptr, st));
+            inv_map_key_ = key;
+        }
+        dd_.inv_map = d_inv_map_.ptr;
+    }
     // On large systems the MAS factor of the fine banks (compute-bound,
     // milliseconds) runs on the work stream while the side stream forms the
     // coarse levels' Galerkin sums and factors the coarse banks, then packs

@@ -168,6 +168,8 @@ class Solver
     DeviceArray<int>       d_sy_row_ptr_, d_sy_col_, d_sy_src_, d_sy_tptr_;
     DeviceArray<int>       d_cp_ptr_, d_cp_col_, d_cp_row_, d_cp_body_, d_cp_sub_;  // 12 x 12 body couplings
     DeviceArray<float>     d_vals_abd_;
+    DeviceArray<unsigned short> d_inv_map_;  // the current inverse layout's position -> element map
+    int inv_map_key_ = -1;
     DeviceArray<float>     d_vals32_;
     DeviceArray<double>    d_vals64_, d_E_, d_b64_, d_tmp64a_, d_tmp64b_;
     DeviceArray<float>     d_x_, d_r_, d_z_, d_p0_, d_p1_, d_Ap_, d_rt_, d_Rc_, d_Zc_, d_part_deep_;
