@@ -31,6 +31,20 @@ namespace
 
 __device__ __forceinline__ int packed_index(int bank, int r, int c) { return bank * r - r * (r + 1) / 2 + c; }
 
+// Row of packed upper-triangle index pos (the largest r with
+// packed_index(BANK, r, r) <= pos): the root of the row-start quadratic,
+// corrected by at most a step either way for the float rounding.
+template <int BANK>
+__device__ __forceinline__ int packed_row(int pos)
+{
+    constexpr int K = 2 * BANK + 1;
+    int r = (int)(((float)K - sqrtf((float)(K * K - 8 * pos))) * 0.5f);
+    r = r < 0 ? 0 : (r > BANK - 1 ? BANK - 1 : r);
+    if(r > 0 && packed_index(BANK, r, r) > pos) --r;
+    if(r + 1 < BANK && packed_index(BANK, r + 1, r + 1) <= pos) ++r;
+    return r;
+}
+
 // blocks stored before block diagonal d of a bank
 __host__ __device__ __forceinline__ constexpr int diag_off(int bank, int d) { return d * bank - d * (d - 1) / 2; }
 
@@ -530,8 +544,7 @@ __global__ void __launch_bounds__(128) k_mas_factor_sym(DevData d, const double*
         {
             const int pos = ent_pos[k] & 0xffff, tr = ent_pos[k] >> 16;
             const int src = ent_src[k];
-            int I = 0;
-            while(I + 1 < BANK && packed_index(BANK, I + 1, I + 1) <= pos) ++I;
+            const int I = packed_row<BANK>(pos);
             const int J = pos - packed_index(BANK, I, I) + I;
             for(int aa = 0; aa < 3; ++aa)
                 for(int bb = 0; bb < 3; ++bb)
@@ -727,8 +740,7 @@ __global__ void __launch_bounds__(64) k_mas_factor_wave(DevData d, const double*
         {
             const int pos = ent_pos[k] & 0xffff, tr = ent_pos[k] >> 16;
             const int src = ent_src[k];
-            int I = 0;
-            while(I + 1 < BANK && packed_index(BANK, I + 1, I + 1) <= pos) ++I;
+            const int I = packed_row<BANK>(pos);
             const int J = pos - packed_index(BANK, I, I) + I;
             for(int aa = 0; aa < 3; ++aa)
                 for(int bb = 0; bb < 3; ++bb)
