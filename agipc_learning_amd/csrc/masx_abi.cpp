@@ -286,6 +286,15 @@ int masx_set_values(masx_solver* s, int sys, const double* d_values, void* strea
     });
 }
 
+int masx_set_values_many(masx_solver* s, int n, const int* systems, const double* const* d_values, void* stream)
+{
+    return guard([&] {
+        require(s && n >= 0 && (n == 0 || (systems && d_values)), "null argument");
+        for(int q = 0; q < n; ++q) require(d_values[q] != nullptr, "null values");
+        s->impl.set_values_many(n, systems, d_values, as_stream(stream));
+    });
+}
+
 int masx_set_values_host(masx_solver* s, int sys, const double* h_values, void* stream)
 {
     return guard([&] {

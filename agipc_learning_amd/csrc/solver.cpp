@@ -681,6 +681,22 @@ void Solver::set_values(int s, const double* d_values, hipStream_t st)
     built_ = false;
 }
 
+void Solver::set_values_many(int n, const int* systems, const double* const* d_values, hipStream_t st)
+{
+    if(!setup_done_) throw std::runtime_error("setup() must precede set_values()");
+    for(int q = 0; q < n; ++q) (void)systems_.at(systems[q]);  // range checks before any copy
+    hipStream_t w = enter(st);
+    for(int q = 0; q < n; ++q)
+    {
+        const int s = systems[q];
+        const long long nb = systems_[s].nb;
+        if(nb) MASX_HIP(hipMemcpyAsync(d_vals64_.ptr + 9 * plan_.src_off[s], d_values[q], 9 * nb * sizeof(double),
+                                       hipMemcpyDeviceToDevice, w));
+    }
+    leave(st);
+    built_ = false;
+}
+
 void Solver::set_values_host(int s, const double* h_values, hipStream_t st)
 {
     if(!setup_done_) throw std::runtime_error("setup() must precede set_values()");

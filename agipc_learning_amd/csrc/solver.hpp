@@ -79,6 +79,8 @@ class Solver
     bool is_setup() const { return setup_done_; }
     bool is_analyzed() const { return analyzed_; }
     void set_values(int s, const double* d_values, hipStream_t st);      // device FP64 col-major
+    // set_values for several systems, one stream hand-off for all of them
+    void set_values_many(int n, const int* systems, const double* const* d_values, hipStream_t st);
     void set_values_host(int s, const double* h_values, hipStream_t st);
     void gather_values(int s, const double* d_values, const uint32_t* d_indices, hipStream_t st);
     void build(int precision, hipStream_t st);

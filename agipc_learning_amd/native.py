@@ -112,6 +112,7 @@ def load():
         "masx_setup": ([vp], C.c_int),
         "masx_set_values": ([vp, C.c_int, vp, vp], C.c_int),
         "masx_set_values_host": ([vp, C.c_int, dp, vp], C.c_int),
+        "masx_set_values_many": ([vp, C.c_int, ip, C.POINTER(vp), vp], C.c_int),
         "masx_build": ([vp, C.c_int, vp], C.c_int),
         "masx_solve": ([vp, vp, vp, C.POINTER(SolveParams), C.POINTER(SolveStats), vp], C.c_int),
         "masx_spmv": ([vp, C.c_double, vp, C.c_double, vp, vp], C.c_int),
@@ -403,6 +404,13 @@ class Solver:
             torch.cuda.synchronize()
         else:
             check(self.L.masx_set_values(self.h, sys, _dev_f64(values, n, "values"), stream))
+
+    def set_values_many(self, values, stream=None):
+        """values: {system: device float64 tensor (nb, 9)}; one library call."""
+        items = sorted(values.items())
+        sys_ids = np.array([k for k, _ in items], np.int32)
+        ptrs = (C.c_void_p * len(items))(*[_dev_f64(v, 9 * self.systems[k]["nb"], "values").value for k, v in items])
+        check(self.L.masx_set_values_many(self.h, len(items), _ptr(sys_ids, C.c_int), ptrs, stream))
 
     def build(self, precision="fp32", stream=None):
         check(self.L.masx_build(self.h, PRECISIONS.get(precision, precision), stream))

@@ -336,12 +336,19 @@ def batch_leg(args):
     log(f"  tiling {tiling}")
 
     bs = rhs_variants(b)
+    torch.cuda.synchronize()
+    # the step runs on one stream, as a Newton loop would call it: values in
+    # (all 64 systems in one call), build and solve are ordered on the stream;
+    # the solve returns after its statistics are read
+    work = torch.cuda.Stream()
+    ws_ = work.cuda_stream
+
+    all_vals = dict(enumerate(vals))
 
     def step(prec, k=0):
-        for i, v in enumerate(vals):
-            s.set_values(i, v)
-        s.build(prec)
-        return s.solve(bs[k % len(bs)], x, rule=1, tol=args.tol)
+        s.set_values_many(all_vals, stream=ws_)
+        s.build(prec, stream=ws_)
+        return s.solve(bs[k % len(bs)], x, rule=1, tol=args.tol, stream=ws_)
 
     precs = ["fp32"] + [p for p in args.precisions.split(",") if p and p != "fp32"]
     out = {"systems": len(ws), "vertices": nv, "dof": 3 * nv, "upper_blocks": int(info["nb_total"]),
@@ -621,11 +628,13 @@ def fem40_leg(args, structure=True):
     x = torch.zeros_like(b)
     bs = rhs_variants(b)
     torch.cuda.synchronize()
+    work = torch.cuda.Stream()  # the step on one stream, as in the batch leg
+    ws_ = work.cuda_stream
 
     def step(prec, k=0):
-        s.set_values(0, vals)
-        s.build(prec)
-        return s.solve(bs[k % len(bs)], x, rule=1, tol=args.tol)[0]
+        s.set_values(0, vals, stream=ws_)
+        s.build(prec, stream=ws_)
+        return s.solve(bs[k % len(bs)], x, rule=1, tol=args.tol, stream=ws_)[0]
 
     def run(prec, steps, warmup):
         for i in range(warmup):

@@ -241,6 +241,33 @@ def test_hybrid_abd_fem_parity(oracle):
     assert iters_close(st["iterations"], so["iterations"]), (st, so)
 
 
+def test_set_values_many_equals_per_system_values():
+    """masx_set_values_many (all systems in one call, on a stream) puts the
+    same values in place as one masx_set_values per system: after the
+    values are overwritten, it restores them and the solve is bit-identical."""
+    ws = [W.fem_grid(n, n_contacts=c, seed=n, young_sigma=0.5) for n, c in ((6, 20), (9, 0), (7, 40))]
+    s = session.solver_for(ws)
+    vals = {i: dev(w["vals"]) for i, w in enumerate(ws)}
+    b = dev(session.rhs_of(ws))
+
+    def solve():
+        s.build("fp32")
+        x = torch.zeros_like(b)
+        st = s.solve(b, x, rule=1, tol=1e-6)
+        return [q["iterations"] for q in st], host(x)
+
+    ref = solve()
+    for i, v in vals.items():
+        s.set_values(i, 2.0 * v)
+    stream = torch.cuda.Stream()
+    s.set_values_many(vals, stream=stream.cuda_stream)
+    got = solve()
+    assert got[0] == ref[0]
+    assert np.array_equal(got[1], ref[1])
+    with pytest.raises(ValueError):
+        s.set_values_many({0: vals[1]})  # wrong length for system 0
+
+
 def test_batched_equals_individual():
     ws = [W.fem_grid(n, n_contacts=c, seed=n, young_sigma=0.5) for n, c in ((6, 20), (11, 150), (9, 0))]
     batch = session.solver_for(ws)
