@@ -458,11 +458,14 @@ def hybrid(args, w):
     out = {"bodies": int(w["n_bodies"]), "fem_vertices": int(w["n_vertices"]), "dof": int(len(b)),
            "upper_blocks": int(s.info()["nb_total"])}
     xs = {}
+    torch.cuda.synchronize()  # inputs in place before the step's own stream reads them
+    work = torch.cuda.Stream()  # the step on one stream, as in the batch leg
+    ws_ = work.cuda_stream
     for prec in ("fp32", "fp16", "bf16"):
         def step():
-            s.set_values(0, vals)
-            s.build(prec)
-            return s.solve(b, x, rule=1, tol=args.tol)[0]
+            s.set_values(0, vals, stream=ws_)
+            s.build(prec, stream=ws_)
+            return s.solve(b, x, rule=1, tol=args.tol, stream=ws_)[0]
         step()
         torch.cuda.synchronize()
         t = time.perf_counter()
