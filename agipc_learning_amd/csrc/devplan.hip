@@ -283,18 +283,49 @@ __global__ void k_dp_ent_bank_counts(int n_fine_banks, int bank, const int* row_
     if(b < n_fine_banks) bank_cnt[b] = row_off[(b + 1) * bank] - row_off[b * bank];
 }
 
-// the greedy tile that starts at r0 (build_spmv_layout's cut, from any r0)
-__global__ void k_dp_sl_next(int n, const int* row_ptr, const int* col, const int* row_sys, const int* row_off,
-                             int cap, int* nxt, int* kept)
+// the greedy tile that starts at r0 (build_spmv_layout's cut, from any r0).
+// A workgroup takes kThreads consecutive r0, whose tiles can only reach the
+// next kThreads + kSpmvTileRows - 1 rows: their row pointers, and their
+// column words when they fit kSlCols, are staged in LDS once, so each
+// thread's per-row searches (two per row it adds) read LDS instead of HBM.
+constexpr int kSlRows = kThreads + kSpmvTileRows;
+constexpr int kSlCols = 8192;
+
+__global__ void __launch_bounds__(kThreads) k_dp_sl_next(int n, const int* row_ptr, const int* col, const int* row_sys,
+                                                         const int* row_off, int cap, int* nxt, int* kept)
 {
-    const int r0 = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ int s_ptr[kSlRows + 1];
+    __shared__ int s_col[kSlCols];
+    const int R = blockIdx.x * kThreads, r0 = R + (int)threadIdx.x;
+    const int rw = min(n, R + kSlRows);  // rows [R, rw) are staged
+    for(int i = threadIdx.x; i <= rw - R; i += kThreads) s_ptr[i] = row_ptr[R + i];
+    __syncthreads();
+    const int e0 = s_ptr[0], ne = s_ptr[rw - R] - e0;
+    const bool cached = ne <= kSlCols;  // uniform
+    if(cached)
+        for(int i = threadIdx.x; i < ne; i += kThreads) s_col[i] = col[e0 + i];
+    __syncthreads();
     if(r0 >= n) return;
+    // first entry of row r (R <= r < rw) whose column is >= c
+    auto first_ge_s = [&](int r, int c) {
+        int lo = s_ptr[r - R], hi = s_ptr[r + 1 - R];
+        while(lo < hi)
+        {
+            const int mid = (lo + hi) >> 1;
+            if((cached ? s_col[mid - e0] : col[mid]) < c)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        return lo;
+    };
     const int r_end = row_off[row_sys[r0] + 1];  // the system's end
-    int k = row_ptr[r0 + 1] - row_ptr[r0];
+    int k = s_ptr[r0 + 1 - R] - s_ptr[r0 - R];
     int r1 = r0 + 1;
     while(r1 < r_end && r1 - r0 < kSpmvTileRows)
     {
-        const int q = (row_ptr[r1 + 1] - first_ge(row_ptr, col, r1, r1)) + (first_ge(row_ptr, col, r1, r0) - row_ptr[r1]);
+        // r1 + 1 <= r0 + kSpmvTileRows <= R + kSlRows - 1 and r1 < n: inside the staged rows
+        const int q = (s_ptr[r1 + 1 - R] - first_ge_s(r1, r1)) + (first_ge_s(r1, r0) - s_ptr[r1 - R]);
         if(k + q > cap) break;
         k += q;
         ++r1;
