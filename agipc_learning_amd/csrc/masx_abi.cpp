@@ -174,7 +174,18 @@ int masx_set_structure(masx_solver* s, int sys, long long nb, const int* rows, c
         d.nb = nb;
         d.rows.assign(rows, rows + nb);
         d.cols.assign(cols, cols + nb);
+        d.rows_on_device = false;
         s->impl.structure_changed(sys);
+    });
+}
+
+int masx_set_structure_device(masx_solver* s, int sys, long long nb, const int* d_rows, const int* d_cols,
+                              void* stream)
+{
+    return guard([&] {
+        require(s && sys >= 0 && sys < s->impl.n_systems(), "bad system index");
+        require(nb >= 0 && (nb == 0 || (d_rows && d_cols)), "null rows/cols");
+        s->impl.set_structure_device(sys, nb, d_rows, d_cols, as_stream(stream));
     });
 }
 

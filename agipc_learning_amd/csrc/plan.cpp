@@ -379,6 +379,12 @@ void build_plan(Plan& P, std::vector<SystemDesc>& sys, bool structure_on_device)
     for(int s = 0; s < S; ++s)
     {
         auto& d = sys[s];
+        if(d.rows_on_device)
+        {
+            // checked on the device by the block CSR build (out-of-range indices are flagged there)
+            if(!structure_on_device) throw std::logic_error("device-held structure needs the device analysis");
+            continue;
+        }
         if((long long)d.rows.size() != d.nb || (long long)d.cols.size() != d.nb)
             throw std::invalid_argument("row/col arrays do not match the block count");
         int bad = 0;

@@ -65,8 +65,10 @@ struct SystemDesc
     bool global_diag = false;
     int  bank       = 32;
     int  levels     = 0;   // <= 0: reference rule
-    // host copies of structure
+    // host copies of structure (empty while rows_on_device: the solver then
+    // holds them on the device, for the device structure analysis)
     std::vector<int>      rows, cols;
+    bool                  rows_on_device = false;
     std::vector<uint32_t> nbr_start, nbr_num, nbr_list;
     std::vector<int>      part_to_real, real_to_part;
     std::vector<int>      pairs;        // int4 per pair

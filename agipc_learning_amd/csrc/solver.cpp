@@ -370,6 +370,7 @@ void Solver::setup()
         }
         else
         {
+            download_device_structures();  // the host analysis reads host rows/cols
             analyze();
             tm.mark("host plan");
         }
@@ -679,6 +680,50 @@ void Solver::set_values(int s, const double* d_values, hipStream_t st)
                                    hipMemcpyDeviceToDevice, w));
     leave(st);
     built_ = false;
+}
+
+void Solver::set_structure_device(int s, long long nb, const int* d_rows, const int* d_cols, hipStream_t user)
+{
+    SystemDesc& d = systems_.at(s);
+    if(nb < 0 || (nb > 0 && (!d_rows || !d_cols))) throw std::invalid_argument("null rows/cols");
+    ensure_stream();
+    while(d_sys_rows_.size() < systems_.size())
+    {
+        d_sys_rows_.push_back(std::make_unique<DeviceArray<int>>());
+        d_sys_cols_.push_back(std::make_unique<DeviceArray<int>>());
+    }
+    d_sys_rows_[s]->alloc(std::max<long long>(nb, 1));
+    d_sys_cols_[s]->alloc(std::max<long long>(nb, 1));
+    hipStream_t w = enter(user);
+    if(nb)
+    {
+        MASX_HIP(hipMemcpyAsync(d_sys_rows_[s]->ptr, d_rows, sizeof(int) * nb, hipMemcpyDeviceToDevice, w));
+        MASX_HIP(hipMemcpyAsync(d_sys_cols_[s]->ptr, d_cols, sizeof(int) * nb, hipMemcpyDeviceToDevice, w));
+    }
+    leave(user);
+    d.nb = nb;
+    d.rows.clear();
+    d.cols.clear();
+    d.rows_on_device = true;
+    structure_changed(s);
+}
+
+void Solver::download_device_structures()
+{
+    for(size_t s = 0; s < systems_.size(); ++s)
+    {
+        SystemDesc& d = systems_[s];
+        if(!d.rows_on_device) continue;
+        d.rows.resize(d.nb);
+        d.cols.resize(d.nb);
+        MASX_HIP(hipStreamSynchronize(work_));
+        if(d.nb)
+        {
+            MASX_HIP(hipMemcpy(d.rows.data(), d_sys_rows_[s]->ptr, sizeof(int) * d.nb, hipMemcpyDeviceToHost));
+            MASX_HIP(hipMemcpy(d.cols.data(), d_sys_cols_[s]->ptr, sizeof(int) * d.nb, hipMemcpyDeviceToHost));
+        }
+        d.rows_on_device = false;
+    }
 }
 
 void Solver::set_values_many(int n, const int* systems, const double* const* d_values, hipStream_t st)

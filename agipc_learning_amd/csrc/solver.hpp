@@ -76,6 +76,10 @@ class Solver
     void setup();
     // topology or contacts of system s changed: the next setup() re-analyzes
     void structure_changed(int s);
+    // system s's upper structure from device arrays (copied on the library's
+    // stream, ordered after `user`); the device structure analysis reads the
+    // copy, the host analysis downloads it if it is ever needed
+    void set_structure_device(int s, long long nb, const int* d_rows, const int* d_cols, hipStream_t user);
     bool is_setup() const { return setup_done_; }
     bool is_analyzed() const { return analyzed_; }
     void set_values(int s, const double* d_values, hipStream_t st);      // device FP64 col-major
@@ -171,6 +175,8 @@ class Solver
     DeviceArray<int>       d_cp_ptr_, d_cp_col_, d_cp_row_, d_cp_body_, d_cp_sub_;  // 12 x 12 body couplings
     DeviceArray<float>     d_vals_abd_;
     DeviceArray<unsigned short> d_inv_map_;  // the current inverse layout's position -> element map
+    std::vector<std::unique_ptr<DeviceArray<int>>> d_sys_rows_, d_sys_cols_;  // device-held structures (set_structure_device)
+    void download_device_structures();  // for the host analysis
     int inv_map_key_ = -1;
     DeviceArray<float>     d_vals32_;
     DeviceArray<double>    d_vals64_, d_E_, d_b64_, d_tmp64a_, d_tmp64b_;

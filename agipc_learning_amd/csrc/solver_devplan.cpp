@@ -103,11 +103,12 @@ void Solver::build_structure_on_device(hipStream_t st)
     {
         const SystemDesc& d = systems_[s];
         if(d.nb <= 0) continue;
-        check(hipMemcpyAsync(d_dp_rows_.ptr + P.src_off[s], d.rows.data(), sizeof(int) * d.nb, hipMemcpyHostToDevice,
-                             st),
+        const bool dev = d.rows_on_device;  // a device-held structure: a device copy, no upload
+        check(hipMemcpyAsync(d_dp_rows_.ptr + P.src_off[s], dev ? (const void*)d_sys_rows_[s]->ptr : d.rows.data(),
+                             sizeof(int) * d.nb, dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st),
               "structure upload");
-        check(hipMemcpyAsync(d_dp_cols_.ptr + P.src_off[s], d.cols.data(), sizeof(int) * d.nb, hipMemcpyHostToDevice,
-                             st),
+        check(hipMemcpyAsync(d_dp_cols_.ptr + P.src_off[s], dev ? (const void*)d_sys_cols_[s]->ptr : d.cols.data(),
+                             sizeof(int) * d.nb, dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st),
               "structure upload");
     }
     const int       bits    = bits_for((long long)n_rows + 1);  // the sentinel row n_rows fits

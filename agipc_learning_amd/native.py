@@ -113,6 +113,7 @@ def load():
         "masx_set_values": ([vp, C.c_int, vp, vp], C.c_int),
         "masx_set_values_host": ([vp, C.c_int, dp, vp], C.c_int),
         "masx_set_values_many": ([vp, C.c_int, ip, C.POINTER(vp), vp], C.c_int),
+        "masx_set_structure_device": ([vp, C.c_int, C.c_longlong, vp, vp, vp], C.c_int),
         "masx_build": ([vp, C.c_int, vp], C.c_int),
         "masx_solve": ([vp, vp, vp, C.POINTER(SolveParams), C.POINTER(SolveStats), vp], C.c_int),
         "masx_spmv": ([vp, C.c_double, vp, C.c_double, vp, vp], C.c_int),
@@ -364,6 +365,20 @@ class Solver:
         check(self.L.masx_set_structure(self.h, sys, len(r), _ptr(r, C.c_int) if len(r) else None,
                                         _ptr(c, C.c_int) if len(c) else None))
         self.systems[sys]["nb"] = len(r)
+
+    def set_structure_device(self, sys, rows, cols, stream=None):
+        """As set_structure, from CUDA int32 tensors (copied on the device)."""
+        import torch
+
+        for t, name in ((rows, "rows"), (cols, "cols")):
+            if not t.is_cuda or t.dtype != torch.int32 or not t.is_contiguous():
+                raise ValueError(f"{name}: expected a contiguous CUDA int32 tensor")
+        if rows.numel() != cols.numel():
+            raise ValueError("rows and cols differ in length")
+        n = rows.numel()
+        check(self.L.masx_set_structure_device(self.h, sys, n, _dev(rows) if n else None, _dev(cols) if n else None,
+                                               stream))
+        self.systems[sys]["nb"] = n
 
     def set_topology(self, sys, nbr_start, nbr_num, nbr_list, part_to_real, real_to_part):
         s, n, l = _u32(nbr_start), _u32(nbr_num), _u32(nbr_list if len(nbr_list) else np.zeros(1))

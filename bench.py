@@ -417,16 +417,24 @@ def batch_leg(args):
     if not args.no_extra:
         # a structure change of every system (new rows/cols and contact pairs,
         # here the same ones: the analysis reruns in full): MAS hierarchies,
-        # block CSR, Galerkin maps, bank entries and SpMV layout on the device,
-        # and for comparison the host analysis
+        # block CSR, Galerkin maps, bank entries and SpMV layout on the device
+        # from host arrays ("ms") and from device-resident arrays
+        # ("device_arrays_ms", no upload of the pattern), and for comparison
+        # the host analysis
         log("step: batch64 structure change")
         restr = {}
-        for key, on_device in (("ms", True), ("host_structure_ms", False)):
+        dev_rc = [(torch.from_numpy(np.ascontiguousarray(w["rows"], np.int32)).cuda(),
+                   torch.from_numpy(np.ascontiguousarray(w["cols"], np.int32)).cuda()) for w in ws]
+        for key, on_device, from_device in (("ms", True, False), ("device_arrays_ms", True, True),
+                                            ("host_structure_ms", False, False)):
             s.set_device_structure(on_device)
             torch.cuda.synchronize()
             t = time.perf_counter()
             for i, w in enumerate(ws):
-                s.set_structure(i, w["rows"], w["cols"])
+                if from_device:
+                    s.set_structure_device(i, *dev_rc[i])
+                else:
+                    s.set_structure(i, w["rows"], w["cols"])
                 s.set_contacts(i, w["pairs"])
             s.setup()
             torch.cuda.synchronize()
@@ -434,6 +442,7 @@ def batch_leg(args):
             if s.device_structure_used() != on_device:
                 raise SystemExit("batch structure analysis ran on the wrong side")
         s.set_device_structure(True)
+        del dev_rc
         out["structure_change"] = restr
         log(f"  structure change: {restr}")
     del s, vals, b, x

@@ -105,6 +105,17 @@ int masx_add_system(masx_solver* s, const masx_system_desc* desc, const int* row
  * whose BCOO is rebuilt each step by Converter::convert
  * (converter.cu:137-284) before MASPreconditioner's set-up. */
 int masx_set_structure(masx_solver* s, int sys, long long nb, const int* rows, const int* cols);
+/* masx_set_structure from device arrays (d_rows, d_cols: nb ints each, as
+ * the reference's BCOO keeps its structure on the GPU in
+ * m_block_row_indices / m_block_col_indices, global_matrix.h:26-27,
+ * read through block_row_indices() / block_col_indices(), :93-107): copied to the
+ * library on `stream` (the arrays may be reused once the stream passes this
+ * call; NULL: the call returns after the copy).  The device structure
+ * analysis (masx_set_device_structure) reads the copy without a host round
+ * trip; an analysis on the host downloads it.  Index range errors are
+ * reported by the next masx_setup. */
+int masx_set_structure_device(masx_solver* s, int sys, long long nb, const int* d_rows, const int* d_cols,
+                              void* stream);
 /* MAS topology: neighbor CSR (load_mesh.cpp:934-1040) and the 32-slot vertex
  * partition (gl_main.cu:1201-1243), host arrays, as uploaded by
  * gl_main.cu:1502-1536 before initPreconditioner_Matrix. */

@@ -65,8 +65,8 @@ def solve(s, w):
     return x.cpu().numpy(), st
 
 
-def compare(sd, sh, label):
-    assert sd.device_structure_used() and not sh.device_structure_used()
+def compare(sd, sh, label, sd_on_device=True):
+    assert sd.device_structure_used() == sd_on_device and not sh.device_structure_used()
     assert sd.info() == sh.info()
     for u in (2, 4, 8):
         sd.set_spmv_tiling(u)
@@ -161,6 +161,52 @@ def test_batch_device_structure_equals_host():
         xs.append((x.cpu().numpy(), [q["iterations"] for q in st]))
     assert xs[0][1] == xs[1][1]
     assert np.array_equal(xs[0][0], xs[1][0])
+    sd.close()
+    sh.close()
+
+
+@pytest.mark.parametrize("device_analysis", [1, 0])
+def test_structure_from_device_arrays(device_analysis):
+    """masx_set_structure_device: a batch whose new contact sets arrive as
+    device arrays gives the same plan and solve as host arrays, with the
+    device analysis (no host round trip of the structure) and with the host
+    analysis (which downloads the device-held structure)."""
+    ws = [W.fem_grid(n, n_contacts=c, seed=40 + n, young_sigma=0.5) for n, c in ((10, 200), (7, 40))]
+    w2 = [W.fem_grid(n, n_contacts=c, seed=50 + n, young_sigma=0.5) for n, c in ((10, 300), (7, 60))]
+    sd = session.describe(ws)
+    sd.set_device_structure(device_analysis)
+    sd.setup()
+    stream = torch.cuda.Stream()
+    for i, w in enumerate(w2):
+        rows = torch.from_numpy(np.ascontiguousarray(w["rows"], np.int32)).cuda()
+        cols = torch.from_numpy(np.ascontiguousarray(w["cols"], np.int32)).cuda()
+        torch.cuda.synchronize()
+        sd.set_structure_device(i, rows, cols, stream=stream.cuda_stream)
+        sd.set_contacts(i, w["pairs"])
+    torch.cuda.synchronize()
+    sd.setup()
+    assert sd.device_structure_used() == bool(device_analysis)
+    sh = session.describe(w2)
+    sh.set_device_structure(0)
+    sh.setup()
+    compare(sd, sh, f"device-held structure, device analysis {device_analysis}", bool(device_analysis))
+    for s in (sd, sh):
+        for i, w in enumerate(w2):
+            s.set_values(i, dev(w["vals"]))
+        s.build("fp32")
+    b = session.rhs_of(w2)
+    xs = []
+    for s in (sd, sh):
+        x = torch.zeros(len(b), dtype=torch.float64, device="cuda")
+        st = s.solve(dev(b), x, rule=1, tol=1e-4)
+        torch.cuda.synchronize()
+        xs.append((x.cpu().numpy(), [q["iterations"] for q in st]))
+    assert xs[0][1] == xs[1][1]
+    assert np.array_equal(xs[0][0], xs[1][0])
+    bad = torch.full((4,), 10**6, dtype=torch.int32, device="cuda")
+    sd.set_structure_device(0, bad, bad)
+    with pytest.raises(RuntimeError):
+        sd.setup()  # out-of-range block indices, found by the analysis
     sd.close()
     sh.close()
 
