@@ -505,6 +505,65 @@ __host__ __device__ constexpr int tile8x16(int rr, int r, int c) { return c * rr
 // scaled by the largest stored entry of the bank's inverse in magnitude, so
 // every stored value lies in [-1, 1] (also for an inverse that floored
 // pivots left indefinite, whose off-diagonal entries can exceed its diagonal).
+// One pass of a bank's dense assembly: the entries' 3 x 3 blocks added into
+// rows [lo, hi) of the packed lower triangle H (LDS, offset by `base`).  A
+// block's lower half only for a diagonal block; fine banks read the FP64
+// input values (transposed for a lower block), coarse banks their Galerkin
+// sums E (row-major).  Each thread takes entries tid, tid + THREADS, ... and
+// loads kFactorEntriesPerRound of them (their words, then their blocks)
+// before adding them in entry order: one dependent memory round trip per
+// round instead of one per entry, and the same sums in the same order.
+constexpr int kFactorEntriesPerRound = 2;  // 64-system batch build: 4 equal, 6 and 8 slower
+
+template <int BANK, int THREADS>
+__device__ __forceinline__ void factor_assemble_pass(const DevData& d, const double* E, bool fine, int e_begin,
+                                                     int e_end, const int* ent_pos, const int* ent_src, int lo,
+                                                     int hi, int base, double* H)
+{
+    constexpr int R = kFactorEntriesPerRound;
+    const double* vsrc = fine ? d.vals64 : E;
+    for(int k = e_begin + (int)threadIdx.x; k < e_end; k += THREADS * R)
+    {
+        int pw[R], sr[R];
+#pragma unroll
+        for(int u = 0; u < R; ++u)
+        {
+            const bool has = k + THREADS * u < e_end;
+            pw[u] = has ? ent_pos[k + THREADS * u] : -1;
+            sr[u] = has ? ent_src[k + THREADS * u] : 0;
+        }
+        double v9[R][9];
+#pragma unroll
+        for(int u = 0; u < R; ++u)
+        {
+            const double* blk = vsrc + 9 * (long long)sr[u];
+#pragma unroll
+            for(int q = 0; q < 9; ++q) v9[u][q] = pw[u] >= 0 ? blk[q] : 0.0;
+        }
+#pragma unroll
+        for(int u = 0; u < R; ++u)
+        {
+            if(pw[u] < 0) break;
+            const int  pos      = pw[u] & 0xffff;
+            const bool rowmajor = !fine || (pw[u] >> 16) != 0;
+            const int  I        = packed_row<BANK>(pos);
+            const int  J        = pos - packed_index(BANK, I, I) + I;
+#pragma unroll
+            for(int aa = 0; aa < 3; ++aa)
+#pragma unroll
+                for(int bb = 0; bb < 3; ++bb)
+                {
+                    const int p = 3 * I + aa, q = 3 * J + bb;
+                    if(I == J && p < q) continue;  // diagonal block: its lower half carries it
+                    const int row = p > q ? p : q, col = p > q ? q : p;
+                    if(row < lo || row >= hi) continue;
+                    const double v = rowmajor ? v9[u][3 * aa + bb] : v9[u][aa + 3 * bb];
+                    atomicAdd(&H[row * (row + 1) / 2 + col - base], v);
+                }
+        }
+    }
+}
+
 template <int BANK, typename T>
 __global__ void __launch_bounds__(128) k_mas_factor_sym(DevData d, const double* E, const int* ent_ptr,
                                                         const int* ent_pos, const int* ent_src, void* out,
@@ -540,27 +599,7 @@ __global__ void __launch_bounds__(128) k_mas_factor_sym(DevData d, const double*
         const int base = lo * (lo + 1) / 2, n = hi * (hi + 1) / 2 - base;
         for(int q = tid; q < n; q += 128) H[q] = 0.0;
         __syncthreads();
-        for(int k = e_begin + tid; k < e_end; k += 128)
-        {
-            const int pos = ent_pos[k] & 0xffff, tr = ent_pos[k] >> 16;
-            const int src = ent_src[k];
-            const int I = packed_row<BANK>(pos);
-            const int J = pos - packed_index(BANK, I, I) + I;
-            for(int aa = 0; aa < 3; ++aa)
-                for(int bb = 0; bb < 3; ++bb)
-                {
-                    const int p = 3 * I + aa, q = 3 * J + bb;
-                    if(I == J && p < q) continue;  // diagonal block: its lower half carries it
-                    const int row = p > q ? p : q, col = p > q ? q : p;
-                    if(row < lo || row >= hi) continue;
-                    double v;
-                    if(fine)
-                        v = tr ? d.vals64[9 * (long long)src + bb + 3 * aa] : d.vals64[9 * (long long)src + aa + 3 * bb];
-                    else
-                        v = E[9 * (long long)src + 3 * aa + bb];
-                    atomicAdd(&H[row * (row + 1) / 2 + col - base], v);
-                }
-        }
+        factor_assemble_pass<BANK, 128>(d, E, fine, e_begin, e_end, ent_pos, ent_src, lo, hi, base, H);
         __syncthreads();
         // zero diagonal -> 1 (padding slots behave as identity)
         for(int q = lo + tid; q < hi; q += 128)
@@ -695,6 +734,7 @@ __global__ void __launch_bounds__(128) k_mas_factor_sym(DevData d, const double*
 // banks share a SIMD.
 __host__ __device__ constexpr int tile8(int r, int c) { return r * (r + 1) / 2 + c; }
 
+
 // value of v in lane `lane` (uniform over the wave)
 __device__ __forceinline__ double readlane_f64(double v, int lane)
 {
@@ -736,27 +776,7 @@ __global__ void __launch_bounds__(64) k_mas_factor_wave(DevData d, const double*
         const int base = lo * (lo + 1) / 2, n = hi * (hi + 1) / 2 - base;
         for(int q = tid; q < n; q += 64) H[q] = 0.0;
         __syncthreads();
-        for(int k = e_begin + tid; k < e_end; k += 64)
-        {
-            const int pos = ent_pos[k] & 0xffff, tr = ent_pos[k] >> 16;
-            const int src = ent_src[k];
-            const int I = packed_row<BANK>(pos);
-            const int J = pos - packed_index(BANK, I, I) + I;
-            for(int aa = 0; aa < 3; ++aa)
-                for(int bb = 0; bb < 3; ++bb)
-                {
-                    const int p = 3 * I + aa, q = 3 * J + bb;
-                    if(I == J && p < q) continue;  // diagonal block: its lower half carries it
-                    const int row = p > q ? p : q, col = p > q ? q : p;
-                    if(row < lo || row >= hi) continue;
-                    double v;
-                    if(fine)
-                        v = tr ? d.vals64[9 * (long long)src + bb + 3 * aa] : d.vals64[9 * (long long)src + aa + 3 * bb];
-                    else
-                        v = E[9 * (long long)src + 3 * aa + bb];
-                    atomicAdd(&H[row * (row + 1) / 2 + col - base], v);
-                }
-        }
+        factor_assemble_pass<BANK, 64>(d, E, fine, e_begin, e_end, ent_pos, ent_src, lo, hi, base, H);
         __syncthreads();
         // zero diagonal -> 1 (padding slots behave as identity)
         for(int q = lo + tid; q < hi; q += 64)
