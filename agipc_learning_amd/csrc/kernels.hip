@@ -2594,6 +2594,39 @@ __global__ void __launch_bounds__(256) k_gather_blocks(const double* src, const 
     dst[t] = src[9 * (long long)idx[k] + t % 9];
 }
 
+// Up to kCopySegments device arrays of doubles copied into one array in a
+// single launch (a batch's per-system values): block b copies chunk
+// b - first[s] of segment s, kCopyChunk doubles, read with nontemporal loads
+// (each value is read once).
+__global__ void __launch_bounds__(256) k_copy_segments(CopySegments c, double* __restrict__ out)
+{
+    const int b = (int)blockIdx.x;
+    int lo = 0, hi = c.count - 1;  // the last segment whose first block is <= b
+    while(lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if(c.first[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    const long long base = (long long)(b - c.first[lo]) * kCopyChunk;
+    const long long m    = min(c.n[lo] - base, (long long)kCopyChunk);
+    const double* src = c.src[lo] + base;
+    double*       dst = out + c.dst[lo] + base;
+    constexpr int U = kCopyChunk / 256;
+    double v[U];
+#pragma unroll
+    for(int u = 0; u < U; ++u)
+    {
+        const int i = (int)threadIdx.x + 256 * u;
+        v[u] = i < m ? __builtin_nontemporal_load(src + i) : 0.0;
+    }
+#pragma unroll
+    for(int u = 0; u < U; ++u)
+    {
+        const int i = (int)threadIdx.x + 256 * u;
+        if(i < m) dst[i] = v[u];
+    }
+}
+
 // ============================================================== launchers
 
 #define MASX_CHECK_LAUNCH() \
@@ -2922,6 +2955,14 @@ hipError_t launch_gather_blocks(const double* src, const uint32_t* idx, double* 
 {
     if(n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_gather_blocks, dim3(blocks_for(9 * n, 256)), dim3(256), 0, st, src, idx, dst, n);
+    MASX_CHECK_LAUNCH();
+    return hipSuccess;
+}
+
+hipError_t launch_copy_segments(const CopySegments& c, double* out, hipStream_t st)
+{
+    if(c.count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy_segments, dim3((unsigned)c.first[c.count]), dim3(256), 0, st, c, out);
     MASX_CHECK_LAUNCH();
     return hipSuccess;
 }

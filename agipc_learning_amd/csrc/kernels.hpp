@@ -59,4 +59,19 @@ hipError_t launch_axpby_int_to_ext(const int* e2i, const double* y_int, double* 
                                    hipStream_t st);
 hipError_t launch_z32_to_ext(const int* e2i, const float* z, double* dst, int n_ext, hipStream_t st);
 hipError_t launch_gather_blocks(const double* src, const uint32_t* idx, double* dst, long long n, hipStream_t st);
+
+// Device arrays of doubles copied into one array by one launch: segment s,
+// n[s] doubles from src[s], goes to out + dst[s]; first[s] is its first block
+// (chunks of kCopyChunk doubles), first[count] the grid size.
+constexpr int kCopySegmentsMax = 64;
+constexpr int kCopyChunk       = 2048;
+struct CopySegments
+{
+    const double* src[kCopySegmentsMax];
+    long long     dst[kCopySegmentsMax];
+    long long     n[kCopySegmentsMax];
+    int           first[kCopySegmentsMax + 1];
+    int           count;
+};
+hipError_t launch_copy_segments(const CopySegments& c, double* out, hipStream_t st);
 }  // namespace masx

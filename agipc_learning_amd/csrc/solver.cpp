@@ -4,6 +4,7 @@
 #include "phase_timer.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -731,13 +732,27 @@ void Solver::set_values_many(int n, const int* systems, const double* const* d_v
     if(!setup_done_) throw std::runtime_error("setup() must precede set_values()");
     for(int q = 0; q < n; ++q) (void)systems_.at(systems[q]);  // range checks before any copy
     hipStream_t w = enter(st);
+    // one launch per kCopySegmentsMax systems instead of one copy per system
+    CopySegments c{};
+    auto flush = [&]() {
+        MASX_HIP(launch_copy_segments(c, d_vals64_.ptr, w));
+        c = CopySegments{};
+    };
     for(int q = 0; q < n; ++q)
     {
         const int s = systems[q];
         const long long nb = systems_[s].nb;
-        if(nb) MASX_HIP(hipMemcpyAsync(d_vals64_.ptr + 9 * plan_.src_off[s], d_values[q], 9 * nb * sizeof(double),
-                                       hipMemcpyDeviceToDevice, w));
+        if(!nb) continue;
+        if(!d_values[q]) throw std::invalid_argument("null values");
+        const long long blocks = (9 * nb + kCopyChunk - 1) / kCopyChunk;
+        if(c.count == kCopySegmentsMax || (long long)c.first[c.count] + blocks > (long long)INT_MAX) flush();
+        c.src[c.count] = d_values[q];
+        c.dst[c.count] = 9 * plan_.src_off[s];
+        c.n[c.count]   = 9 * nb;
+        c.first[c.count + 1] = c.first[c.count] + (int)blocks;
+        ++c.count;
     }
+    flush();
     leave(st);
     built_ = false;
 }

@@ -151,7 +151,8 @@ int masx_set_values(masx_solver* s, int sys, const double* d_values, void* strea
 int masx_set_values_host(masx_solver* s, int sys, const double* h_values, void* stream);
 /* masx_set_values for n systems at once (systems[q] takes d_values[q]; both
  * arrays on the host, the values on the device): one hand-off to the
- * library's stream instead of one per system, for batches.  No counterpart
+ * library's stream instead of one per system, and one copy launch per 64
+ * systems instead of one copy per system, for batches.  No counterpart
  * in the reference (one system per solve there). */
 int masx_set_values_many(masx_solver* s, int n, const int* systems, const double* const* d_values, void* stream);
 

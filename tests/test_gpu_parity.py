@@ -268,6 +268,30 @@ def test_set_values_many_equals_per_system_values():
         s.set_values_many({0: vals[1]})  # wrong length for system 0
 
 
+def test_set_values_many_beyond_one_launch():
+    """70 systems: more than one copy launch takes (64 arrays each); the
+    values in place equal per-system copies, seen through a bit-identical
+    solve of every system."""
+    ws = [W.fem_grid(3 + i % 3, n_contacts=0, seed=i, young_sigma=0.5) for i in range(70)]
+    s = session.solver_for(ws)
+    vals = {i: dev(w["vals"]) for i, w in enumerate(ws)}
+    b = dev(session.rhs_of(ws))
+
+    def solve():
+        s.build("fp32")
+        x = torch.zeros_like(b)
+        st = s.solve(b, x, rule=1, tol=1e-6)
+        return [q["iterations"] for q in st], host(x)
+
+    ref = solve()
+    for i, v in vals.items():
+        s.set_values(i, 0.5 * v)
+    s.set_values_many(vals)
+    got = solve()
+    assert got[0] == ref[0]
+    assert np.array_equal(got[1], ref[1])
+
+
 def test_batched_equals_individual():
     ws = [W.fem_grid(n, n_contacts=c, seed=n, young_sigma=0.5) for n, c in ((6, 20), (11, 150), (9, 0))]
     batch = session.solver_for(ws)
